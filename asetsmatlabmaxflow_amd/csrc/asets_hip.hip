@@ -1,0 +1,723 @@
+// asets_hip.hip -- libasets_hip.so: host orchestration and the extern "C"
+// entry points declared in include/asets.h.
+//
+// A solve is: init kernel(s) -> max_iter iterations, each a few kernels plus a
+// finalize kernel that reduces the per-block FP64 |e| partials in a fixed
+// order, stores cvg[it] and raises a device stop flag when cvg <= err_bound
+// (asetsBinaryMF3D_mex.c:183-188).  Every kernel of a later iteration reads the
+// flag first and returns at once when it is set, so the host can enqueue many
+// iterations ahead and poll the flag only every poll_every iterations; u is
+// the state after iteration it_num, exactly as in the reference loop.
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "asets.h"
+#include "common.h"
+#include "kernels_fused.h"
+#include "kernels_passes.h"
+
+using namespace asets;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+asets_status fail(asets_status s, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return s;
+}
+
+#define HIPCHK(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(ASETS_EDEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kBlock = 256;
+constexpr int kMaxDevices = 64;
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Per-device cached workspace, grown on demand and reused across calls.  A
+// per-device mutex serialises solves on one device (they share the buffers).
+struct DeviceCache {
+    std::mutex mu;     // guards dev and host_flag for the duration of a solve
+    std::mutex io_mu;  // guards io for the duration of a host-pointer call
+    void *dev = nullptr;
+    size_t dev_bytes = 0;
+    void *io = nullptr;  // staging for the host-pointer entry points
+    size_t io_bytes = 0;
+    int32_t *host_flag = nullptr;  // pinned
+};
+DeviceCache g_cache[kMaxDevices];
+
+asets_status ensure(void **p, size_t *cap, size_t need) {
+    if (*cap >= need) return ASETS_OK;
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+        *cap = 0;
+    }
+    if (hipMalloc(p, need) != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(ASETS_ENOMEM, "hipMalloc of %zu bytes failed", need);
+    }
+    *cap = need;
+    return ASETS_OK;
+}
+
+// Bump allocator over a workspace; 256-byte aligned slices.
+struct Carver {
+    char *base;
+    size_t off = 0;
+    explicit Carver(void *b) : base((char *)b) {}
+    template <typename T>
+    T *take(int64_t n) {
+        T *p = (T *)(base + off);
+        off += ((size_t)n * sizeof(T) + 255) & ~(size_t)255;
+        return p;
+    }
+    static size_t sz(int64_t n, size_t es) { return ((size_t)n * es + 255) & ~(size_t)255; }
+};
+
+asets_status check_params(const asets_params *p) {
+    if (!p) return fail(ASETS_EBADARG, "params is NULL");
+    if (p->max_iter < 0) return fail(ASETS_EBADARG, "max_iter must be >= 0 (got %d)", p->max_iter);
+    if (!(p->cc > 0.0f) && !(p->cc < 0.0f))
+        return fail(ASETS_EBADARG, "cc must be non-zero");
+    return ASETS_OK;
+}
+
+asets_status check_dims(int64_t ny, int64_t nx, int64_t nz, int64_t L) {
+    if (ny <= 0 || nx <= 0 || nz <= 0 || L <= 0)
+        return fail(ASETS_EBADARG, "sizes must be positive (ny=%lld nx=%lld nz=%lld L=%lld)",
+                    (long long)ny, (long long)nx, (long long)nz, (long long)L);
+    if (ny * nx > (int64_t)INT32_MAX / 2)
+        return fail(ASETS_ETOOBIG, "a plane of %lld voxels exceeds the 2^30 limit",
+                    (long long)(ny * nx));
+    if (nz > (int64_t)INT32_MAX / 2 || ny * nx * nz * L > ((int64_t)1 << 40))
+        return fail(ASETS_ETOOBIG, "volume too large");
+    return ASETS_OK;
+}
+
+// alpha <= 0 anywhere is an error when at least one iteration runs
+// (asetsBinaryMF3D_mex.c:317-319 raises it inside the first projection pass).
+__global__ void k_alpha_check(const float *alpha, int64_t n, int32_t *bad) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int found = 0;
+    for (; i < n; i += stride)
+        if (alpha[i] <= 0.0f) found = 1;
+    if (found) atomicOr(bad, 1);
+}
+
+asets_status alpha_ok(const float *alpha, int64_t n, hipStream_t s, DeviceCache &dc,
+                      int32_t *scratch_i32) {
+    HIPCHK(hipMemsetAsync(scratch_i32, 0, sizeof(int32_t), s));
+    int64_t blocks = std::min<int64_t>(cdiv(n, kBlock), 4096);
+    k_alpha_check<<<(unsigned)blocks, kBlock, 0, s>>>(alpha, n, scratch_i32);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dc.host_flag, scratch_i32, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (*dc.host_flag) return fail(ASETS_EALPHA, "alpha(x) must be positive. Exiting...");
+    return ASETS_OK;
+}
+
+inline void fill(float *p, float v, int64_t n, hipStream_t s) {
+    if (n > 0) k_fill_f32<<<(unsigned)cdiv(n, kBlock), kBlock, 0, s>>>(p, v, n);
+}
+
+// Enqueue iterations [0, max_iter) via enqueue_iter(it), polling the stop flag
+// on the host every poll iterations.  Returns the number of iterations whose
+// kernels were enqueued.
+template <typename F>
+asets_status run_loop(int32_t max_iter, float err_bound, int32_t poll_opt, hipStream_t s,
+                      DeviceCache &dc, Flags *flags, F &&enqueue_iter) {
+    int32_t poll = poll_opt > 0 ? poll_opt : (err_bound > 0.0f ? 16 : 256);
+    int32_t it = 0;
+    while (it < max_iter) {
+        int32_t k = std::min(poll, max_iter - it);
+        for (int32_t j = 0; j < k; ++j) {
+            asets_status st = enqueue_iter(it + j);
+            if (st != ASETS_OK) return st;
+        }
+        it += k;
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(dc.host_flag, &flags->done, sizeof(int32_t),
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (*dc.host_flag) break;
+        if (poll < 128 && err_bound > 0.0f && poll_opt <= 0) poll *= 2;
+    }
+    return ASETS_OK;
+}
+
+struct TileCfg {
+    int tx;
+    int zchunk;
+};
+
+// Tile shape for the fused kernel: chosen so that a 256^3 volume gives a few
+// workgroups per CU (256 CUs) while keeping the z prologue overhead
+// (4 extra gk planes per chunk) small.  Overridden by asets_opts.
+TileCfg pick_tiles(int64_t ny, int64_t nx, int64_t nz, const asets_opts *o) {
+    TileCfg t;
+    t.tx = (o && o->tile_x > 0) ? o->tile_x : 16;
+    if (t.tx != 8 && t.tx != 16 && t.tx != 32) t.tx = 16;
+    int64_t tiles = cdiv(ny, 64) * cdiv(nx, t.tx);
+    int zc;
+    if (o && o->zchunk > 0) {
+        zc = o->zchunk;
+    } else {
+        // aim for ~1024 workgroups, chunks of at least 16 planes
+        int64_t want_chunks = std::max<int64_t>(1, cdiv(1024, tiles));
+        zc = (int)std::max<int64_t>(16, cdiv(nz, want_chunks));
+    }
+    t.zchunk = (int)std::min<int64_t>(zc, nz);
+    return t;
+}
+
+template <bool IS3D, int TX, typename T>
+void launch_fused_t(const FusedArgs<T> &a, int grid, hipStream_t s) {
+    kf_binary<IS3D, TX, T><<<grid, 256, 0, s>>>(a);
+}
+
+template <typename T>
+void launch_fused(bool is3d, int tx, const FusedArgs<T> &a, int grid, hipStream_t s) {
+    if (is3d) {
+        if (tx == 8) launch_fused_t<true, 8, T>(a, grid, s);
+        else if (tx == 32) launch_fused_t<true, 32, T>(a, grid, s);
+        else launch_fused_t<true, 16, T>(a, grid, s);
+    } else {
+        if (tx == 8) launch_fused_t<false, 8, T>(a, grid, s);
+        else if (tx == 32) launch_fused_t<false, 32, T>(a, grid, s);
+        else launch_fused_t<false, 16, T>(a, grid, s);
+    }
+}
+
+__global__ void k_copy_f32(float *dst, const float *src, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// binary model on device memory
+// ---------------------------------------------------------------------------
+asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float *alpha,
+                        int64_t ny, int64_t nx, int64_t nz, const asets_params *p,
+                        const float *u0, const float *pt0, float *u, float *cvg,
+                        int32_t *it_num, double *seconds, float *o_ps, float *o_pt,
+                        float *o_bx, float *o_by, float *o_bz, const asets_opts *o) {
+    auto t_start = std::chrono::steady_clock::now();
+    if (ndim != 2 && ndim != 3) return fail(ASETS_EBADARG, "ndim must be 2 or 3");
+    if (ndim == 2 && nz != 1) return fail(ASETS_EBADARG, "2D solve needs nz == 1");
+    asets_status st = check_params(p);
+    if (st) return st;
+    st = check_dims(ny, nx, nz, 1);
+    if (st) return st;
+    if (!Cs || !Ct || !alpha || !u || !it_num || (p->max_iter > 0 && !cvg))
+        return fail(ASETS_EBADARG, "null array argument");
+    if ((u0 == nullptr) != (pt0 == nullptr))
+        return fail(ASETS_EBADARG, "warm start needs both u0 and pt0");
+    const int flow16 = o ? o->flow_dtype : ASETS_FLOW_F32;
+    if (flow16 != ASETS_FLOW_F32 && flow16 != ASETS_FLOW_F16)
+        return fail(ASETS_EBADARG, "unknown flow_dtype %d", flow16);
+    const int kern = o ? o->kernel : ASETS_KERNEL_AUTO;
+    const bool fused = (kern != ASETS_KERNEL_PASSES);
+    if (flow16 == ASETS_FLOW_F16 && !fused)
+        return fail(ASETS_EBADARG, "fp16 flow storage needs the fused kernel");
+
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(ASETS_EDEVICE, "device id %d", dev);
+    DeviceCache &dc = g_cache[dev];
+    std::lock_guard<std::mutex> lock(dc.mu);
+    hipStream_t s = o ? (hipStream_t)o->stream : nullptr;
+    if (!dc.host_flag) HIPCHK(hipHostMalloc((void **)&dc.host_flag, 64));
+
+    const bool is3d = (ndim == 3);
+    const int64_t N = ny * nx * nz;
+    const Geo G{ny, nx, nz, N, 1};
+    const int32_t max_iter = p->max_iter;
+    TileCfg tc = pick_tiles(ny, nx, nz, o);
+    const int ntile_y = (int)cdiv(ny, 64), ntile_x = (int)cdiv(nx, tc.tx);
+    const int nzc = (int)cdiv(nz, tc.zchunk);
+    const int nblocks = ntile_y * ntile_x * nzc;
+    const int grid_fused = (int)cdiv(nblocks, 8) * 8;
+    const int64_t nparts = fused ? grid_fused : cdiv(N, kBlock);
+
+    // workspace
+    const size_t es = (flow16 == ASETS_FLOW_F16) ? sizeof(__half) : sizeof(float);
+    size_t need = Carver::sz(1, sizeof(Flags)) + Carver::sz(nparts, sizeof(double)) +
+                  Carver::sz(1, sizeof(int32_t));
+    if (fused)
+        need += 2 * (Carver::sz(N, sizeof(float)) + 5 * Carver::sz(N, es));
+    else
+        need += 7 * Carver::sz(N, sizeof(float));
+    st = ensure(&dc.dev, &dc.dev_bytes, need);
+    if (st) return st;
+    Carver cv(dc.dev);
+    Flags *flags = cv.take<Flags>(1);
+    double *partials = cv.take<double>(nparts);
+    int32_t *scratch_i32 = cv.take<int32_t>(1);
+
+    if (max_iter > 0) {
+        st = alpha_ok(alpha, N, s, dc, scratch_i32);
+        if (st) return st;
+    }
+
+    k_fill_i32<<<1, 64, 0, s>>>((int32_t *)flags, 0, sizeof(Flags) / 4);
+    k_fill_i32<<<1, 1, 0, s>>>(it_num, max_iter, 1);
+    fill(cvg, 0.0f, max_iter, s);
+    HIPCHK(hipGetLastError());
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (o && o->kernel_ms) {
+        HIPCHK(hipEventCreate(&ev0));
+        HIPCHK(hipEventCreate(&ev1));
+    }
+
+    if (!fused) {
+        float *ps = cv.take<float>(N), *pt = cv.take<float>(N), *bx = cv.take<float>(N);
+        float *by = cv.take<float>(N), *bz = cv.take<float>(N), *dv = cv.take<float>(N);
+        float *gk = cv.take<float>(N);
+        fill(bx, 0.0f, N, s);
+        fill(by, 0.0f, N, s);
+        fill(bz, 0.0f, N, s);
+        fill(dv, 0.0f, N, s);
+        k_init_binary<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Cs, Ct, u0, pt0, u, ps, pt, N);
+        HIPCHK(hipGetLastError());
+        const unsigned nb = (unsigned)cdiv(N, kBlock);
+        if (ev0) HIPCHK(hipEventRecord(ev0, s));
+        auto iter = [&](int32_t it) -> asets_status {
+            kp_gk<<<nb, kBlock, 0, s>>>(dv, ps, pt, u, gk, p->cc, G, flags);
+            if (is3d) {
+                kp_step<true><<<nb, kBlock, 0, s>>>(gk, bx, by, bz, p->steps, G, flags);
+                kp_proj<true><<<nb, kBlock, 0, s>>>(bx, by, bz, alpha, gk, G, flags);
+                kp_rescale<true><<<nb, kBlock, 0, s>>>(gk, bx, by, bz, G, flags);
+                kp_update_binary<true><<<nb, kBlock, 0, s>>>(bx, by, bz, dv, ps, pt, u, Cs, Ct,
+                                                            p->cc, G, partials, flags);
+            } else {
+                kp_step<false><<<nb, kBlock, 0, s>>>(gk, bx, by, bz, p->steps, G, flags);
+                kp_proj<false><<<nb, kBlock, 0, s>>>(bx, by, bz, alpha, gk, G, flags);
+                kp_rescale<false><<<nb, kBlock, 0, s>>>(gk, bx, by, bz, G, flags);
+                kp_update_binary<false><<<nb, kBlock, 0, s>>>(bx, by, bz, dv, ps, pt, u, Cs, Ct,
+                                                             p->cc, G, partials, flags);
+            }
+            k_finalize<<<1, 1024, 0, s>>>(partials, nparts, (double)N, it, p->err_bound, cvg,
+                                          it_num, flags);
+            return ASETS_OK;
+        };
+        st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc, flags, iter);
+        if (st) return st;
+        if (ev1) HIPCHK(hipEventRecord(ev1, s));
+        if (o_ps) HIPCHK(hipMemcpyAsync(o_ps, ps, N * 4, hipMemcpyDeviceToDevice, s));
+        if (o_pt) HIPCHK(hipMemcpyAsync(o_pt, pt, N * 4, hipMemcpyDeviceToDevice, s));
+        if (o_bx) HIPCHK(hipMemcpyAsync(o_bx, bx, N * 4, hipMemcpyDeviceToDevice, s));
+        if (o_by) HIPCHK(hipMemcpyAsync(o_by, by, N * 4, hipMemcpyDeviceToDevice, s));
+        if (o_bz) HIPCHK(hipMemcpyAsync(o_bz, bz, N * 4, hipMemcpyDeviceToDevice, s));
+    } else {
+        // two state buffers; iteration i reads buf[i&1] and writes buf[(i+1)&1]
+        float *ub[2];
+        void *fb[2][5];
+        for (int b = 0; b < 2; ++b) {
+            ub[b] = cv.take<float>(N);
+            for (int k = 0; k < 5; ++k) fb[b][k] = cv.take<char>(N * (int64_t)es);
+        }
+        // init buffer 0 (fp32 scratch in buffer 1's ps/pt when storing fp16)
+        if (flow16 == ASETS_FLOW_F32) {
+            float **f0 = (float **)fb[0];
+            k_init_binary<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Cs, Ct, u0, pt0, ub[0],
+                                                                      f0[0], f0[1], N);
+            for (int b = 0; b < 2; ++b)
+                for (int k = 2; k < 5; ++k) fill((float *)fb[b][k], 0.0f, N, s);
+        } else {
+            return fail(ASETS_EBADARG, "fp16 flow storage is not available in this build");
+        }
+        HIPCHK(hipGetLastError());
+
+        FusedArgs<float> fa{};
+        fa.Cs = Cs;
+        fa.Ct = Ct;
+        fa.alpha = alpha;
+        fa.ny = (int)ny;
+        fa.nx = (int)nx;
+        fa.nz = (int)nz;
+        fa.plane = ny * nx;
+        fa.cc = p->cc;
+        fa.steps = p->steps;
+        fa.zchunk = tc.zchunk;
+        fa.ntile_y = ntile_y;
+        fa.ntile_x = ntile_x;
+        fa.nzc = nzc;
+        fa.nblocks = nblocks;
+        fa.partials = partials;
+        fa.flags = flags;
+        auto state = [&](int b) {
+            FusedState<float> x;
+            x.u = ub[b];
+            x.ps = (float *)fb[b][0];
+            x.pt = (float *)fb[b][1];
+            x.bx = (float *)fb[b][2];
+            x.by = (float *)fb[b][3];
+            x.bz = (float *)fb[b][4];
+            return x;
+        };
+        if (ev0) HIPCHK(hipEventRecord(ev0, s));
+        auto iter = [&](int32_t it) -> asets_status {
+            fa.in = state(it & 1);
+            fa.out = state((it + 1) & 1);
+            launch_fused<float>(is3d, tc.tx, fa, grid_fused, s);
+            k_finalize<<<1, 1024, 0, s>>>(partials, nparts, (double)N, it, p->err_bound, cvg,
+                                          it_num, flags);
+            return ASETS_OK;
+        };
+        st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc, flags, iter);
+        if (st) return st;
+        if (ev1) HIPCHK(hipEventRecord(ev1, s));
+        // final state lives in buf[n_exec & 1]
+        int32_t itn = 0;
+        HIPCHK(hipMemcpyAsync(&itn, it_num, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const int32_t n_exec = (itn < max_iter) ? itn + 1 : max_iter;
+        FusedState<float> fin = state(n_exec & 1);
+        HIPCHK(hipMemcpyAsync(u, fin.u, N * 4, hipMemcpyDeviceToDevice, s));
+        if (o_ps) HIPCHK(hipMemcpyAsync(o_ps, fin.ps, N * 4, hipMemcpyDeviceToDevice, s));
+        if (o_pt) HIPCHK(hipMemcpyAsync(o_pt, fin.pt, N * 4, hipMemcpyDeviceToDevice, s));
+        if (o_bx) HIPCHK(hipMemcpyAsync(o_bx, fin.bx, N * 4, hipMemcpyDeviceToDevice, s));
+        if (o_by) HIPCHK(hipMemcpyAsync(o_by, fin.by, N * 4, hipMemcpyDeviceToDevice, s));
+        if (o_bz) HIPCHK(hipMemcpyAsync(o_bz, fin.bz, N * 4, hipMemcpyDeviceToDevice, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    if (ev0) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        int32_t itn = 0;
+        HIPCHK(hipMemcpy(&itn, it_num, sizeof(int32_t), hipMemcpyDeviceToHost));
+        const int32_t n_exec = (itn < max_iter) ? itn + 1 : max_iter;
+        *o->kernel_ms = n_exec > 0 ? (double)ms / n_exec : 0.0;
+        (void)hipEventDestroy(ev0);
+        (void)hipEventDestroy(ev1);
+    }
+    if (seconds)
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    return ASETS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Potts model on device memory
+// ---------------------------------------------------------------------------
+asets_status potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny, int64_t nx,
+                       int64_t nz, int64_t L, const asets_params *p, float *u, float *cvg,
+                       int32_t *it_num, double *seconds, const asets_opts *o) {
+    auto t_start = std::chrono::steady_clock::now();
+    if (ndim != 2 && ndim != 3) return fail(ASETS_EBADARG, "ndim must be 2 or 3");
+    if (ndim == 2 && nz != 1) return fail(ASETS_EBADARG, "2D solve needs nz == 1");
+    asets_status st = check_params(p);
+    if (st) return st;
+    st = check_dims(ny, nx, nz, L);
+    if (st) return st;
+    if (!Ct || !alpha || !u || !it_num || (p->max_iter > 0 && !cvg))
+        return fail(ASETS_EBADARG, "null array argument");
+    if (o && o->flow_dtype != ASETS_FLOW_F32)
+        return fail(ASETS_EBADARG, "fp16 flow storage is available for the binary model only");
+
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(ASETS_EDEVICE, "device id %d", dev);
+    DeviceCache &dc = g_cache[dev];
+    std::lock_guard<std::mutex> lock(dc.mu);
+    hipStream_t s = o ? (hipStream_t)o->stream : nullptr;
+    if (!dc.host_flag) HIPCHK(hipHostMalloc((void **)&dc.host_flag, 64));
+
+    const bool is3d = (ndim == 3);
+    const int64_t N = ny * nx * nz, NL = N * L;
+    const Geo G{ny, nx, nz, N, L};
+    const int32_t max_iter = p->max_iter;
+    const int64_t nparts = cdiv(N, kBlock);
+    size_t need = Carver::sz(1, sizeof(Flags)) + Carver::sz(nparts, sizeof(double)) +
+                  Carver::sz(1, sizeof(int32_t)) + 6 * Carver::sz(NL, 4) + Carver::sz(N, 4);
+    st = ensure(&dc.dev, &dc.dev_bytes, need);
+    if (st) return st;
+    Carver cv(dc.dev);
+    Flags *flags = cv.take<Flags>(1);
+    double *partials = cv.take<double>(nparts);
+    int32_t *scratch_i32 = cv.take<int32_t>(1);
+    float *bx = cv.take<float>(NL), *by = cv.take<float>(NL), *bz = cv.take<float>(NL);
+    float *dv = cv.take<float>(NL), *pt = cv.take<float>(NL), *gk = cv.take<float>(NL);
+    float *ps = cv.take<float>(N);
+
+    if (max_iter > 0) {
+        st = alpha_ok(alpha, NL, s, dc, scratch_i32);
+        if (st) return st;
+    }
+    k_fill_i32<<<1, 64, 0, s>>>((int32_t *)flags, 0, sizeof(Flags) / 4);
+    k_fill_i32<<<1, 1, 0, s>>>(it_num, max_iter, 1);
+    fill(cvg, 0.0f, max_iter, s);
+    fill(bx, 0.0f, NL, s);
+    fill(by, 0.0f, NL, s);
+    fill(bz, 0.0f, NL, s);
+    fill(dv, 0.0f, NL, s);
+    k_init_potts<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Ct, u, ps, pt, N, L);
+    HIPCHK(hipGetLastError());
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (o && o->kernel_ms) {
+        HIPCHK(hipEventCreate(&ev0));
+        HIPCHK(hipEventCreate(&ev1));
+        HIPCHK(hipEventRecord(ev0, s));
+    }
+    const unsigned nbl = (unsigned)cdiv(NL, kBlock), nbv = (unsigned)cdiv(N, kBlock);
+    auto iter = [&](int32_t it) -> asets_status {
+        kp_gk<<<nbl, kBlock, 0, s>>>(dv, ps, pt, u, gk, p->cc, G, flags);
+        if (is3d) {
+            kp_step<true><<<nbl, kBlock, 0, s>>>(gk, bx, by, bz, p->steps, G, flags);
+            kp_proj<true><<<nbl, kBlock, 0, s>>>(bx, by, bz, alpha, gk, G, flags);
+            kp_rescale<true><<<nbl, kBlock, 0, s>>>(gk, bx, by, bz, G, flags);
+            kp_ptdiv<true><<<nbl, kBlock, 0, s>>>(bx, by, bz, dv, ps, pt, u, Ct, p->cc, G, flags);
+        } else {
+            kp_step<false><<<nbl, kBlock, 0, s>>>(gk, bx, by, bz, p->steps, G, flags);
+            kp_proj<false><<<nbl, kBlock, 0, s>>>(bx, by, bz, alpha, gk, G, flags);
+            kp_rescale<false><<<nbl, kBlock, 0, s>>>(gk, bx, by, bz, G, flags);
+            kp_ptdiv<false><<<nbl, kBlock, 0, s>>>(bx, by, bz, dv, ps, pt, u, Ct, p->cc, G, flags);
+        }
+        kp_psu<<<nbv, kBlock, 0, s>>>(dv, pt, u, ps, p->cc, G, partials, flags);
+        k_finalize<<<1, 1024, 0, s>>>(partials, nparts, (double)NL, it, p->err_bound, cvg, it_num,
+                                      flags);
+        return ASETS_OK;
+    };
+    st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc, flags, iter);
+    if (st) return st;
+    if (ev1) HIPCHK(hipEventRecord(ev1, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (ev0) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        int32_t itn = 0;
+        HIPCHK(hipMemcpy(&itn, it_num, sizeof(int32_t), hipMemcpyDeviceToHost));
+        const int32_t n_exec = (itn < max_iter) ? itn + 1 : max_iter;
+        *o->kernel_ms = n_exec > 0 ? (double)ms / n_exec : 0.0;
+        (void)hipEventDestroy(ev0);
+        (void)hipEventDestroy(ev1);
+    }
+    if (seconds)
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    return ASETS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host-pointer wrappers (the C-MEX call surface)
+// ---------------------------------------------------------------------------
+struct HostIo {
+    float *d_in[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    float *d_u = nullptr, *d_cvg = nullptr;
+    int32_t *d_it = nullptr;
+};
+
+// Device staging for host arrays; a separate cache from the solver workspace.
+asets_status stage(int nin, const float *const *h_in, int64_t n_in, int64_t n_u,
+                   int32_t max_iter, HostIo &io, hipStream_t s) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    DeviceCache &dc = g_cache[dev];
+    size_t need = nin * Carver::sz(n_in, 4) + Carver::sz(n_u, 4) +
+                  Carver::sz(std::max(max_iter, 1), 4) + Carver::sz(1, 4);
+    asets_status st = ensure(&dc.io, &dc.io_bytes, need);
+    if (st) return st;
+    Carver cv(dc.io);
+    for (int i = 0; i < nin; ++i) {
+        io.d_in[i] = cv.take<float>(n_in);
+        HIPCHK(hipMemcpyAsync(io.d_in[i], h_in[i], n_in * 4, hipMemcpyHostToDevice, s));
+    }
+    io.d_u = cv.take<float>(n_u);
+    io.d_cvg = cv.take<float>(std::max(max_iter, 1));
+    io.d_it = cv.take<int32_t>(1);
+    return ASETS_OK;
+}
+
+asets_status unstage(const HostIo &io, int64_t n_u, int32_t max_iter, float *u, float *cvg,
+                     int32_t *it_num, hipStream_t s) {
+    HIPCHK(hipMemcpyAsync(u, io.d_u, n_u * 4, hipMemcpyDeviceToHost, s));
+    if (max_iter > 0 && cvg)
+        HIPCHK(hipMemcpyAsync(cvg, io.d_cvg, (size_t)max_iter * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(it_num, io.d_it, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return ASETS_OK;
+}
+
+asets_status binary_host(int ndim, const float *Cs, const float *Ct, const float *alpha,
+                         int64_t ny, int64_t nx, int64_t nz, const asets_params *p,
+                         const float *u0, const float *pt0, float *u, float *cvg,
+                         int32_t *it_num, double *seconds, const asets_opts *o) {
+    auto t0 = std::chrono::steady_clock::now();
+    asets_status st = check_params(p);
+    if (st) return st;
+    st = check_dims(ny, nx, nz, 1);
+    if (st) return st;
+    if (!Cs || !Ct || !alpha || !u || !it_num || (p->max_iter > 0 && !cvg))
+        return fail(ASETS_EBADARG, "null array argument");
+    if ((u0 == nullptr) != (pt0 == nullptr))
+        return fail(ASETS_EBADARG, "warm start needs both u0 and pt0");
+    const int64_t N = ny * nx * nz;
+    hipStream_t s = o ? (hipStream_t)o->stream : nullptr;
+    const float *in[5] = {Cs, Ct, alpha, u0, pt0};
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> io_lock(g_cache[dev].io_mu);
+    HostIo io;
+    st = stage(u0 ? 5 : 3, in, N, N, p->max_iter, io, s);
+    if (st) return st;
+    st = binary_dev(ndim, io.d_in[0], io.d_in[1], io.d_in[2], ny, nx, nz, p, io.d_in[3], io.d_in[4],
+                    io.d_u, io.d_cvg, io.d_it, nullptr, nullptr, nullptr, nullptr, nullptr,
+                    nullptr, o);
+    if (st) return st;
+    st = unstage(io, N, p->max_iter, u, cvg, it_num, s);
+    if (st) return st;
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return ASETS_OK;
+}
+
+asets_status potts_host(int ndim, const float *Ct, const float *alpha, int64_t ny, int64_t nx,
+                        int64_t nz, int64_t L, const asets_params *p, float *u, float *cvg,
+                        int32_t *it_num, double *seconds, const asets_opts *o) {
+    auto t0 = std::chrono::steady_clock::now();
+    asets_status st = check_params(p);
+    if (st) return st;
+    st = check_dims(ny, nx, nz, L);
+    if (st) return st;
+    if (!Ct || !alpha || !u || !it_num || (p->max_iter > 0 && !cvg))
+        return fail(ASETS_EBADARG, "null array argument");
+    const int64_t NL = ny * nx * nz * L;
+    hipStream_t s = o ? (hipStream_t)o->stream : nullptr;
+    const float *in[2] = {Ct, alpha};
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> io_lock(g_cache[dev].io_mu);
+    HostIo io;
+    st = stage(2, in, NL, NL, p->max_iter, io, s);
+    if (st) return st;
+    st = potts_dev(ndim, io.d_in[0], io.d_in[1], ny, nx, nz, L, p, io.d_u, io.d_cvg, io.d_it,
+                   nullptr, o);
+    if (st) return st;
+    st = unstage(io, NL, p->max_iter, u, cvg, it_num, s);
+    if (st) return st;
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return ASETS_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// extern "C" entry points
+// ---------------------------------------------------------------------------
+extern "C" {
+
+asets_status asets_binary2d(const float *Cs, const float *Ct, const float *alpha, int64_t ny,
+                            int64_t nx, const asets_params *p, float *u, float *cvg,
+                            int32_t *it_num, double *seconds, const asets_opts *o) {
+    return binary_host(2, Cs, Ct, alpha, ny, nx, 1, p, nullptr, nullptr, u, cvg, it_num, seconds,
+                       o);
+}
+
+asets_status asets_binary3d(const float *Cs, const float *Ct, const float *alpha, int64_t ny,
+                            int64_t nx, int64_t nz, const asets_params *p, float *u, float *cvg,
+                            int32_t *it_num, double *seconds, const asets_opts *o) {
+    return binary_host(3, Cs, Ct, alpha, ny, nx, nz, p, nullptr, nullptr, u, cvg, it_num,
+                       seconds, o);
+}
+
+asets_status asets_binary_warm(int ndim, const float *Cs, const float *Ct, const float *alpha,
+                               int64_t ny, int64_t nx, int64_t nz, const asets_params *p,
+                               const float *u0, const float *pt0, float *u, float *cvg,
+                               int32_t *it_num, double *seconds, const asets_opts *o) {
+    if (ndim != 2 && ndim != 3) return fail(ASETS_EBADARG, "ndim must be 2 or 3");
+    if (ndim == 2 && nz != 1) return fail(ASETS_EBADARG, "2D solve needs nz == 1");
+    return binary_host(ndim, Cs, Ct, alpha, ny, nx, nz, p, u0, pt0, u, cvg, it_num, seconds, o);
+}
+
+asets_status asets_potts2d(const float *Ct, const float *alpha, int64_t ny, int64_t nx,
+                           int64_t nlab, const asets_params *p, float *u, float *cvg,
+                           int32_t *it_num, double *seconds, const asets_opts *o) {
+    return potts_host(2, Ct, alpha, ny, nx, 1, nlab, p, u, cvg, it_num, seconds, o);
+}
+
+asets_status asets_potts3d(const float *Ct, const float *alpha, int64_t ny, int64_t nx,
+                           int64_t nz, int64_t nlab, const asets_params *p, float *u,
+                           float *cvg, int32_t *it_num, double *seconds, const asets_opts *o) {
+    return potts_host(3, Ct, alpha, ny, nx, nz, nlab, p, u, cvg, it_num, seconds, o);
+}
+
+asets_status asets_binary_dev(int ndim, const float *Cs, const float *Ct, const float *alpha,
+                              int64_t ny, int64_t nx, int64_t nz, const asets_params *p,
+                              const float *u0, const float *pt0, float *u, float *cvg,
+                              int32_t *it_num, double *seconds, const asets_opts *o) {
+    return binary_dev(ndim, Cs, Ct, alpha, ny, nx, nz, p, u0, pt0, u, cvg, it_num, seconds,
+                      nullptr, nullptr, nullptr, nullptr, nullptr, o);
+}
+
+asets_status asets_binary_dev_state(int ndim, const float *Cs, const float *Ct,
+                                    const float *alpha, int64_t ny, int64_t nx, int64_t nz,
+                                    const asets_params *p, const float *u0, const float *pt0,
+                                    float *u, float *cvg, int32_t *it_num, double *seconds,
+                                    float *ps, float *pt, float *bx, float *by, float *bz,
+                                    const asets_opts *o) {
+    return binary_dev(ndim, Cs, Ct, alpha, ny, nx, nz, p, u0, pt0, u, cvg, it_num, seconds, ps,
+                      pt, bx, by, bz, o);
+}
+
+asets_status asets_potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny,
+                             int64_t nx, int64_t nz, int64_t nlab, const asets_params *p,
+                             float *u, float *cvg, int32_t *it_num, double *seconds,
+                             const asets_opts *o) {
+    return potts_dev(ndim, Ct, alpha, ny, nx, nz, nlab, p, u, cvg, it_num, seconds, o);
+}
+
+asets_status asets_binary3d_batched_dev(int32_t nvol, const asets_volume *vols,
+                                        const asets_params *p, double *seconds,
+                                        const asets_opts *o) {
+    (void)nvol;
+    (void)vols;
+    (void)p;
+    (void)seconds;
+    (void)o;
+    return fail(ASETS_EBADARG, "batched solve is not available in this build");
+}
+
+const char *asets_last_error(void) { return g_last_error.c_str(); }
+
+const char *asets_version(void) { return "asets-mi355x 0.1 (gfx950)"; }
+
+void asets_release(void) {
+    for (int d = 0; d < kMaxDevices; ++d) {
+        DeviceCache &dc = g_cache[d];
+        std::lock_guard<std::mutex> lock(dc.mu);
+        if (!dc.dev && !dc.io && !dc.host_flag) continue;
+        int prev = 0;
+        (void)hipGetDevice(&prev);
+        (void)hipSetDevice(d);
+        if (dc.dev) (void)hipFree(dc.dev);
+        if (dc.io) (void)hipFree(dc.io);
+        if (dc.host_flag) (void)hipHostFree(dc.host_flag);
+        dc.dev = dc.io = nullptr;
+        dc.host_flag = nullptr;
+        dc.dev_bytes = dc.io_bytes = 0;
+        (void)hipSetDevice(prev);
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+"""Device-memory solver API over the C ABI (torch tensors are used as device buffers only).
+
+Volumes are flat float32 tensors holding the column-major (MATLAB) layout: index
+g = z*Nx*Ny + x*Ny + y, Ny fastest; a Potts field stores label l at offset l*N.  Use
+``to_device``/``from_device`` to move numpy arrays of shape (Ny, Nx[, Nz][, L]) in and out.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as nat
+
+try:
+    import torch
+except ImportError:  # pragma: no cover - torch is part of the image
+    torch = None
+
+
+@dataclass
+class Result:
+    u: "torch.Tensor"
+    cvg: "torch.Tensor"
+    it_num: int
+    seconds: float
+    kernel_ms: float | None = None
+    state: dict = field(default_factory=dict)
+
+    @property
+    def iterations(self) -> int:
+        """Iterations executed: it_num + 1 if the stop test fired, else max_iter."""
+        return self._executed
+
+    _executed: int = 0
+
+
+def to_device(a: np.ndarray, device="cuda") -> "torch.Tensor":
+    flat = np.asarray(a, dtype=np.float32).ravel(order="F")
+    return torch.from_numpy(np.ascontiguousarray(flat)).to(device)
+
+
+def from_device(t: "torch.Tensor", shape) -> np.ndarray:
+    return t.detach().cpu().numpy().reshape(shape, order="F")
+
+
+def _ptr(t) -> int | None:
+    if t is None:
+        return None
+    if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+        raise ValueError("expected a contiguous float32 CUDA tensor")
+    return t.data_ptr()
+
+
+_KERNELS = {"auto": nat.KERNEL_AUTO, "fused": nat.KERNEL_FUSED, "passes": nat.KERNEL_PASSES}
+_FLOWS = {"f32": nat.FLOW_F32, "f16": nat.FLOW_F16}
+
+
+def _stream_handle(stream) -> int | None:
+    if stream is None:
+        return torch.cuda.current_stream().cuda_stream
+    return stream.cuda_stream
+
+
+def binary(Cs, Ct, alpha, shape, max_iter: int, err_bound: float, cc: float, steps: float, *,
+           u0=None, pt0=None, kernel: str = "auto", flow: str = "f32", tile_x: int = 0,
+           zchunk: int = 0, poll_every: int = 0, stream=None, want_state: bool = False,
+           time_kernel: bool = False, out=None) -> Result:
+    """Binary continuous max-flow (asetsBinaryMF2D/3D_mex semantics) on device tensors."""
+    if len(shape) == 2:
+        ndim, (ny, nx), nz = 2, shape, 1
+    elif len(shape) == 3:
+        ndim, (ny, nx, nz) = 3, shape
+    else:
+        raise ValueError("shape must be (Ny, Nx) or (Ny, Nx, Nz)")
+    N = ny * nx * nz
+    for t in (Cs, Ct, alpha):
+        if t.numel() != N:
+            raise ValueError(f"array has {t.numel()} elements, shape implies {N}")
+    dev = Cs.device
+    u = out if out is not None else torch.empty(N, dtype=torch.float32, device=dev)
+    cvg = torch.empty(max(max_iter, 1), dtype=torch.float32, device=dev)
+    itn = torch.empty(1, dtype=torch.int32, device=dev)
+    secs = C.c_double(0.0)
+    kms = C.c_double(0.0) if time_kernel else None
+    opts = nat.make_opts(_stream_handle(stream), _FLOWS[flow], _KERNELS[kernel], poll_every,
+                         tile_x, zchunk, kms)
+    p = nat.make_params(max_iter, err_bound, cc, steps)
+    L = nat.lib()
+    if want_state:
+        st = {k: torch.empty(N, dtype=torch.float32, device=dev)
+              for k in ("ps", "pt", "bx", "by", "bz")}
+        rc = L.asets_binary_dev_state(ndim, _ptr(Cs), _ptr(Ct), _ptr(alpha), ny, nx, nz,
+                                      C.byref(p), _ptr(u0), _ptr(pt0), _ptr(u), _ptr(cvg),
+                                      itn.data_ptr(), C.byref(secs), _ptr(st["ps"]),
+                                      _ptr(st["pt"]), _ptr(st["bx"]), _ptr(st["by"]),
+                                      _ptr(st["bz"]), C.byref(opts))
+    else:
+        st = {}
+        rc = L.asets_binary_dev(ndim, _ptr(Cs), _ptr(Ct), _ptr(alpha), ny, nx, nz, C.byref(p),
+                                _ptr(u0), _ptr(pt0), _ptr(u), _ptr(cvg), itn.data_ptr(),
+                                C.byref(secs), C.byref(opts))
+    nat.check(rc)
+    it = int(itn.item())
+    r = Result(u=u, cvg=cvg[:max_iter], it_num=it, seconds=secs.value,
+               kernel_ms=kms.value if kms is not None else None, state=st)
+    r._executed = it + 1 if it < max_iter else max_iter
+    return r
+
+
+def potts(Ct, alpha, shape, nlab: int, max_iter: int, err_bound: float, cc: float,
+          steps: float, *, kernel: str = "auto", poll_every: int = 0, stream=None,
+          time_kernel: bool = False) -> Result:
+    """Potts continuous max-flow (asetsPotts2D/3D_mex semantics) on device tensors."""
+    if len(shape) == 2:
+        ndim, (ny, nx), nz = 2, shape, 1
+    elif len(shape) == 3:
+        ndim, (ny, nx, nz) = 3, shape
+    else:
+        raise ValueError("shape must be (Ny, Nx) or (Ny, Nx, Nz)")
+    NL = ny * nx * nz * nlab
+    for t in (Ct, alpha):
+        if t.numel() != NL:
+            raise ValueError(f"array has {t.numel()} elements, shape implies {NL}")
+    dev = Ct.device
+    u = torch.empty(NL, dtype=torch.float32, device=dev)
+    cvg = torch.empty(max(max_iter, 1), dtype=torch.float32, device=dev)
+    itn = torch.empty(1, dtype=torch.int32, device=dev)
+    secs = C.c_double(0.0)
+    kms = C.c_double(0.0) if time_kernel else None
+    opts = nat.make_opts(_stream_handle(stream), nat.FLOW_F32, _KERNELS[kernel], poll_every,
+                         0, 0, kms)
+    p = nat.make_params(max_iter, err_bound, cc, steps)
+    rc = nat.lib().asets_potts_dev(ndim, _ptr(Ct), _ptr(alpha), ny, nx, nz, nlab, C.byref(p),
+                                   _ptr(u), _ptr(cvg), itn.data_ptr(), C.byref(secs),
+                                   C.byref(opts))
+    nat.check(rc)
+    it = int(itn.item())
+    r = Result(u=u, cvg=cvg[:max_iter], it_num=it, seconds=secs.value,
+               kernel_ms=kms.value if kms is not None else None)
+    r._executed = it + 1 if it < max_iter else max_iter
+    return r

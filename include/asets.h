@@ -1,0 +1,174 @@
+/*
+ * asets.h -- C ABI of libasets_hip.so, the gfx950 (MI355X) implementation of
+ * ASETS' continuous max-flow primal-dual iteration.
+ *
+ * No HIP, torch or C++ types cross this boundary: plain pointers, sizes and
+ * POD structs.  Every entry point returns an asets_status and never aborts the
+ * process; asets_last_error() gives the message of the calling thread's last
+ * failure.
+ *
+ * Array convention (as the MATLAB C-MEX solvers it replaces): float32,
+ * column-major, index g = z*Nx*Ny + x*Ny + y with Ny (rows, pars[0]) fastest;
+ * Potts label l at offset l*Nx*Ny*Nz.
+ *
+ * Outputs follow maxflow/asetsBinaryMF3D_mex.c:91-118:
+ *   u       [Ny Nx (Nz) (L)]  labelling function
+ *   cvg     [max_iter]        per-iteration mean |delta u|; entries after the
+ *                             stop index are 0
+ *   it_num                    0-based index of the iteration at which
+ *                             cvg <= err_bound held, or max_iter if none did
+ *                             (asetsBinaryMF3D_mex.c:163-192)
+ *   seconds                   wall time of the solve (the reference reports
+ *                             process CPU time, asetsBinaryMF3D_mex.c:122-131)
+ * cvg is the FP64 sum of |e| over the volume divided by N and rounded to
+ * float; the reference sums in float32 sequentially, so near err_bound the two
+ * can stop on different iterations.  At a fixed iteration count u is
+ * bit-identical to the reference operation order.
+ */
+#ifndef ASETS_H
+#define ASETS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum asets_status {
+    ASETS_OK = 0,
+    ASETS_EBADARG = 1,   /* null pointer, non-positive size, bad option       */
+    ASETS_EALPHA = 2,    /* some alpha <= 0 (asetsBinaryMF3D_mex.c:317-319)   */
+    ASETS_ENOMEM = 3,    /* device or pinned-host allocation failed           */
+    ASETS_EDEVICE = 4,   /* HIP runtime error                                 */
+    ASETS_ETOOBIG = 5    /* size beyond a documented limit                    */
+} asets_status;
+
+/* Solver parameters: the numeric tail of the reference's pars vector
+ * (asetsBinaryMF3D_mex.c:82-85, asetsPotts3D_mex.c:88-91). */
+typedef struct asets_params {
+    int32_t max_iter;    /* pars[maxIt]     */
+    float err_bound;     /* pars[errBound]  */
+    float cc;            /* pars[cc]        step size of the augmented Lagrangian */
+    float steps;         /* pars[steps]     step size of the flow gradient step   */
+} asets_params;
+
+/* Flow storage precision. */
+#define ASETS_FLOW_F32 0
+#define ASETS_FLOW_F16 1   /* ps, pt, bx, by, bz stored in fp16; arithmetic in fp32 */
+
+/* Kernel family. */
+#define ASETS_KERNEL_AUTO 0
+#define ASETS_KERNEL_FUSED 1    /* one z-streaming kernel per iteration (binary 3D/2D) */
+#define ASETS_KERNEL_PASSES 2   /* one kernel per reference pass (all models)          */
+
+/* Optional execution settings; pass NULL for defaults. */
+typedef struct asets_opts {
+    void *stream;          /* hipStream_t to run on (device-pointer entries); NULL = null stream */
+    int32_t flow_dtype;    /* ASETS_FLOW_*                       (binary only)                     */
+    int32_t kernel;        /* ASETS_KERNEL_*                                                        */
+    int32_t poll_every;    /* iterations enqueued between host polls of the stop flag; 0 = auto     */
+    int32_t tile_x;        /* fused kernel tile width along x; 0 = autotuned                        */
+    int32_t zchunk;        /* fused kernel planes per workgroup; 0 = autotuned                      */
+    int32_t reserved[3];
+    double *kernel_ms;     /* out (optional): mean device time per iteration of the solve loop      */
+} asets_opts;
+
+/* ---- host-pointer entry points: drop-in for the four C-MEX solvers ------- */
+
+/* asetsBinaryMF2D_mex(Cs, Ct, alpha, [Ny Nx maxIt errBound cc steps])
+ * replaces maxflow/asetsBinaryMF2D_mex.c:36-129 (mexFunction) and :132-358. */
+asets_status asets_binary2d(const float *Cs, const float *Ct, const float *alpha,
+                            int64_t ny, int64_t nx, const asets_params *p,
+                            float *u, float *cvg, int32_t *it_num, double *seconds,
+                            const asets_opts *o);
+
+/* asetsBinaryMF3D_mex(Cs, Ct, alpha, [Ny Nx Nz maxIt errBound cc steps])
+ * replaces maxflow/asetsBinaryMF3D_mex.c:39-134 (mexFunction) and :137-438. */
+asets_status asets_binary3d(const float *Cs, const float *Ct, const float *alpha,
+                            int64_t ny, int64_t nx, int64_t nz, const asets_params *p,
+                            float *u, float *cvg, int32_t *it_num, double *seconds,
+                            const asets_opts *o);
+
+/* Binary 2D (ndim 2, nz == 1) or 3D solve on host arrays with an optional warm
+ * start: u0/pt0 both NULL reproduces asets_binary2d/3d; both given start from
+ * u = u0, ps = pt = pt0 and zero spatial flows.  The warm start has no
+ * reference counterpart (asetsBinaryMF3D_mex.c:205-224 always cold-starts);
+ * it backs the six-argument form of the mex host files. */
+asets_status asets_binary_warm(int ndim, const float *Cs, const float *Ct, const float *alpha,
+                               int64_t ny, int64_t nx, int64_t nz, const asets_params *p,
+                               const float *u0, const float *pt0,
+                               float *u, float *cvg, int32_t *it_num, double *seconds,
+                               const asets_opts *o);
+
+/* asetsPotts2D_mex(Ct, alpha, [Ny Nx L maxIt errBound cc steps])
+ * replaces maxflow/asetsPotts2D_mex.c:42-135 (mexFunction) and :138-434. */
+asets_status asets_potts2d(const float *Ct, const float *alpha,
+                           int64_t ny, int64_t nx, int64_t nlab, const asets_params *p,
+                           float *u, float *cvg, int32_t *it_num, double *seconds,
+                           const asets_opts *o);
+
+/* asetsPotts3D_mex(Ct, alpha, [Ny Nx Nz L maxIt errBound cc steps])
+ * replaces maxflow/asetsPotts3D_mex.c:44-141 (mexFunction) and :144-521. */
+asets_status asets_potts3d(const float *Ct, const float *alpha,
+                           int64_t ny, int64_t nx, int64_t nz, int64_t nlab,
+                           const asets_params *p,
+                           float *u, float *cvg, int32_t *it_num, double *seconds,
+                           const asets_opts *o);
+
+/* ---- device-pointer entry points (inputs already resident in HBM) -------- */
+
+/* Binary solve on device memory.  ndim is 2 (nz must be 1; the 2D operation
+ * order of asetsBinaryMF2D_mex.c) or 3.  u0/pt0 (device, optional, both or
+ * neither): warm start u = u0, ps = pt = pt0, spatial flows 0 -- a superset of
+ * the reference, which always starts from asetsBinaryMF3D_mex.c:205-224.
+ * cvg and it_num are device pointers; seconds is a host pointer (optional). */
+asets_status asets_binary_dev(int ndim, const float *Cs, const float *Ct, const float *alpha,
+                              int64_t ny, int64_t nx, int64_t nz, const asets_params *p,
+                              const float *u0, const float *pt0,
+                              float *u, float *cvg, int32_t *it_num, double *seconds,
+                              const asets_opts *o);
+
+/* Same as asets_binary_dev, additionally returning the final flows
+ * (device pointers, N floats each, any may be NULL): ps, pt and the three
+ * face fields in own-face convention (bx[g] is the face between x-1 and x). */
+asets_status asets_binary_dev_state(int ndim, const float *Cs, const float *Ct,
+                                    const float *alpha, int64_t ny, int64_t nx, int64_t nz,
+                                    const asets_params *p, const float *u0, const float *pt0,
+                                    float *u, float *cvg, int32_t *it_num, double *seconds,
+                                    float *ps, float *pt, float *bx, float *by, float *bz,
+                                    const asets_opts *o);
+
+/* Potts solve on device memory; ndim 2 (nz == 1) or 3. */
+asets_status asets_potts_dev(int ndim, const float *Ct, const float *alpha,
+                             int64_t ny, int64_t nx, int64_t nz, int64_t nlab,
+                             const asets_params *p,
+                             float *u, float *cvg, int32_t *it_num, double *seconds,
+                             const asets_opts *o);
+
+/* ---- batched binary 3D: many volumes of uneven size, one launch per iteration
+ * No reference counterpart (the reference solves one volume per call).  Each
+ * volume has its own sizes, arrays, cvg/it_num outputs and stop decision; a
+ * converged volume drops out of the work list. */
+typedef struct asets_volume {
+    const float *Cs, *Ct, *alpha;   /* device */
+    int64_t ny, nx, nz;
+    float *u;                       /* device, out */
+    float *cvg;                     /* device, out [max_iter] */
+    int32_t *it_num;                /* device, out */
+} asets_volume;
+
+asets_status asets_binary3d_batched_dev(int32_t nvol, const asets_volume *vols,
+                                        const asets_params *p, double *seconds,
+                                        const asets_opts *o);
+
+/* ---- misc ---------------------------------------------------------------- */
+const char *asets_last_error(void);
+const char *asets_version(void);
+/* Free the library's cached device workspaces (all devices). */
+void asets_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ASETS_H */

@@ -1,0 +1,132 @@
+"""ctypes wrapper of oracle/liboracle.so -- the CPU checker (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py use this module.
+Inputs are numpy arrays in MATLAB layout (shape (Ny, Nx[, Nz][, L]), any order; they are
+flattened in Fortran order).  See oracle/asets_oracle.c for what each function follows
+in the reference.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+_lib = None
+
+ORC_OK, ORC_EBADARG, ORC_EALPHA, ORC_ENOMEM = range(4)
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code):
+        super().__init__(f"oracle error {code}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            subprocess.run(["make", "-s", "-C", str(HERE), "CC=gcc"], check=True)
+        L = C.CDLL(str(LIB))
+        P = C.c_void_p
+        I64 = C.c_int64
+        for name, real in (("orc_binary_f32", C.c_float), ("orc_binary_f64", C.c_double)):
+            f = getattr(L, name)
+            f.argtypes = [P, P, P, I64, I64, I64, C.c_int, C.c_int, real, real, real, P, P,
+                          C.c_int, P, P, P, P, P, P, P, P, P]
+            f.restype = C.c_int
+        for name, real in (("orc_potts_f32", C.c_float), ("orc_potts_f64", C.c_double)):
+            f = getattr(L, name)
+            f.argtypes = [P, P, I64, I64, I64, I64, C.c_int, C.c_int, real, real, real, C.c_int,
+                          P, P, P, P, P, P, P, P, P]
+            f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+@dataclass
+class OracleResult:
+    u: np.ndarray
+    cvg_ref: np.ndarray
+    cvg64: np.ndarray
+    it_num: int
+    state: dict = field(default_factory=dict)
+
+    @property
+    def executed(self) -> int:
+        return self.it_num + 1 if self.it_num < len(self.cvg_ref) else len(self.cvg_ref)
+
+
+def _flat(a, dt):
+    return np.ascontiguousarray(np.asarray(a, dtype=dt).ravel(order="F"))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def binary(Cs, Ct, alpha, shape, max_iter, err_bound, cc, steps, *, precision="f32",
+           u0=None, pt0=None, stop_rule=0, want_state=False, is3d=None) -> OracleResult:
+    """Binary model.  shape (Ny, Nx) uses the 2D operation order of asetsBinaryMF2D_mex.c,
+    (Ny, Nx, Nz) the 3D order of asetsBinaryMF3D_mex.c (is3d may force the 3D forms on a
+    2-tuple-with-Nz=1 volume)."""
+    dt = np.float32 if precision == "f32" else np.float64
+    if len(shape) == 2:
+        ny, nx = shape
+        nz = 1
+        d3 = False if is3d is None else is3d
+    else:
+        ny, nx, nz = shape
+        d3 = True if is3d is None else is3d
+    N = ny * nx * nz
+    Cs, Ct, alpha = (_flat(x, dt) for x in (Cs, Ct, alpha))
+    u0 = None if u0 is None else _flat(u0, dt)
+    pt0 = None if pt0 is None else _flat(pt0, dt)
+    u = np.zeros(N, dtype=dt)
+    cr = np.zeros(max(max_iter, 1), dtype=np.float32)
+    c64 = np.zeros(max(max_iter, 1), dtype=np.float64)
+    itn = C.c_int32(0)
+    st = {k: np.zeros(N, dtype=dt) for k in ("ps", "pt", "bx", "by", "bz")} if want_state else {}
+    f = lib().orc_binary_f32 if dt == np.float32 else lib().orc_binary_f64
+    rc = f(Cs.ctypes.data, Ct.ctypes.data, alpha.ctypes.data, ny, nx, nz, int(d3),
+           int(max_iter), err_bound, cc, steps, _p(u0), _p(pt0), stop_rule, u.ctypes.data,
+           cr.ctypes.data, c64.ctypes.data, C.byref(itn), *(_p(st.get(k)) for k in
+                                                          ("ps", "pt", "bx", "by", "bz")))
+    if rc != ORC_OK:
+        raise OracleError(rc)
+    return OracleResult(u, cr[:max_iter], c64[:max_iter], itn.value, st)
+
+
+def potts(Ct, alpha, shape, nlab, max_iter, err_bound, cc, steps, *, precision="f32",
+          stop_rule=0, want_state=False, is3d=None) -> OracleResult:
+    dt = np.float32 if precision == "f32" else np.float64
+    if len(shape) == 2:
+        ny, nx = shape
+        nz = 1
+        d3 = False if is3d is None else is3d
+    else:
+        ny, nx, nz = shape
+        d3 = True if is3d is None else is3d
+    N = ny * nx * nz
+    NL = N * nlab
+    Ct, alpha = (_flat(x, dt) for x in (Ct, alpha))
+    u = np.zeros(NL, dtype=dt)
+    cr = np.zeros(max(max_iter, 1), dtype=np.float32)
+    c64 = np.zeros(max(max_iter, 1), dtype=np.float64)
+    itn = C.c_int32(0)
+    st = {}
+    if want_state:
+        st = {"ps": np.zeros(N, dtype=dt)}
+        st.update({k: np.zeros(NL, dtype=dt) for k in ("pt", "bx", "by", "bz")})
+    f = lib().orc_potts_f32 if dt == np.float32 else lib().orc_potts_f64
+    rc = f(Ct.ctypes.data, alpha.ctypes.data, ny, nx, nz, nlab, int(d3), int(max_iter),
+           err_bound, cc, steps, stop_rule, u.ctypes.data, cr.ctypes.data, c64.ctypes.data,
+           C.byref(itn), *(_p(st.get(k)) for k in ("ps", "pt", "bx", "by", "bz")))
+    if rc != ORC_OK:
+        raise OracleError(rc)
+    return OracleResult(u, cr[:max_iter], c64[:max_iter], itn.value, st)

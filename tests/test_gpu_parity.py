@@ -1,0 +1,236 @@
+"""GPU parity: the HIP kernels against the CPU oracle (reference operation order).
+
+Bar: u bit-identical to the oracle's FP32 reference-order run at a fixed iteration count
+(stronger than the max|u - u_ref| <= 1e-4 tolerance); cvg within 1e-6 relative of the
+oracle's FP64-summed trace (the GPU sums |e| in FP64 in a different order); it_num equal
+under the same stop rule.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as orc
+from asetsmatlabmaxflow_amd import mex, solver, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits_equal(a, b):
+    a = np.asarray(a, dtype=np.float32).ravel(order="F")
+    b = np.asarray(b, dtype=np.float32).ravel(order="F")
+    return np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def _report(a, b):
+    a = np.asarray(a, dtype=np.float64).ravel(order="F")
+    b = np.asarray(b, dtype=np.float64).ravel(order="F")
+    d = np.abs(a - b)
+    return f"max|diff|={d.max():.3e} at {int(d.argmax())}, n_diff={int((a != b).sum())}"
+
+
+def _run_binary(Cs, Ct, a, shape, maxit, eb, cc, steps, **kw):
+    d = [solver.to_device(x) for x in (Cs, Ct, a)]
+    return solver.binary(*d, shape, maxit, eb, cc, steps, **kw)
+
+
+SHAPES_3D = [(1, 1, 1), (2, 3, 4), (5, 7, 3), (17, 13, 9), (64, 16, 20), (70, 33, 41),
+             (130, 40, 37)]
+SHAPES_2D = [(1, 1), (17, 13), (64, 64), (130, 77), (256, 256)]
+
+
+@pytest.mark.parametrize("kernel", ["fused", "passes"])
+@pytest.mark.parametrize("shape", SHAPES_3D)
+def test_binary3d_bitexact(cuda, kernel, shape):
+    Cs, Ct, a = workloads.random_smooth_binary(shape, seed=11 + sum(shape), alpha=0.1)
+    maxit = 40
+    ref = orc.binary(Cs, Ct, a, shape, maxit, 0.0, 0.25, 0.11)
+    r = _run_binary(Cs, Ct, a, shape, maxit, 0.0, 0.25, 0.11, kernel=kernel)
+    u = r.u.cpu().numpy()
+    # a volume that starts at its fixed point (e.g. 1 voxel) has cvg[0] == 0 <= 0 and
+    # stops at iteration 0, as the reference loop does
+    assert r.it_num == ref.it_num
+    assert _bits_equal(u, ref.u), _report(u, ref.u)
+    np.testing.assert_allclose(r.cvg.cpu().numpy(), ref.cvg64.astype(np.float32), rtol=1e-6,
+                               atol=0)
+
+
+@pytest.mark.parametrize("kernel", ["fused", "passes"])
+@pytest.mark.parametrize("shape", SHAPES_2D)
+def test_binary2d_bitexact(cuda, kernel, shape):
+    Cs, Ct, a = workloads.random_smooth_binary(shape, seed=7 + sum(shape), alpha=0.25)
+    maxit = 60
+    ref = orc.binary(Cs, Ct, a, shape, maxit, 0.0, 0.2, 0.16)
+    r = _run_binary(Cs, Ct, a, shape, maxit, 0.0, 0.2, 0.16, kernel=kernel)
+    u = r.u.cpu().numpy()
+    assert _bits_equal(u, ref.u), _report(u, ref.u)
+    np.testing.assert_allclose(r.cvg.cpu().numpy(), ref.cvg64.astype(np.float32), rtol=1e-6,
+                               atol=0)
+
+
+@pytest.mark.parametrize("tile_x,zchunk", [(8, 1), (8, 5), (16, 3), (16, 64), (32, 2),
+                                           (32, 17)])
+def test_fused_tilings_bitexact(cuda, tile_x, zchunk):
+    shape = (70, 45, 23)
+    Cs, Ct, a, _ = workloads.binary3d_ellipsoids(shape, seed=5, count=4)
+    maxit = 30
+    ref = orc.binary(Cs, Ct, a, shape, maxit, 0.0, workloads.BINARY3D_CC,
+                     workloads.BINARY3D_STEPS)
+    r = _run_binary(Cs, Ct, a, shape, maxit, 0.0, workloads.BINARY3D_CC,
+                    workloads.BINARY3D_STEPS, kernel="fused", tile_x=tile_x, zchunk=zchunk)
+    u = r.u.cpu().numpy()
+    assert _bits_equal(u, ref.u), _report(u, ref.u)
+
+
+def test_binary3d_contrast_alpha_long_run(cuda):
+    """The 3D workload generator at reduced size, 150 iterations."""
+    shape = (96, 80, 72)
+    Cs, Ct, a, _ = workloads.binary3d_ellipsoids(shape, seed=1002, count=6)
+    maxit = 150
+    ref = orc.binary(Cs, Ct, a, shape, maxit, 0.0, workloads.BINARY3D_CC,
+                     workloads.BINARY3D_STEPS)
+    for kernel in ("fused", "passes"):
+        r = _run_binary(Cs, Ct, a, shape, maxit, 0.0, workloads.BINARY3D_CC,
+                        workloads.BINARY3D_STEPS, kernel=kernel)
+        u = r.u.cpu().numpy()
+        assert _bits_equal(u, ref.u), (kernel, _report(u, ref.u))
+
+
+@pytest.mark.parametrize("kernel", ["fused", "passes"])
+def test_stop_rule_matches(cuda, kernel):
+    """errBound > 0: stop on the first iteration with cvg <= errBound (<=, 0-based it_num,
+    asetsBinaryMF3D_mex.c:183-192); entries after the stop index stay 0."""
+    shape = (33, 29, 17)
+    Cs, Ct, a = workloads.random_smooth_binary(shape, seed=3, alpha=0.2)
+    full = orc.binary(Cs, Ct, a, shape, 200, 0.0, 0.25, 0.11, stop_rule=1)
+    k = 57
+    eb = float(np.float32(full.cvg64[k]))
+    ref = orc.binary(Cs, Ct, a, shape, 200, eb, 0.25, 0.11, stop_rule=1)
+    r = _run_binary(Cs, Ct, a, shape, 200, eb, 0.25, 0.11, kernel=kernel, poll_every=7)
+    assert r.it_num == ref.it_num <= k
+    cvg = r.cvg.cpu().numpy()
+    assert np.all(cvg[r.it_num + 1:] == 0)
+    assert _bits_equal(r.u.cpu().numpy(), ref.u)
+
+
+@pytest.mark.parametrize("kernel", ["fused", "passes"])
+def test_warm_start(cuda, kernel):
+    shape = (40, 21, 13)
+    Cs, Ct, a = workloads.random_smooth_binary(shape, seed=9, alpha=0.15)
+    rng = np.random.default_rng(0)
+    u0 = rng.random(shape).astype(np.float32)
+    pt0 = np.minimum(Cs, Ct) + rng.normal(0, 0.01, shape).astype(np.float32)
+    ref = orc.binary(Cs, Ct, a, shape, 25, 0.0, 0.25, 0.11, u0=u0, pt0=pt0)
+    d = [solver.to_device(x) for x in (Cs, Ct, a, u0, pt0)]
+    r = solver.binary(d[0], d[1], d[2], shape, 25, 0.0, 0.25, 0.11, u0=d[3], pt0=d[4],
+                      kernel=kernel)
+    assert _bits_equal(r.u.cpu().numpy(), ref.u)
+
+
+@pytest.mark.parametrize("kernel", ["fused", "passes"])
+def test_final_state_matches(cuda, kernel):
+    shape = (19, 11, 7)
+    Cs, Ct, a = workloads.random_smooth_binary(shape, seed=4, alpha=0.1)
+    ref = orc.binary(Cs, Ct, a, shape, 12, 0.0, 0.25, 0.11, want_state=True)
+    r = _run_binary(Cs, Ct, a, shape, 12, 0.0, 0.25, 0.11, kernel=kernel, want_state=True)
+    for k in ("ps", "pt", "bx", "by", "bz"):
+        assert _bits_equal(r.state[k].cpu().numpy(), ref.state[k]), k
+
+
+@pytest.mark.parametrize("shape,nlab", [((1, 1, 1), 2), ((17, 13, 9), 3), ((33, 20, 11), 6),
+                                        ((70, 9, 5), 10)])
+def test_potts3d_bitexact(cuda, shape, nlab):
+    Ct, a = workloads.random_smooth_potts(shape, nlab, seed=sum(shape) + nlab)
+    ref = orc.potts(Ct, a, shape, nlab, 30, 0.0, 0.25, 0.11)
+    d = [solver.to_device(x) for x in (Ct, a)]
+    r = solver.potts(*d, shape, nlab, 30, 0.0, 0.25, 0.11)
+    u = r.u.cpu().numpy()
+    assert _bits_equal(u, ref.u), _report(u, ref.u)
+    np.testing.assert_allclose(r.cvg.cpu().numpy(), ref.cvg64.astype(np.float32), rtol=1e-6)
+
+
+@pytest.mark.parametrize("shape,nlab", [((17, 13), 3), ((64, 48), 4), ((100, 77), 10)])
+def test_potts2d_bitexact(cuda, shape, nlab):
+    Ct, a = workloads.random_smooth_potts(shape, nlab, seed=sum(shape), alpha_step=0.05)
+    ref = orc.potts(Ct, a, shape, nlab, 40, 0.0, 0.2, 0.16)
+    d = [solver.to_device(x) for x in (Ct, a)]
+    r = solver.potts(*d, shape, nlab, 40, 0.0, 0.2, 0.16)
+    assert _bits_equal(r.u.cpu().numpy(), ref.u), _report(r.u.cpu().numpy(), ref.u)
+
+
+# ---- the mex-ABI boundary -------------------------------------------------------------
+
+def test_mex_binary3d(cuda):
+    mex.set_quiet(True)
+    shape = (21, 17, 9)
+    Cs, Ct, a = workloads.random_smooth_binary(shape, seed=21, alpha=0.1)
+    pars = np.array([21, 17, 9, 35, 0.0, 0.25, 0.11], dtype=np.float32)
+    u, cvg, it, t = mex.asetsBinaryMF3D_mex(Cs, Ct, a, pars)
+    ref = orc.binary(Cs, Ct, a, shape, 35, 0.0, 0.25, 0.11)
+    assert u.dtype == np.float32 and u.shape == shape
+    assert cvg.dtype == np.float32 and cvg.shape == (35, 1)
+    assert it.dtype == np.uint16 and it.shape == (1, 1) and int(it[0, 0]) == 35
+    assert t.dtype == np.float32 and t.shape == (1, 1) and t[0, 0] > 0
+    assert _bits_equal(u, ref.u)
+
+
+def test_mex_binary2d_config1(cuda):
+    """256x256 ellipse image with the pars of the 2D workload, 300 fixed iterations."""
+    mex.set_quiet(True)
+    Cs, Ct, a, pars, _ = workloads.binary2d_ellipses()
+    u, cvg, it, t = mex.asetsBinaryMF2D_mex(Cs, Ct, a, pars)
+    ref = orc.binary(Cs, Ct, a, (256, 256), 300, 0.0, 0.3, 0.16)
+    assert u.shape == (256, 256) and int(it[0, 0]) == ref.it_num
+    assert _bits_equal(u, ref.u), _report(u, ref.u)
+    lab, lab_ref = u > 0.5, ref.u.reshape(256, 256, order="F") > 0.5
+    assert (lab == lab_ref).mean() == 1.0
+
+
+def test_mex_potts(cuda):
+    mex.set_quiet(True)
+    shape, nlab = (15, 12, 6), 4
+    Ct, a = workloads.random_smooth_potts(shape, nlab, seed=2)
+    pars = np.array([15, 12, 6, nlab, 20, 0.0, 0.25, 0.11], dtype=np.float32)
+    u, cvg, it, t = mex.asetsPotts3D_mex(Ct, a, pars)
+    ref = orc.potts(Ct, a, shape, nlab, 20, 0.0, 0.25, 0.11)
+    assert u.shape == shape + (nlab,)
+    assert _bits_equal(u, ref.u)
+    Ct2, a2 = workloads.random_smooth_potts((15, 12), 3, seed=3)
+    pars2 = np.array([15, 12, 3, 20, 0.0, 0.2, 0.16], dtype=np.float32)
+    u2, _, _, _ = mex.asetsPotts2D_mex(Ct2, a2, pars2)
+    ref2 = orc.potts(Ct2, a2, (15, 12), 3, 20, 0.0, 0.2, 0.16)
+    assert u2.shape == (15, 12, 3) and _bits_equal(u2, ref2.u)
+
+
+def test_mex_errors_and_edges(cuda):
+    mex.set_quiet(True)
+    shape = (8, 6, 4)
+    Cs, Ct, a = workloads.random_smooth_binary(shape, seed=1, alpha=0.1)
+    bad = a.copy()
+    bad[3, 2, 1] = 0.0
+    pars = np.array([8, 6, 4, 10, 0.0, 0.25, 0.11], dtype=np.float32)
+    with pytest.raises(mex.MexError, match="alpha"):
+        mex.asetsBinaryMF3D_mex(Cs, Ct, bad, pars)
+    # maxIt = 0: no iteration, u is the initial cut, numIter 0, no alpha check
+    pars0 = pars.copy()
+    pars0[3] = 0
+    u, cvg, it, _ = mex.asetsBinaryMF3D_mex(Cs, Ct, bad, pars0)
+    assert int(it[0, 0]) == 0 and cvg.shape == (0, 1)
+    np.testing.assert_array_equal(u, (Cs - Ct >= 0).astype(np.float32))
+    # the library is still usable after an error
+    u, _, it, _ = mex.asetsBinaryMF3D_mex(Cs, Ct, a, pars)
+    assert int(it[0, 0]) == 10
+
+
+def test_fused_equals_passes_full_size(cuda):
+    """256^3 workload: the fused kernel and the per-pass kernels agree bit for bit over
+    50 iterations (two independent GPU implementations of the same operation order)."""
+    shape = (256, 256, 256)
+    Cs, Ct, a, _ = workloads.binary3d_ellipsoids(shape, seed=1002)
+    d = [solver.to_device(x) for x in (Cs, Ct, a)]
+    del Cs, Ct, a
+    rf = solver.binary(*d, shape, 50, 0.0, workloads.BINARY3D_CC, workloads.BINARY3D_STEPS,
+                       kernel="fused")
+    rp = solver.binary(*d, shape, 50, 0.0, workloads.BINARY3D_CC, workloads.BINARY3D_STEPS,
+                       kernel="passes")
+    assert torch.equal(rf.u.view(torch.int32), rp.u.view(torch.int32))
+    torch.testing.assert_close(rf.cvg, rp.cvg, rtol=1e-6, atol=0)
