@@ -1,0 +1,118 @@
+"""The C-ABI library, the mx shim and the mex host files load and export what they
+declare; argument validation that happens before any device call (CPU-only checks)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from asetsmatlabmaxflow_amd import _native as nat
+from asetsmatlabmaxflow_amd import mex
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared(header: Path):
+    text = header.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(asets_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = nat.lib()
+    declared = _declared(ROOT / "include" / "asets.h")
+    assert declared, "no declarations parsed"
+    for name in declared:
+        assert hasattr(lib, name), f"libasets_hip.so does not export {name}"
+    assert set(declared) == set(nat.EXPORTED)
+    assert lib.asets_version().decode().startswith("asets-mi355x")
+
+
+def test_mex_modules_export_mexFunction():
+    for name in mex.MEX_DIR.glob("asets*_mex.so"):
+        assert hasattr(C.CDLL(str(name)), "mexFunction"), name
+    assert len(list(mex.MEX_DIR.glob("asets*_mex.so"))) == 4
+    shim = C.CDLL(str(mex.MEX_DIR / "libmexshim.so"))
+    for sym in ("mxCreateNumericArray", "mxGetData", "mxGetNumberOfElements", "mexErrMsgTxt",
+                "mexPrintf", "mexshim_call"):
+        assert hasattr(shim, sym)
+
+
+def _status(fn, *args):
+    return fn(*args)
+
+
+def test_bad_arguments_rejected_before_device_work():
+    L = nat.lib()
+    p = nat.make_params(10, 0.0, 0.25, 0.11)
+    buf = np.zeros(64, np.float32)
+    itn = C.c_int32(0)
+    ptr = buf.ctypes.data
+    # non-positive size
+    assert L.asets_binary3d(ptr, ptr, ptr, 0, 4, 4, C.byref(p), ptr, ptr, C.byref(itn), None,
+                            None) == nat.ASETS_EBADARG
+    assert b"positive" in L.asets_last_error()
+    # null array
+    assert L.asets_binary3d(None, ptr, ptr, 4, 4, 4, C.byref(p), ptr, ptr, C.byref(itn), None,
+                            None) == nat.ASETS_EBADARG
+    # negative max_iter, zero cc
+    bad = nat.make_params(-1, 0.0, 0.25, 0.11)
+    assert L.asets_binary2d(ptr, ptr, ptr, 4, 4, C.byref(bad), ptr, ptr, C.byref(itn), None,
+                            None) == nat.ASETS_EBADARG
+    bad = nat.make_params(5, 0.0, 0.0, 0.11)
+    assert L.asets_potts3d(ptr, ptr, 2, 2, 2, 2, C.byref(bad), ptr, ptr, C.byref(itn), None,
+                           None) == nat.ASETS_EBADARG
+    assert L.asets_binary_warm(2, ptr, ptr, ptr, 4, 4, 2, C.byref(p), None, None, ptr, ptr,
+                               C.byref(itn), None, None) == nat.ASETS_EBADARG
+    assert L.asets_binary_warm(3, ptr, ptr, ptr, 4, 4, 2, C.byref(p), ptr, None, ptr, ptr,
+                               C.byref(itn), None, None) == nat.ASETS_EBADARG
+    # oversized plane
+    assert L.asets_binary3d(ptr, ptr, ptr, 1 << 16, 1 << 15, 1, C.byref(p), ptr, ptr,
+                            C.byref(itn), None, None) == nat.ASETS_ETOOBIG
+
+
+def test_mex_argument_validation():
+    mex.set_quiet(True)
+    f = np.zeros((4, 3, 2), np.float32)
+    pars = np.array([4, 3, 2, 10, 0.0, 0.25, 0.11], np.float32)
+    with pytest.raises(mex.MexError, match="expected"):
+        mex.call("asetsBinaryMF3D_mex", f, f, f)
+    with pytest.raises(mex.MexError, match="Cs has 24 elements"):
+        mex.asetsBinaryMF3D_mex(f, f, f, np.array([5, 3, 2, 10, 0, 0.25, 0.11], np.float32))
+    with pytest.raises(mex.MexError, match="single"):
+        mex.call("asetsBinaryMF3D_mex", f, f, f, np.array([4, 3, 2, 10, 0, 0, 0], np.uint16))
+    with pytest.raises(mex.MexError, match="pars needs 7"):
+        mex.asetsBinaryMF3D_mex(f, f, f, pars[:5])
+    neg = pars.copy()
+    neg[3] = -1
+    with pytest.raises(mex.MexError, match="maxIt"):
+        mex.asetsBinaryMF3D_mex(f, f, f, neg)
+    with pytest.raises(mex.MexError, match="Ny"):
+        mex.asetsBinaryMF2D_mex(f, f, f, np.array([0, 3, 10, 0, 0.2, 0.16], np.float32))
+    with pytest.raises(mex.MexError, match="expected"):
+        mex.call("asetsPotts3D_mex", f, f)
+    with pytest.raises(mex.MexError, match="Ct has 24"):
+        mex.asetsPotts3D_mex(f, f, np.array([4, 3, 2, 2, 10, 0, 0.25, 0.11], np.float32))
+    with pytest.raises(mex.MexError, match="u0"):
+        mex.asetsBinaryMF3D_mex(f, f, f, pars, u0=f[:2], pt0=f)
+
+
+def test_mx_roundtrip():
+    a = np.asfortranarray(np.arange(24, dtype=np.float32).reshape(4, 3, 2))
+    m = mex._to_mx(a)
+    back = mex._from_mx(m)
+    mex._load_shim().mxDestroyArray(m)
+    assert back.shape == (4, 3, 2) and np.array_equal(back, a)
+
+
+def test_product_does_not_reference_the_oracle():
+    """The product path (package + HIP sources) never imports, links or loads oracle/.
+    (build.py only compiles the checker next to the product.)"""
+    pkg = ROOT / "asetsmatlabmaxflow_amd"
+    for f in list(pkg.rglob("*.py")) + list(pkg.rglob("*.hip")) + list(pkg.rglob("*.c")) + \
+            list(pkg.rglob("*.h")):
+        if f.name == "build.py":
+            continue
+        text = f.read_text()
+        assert "import oracle" not in text and "liboracle" not in text, f
