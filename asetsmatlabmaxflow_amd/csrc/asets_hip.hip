@@ -12,6 +12,7 @@
 #include <hip/hip_fp16.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -107,8 +108,8 @@ asets_status check_dims(int64_t ny, int64_t nx, int64_t nz, int64_t L) {
     if (ny <= 0 || nx <= 0 || nz <= 0 || L <= 0)
         return fail(ASETS_EBADARG, "sizes must be positive (ny=%lld nx=%lld nz=%lld L=%lld)",
                     (long long)ny, (long long)nx, (long long)nz, (long long)L);
-    if (ny * nx > (int64_t)INT32_MAX / 2)
-        return fail(ASETS_ETOOBIG, "a plane of %lld voxels exceeds the 2^30 limit",
+    if (ny * nx > ((int64_t)1 << 28))
+        return fail(ASETS_ETOOBIG, "a plane of %lld voxels exceeds the 2^28 limit",
                     (long long)(ny * nx));
     if (nz > (int64_t)INT32_MAX / 2 || ny * nx * nz * L > ((int64_t)1 << 40))
         return fail(ASETS_ETOOBIG, "volume too large");
@@ -172,9 +173,10 @@ struct TileCfg {
     int zchunk;
 };
 
-// Tile shape for the fused kernel: chosen so that a 256^3 volume gives a few
-// workgroups per CU (256 CUs) while keeping the z prologue overhead
-// (4 extra gk planes per chunk) small.  Overridden by asets_opts.
+// Tile shape for the fused kernel.  Default: 64 x 16 tiles (512 threads, two
+// workgroups per CU by LDS) and z chunks sized so the grid is about one
+// resident round of 2 workgroups x 256 CUs; a chunk is at least 8 planes so
+// the 4-plane z prologue stays a small overhead.  Overridden by asets_opts.
 TileCfg pick_tiles(int64_t ny, int64_t nx, int64_t nz, const asets_opts *o) {
     TileCfg t;
     t.tx = (o && o->tile_x > 0) ? o->tile_x : 16;
@@ -184,30 +186,56 @@ TileCfg pick_tiles(int64_t ny, int64_t nx, int64_t nz, const asets_opts *o) {
     if (o && o->zchunk > 0) {
         zc = o->zchunk;
     } else {
-        // aim for ~1024 workgroups, chunks of at least 16 planes
-        int64_t want_chunks = std::max<int64_t>(1, cdiv(1024, tiles));
-        zc = (int)std::max<int64_t>(16, cdiv(nz, want_chunks));
+        int64_t nzc = std::max<int64_t>(1, (512 + tiles / 2) / tiles);
+        zc = (int)std::max<int64_t>(8, cdiv(nz, nzc));
     }
     t.zchunk = (int)std::min<int64_t>(zc, nz);
     return t;
 }
 
-template <bool IS3D, int TX, typename T>
+// Fused kernel instances: (tile width TX, threads per workgroup).  Wider tiles
+// use more threads so each thread keeps few positions in registers.
+template <bool IS3D, int TX, int NT, typename T>
 void launch_fused_t(const FusedArgs<T> &a, int grid, hipStream_t s) {
-    kf_binary<IS3D, TX, T><<<grid, 256, 0, s>>>(a);
+    // occupancy target: 4 waves per SIMD (<= 128 VGPRs) for the 512- and
+    // 1024-thread tiles, 3 for the 256-thread tile (LDS allows 3 per CU)
+    constexpr int W = (NT >= 512) ? 4 : 3;
+    kf_binary<IS3D, TX, NT, W, T><<<grid, NT, 0, s>>>(a);
 }
 
 template <typename T>
 void launch_fused(bool is3d, int tx, const FusedArgs<T> &a, int grid, hipStream_t s) {
     if (is3d) {
-        if (tx == 8) launch_fused_t<true, 8, T>(a, grid, s);
-        else if (tx == 32) launch_fused_t<true, 32, T>(a, grid, s);
-        else launch_fused_t<true, 16, T>(a, grid, s);
+        if (tx == 8) launch_fused_t<true, 8, 256, T>(a, grid, s);
+        else if (tx == 32) launch_fused_t<true, 32, 1024, T>(a, grid, s);
+        else launch_fused_t<true, 16, 512, T>(a, grid, s);
     } else {
-        if (tx == 8) launch_fused_t<false, 8, T>(a, grid, s);
-        else if (tx == 32) launch_fused_t<false, 32, T>(a, grid, s);
-        else launch_fused_t<false, 16, T>(a, grid, s);
+        if (tx == 8) launch_fused_t<false, 8, 256, T>(a, grid, s);
+        else if (tx == 32) launch_fused_t<false, 32, 1024, T>(a, grid, s);
+        else launch_fused_t<false, 16, 512, T>(a, grid, s);
     }
+}
+
+__global__ void k_selftest_div(int mode, float d, unsigned long long *count) {
+    unsigned long long bad = 0;
+    const float rd = rcp_refined(d);
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < (1ull << 32);
+         b += (uint64_t)gridDim.x * blockDim.x) {
+        const float x = __uint_as_float((uint32_t)b);
+        float got, want;
+        if (mode == 0) {
+            got = div_fast(x, d, rd);
+            want = x / d;
+        } else {
+            if (!div_fast_ok(x)) continue;
+            got = div_fast(1.0f, x, rcp_refined(x));
+            want = 1.0f / x;
+        }
+        const bool same = (__float_as_uint(got) == __float_as_uint(want)) ||
+                          (got != got && want != want);
+        bad += same ? 0 : 1;
+    }
+    if (bad) atomicAdd(count, bad);
 }
 
 __global__ void k_copy_f32(float *dst, const float *src, int64_t n) {
@@ -358,7 +386,8 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
         fa.ny = (int)ny;
         fa.nx = (int)nx;
         fa.nz = (int)nz;
-        fa.plane = ny * nx;
+        fa.plane = (int)(ny * nx);
+        fa.fast_cc = (std::fabs(p->cc) >= 0x1p-16f && std::fabs(p->cc) <= 0x1p16f) ? 1 : 0;
         fa.cc = p->cc;
         fa.steps = p->steps;
         fa.zchunk = tc.zchunk;
@@ -696,6 +725,25 @@ asets_status asets_binary3d_batched_dev(int32_t nvol, const asets_volume *vols,
     (void)seconds;
     (void)o;
     return fail(ASETS_EBADARG, "batched solve is not available in this build");
+}
+
+asets_status asets_selftest_division(int mode, float d, uint64_t *mismatches) {
+    if (!mismatches) return fail(ASETS_EBADARG, "mismatches is NULL");
+    if (mode != 0 && mode != 1) return fail(ASETS_EBADARG, "mode must be 0 or 1");
+    if (mode == 0 && !(std::fabs(d) >= 0x1p-16f && std::fabs(d) <= 0x1p16f))
+        return fail(ASETS_EBADARG, "divisor outside [2^-16, 2^16]");
+    unsigned long long *dcount = nullptr;
+    HIPCHK(hipMalloc(&dcount, sizeof(unsigned long long)));
+    HIPCHK(hipMemset(dcount, 0, sizeof(unsigned long long)));
+    k_selftest_div<<<4096, 256>>>(mode, d, dcount);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    unsigned long long h = 0;
+    if (e == hipSuccess) e = hipMemcpy(&h, dcount, sizeof h, hipMemcpyDeviceToHost);
+    (void)hipFree(dcount);
+    if (e != hipSuccess) return fail(ASETS_EDEVICE, "selftest: %s", hipGetErrorString(e));
+    *mismatches = h;
+    return ASETS_OK;
 }
 
 const char *asets_last_error(void) { return g_last_error.c_str(); }

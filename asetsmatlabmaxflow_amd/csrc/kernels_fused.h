@@ -3,32 +3,39 @@
 // updateBX/BY/BZ, updateDIVPSPTU), fused by streaming along z.
 //
 // Dependency radius of one iteration at voxel v (L1 distance):
-//   gk      = dv - ((ps - pt) + u/cc)           needed within radius 2 of v
-//   b1      = steps*grad(gk) + b                faces within radius 1
-//   g       = 1/max(|b1|/alpha, 1)              within radius 1
-//   b2      = 0.5*(g_lo + g_hi)*b1              v's six faces
-//   dv, ps, pt, u                               at v
-// dv at the start of an iteration equals div(b) of the stored faces (it is
-// computed from them at the end of the previous iteration with the same
-// expression), so it is recomputed instead of stored: per voxel-iteration the
-// kernel reads Cs, Ct, alpha and u, ps, pt, bx, by, bz and writes the latter
-// six -- 60 bytes of compulsory HBM traffic in fp32.
+//   gk  = dv - ((ps - pt) + u/cc)        within radius 2 of v
+//   b1  = steps*grad(gk) + b             faces within radius 1
+//   g   = 1/max(|b1|/alpha, 1)           within radius 1
+//   b2  = 0.5*(g_lo + g_hi)*b1           v's six faces
+//   dv, ps, pt, u                        at v
+// dv at the start of an iteration equals div(b) of the stored faces (computed
+// from them with the same expression at the end of the previous iteration), so
+// it is recomputed, not stored.  Per voxel-iteration the compulsory traffic is
+// reading Cs, Ct, alpha and u, ps, pt, bx, by, bz and writing the latter six:
+// 60 bytes in fp32.
 //
-// Work decomposition: a workgroup owns an in-plane tile of TY (=64, the
-// contiguous y axis, one wavefront) by TX voxels and a range of z planes.  It
-// streams through the planes keeping
-//   GK[4][TX+4][TY+4]  gk of planes p..p+3 over the tile + 2-voxel halo
-//   G [2][TX+2][TY+2]  g  of planes p, p+1 over the tile + 1-voxel halo
-// in LDS.  Step p: (A) g of plane p+1; barrier; (B) output plane p and gk of
-// plane p+3; barrier.  The z face below plane p is carried in registers from
-// the step before.  Halo values are recomputed by the neighbouring workgroups
-// (same inputs, same operation order, same bits).
+// Work decomposition.  A workgroup (NT threads) owns an in-plane tile of
+// TY (=64, the contiguous y axis) x TX voxels and a range of z planes, and
+// streams through the planes.  Every raw plane is loaded from HBM once per
+// workgroup; its loads are issued at the start of a step and consumed after
+// the step's output stage.  Per plane step:
+//   S1 (plane s+2)  gk over the tile + 2-voxel halo -> LDS GK ring (2), and
+//                   the stepped z face below plane s+2 -> LDS B1Z ring (3)
+//   S2a(plane s+2)  stepped in-plane faces b1 from gk -> LDS B1X/B1Y ring (3)
+//   S2b(plane s+1)  projection factor g over tile + 1 halo -> LDS G ring (2)
+//   S3 (plane s)    rescale, divergence, source/sink/multiplier update, store
+// in two barrier phases: {S3(s-1), S1(s+2)} | {S2a(s+2), S2b(s+1)}.  The
+// rescaled z face below the current output plane is carried in registers.  Halo values are recomputed by neighbouring
+// workgroups from identical inputs in the same operation order (same bits).
 //
-// The state is double-buffered (in -> out) because neighbouring workgroups
-// read the halo of the previous iterate.  Every face with axis index 0 is
-// written as 0 and high faces past the last voxel read as 0: zero-flux
-// boundary as in the reference.
+// Global reads go through buffer descriptors built per plane (base = field +
+// plane offset, size = one plane): 32-bit in-plane offsets, and reads past the
+// plane return 0 -- which is exactly the zero-flux value of a high face past
+// the last voxel.  The state is double-buffered (in -> out) because
+// neighbouring workgroups read the halo of the previous iterate.
 #pragma once
+#include <hip/hip_fp16.h>
+
 #include "common.h"
 
 namespace asets {
@@ -43,40 +50,185 @@ struct FusedArgs {
     const float *Cs, *Ct, *alpha;
     FusedState<T> in, out;
     int ny, nx, nz;
-    int64_t plane;          // ny*nx
+    int plane;              // ny*nx (< 2^30)
     float cc, steps;
     int zchunk;             // planes per workgroup
     int ntile_y, ntile_x, nzc;
     int nblocks;            // ntile_y*ntile_x*nzc (logical)
+    int fast_cc;            // cc is in the range where div_fast is exact
     double *partials;       // [gridDim.x]
     const Flags *flags;
 };
 
-template <typename T> __device__ __forceinline__ float ld(const T *p);
-template <> __device__ __forceinline__ float ld<float>(const float *p) { return *p; }
-template <> __device__ __forceinline__ float ld<__half>(const __half *p) { return __half2float(*p); }
-template <typename T> __device__ __forceinline__ void st(T *p, float v);
-template <> __device__ __forceinline__ void st<float>(float *p, float v) { *p = v; }
-template <> __device__ __forceinline__ void st<__half>(__half *p, float v) { *p = __float2half_rn(v); }
+// ---- buffer access --------------------------------------------------------
 
-template <bool IS3D, int TX, typename T>
-__global__ __launch_bounds__(256) void kf_binary(FusedArgs<T> a) {
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *base, int64_t elem_off,
+                                                             int bytes_per_elem, int plane) {
+    const char *p = (const char *)base + elem_off * bytes_per_elem;
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, plane * bytes_per_elem,
+                                             0x00020000);
+}
+
+// Descriptor of plane q of a field, or an empty one (every load returns 0)
+// when q is outside [0, nz): loads never need a branch, which keeps the
+// prefetched loads countable by the compiler's wait-counter insertion.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_q(const void *base, int q, int nz,
+                                                          int bytes_per_elem, int plane) {
+    const bool in = q >= 0 && q < nz;
+    const int qq = in ? q : 0;
+    return __builtin_amdgcn_make_buffer_rsrc(
+        (void *)((const char *)base + (int64_t)qq * plane * bytes_per_elem), (short)0,
+        in ? plane * bytes_per_elem : 0, 0x00020000);
+}
+
+template <typename T> struct Io;
+template <> struct Io<float> {
+    static constexpr int kBytes = 4;
+    // the b32 buffer builtins move raw 32-bit words: bit-cast, never convert
+    static __device__ __forceinline__ float load(__amdgpu_buffer_rsrc_t r, int off) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off * 4, 0, 0));
+    }
+    static __device__ __forceinline__ void store(__amdgpu_buffer_rsrc_t r, int off, float v) {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off * 4, 0, 0);
+    }
+};
+template <> struct Io<__half> {
+    static constexpr int kBytes = 2;
+    static __device__ __forceinline__ float load(__amdgpu_buffer_rsrc_t r, int off) {
+        unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, off * 2, 0, 0);
+        return __half2float(__ushort_as_half(b));
+    }
+    static __device__ __forceinline__ void store(__amdgpu_buffer_rsrc_t r, int off, float v) {
+        __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(__float2half_rn(v)), r, off * 2,
+                                              0, 0);
+    }
+};
+
+__device__ __forceinline__ float ldf(__amdgpu_buffer_rsrc_t r, int off) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off * 4, 0, 0));
+}
+
+// ---- correctly rounded division, short form ------------------------------
+// The compiler's IEEE fp32 division is: refine a hardware reciprocal of the
+// (scaled) divisor with two FMAs, multiply, then two remainder/correction FMA
+// steps, a scale-back and a special-case fixup.  When the exponents of x and d
+// are moderate the scaling is the identity and the fixup passes the value
+// through, so the same sequence without them returns the same correctly
+// rounded quotient.  div_fast applies it when |x| is in [2^-64, 2^64] and d
+// (precondition, checked by the caller) is in [2^-16, 2^16], and falls back to
+// '/' otherwise (tiny, huge, inf, NaN).  rcp_refined(d) is the refined
+// reciprocal of that sequence.
+__device__ __forceinline__ float rcp_refined(float d) {
+    const float e0 = __builtin_amdgcn_rcpf(d);
+    const float t = __builtin_fmaf(-d, e0, 1.0f);
+    return __builtin_fmaf(t, e0, e0);
+}
+
+__device__ __forceinline__ bool div_fast_ok(float d) {
+    const float ad = __builtin_fabsf(d);
+    return ad >= 0x1p-16f && ad <= 0x1p16f;
+}
+
+// True on every active lane of the wavefront: lets a rare exact fallback be a
+// wave-uniform branch instead of exec-masked divergent code.
+__device__ __forceinline__ bool wave_all(bool p) { return __all(p ? 1 : 0) != 0; }
+
+// x/d with d's refined reciprocal r; exact fallback to '/' only when some lane
+// has x outside the short form's range (or d outside it: d_ok false).
+__device__ __forceinline__ float div_fast(float x, float d, float r, bool d_ok = true) {
+    const float ax = __builtin_fabsf(x);
+    const bool ok = d_ok && ((ax >= 0x1p-64f && ax <= 0x1p64f) || ax == 0.0f);
+    const float q0 = x * r;
+    const float e0 = __builtin_fmaf(-d, q0, x);
+    const float q1 = __builtin_fmaf(e0, r, q0);
+    const float e1 = __builtin_fmaf(-d, q1, x);
+    // x = +-0: x*r already is the signed zero x/d (the corrections could turn
+    // -0 into +0)
+    float q = (ax == 0.0f) ? q0 : __builtin_fmaf(e1, r, q1);
+    if (!wave_all(ok)) q = ok ? q : x / d;
+    return q;
+}
+
+// Projection factor (asetsBinaryMF3D_mex.c:321-331).  The sum of squares is
+// exact-per-term in double and accumulated high face first, as the reference's
+// pow() chain.  If t*0.5 <= alpha^2 then sqrt(t*0.5) <= alpha, the correctly
+// rounded double and then float square root cannot exceed alpha, the branch
+// n > alpha is not taken and the factor is exactly 1: the square root and both
+// divisions are needed only when some lane of the wave is above that bound.
+// Lanes that do not use a division feed it benign operands (1) so they never
+// trigger the exact fallback.
+template <bool IS3D>
+__device__ __forceinline__ float proj_factor_fast(float xr, float xl, float yr, float yl,
+                                                  float zr, float zl, float a) {
+    double t = (double)xr * (double)xr;
+    t = __builtin_fma((double)xl, (double)xl, t);
+    t = __builtin_fma((double)yr, (double)yr, t);
+    t = __builtin_fma((double)yl, (double)yl, t);
+    if (IS3D) {
+        t = __builtin_fma((double)zr, (double)zr, t);
+        t = __builtin_fma((double)zl, (double)zl, t);
+    }
+    const double h = t * 0.5;
+    const double ad = (double)a;
+    const bool maybe = !(h <= ad * ad);
+    float g = 1.0f;
+    if (!wave_all(!maybe)) {
+        const float n = (float)__builtin_sqrt(h);
+        const bool big = maybe && (n > a);
+        const float nn = big ? n : 1.0f, aa = big ? a : 1.0f;
+        const float q = div_fast(nn, aa, rcp_refined(aa), div_fast_ok(aa));   // n/alpha
+        const float qq = big ? q : 1.0f;
+        const float r = div_fast(1.0f, qq, rcp_refined(qq), div_fast_ok(qq)); // 1/(n/alpha)
+        g = big ? r : 1.0f;
+    }
+    return g;
+}
+
+// Face rescale (g_hi + g_lo)*0.5*b with the reference's double literal: one
+// rounding of the exact product.  (s*0.5f)*b gives the same float whenever the
+// halving is exact, i.e. unless 0 < |s| < 2^-125.  Projection factors are >= 0,
+// so s >= max(g_hi, g_lo): a voxel whose own factor is >= 2^-124 can take the
+// float form for all six of its faces (rescale_f32); otherwise rescale_f64.
+__device__ __forceinline__ float rescale_f32(float g_hi, float g_lo, float b) {
+    return ((g_hi + g_lo) * 0.5f) * b;
+}
+__device__ __forceinline__ float rescale_f64(float g_hi, float g_lo, float b) {
+    return (float)((double)(g_hi + g_lo) * 0.5 * (double)b);
+}
+// rescale with the per-voxel choice made wave-uniform in the common case
+__device__ __forceinline__ float rescale(float g_hi, float g_lo, float b, bool tiny,
+                                         bool any_tiny) {
+    const float f = rescale_f32(g_hi, g_lo, b);
+    if (any_tiny) return tiny ? rescale_f64(g_hi, g_lo, b) : f;
+    return f;
+}
+
+// ---- the kernel -------------------------------------------------------------
+
+template <bool IS3D, int TX, int NT, int WAVES, typename T>
+__global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
+    using IO = Io<T>;
     constexpr int TY = 64;
-    constexpr int NT = 256;
-    constexpr int W2 = TY + 4, H2 = TX + 4;   // gk region (radius 2)
-    constexpr int W1 = TY + 2, H1 = TX + 2;   // g region (radius 1)
-    constexpr int KOUT = (TY * TX) / NT;      // output voxels per thread per plane
+    constexpr int W2 = TY + 4, H2 = TX + 4;   // gk / b1 region (tile + radius 2)
+    constexpr int W1 = TY + 2, H1 = TX + 2;   // g region (tile + radius 1)
+    constexpr int N2 = W2 * H2, N1 = W1 * H1;
+    constexpr int K2 = (N2 + NT - 1) / NT;    // radius-2 positions per thread
+    constexpr int K1 = (N1 + NT - 1) / NT;    // radius-1 positions per thread
+    constexpr int KO = (TY * TX) / NT;        // output voxels per thread
     static_assert((TY * TX) % NT == 0, "tile must be a multiple of the block");
 
-    __shared__ float GK[4][H2][W2];
-    __shared__ float G[2][H1][W1];
+    __shared__ float GK[2][N2];
+    __shared__ float B1X[3][N2], B1Y[3][N2], B1Z[IS3D ? 3 : 1][IS3D ? N2 : 1];
+    __shared__ float G[2][N1];
+    // only the last position slot of a thread can fall past the region
+    auto in2 = [&](int k, int idx) { return k < K2 - 1 || (N2 % NT) == 0 || idx < N2; };
+    auto in1 = [&](int k, int idx) { return k < K1 - 1 || (N1 % NT) == 0 || idx < N1; };
     __shared__ double red[16];
 
     if (a.flags->done) return;
 
-    // XCD-aware block order: consecutive logical tiles land on the same XCD
-    // (hardware dispatches block b to XCD b % 8) so in-plane halos are shared
-    // through one L2.
+    // XCD-aware block order: hardware dispatches block b to XCD b % 8; remap
+    // so that consecutive logical tiles share an XCD and its L2.
     const int nb = gridDim.x;
     int lb = blockIdx.x;
     if ((nb & 7) == 0) lb = (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
@@ -90,173 +242,299 @@ __global__ __launch_bounds__(256) void kf_binary(FusedArgs<T> a) {
         const int y0 = tyi * TY, x0 = txi * TX;
         const int z0 = zci * a.zchunk;
         const int z1 = min(z0 + a.zchunk, a.nz);
-        const int ny = a.ny, nx = a.nx, nz = a.nz;
-        const int64_t plane = a.plane;
+        const int ny = a.ny, nx = a.nx, nz = a.nz, plane = a.plane;
         const float cc = a.cc, steps = a.steps;
+        const float rcc = rcp_refined(cc);
+        const float one_cc = 1.0f / cc;
+        const bool fast_cc = a.fast_cc != 0;
 
-        // ---- stage S1: gk of plane q over the radius-2 region ------------
-        auto s1 = [&](int q) {
-            float(*gk)[W2] = GK[(q + 4) & 3];
-            const bool qin = (q >= 0) && (q < nz);
-            const int64_t base = (int64_t)q * plane;
-            for (int idx = tid; idx < W2 * H2; idx += NT) {
-                const int i = idx % W2, j = idx / W2;
-                const int y = y0 - 2 + i, x = x0 - 2 + j;
-                float v = 0.0f;
-                if (qin && y >= 0 && y < ny && x >= 0 && x < nx) {
-                    const int64_t o = base + (int64_t)x * ny + y;
-                    const float bxl = ld(a.in.bx + o);
-                    const float bxr = (x + 1 < nx) ? ld(a.in.bx + o + ny) : 0.0f;
-                    const float byl = ld(a.in.by + o);
-                    const float byr = (y + 1 < ny) ? ld(a.in.by + o + 1) : 0.0f;
-                    float d = ((byr - byl) + bxr) - bxl;
-                    if (IS3D) {
-                        const float bzl = ld(a.in.bz + o);
-                        const float bzr = (q + 1 < nz) ? ld(a.in.bz + o + plane) : 0.0f;
-                        d = (d + bzr) - bzl;
-                    }
-                    v = d - ((ld(a.in.ps + o) - ld(a.in.pt + o)) + ld(a.in.u + o) / cc);
-                }
-                gk[j][i] = v;
-            }
-        };
-
-        // ---- stage S2: projection factor g of plane q over radius 1 -------
-        auto s2 = [&](int q) {
-            float(*gq)[W2] = GK[(q + 4) & 3];
-            float(*gm)[W2] = GK[(q + 3) & 3];
-            float(*gp)[W2] = GK[(q + 5) & 3];
-            float(*go)[W1] = G[q & 1];
-            const bool qin = (q >= 0) && (q < nz);
-            const int64_t base = (int64_t)q * plane;
-            for (int idx = tid; idx < W1 * H1; idx += NT) {
-                const int i1 = idx % W1, j1 = idx / W1;
-                const int y = y0 - 1 + i1, x = x0 - 1 + j1;
-                float v = 1.0f;
-                if (qin && y >= 0 && y < ny && x >= 0 && x < nx) {
-                    const int i = i1 + 1, j = j1 + 1;
-                    const int64_t o = base + (int64_t)x * ny + y;
-                    const float c = gq[j][i];
-                    const float bxl = (x > 0) ? steps * (c - gq[j - 1][i]) + ld(a.in.bx + o) : 0.0f;
-                    const float bxr = (x + 1 < nx) ? steps * (gq[j + 1][i] - c) + ld(a.in.bx + o + ny) : 0.0f;
-                    const float byl = (y > 0) ? steps * (c - gq[j][i - 1]) + ld(a.in.by + o) : 0.0f;
-                    const float byr = (y + 1 < ny) ? steps * (gq[j][i + 1] - c) + ld(a.in.by + o + 1) : 0.0f;
-                    float bzl = 0.0f, bzr = 0.0f;
-                    if (IS3D) {
-                        bzl = (q > 0) ? steps * (c - gm[j][i]) + ld(a.in.bz + o) : 0.0f;
-                        bzr = (q + 1 < nz) ? steps * (gp[j][i] - c) + ld(a.in.bz + o + plane) : 0.0f;
-                    }
-                    v = proj_factor<IS3D>(bxr, bxl, byr, byl, bzr, bzl, a.alpha[o]);
-                }
-                go[j1][i1] = v;
-            }
-        };
-
-        float carry[KOUT];   // rescaled z face below plane p, per output voxel
-
-        // ---- prologue: gk of z0-2..z0+1, g of z0-1 and z0, carry ----------
-        s1(z0 - 2); s1(z0 - 1); s1(z0); s1(z0 + 1);
-        __syncthreads();
-        s2(z0 - 1);
-        __syncthreads();
-        s2(z0);
-        __syncthreads();
-        {
-            float(*gq)[W2] = GK[(z0 + 4) & 3];
-            float(*gm)[W2] = GK[(z0 + 3) & 3];
-            float(*g0)[W1] = G[z0 & 1];
-            float(*gmm)[W1] = G[(z0 + 1) & 1];   // plane z0-1
-            const int64_t base = (int64_t)z0 * plane;
+        // ---- per-thread position tables (fixed for all planes) ------------
+        // radius-2 positions: in-plane offset and flags
+        //   bit0 inside the volume, bit1 x>0, bit2 y>0, bit3 i>=1 && j>=1
+        static_assert(K2 <= 16, "flag word holds 16 positions");
+        int off2[K2];
+        uint64_t fl2 = 0;
 #pragma unroll
-            for (int k = 0; k < KOUT; ++k) {
+        for (int k = 0; k < K2; ++k) {
+            const int idx = tid + k * NT;
+            const int i = idx % W2, j = idx / W2;
+            const int y = y0 - 2 + i, x = x0 - 2 + j;
+            off2[k] = x * ny + y;
+            const bool in = idx < N2 && y >= 0 && y < ny && x >= 0 && x < nx;
+            const uint64_t f = (in ? 1u : 0u) | (x > 0 ? 2u : 0u) | (y > 0 ? 4u : 0u) |
+                               ((idx < N2 && i >= 1 && j >= 1) ? 8u : 0u);
+            fl2 |= f << (4 * k);
+        }
+        // radius-1 positions: LDS index (in the N2 layout), offset, inside flag
+        static_assert(K1 <= 32, "flag word holds 32 positions");
+        int off1[K1], l2i1[K1];
+        uint32_t fl1 = 0;
+#pragma unroll
+        for (int k = 0; k < K1; ++k) {
+            const int idx = tid + k * NT;
+            const int i1 = idx % W1, j1 = idx / W1;
+            const int y = y0 - 1 + i1, x = x0 - 1 + j1;
+            off1[k] = x * ny + y;
+            l2i1[k] = idx < N1 ? (j1 + 1) * W2 + (i1 + 1) : W2 + 1;
+            const bool in = idx < N1 && y >= 0 && y < ny && x >= 0 && x < nx;
+            fl1 |= (in ? 1u : 0u) << k;
+        }
+
+        // ---- raw plane registers (prefetched one step ahead) --------------
+        struct Raw {
+            float u[K2], ps[K2], pt[K2], bxl[K2], bxr[K2], byl[K2], byr[K2], bzn[K2];
+            float al[K1];
+        };
+        // bz own face of the plane S1 is working on; the next plane's own bz
+        // (loaded with the raw plane) becomes it one step later.
+        float bz_cur[K2];
+#pragma unroll
+        for (int k = 0; k < K2; ++k) bz_cur[k] = 0.0f;
+
+        // load raw plane q (u, ps, pt, bx own/high, by own/high), bz own of
+        // plane q+1, and alpha of plane q-1 (for S2b one step later)
+        auto load_raw = [&](int q, Raw &r) {
+            const auto ru = plane_q(a.in.u, q, nz, IO::kBytes, plane);
+            const auto rps = plane_q(a.in.ps, q, nz, IO::kBytes, plane);
+            const auto rpt = plane_q(a.in.pt, q, nz, IO::kBytes, plane);
+            const auto rbx = plane_q(a.in.bx, q, nz, IO::kBytes, plane);
+            const auto rby = plane_q(a.in.by, q, nz, IO::kBytes, plane);
+#pragma unroll
+            for (int k = 0; k < K2; ++k) {
+                const int o = off2[k];
+                r.u[k] = IO::load(ru, o);
+                r.ps[k] = IO::load(rps, o);
+                r.pt[k] = IO::load(rpt, o);
+                r.bxl[k] = IO::load(rbx, o);
+                r.bxr[k] = IO::load(rbx, o + ny);   // past the plane -> 0
+                r.byl[k] = IO::load(rby, o);
+                r.byr[k] = IO::load(rby, o + 1);    // next column's y=0 face is 0
+            }
+            if (IS3D) {
+                const auto rbz = plane_q(a.in.bz, q + 1, nz, IO::kBytes, plane);
+#pragma unroll
+                for (int k = 0; k < K2; ++k) r.bzn[k] = IO::load(rbz, off2[k]);
+            }
+            const auto ra = plane_q(a.alpha, q - 1, nz, 4, plane);
+#pragma unroll
+            for (int k = 0; k < K1; ++k) r.al[k] = ldf(ra, off1[k]);
+        };
+
+        // S1: gk of plane q over the radius-2 region, and the stepped z face
+        // below plane q (needs gk of q-1 at the same position: kept in gkp)
+        float gkp[K2];
+#pragma unroll
+        for (int k = 0; k < K2; ++k) gkp[k] = 0.0f;
+        auto s1 = [&](int q, const Raw &r) {
+            float *gk = GK[q & 1];
+            const int sl = (q + 3) % 3;
+            const bool qin = q >= 0 && q < nz;
+#pragma unroll
+            for (int k = 0; k < K2; ++k) {
                 const int idx = tid + k * NT;
-                const int i = idx % TY, j = idx / TY;
-                const int y = y0 + i, x = x0 + j;
-                float v = 0.0f;
-                if (IS3D && z0 > 0 && y < ny && x < nx) {
-                    const int64_t o = base + (int64_t)x * ny + y;
-                    const float b1 = steps * (gq[j + 2][i + 2] - gm[j + 2][i + 2]) + ld(a.in.bz + o);
-                    v = rescale_face(g0[j + 1][i + 1], gmm[j + 1][i + 1], b1);
-                }
-                carry[k] = v;
-            }
-        }
-        s1(z0 + 2);
-        __syncthreads();
-
-        // ---- main loop ----------------------------------------------------
-        for (int p = z0; p < z1; ++p) {
-            s2(p + 1);
-            __syncthreads();
-            {
-                float(*gq)[W2] = GK[(p + 4) & 3];
-                float(*gn)[W2] = GK[(p + 5) & 3];
-                float(*g0)[W1] = G[p & 1];
-                float(*g1)[W1] = G[(p + 1) & 1];
-                const int64_t base = (int64_t)p * plane;
-#pragma unroll
-                for (int k = 0; k < KOUT; ++k) {
-                    const int idx = tid + k * NT;
-                    const int i = idx % TY, j = idx / TY;
-                    const int y = y0 + i, x = x0 + j;
-                    if (y < ny && x < nx) {
-                        const int64_t o = base + (int64_t)x * ny + y;
-                        const int I = i + 2, J = j + 2;      // gk coordinates
-                        const int I1 = i + 1, J1 = j + 1;    // g coordinates
-                        const float c = gq[J][I];
-                        const float gc = g0[J1][I1];
-                        // x faces
-                        float bxl = 0.0f, bxr = 0.0f;
-                        if (x > 0) {
-                            const float b1 = steps * (c - gq[J - 1][I]) + ld(a.in.bx + o);
-                            bxl = rescale_face(gc, g0[J1 - 1][I1], b1);
-                        }
-                        if (x + 1 < nx) {
-                            const float b1 = steps * (gq[J + 1][I] - c) + ld(a.in.bx + o + ny);
-                            bxr = rescale_face(g0[J1 + 1][I1], gc, b1);
-                        }
-                        // y faces
-                        float byl = 0.0f, byr = 0.0f;
-                        if (y > 0) {
-                            const float b1 = steps * (c - gq[J][I - 1]) + ld(a.in.by + o);
-                            byl = rescale_face(gc, g0[J1][I1 - 1], b1);
-                        }
-                        if (y + 1 < ny) {
-                            const float b1 = steps * (gq[J][I + 1] - c) + ld(a.in.by + o + 1);
-                            byr = rescale_face(g0[J1][I1 + 1], gc, b1);
-                        }
-                        float d = ((byr - byl) + bxr) - bxl;
-                        float bzl = 0.0f, bzr = 0.0f;
-                        if (IS3D) {
-                            bzl = carry[k];
-                            if (p + 1 < nz) {
-                                const float b1 = steps * (gn[J][I] - c) + ld(a.in.bz + o + plane);
-                                bzr = rescale_face(g1[J1][I1], gc, b1);
-                            }
-                            d = (d + bzr) - bzl;
-                            carry[k] = bzr;
-                        }
-                        // source/sink flows and multiplier (asetsBinaryMF3D_mex.c:419-432)
-                        const float uo = ld(a.in.u + o);
-                        const float uc = uo / cc;
-                        float f = ((ld(a.in.pt + o) - uc) + d) + 1.0f / cc;
-                        const float nps = fmin_ref(f, a.Cs[o]);
-                        f = (nps + uc) - d;
-                        const float npt = fmin_ref(f, a.Ct[o]);
-                        const float e = cc * ((npt + d) - nps);
-                        e_acc += fabs((double)e);
-                        st(a.out.u + o, uo - e);
-                        st(a.out.ps + o, nps);
-                        st(a.out.pt + o, npt);
-                        st(a.out.bx + o, bxl);
-                        st(a.out.by + o, byl);
-                        if (IS3D) st(a.out.bz + o, bzl);
+                const uint32_t f = (uint32_t)(fl2 >> (4 * k));
+                const bool live = (f & 1) && qin;
+                float d = ((r.byr[k] - r.byl[k]) + r.bxr[k]) - r.bxl[k];
+                if (IS3D) d = (d + r.bzn[k]) - bz_cur[k];
+                const float uc = div_fast(r.u[k], cc, rcc, fast_cc);
+                const float v = live ? d - ((r.ps[k] - r.pt[k]) + uc) : 0.0f;
+                if (in2(k, idx)) {
+                    gk[idx] = v;
+                    if (IS3D) {
+                        const float bz1 =
+                            (live && q > 0) ? steps * (v - gkp[k]) + bz_cur[k] : 0.0f;
+                        B1Z[sl][idx] = bz1;
                     }
                 }
+                gkp[k] = v;
             }
-            s1(p + 3);
+        };
+
+        // S2a: stepped in-plane own faces of plane q (needs gk of q only)
+        auto s2a = [&](int q, const Raw &r) {
+            const float *gq = GK[q & 1];
+            const int sl = (q + 3) % 3;
+            const bool qin = q >= 0 && q < nz;
+#pragma unroll
+            for (int k = 0; k < K2; ++k) {
+                const int idx = tid + k * NT;
+                const uint32_t f = (uint32_t)(fl2 >> (4 * k));
+                const bool live = (f & 1) && qin;
+                // positions on the first row/column of the region are never read
+                if (in2(k, idx)) {
+                    const float c = gq[idx];
+                    const float cx = gq[(f & 8) ? idx - W2 : idx];
+                    const float cy = gq[(f & 8) ? idx - 1 : idx];
+                    const float bx1 = (live && (f & 2)) ? steps * (c - cx) + r.bxl[k] : 0.0f;
+                    const float by1 = (live && (f & 4)) ? steps * (c - cy) + r.byl[k] : 0.0f;
+                    B1X[sl][idx] = bx1;
+                    B1Y[sl][idx] = by1;
+                }
+            }
+        };
+
+        // S2b: projection factor g of plane q over the radius-1 region
+        // (alpha of plane q arrives with the raw plane q+1)
+        auto s2b = [&](int q, const Raw &r) {
+            const int sl = (q + 3) % 3, sn = (q + 4) % 3;
+            float *go = G[q & 1];
+            const bool qin = q >= 0 && q < nz;
+#pragma unroll
+            for (int k = 0; k < K1; ++k) {
+                const int idx = tid + k * NT;
+                const int l = l2i1[k];
+                const float xl = B1X[sl][l], xr = B1X[sl][l + W2];
+                const float yl = B1Y[sl][l], yr = B1Y[sl][l + 1];
+                float zl = 0.0f, zr = 0.0f;
+                if (IS3D) {
+                    zl = B1Z[sl][l];
+                    zr = B1Z[sn][l];
+                }
+                const float v = proj_factor_fast<IS3D>(xr, xl, yr, yl, zr, zl, r.al[k]);
+                if (in1(k, idx)) go[idx] = (((fl1 >> k) & 1) && qin) ? v : 1.0f;
+            }
+        };
+
+        // S3 operands of plane s at the output positions
+        struct Out {
+            float u[KO], pt[KO], cs[KO], ct[KO];
+        };
+        auto load_out = [&](int s, Out &o) {
+            const int sv = (s >= z0 && s < z1) ? s : -1;   // outside: empty descriptors
+            const auto ru = plane_q(a.in.u, sv, nz, IO::kBytes, plane);
+            const auto rpt = plane_q(a.in.pt, sv, nz, IO::kBytes, plane);
+            const auto rcs = plane_q(a.Cs, sv, nz, 4, plane);
+            const auto rct = plane_q(a.Ct, sv, nz, 4, plane);
+#pragma unroll
+            for (int k = 0; k < KO; ++k) {
+                const int i = tid % TY, j = tid / TY + k * (NT / TY);
+                const int off = (x0 + j) * ny + (y0 + i);
+                o.u[k] = IO::load(ru, off);
+                o.pt[k] = IO::load(rpt, off);
+                o.cs[k] = ldf(rcs, off);
+                o.ct[k] = ldf(rct, off);
+            }
+        };
+
+        float carry[KO];   // rescaled z face below plane s
+#pragma unroll
+        for (int k = 0; k < KO; ++k) carry[k] = 0.0f;
+
+        // the rescaled z face above plane s, at output position k
+        auto z_face_up = [&](int s, float gc, float gup, int l2) {
+            const int sn = (s + 4) % 3;
+            if (!(IS3D && s + 1 < nz)) return 0.0f;
+            const bool tiny = !(gc >= 0x1p-124f);
+            return rescale(gup, gc, B1Z[sn][l2], tiny, !wave_all(!tiny));
+        };
+
+        // S3 light: only the carried z face for plane s+1 (prologue)
+        auto s3_carry = [&](int s) {
+            const float *g0 = G[s & 1];
+            const float *g1 = G[(s + 1) & 1];
+#pragma unroll
+            for (int k = 0; k < KO; ++k) {
+                const int i = tid % TY, j = tid / TY + k * (NT / TY);
+                const int l2 = (j + 2) * W2 + (i + 2);
+                const int l1 = (j + 1) * W1 + (i + 1);
+                carry[k] = z_face_up(s, g0[l1], g1[l1], l2);
+            }
+        };
+
+        // S3: output plane s
+        auto s3 = [&](int s, const Out &o) {
+            const int sl = (s + 3) % 3;
+            const float *g0 = G[s & 1];
+            const float *g1 = G[(s + 1) & 1];
+            const int64_t base = (int64_t)s * plane;
+            const auto wu = plane_rsrc(a.out.u, base, IO::kBytes, plane);
+            const auto wps = plane_rsrc(a.out.ps, base, IO::kBytes, plane);
+            const auto wpt = plane_rsrc(a.out.pt, base, IO::kBytes, plane);
+            const auto wbx = plane_rsrc(a.out.bx, base, IO::kBytes, plane);
+            const auto wby = plane_rsrc(a.out.by, base, IO::kBytes, plane);
+            const auto wbz = plane_rsrc(a.out.bz, base, IO::kBytes, plane);
+#pragma unroll
+            for (int k = 0; k < KO; ++k) {
+                const int i = tid % TY, j = tid / TY + k * (NT / TY);
+                const int y = y0 + i, x = x0 + j;
+                const int l2 = (j + 2) * W2 + (i + 2);   // b1 (N2 layout)
+                const int l1 = (j + 1) * W1 + (i + 1);   // g  (N1 layout)
+                const float gc = g0[l1];
+                const float bzr = z_face_up(s, gc, g1[l1], l2);
+                const float b_xl = B1X[sl][l2], b_xr = B1X[sl][l2 + W2];
+                const float b_yl = B1Y[sl][l2], b_yr = B1Y[sl][l2 + 1];
+                const float g_xl = g0[l1 - W1], g_xr = g0[l1 + W1];
+                const float g_yl = g0[l1 - 1], g_yr = g0[l1 + 1];
+                const bool tiny = !(gc >= 0x1p-124f);
+                const bool any_tiny = !wave_all(!tiny);
+                float bxl = rescale(gc, g_xl, b_xl, tiny, any_tiny);
+                float bxr = rescale(g_xr, gc, b_xr, tiny, any_tiny);
+                float byl = rescale(gc, g_yl, b_yl, tiny, any_tiny);
+                float byr = rescale(g_yr, gc, b_yr, tiny, any_tiny);
+                // index-0 faces and high faces past the volume are zero
+                bxl = (x > 0) ? bxl : 0.0f;
+                bxr = (x + 1 < nx) ? bxr : 0.0f;
+                byl = (y > 0) ? byl : 0.0f;
+                byr = (y + 1 < ny) ? byr : 0.0f;
+                float d = ((byr - byl) + bxr) - bxl;
+                float bzl = 0.0f;
+                if (IS3D) {
+                    bzl = carry[k];
+                    d = (d + bzr) - bzl;
+                    carry[k] = bzr;
+                }
+                // source/sink flows and multiplier (asetsBinaryMF3D_mex.c:419-432)
+                const float uo = o.u[k];
+                const float uc = div_fast(uo, cc, rcc, fast_cc);
+                float f = ((o.pt[k] - uc) + d) + one_cc;
+                const float nps = fmin_ref(f, o.cs[k]);
+                f = (nps + uc) - d;
+                const float npt = fmin_ref(f, o.ct[k]);
+                const float e = cc * ((npt + d) - nps);
+                if (y < ny && x < nx) {
+                    e_acc += fabs((double)e);
+                    const int off = x * ny + y;
+                    IO::store(wu, off, uo - e);
+                    IO::store(wps, off, nps);
+                    IO::store(wpt, off, npt);
+                    IO::store(wbx, off, bxl);
+                    IO::store(wby, off, byl);
+                    if (IS3D) IO::store(wbz, off, bzl);
+                }
+            }
+        };
+
+        // ---- pipeline ----------------------------------------------------
+        // Step s, phase X: S3(s-1) and S1(s+2); barrier; phase Y: S2a(s+2) and
+        // S2b(s+1); barrier.  Steps run from z0-4 (prologue: gk, b1 and g of
+        // the planes below the chunk) to z1; the carried z face of plane z0 is
+        // set at step z0.  A raw plane's loads are issued at the start of the
+        // step that consumes them (S3 of the previous plane covers their
+        // latency, together with the other resident waves); the S3 operands
+        // are loaded one step ahead.
+        auto step = [&](int s, Raw &r, Out &o) {
+            if (s + 2 <= z1 + 1) load_raw(s + 2, r);   // consumed after S3 below
+            if (s - 1 >= z0) s3(s - 1, o);
+            else if (s - 1 == z0 - 1) s3_carry(s - 1);
+            if (s + 2 <= z1 + 1) s1(s + 2, r);
             __syncthreads();
+            if (s + 2 <= z1 + 1) s2a(s + 2, r);
+            if (s + 1 >= z0 - 1 && s + 1 <= z1) s2b(s + 1, r);
+            load_out(s, o);   // consumed by S3(s) at the next step
+            __syncthreads();
+            // the next plane's own bz face becomes current
+#pragma unroll
+            for (int k = 0; k < K2; ++k) bz_cur[k] = r.bzn[k];
+        };
+        Raw r;
+        Out o;
+        // bz own of plane z0-2 (raw plane z0-3 would carry it as its "next" bz)
+        if (IS3D) {
+            const auto rbz = plane_q(a.in.bz, z0 - 2, nz, IO::kBytes, plane);
+#pragma unroll
+            for (int k = 0; k < K2; ++k) bz_cur[k] = IO::load(rbz, off2[k]);
         }
+        for (int s = z0 - 4; s <= z1; ++s) step(s, r, o);
     }
     const double s = block_sum(e_acc, red);
     if (tid == 0) a.partials[blockIdx.x] = s;

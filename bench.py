@@ -81,6 +81,13 @@ def main():
     value = N * args.iters * args.steps / wall
     k_ms = float(np.mean(kms))
     achieved = BYTES_PER_VOXEL_ITER * N / (k_ms * 1e-3) / 1e9
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs of
+    # this kernel and workload (scaled by a known-size copy), committed under profiles/
+    traffic = None
+    prof = ROOT / "profiles" / "kf_binary_256cubed.json"
+    if prof.exists() and n == 256 and args.kernel == "fused" and not args.tile_x \
+            and not args.zchunk:
+        traffic = json.loads(prof.read_text())["traffic_bytes_per_launch"]
 
     out = {
         "metric": "voxel-iters/s, 256^3 binary MF (plain, 200 fixed iterations); "
@@ -99,7 +106,7 @@ def main():
         "config": {"workload": f"binary3d {n}^3 ellipsoids, contrast alpha, cc 0.35 steps 0.11",
                    "shape": list(shape), "iters_per_step": args.iters, "kernel": args.kernel},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel_ms_per_iter": k_ms,
                      "bytes_per_voxel_iter": BYTES_PER_VOXEL_ITER},
         "gen_s": gen_s,

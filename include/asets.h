@@ -163,6 +163,13 @@ asets_status asets_binary3d_batched_dev(int32_t nvol, const asets_volume *vols,
                                         const asets_opts *o);
 
 /* ---- misc ---------------------------------------------------------------- */
+/* Self-check of the short correctly-rounded division form the kernels use for
+ * x/cc, n/alpha and 1/n (no reference counterpart): compares it with IEEE
+ * division over all 2^32 float bit patterns x.  mode 0: x/d for the divisor d
+ * (|d| in [2^-16, 2^16]); mode 1: 1/x for every x in that range.  Returns the
+ * number of bitwise mismatches in *mismatches (0 expected). */
+asets_status asets_selftest_division(int mode, float d, uint64_t *mismatches);
+
 const char *asets_last_error(void);
 const char *asets_version(void);
 /* Free the library's cached device workspaces (all devices). */

@@ -68,7 +68,7 @@ def test_binary2d_bitexact(cuda, kernel, shape):
 
 
 @pytest.mark.parametrize("tile_x,zchunk", [(8, 1), (8, 5), (16, 3), (16, 64), (32, 2),
-                                           (32, 17)])
+                                           (32, 17), (8, 23), (16, 7)])
 def test_fused_tilings_bitexact(cuda, tile_x, zchunk):
     shape = (70, 45, 23)
     Cs, Ct, a, _ = workloads.binary3d_ellipsoids(shape, seed=5, count=4)
@@ -234,3 +234,17 @@ def test_fused_equals_passes_full_size(cuda):
                        kernel="passes")
     assert torch.equal(rf.u.view(torch.int32), rp.u.view(torch.int32))
     torch.testing.assert_close(rf.cvg, rp.cvg, rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("mode,d", [(0, 0.35), (0, 0.25), (0, 0.3), (0, 0.2), (0, 1.0 / 3.0),
+                                    (0, 0.11), (0, 7.0), (0, 1.7e-3), (1, 1.0)])
+def test_short_division_is_ieee(cuda, mode, d):
+    """The kernels' short division form equals IEEE division for every float numerator
+    (mode 0: x/d) and every reciprocal in its range (mode 1: 1/x): 2^32 cases each."""
+    import ctypes as C
+
+    from asetsmatlabmaxflow_amd import _native as nat
+
+    bad = C.c_uint64(0)
+    nat.check(nat.lib().asets_selftest_division(mode, float(np.float32(d)), C.byref(bad)))
+    assert bad.value == 0
