@@ -15,13 +15,16 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
 
 #include "asets.h"
 #include "common.h"
+#include "kernels_batched.h"
 #include "kernels_fused.h"
 #include "kernels_passes.h"
 
@@ -243,6 +246,129 @@ __global__ void k_copy_f32(float *dst, const float *src, int64_t n) {
     if (i < n) dst[i] = src[i];
 }
 
+template <typename T>
+__global__ void k_init_binary_t(const float *Cs, const float *Ct, const float *u0,
+                                const float *pt0, float *u, T *ps, T *pt, int64_t N) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    float uu, m;
+    if (u0) {
+        uu = u0[g];
+        m = pt0[g];
+    } else {
+        const float cs = Cs[g], ct = Ct[g];
+        uu = (cs - ct >= 0.0f) ? 1.0f : 0.0f;   // asetsBinaryMF3D_mex.c:215-219
+        m = fmin_ref(cs, ct);
+    }
+    u[g] = uu;
+    Io<T>::put(ps + g, m);
+    Io<T>::put(pt + g, m);
+}
+
+template <typename T>
+__global__ void k_fill_t(T *p, float v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) Io<T>::put(p + i, v);
+}
+
+template <typename T>
+__global__ void k_to_f32(float *dst, const T *src, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = Io<T>::get(src + i);
+}
+
+// Everything the fused solve loop needs (one volume).
+struct FusedRun {
+    bool is3d;
+    const float *Cs, *Ct, *alpha;
+    int64_t ny, nx, nz;
+    const asets_params *p;
+    const float *u0, *pt0;
+    float *u, *cvg;
+    int32_t *it_num;
+    float *o_ps, *o_pt, *o_bx, *o_by, *o_bz;
+    TileCfg tc;
+    int ntile_y, ntile_x, nzc, nblocks, grid;
+    int64_t nparts;
+    double *partials;
+    Flags *flags;
+    hipStream_t s;
+    hipEvent_t ev0, ev1;
+    int32_t poll;
+};
+
+// Workspace bytes of a fused solve with flow element size es.
+size_t fused_bytes(int64_t N, size_t es) {
+    return 2 * (Carver::sz(N, sizeof(float)) + 5 * Carver::sz(N, es));
+}
+
+template <typename T>
+asets_status run_fused(const FusedRun &r, Carver &cv, DeviceCache &dc) {
+    const int64_t N = r.ny * r.nx * r.nz;
+    const hipStream_t s = r.s;
+    const unsigned nb = (unsigned)cdiv(N, kBlock);
+    // two state buffers; iteration i reads buf[i&1] and writes buf[(i+1)&1]
+    FusedState<T> buf[2];
+    for (int b = 0; b < 2; ++b) {
+        buf[b].u = cv.take<float>(N);
+        buf[b].ps = cv.take<T>(N);
+        buf[b].pt = cv.take<T>(N);
+        buf[b].bx = cv.take<T>(N);
+        buf[b].by = cv.take<T>(N);
+        buf[b].bz = cv.take<T>(N);
+    }
+    k_init_binary_t<T><<<nb, kBlock, 0, s>>>(r.Cs, r.Ct, r.u0, r.pt0, buf[0].u, buf[0].ps,
+                                            buf[0].pt, N);
+    for (int b = 0; b < 2; ++b)
+        for (T *f : {buf[b].bx, buf[b].by, buf[b].bz}) k_fill_t<T><<<nb, kBlock, 0, s>>>(f, 0.0f, N);
+    HIPCHK(hipGetLastError());
+
+    FusedArgs<T> fa{};
+    fa.Cs = r.Cs;
+    fa.Ct = r.Ct;
+    fa.alpha = r.alpha;
+    fa.ny = (int)r.ny;
+    fa.nx = (int)r.nx;
+    fa.nz = (int)r.nz;
+    fa.plane = (int)(r.ny * r.nx);
+    fa.fast_cc = (std::fabs(r.p->cc) >= 0x1p-16f && std::fabs(r.p->cc) <= 0x1p16f) ? 1 : 0;
+    fa.cc = r.p->cc;
+    fa.steps = r.p->steps;
+    fa.zchunk = r.tc.zchunk;
+    fa.ntile_y = r.ntile_y;
+    fa.ntile_x = r.ntile_x;
+    fa.nzc = r.nzc;
+    fa.nblocks = r.nblocks;
+    fa.partials = r.partials;
+    fa.flags = r.flags;
+    const int32_t max_iter = r.p->max_iter;
+    if (r.ev0) HIPCHK(hipEventRecord(r.ev0, s));
+    auto iter = [&](int32_t it) -> asets_status {
+        fa.in = buf[it & 1];
+        fa.out = buf[(it + 1) & 1];
+        launch_fused<T>(r.is3d, r.tc.tx, fa, r.grid, s);
+        k_finalize<<<1, 1024, 0, s>>>(r.partials, r.nparts, (double)N, it, r.p->err_bound, r.cvg,
+                                      r.it_num, r.flags);
+        return ASETS_OK;
+    };
+    asets_status st = run_loop(max_iter, r.p->err_bound, r.poll, s, dc, r.flags, iter);
+    if (st) return st;
+    if (r.ev1) HIPCHK(hipEventRecord(r.ev1, s));
+    // final state lives in buf[n_exec & 1]
+    int32_t itn = 0;
+    HIPCHK(hipMemcpyAsync(&itn, r.it_num, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const int32_t n_exec = (itn < max_iter) ? itn + 1 : max_iter;
+    const FusedState<T> &fin = buf[n_exec & 1];
+    HIPCHK(hipMemcpyAsync(r.u, fin.u, N * 4, hipMemcpyDeviceToDevice, s));
+    const T *srcs[5] = {fin.ps, fin.pt, fin.bx, fin.by, fin.bz};
+    float *dsts[5] = {r.o_ps, r.o_pt, r.o_bx, r.o_by, r.o_bz};
+    for (int k = 0; k < 5; ++k)
+        if (dsts[k]) k_to_f32<T><<<nb, kBlock, 0, s>>>(dsts[k], srcs[k], N);
+    HIPCHK(hipGetLastError());
+    return ASETS_OK;
+}
+
 // ---------------------------------------------------------------------------
 // binary model on device memory
 // ---------------------------------------------------------------------------
@@ -294,7 +420,7 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
     size_t need = Carver::sz(1, sizeof(Flags)) + Carver::sz(nparts, sizeof(double)) +
                   Carver::sz(1, sizeof(int32_t));
     if (fused)
-        need += 2 * (Carver::sz(N, sizeof(float)) + 5 * Carver::sz(N, es));
+        need += fused_bytes(N, es);
     else
         need += 7 * Carver::sz(N, sizeof(float));
     st = ensure(&dc.dev, &dc.dev_bytes, need);
@@ -360,77 +486,12 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
         if (o_by) HIPCHK(hipMemcpyAsync(o_by, by, N * 4, hipMemcpyDeviceToDevice, s));
         if (o_bz) HIPCHK(hipMemcpyAsync(o_bz, bz, N * 4, hipMemcpyDeviceToDevice, s));
     } else {
-        // two state buffers; iteration i reads buf[i&1] and writes buf[(i+1)&1]
-        float *ub[2];
-        void *fb[2][5];
-        for (int b = 0; b < 2; ++b) {
-            ub[b] = cv.take<float>(N);
-            for (int k = 0; k < 5; ++k) fb[b][k] = cv.take<char>(N * (int64_t)es);
-        }
-        // init buffer 0 (fp32 scratch in buffer 1's ps/pt when storing fp16)
-        if (flow16 == ASETS_FLOW_F32) {
-            float **f0 = (float **)fb[0];
-            k_init_binary<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Cs, Ct, u0, pt0, ub[0],
-                                                                      f0[0], f0[1], N);
-            for (int b = 0; b < 2; ++b)
-                for (int k = 2; k < 5; ++k) fill((float *)fb[b][k], 0.0f, N, s);
-        } else {
-            return fail(ASETS_EBADARG, "fp16 flow storage is not available in this build");
-        }
-        HIPCHK(hipGetLastError());
-
-        FusedArgs<float> fa{};
-        fa.Cs = Cs;
-        fa.Ct = Ct;
-        fa.alpha = alpha;
-        fa.ny = (int)ny;
-        fa.nx = (int)nx;
-        fa.nz = (int)nz;
-        fa.plane = (int)(ny * nx);
-        fa.fast_cc = (std::fabs(p->cc) >= 0x1p-16f && std::fabs(p->cc) <= 0x1p16f) ? 1 : 0;
-        fa.cc = p->cc;
-        fa.steps = p->steps;
-        fa.zchunk = tc.zchunk;
-        fa.ntile_y = ntile_y;
-        fa.ntile_x = ntile_x;
-        fa.nzc = nzc;
-        fa.nblocks = nblocks;
-        fa.partials = partials;
-        fa.flags = flags;
-        auto state = [&](int b) {
-            FusedState<float> x;
-            x.u = ub[b];
-            x.ps = (float *)fb[b][0];
-            x.pt = (float *)fb[b][1];
-            x.bx = (float *)fb[b][2];
-            x.by = (float *)fb[b][3];
-            x.bz = (float *)fb[b][4];
-            return x;
-        };
-        if (ev0) HIPCHK(hipEventRecord(ev0, s));
-        auto iter = [&](int32_t it) -> asets_status {
-            fa.in = state(it & 1);
-            fa.out = state((it + 1) & 1);
-            launch_fused<float>(is3d, tc.tx, fa, grid_fused, s);
-            k_finalize<<<1, 1024, 0, s>>>(partials, nparts, (double)N, it, p->err_bound, cvg,
-                                          it_num, flags);
-            return ASETS_OK;
-        };
-        st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc, flags, iter);
+        FusedRun fr{is3d, Cs, Ct, alpha, ny, nx, nz, p, u0, pt0, u, cvg, it_num,
+                    o_ps, o_pt, o_bx, o_by, o_bz, tc, ntile_y, ntile_x, nzc, nblocks,
+                    grid_fused, nparts, partials, flags, s, ev0, ev1,
+                    o ? o->poll_every : 0};
+        st = (flow16 == ASETS_FLOW_F16) ? run_fused<__half>(fr, cv, dc) : run_fused<float>(fr, cv, dc);
         if (st) return st;
-        if (ev1) HIPCHK(hipEventRecord(ev1, s));
-        // final state lives in buf[n_exec & 1]
-        int32_t itn = 0;
-        HIPCHK(hipMemcpyAsync(&itn, it_num, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        const int32_t n_exec = (itn < max_iter) ? itn + 1 : max_iter;
-        FusedState<float> fin = state(n_exec & 1);
-        HIPCHK(hipMemcpyAsync(u, fin.u, N * 4, hipMemcpyDeviceToDevice, s));
-        if (o_ps) HIPCHK(hipMemcpyAsync(o_ps, fin.ps, N * 4, hipMemcpyDeviceToDevice, s));
-        if (o_pt) HIPCHK(hipMemcpyAsync(o_pt, fin.pt, N * 4, hipMemcpyDeviceToDevice, s));
-        if (o_bx) HIPCHK(hipMemcpyAsync(o_bx, fin.bx, N * 4, hipMemcpyDeviceToDevice, s));
-        if (o_by) HIPCHK(hipMemcpyAsync(o_by, fin.by, N * 4, hipMemcpyDeviceToDevice, s));
-        if (o_bz) HIPCHK(hipMemcpyAsync(o_bz, fin.bz, N * 4, hipMemcpyDeviceToDevice, s));
     }
     HIPCHK(hipStreamSynchronize(s));
     if (ev0) {
@@ -541,6 +602,339 @@ asets_status potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny
         HIPCHK(hipMemcpy(&itn, it_num, sizeof(int32_t), hipMemcpyDeviceToHost));
         const int32_t n_exec = (itn < max_iter) ? itn + 1 : max_iter;
         *o->kernel_ms = n_exec > 0 ? (double)ms / n_exec : 0.0;
+        (void)hipEventDestroy(ev0);
+        (void)hipEventDestroy(ev1);
+    }
+    if (seconds)
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    return ASETS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: tile shape of the fused kernel per shape bucket
+// ---------------------------------------------------------------------------
+// Candidates are timed on the caller's own data (a warm-up iteration and two
+// timed iterations each, state re-initialised afterwards) and the fastest is
+// cached in memory under a key made of the volume count, the total voxel count
+// rounded to a quarter octave and the flow storage type.  With ASETS_TUNE_CACHE
+// set to a file path the cache is also read from and appended to that file.
+struct TuneCache {
+    std::mutex mu;
+    std::map<std::string, TileCfg> m;
+    bool loaded = false;
+};
+TuneCache g_tune;
+
+std::string tune_key(const char *kind, int32_t nvol, int64_t vox, int es) {
+    int bucket = (int)std::lround(4.0 * std::log2((double)std::max<int64_t>(vox, 1)));
+    char buf[128];
+    snprintf(buf, sizeof buf, "%s:nvol=%d:vox4log2=%d:es=%d", kind, nvol, bucket, es);
+    return buf;
+}
+
+bool tune_lookup(const std::string &key, TileCfg *out) {
+    std::lock_guard<std::mutex> lk(g_tune.mu);
+    const char *path = std::getenv("ASETS_TUNE_CACHE");
+    if (!g_tune.loaded && path) {
+        g_tune.loaded = true;
+        if (FILE *f = std::fopen(path, "r")) {
+            char k[256];
+            int tx, zc;
+            while (std::fscanf(f, "%255s %d %d", k, &tx, &zc) == 3) g_tune.m[k] = TileCfg{tx, zc};
+            std::fclose(f);
+        }
+    }
+    auto it = g_tune.m.find(key);
+    if (it == g_tune.m.end()) return false;
+    *out = it->second;
+    return true;
+}
+
+void tune_store(const std::string &key, TileCfg c) {
+    std::lock_guard<std::mutex> lk(g_tune.mu);
+    g_tune.m[key] = c;
+    if (const char *path = std::getenv("ASETS_TUNE_CACHE")) {
+        if (FILE *f = std::fopen(path, "a")) {
+            std::fprintf(f, "%s %d %d\n", key.c_str(), c.tx, c.zchunk);
+            std::fclose(f);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// batched binary 3D
+// ---------------------------------------------------------------------------
+template <int TX, int NT, typename T>
+int batched_grid() {
+    constexpr int W = (NT >= 512) ? 4 : 3;
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kf_binary_batched<TX, NT, W, T>, NT,
+                                                       0);
+    return std::max(1, cus) * std::max(1, per);
+}
+
+template <typename T>
+void launch_batched(int tx, const BatchVol<T> *vols, const int2 *work, int nwork,
+                    const int32_t *vdone, const int32_t *viter, double *partials, BatchCtl *ctl,
+                    float cc, float steps, int fast_cc, hipStream_t s) {
+    if (tx == 8) {
+        const int g = std::min(nwork, batched_grid<8, 256, T>());
+        kf_binary_batched<8, 256, 3, T><<<g, 256, 0, s>>>(vols, work, nwork, vdone, viter,
+                                                          partials, ctl, cc, steps, fast_cc);
+    } else if (tx == 32) {
+        const int g = std::min(nwork, batched_grid<32, 1024, T>());
+        kf_binary_batched<32, 1024, 4, T><<<g, 1024, 0, s>>>(vols, work, nwork, vdone, viter,
+                                                             partials, ctl, cc, steps, fast_cc);
+    } else {
+        const int g = std::min(nwork, batched_grid<16, 512, T>());
+        kf_binary_batched<16, 512, 4, T><<<g, 512, 0, s>>>(vols, work, nwork, vdone, viter,
+                                                           partials, ctl, cc, steps, fast_cc);
+    }
+}
+
+__global__ void k_batch_reset(int32_t *vdone, int32_t *viter, int nvol, int32_t max_iter,
+                              BatchCtl *ctl) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < nvol) {
+        vdone[v] = (max_iter <= 0) ? 1 : 0;
+        viter[v] = 0;
+    }
+    if (v == 0) {
+        ctl->remaining = (max_iter <= 0) ? 0 : nvol;
+        ctl->work_next = 0;
+    }
+}
+
+template <typename T>
+asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_params *p,
+                         double *seconds, const asets_opts *o) {
+    auto t_start = std::chrono::steady_clock::now();
+    if (nvol <= 0 || !vols) return fail(ASETS_EBADARG, "need at least one volume");
+    asets_status st = check_params(p);
+    if (st) return st;
+    int64_t total = 0;
+    for (int v = 0; v < nvol; ++v) {
+        const asets_volume &V = vols[v];
+        st = check_dims(V.ny, V.nx, V.nz, 1);
+        if (st) return fail(st, "volume %d: %s", v, asets_last_error());
+        if (!V.Cs || !V.Ct || !V.alpha || !V.u || !V.it_num || (p->max_iter > 0 && !V.cvg))
+            return fail(ASETS_EBADARG, "volume %d: null array", v);
+        total += V.ny * V.nx * V.nz;
+    }
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    DeviceCache &dc = g_cache[dev];
+    std::lock_guard<std::mutex> lock(dc.mu);
+    if (!dc.host_flag) HIPCHK(hipHostMalloc((void **)&dc.host_flag, 64));
+    hipStream_t us = o ? (hipStream_t)o->stream : nullptr;
+    hipStream_t s = us;
+    hipStream_t own = nullptr;
+    if (!s) {   // graphs cannot be captured on the null stream
+        HIPCHK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+        s = own;
+    }
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() {
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } sg{own};
+    const int32_t max_iter = p->max_iter;
+    const int fast_cc = (std::fabs(p->cc) >= 0x1p-16f && std::fabs(p->cc) <= 0x1p16f) ? 1 : 0;
+
+    // work-list capacity: the finest tiling the tuner may pick (tx 8, zchunk 8)
+    int64_t max_items = 0;
+    for (int v = 0; v < nvol; ++v)
+        max_items += cdiv(vols[v].ny, 64) * cdiv(vols[v].nx, 8) * cdiv(vols[v].nz, 8);
+    size_t need = Carver::sz(1, sizeof(BatchCtl)) + Carver::sz(nvol, sizeof(BatchVol<T>)) +
+                  Carver::sz(nvol, sizeof(BatchMeta)) + 2 * Carver::sz(nvol, 4) +
+                  Carver::sz(max_items, sizeof(int2)) + Carver::sz(max_items, sizeof(double)) +
+                  Carver::sz(1, 4);
+    for (int v = 0; v < nvol; ++v) need += fused_bytes(vols[v].ny * vols[v].nx * vols[v].nz, sizeof(T));
+    st = ensure(&dc.dev, &dc.dev_bytes, need);
+    if (st) return st;
+    Carver cv(dc.dev);
+    BatchCtl *ctl = cv.take<BatchCtl>(1);
+    BatchVol<T> *d_vols = cv.take<BatchVol<T>>(nvol);
+    BatchMeta *d_meta = cv.take<BatchMeta>(nvol);
+    int32_t *vdone = cv.take<int32_t>(nvol);
+    int32_t *viter = cv.take<int32_t>(nvol);
+    int2 *d_work = cv.take<int2>(max_items);
+    double *partials = cv.take<double>(max_items);
+    int32_t *scratch_i32 = cv.take<int32_t>(1);
+
+    std::vector<BatchVol<T>> hv(nvol);
+    std::vector<BatchMeta> hm(nvol);
+    for (int v = 0; v < nvol; ++v) {
+        const asets_volume &V = vols[v];
+        const int64_t N = V.ny * V.nx * V.nz;
+        BatchVol<T> &b = hv[v];
+        b.Cs = V.Cs;
+        b.Ct = V.Ct;
+        b.alpha = V.alpha;
+        for (int k = 0; k < 2; ++k) {
+            b.u[k] = cv.take<float>(N);
+            b.ps[k] = cv.take<T>(N);
+            b.pt[k] = cv.take<T>(N);
+            b.bx[k] = cv.take<T>(N);
+            b.by[k] = cv.take<T>(N);
+            b.bz[k] = cv.take<T>(N);
+        }
+        b.ny = (int)V.ny;
+        b.nx = (int)V.nx;
+        b.nz = (int)V.nz;
+        hm[v].n = (double)N;
+        hm[v].cvg = V.cvg;
+        hm[v].it_num = V.it_num;
+        if (max_iter > 0) {
+            st = alpha_ok(V.alpha, N, s, dc, scratch_i32);
+            if (st) return fail(st, "volume %d: %s", v, asets_last_error());
+        }
+    }
+
+    // (re)initialise every volume's state and the control words
+    auto init_all = [&]() -> asets_status {
+        for (int v = 0; v < nvol; ++v) {
+            const BatchVol<T> &b = hv[v];
+            const int64_t N = (int64_t)b.ny * b.nx * b.nz;
+            const unsigned nb = (unsigned)cdiv(N, kBlock);
+            k_init_binary_t<T><<<nb, kBlock, 0, s>>>(b.Cs, b.Ct, nullptr, nullptr, b.u[0], b.ps[0],
+                                                    b.pt[0], N);
+            for (int k = 0; k < 2; ++k)
+                for (T *f : {b.bx[k], b.by[k], b.bz[k]}) k_fill_t<T><<<nb, kBlock, 0, s>>>(f, 0.0f, N);
+            fill(vols[v].cvg, 0.0f, max_iter, s);
+            k_fill_i32<<<1, 1, 0, s>>>(vols[v].it_num, max_iter, 1);
+        }
+        k_batch_reset<<<(unsigned)cdiv(nvol, 256), 256, 0, s>>>(vdone, viter, nvol, max_iter, ctl);
+        HIPCHK(hipGetLastError());
+        return ASETS_OK;
+    };
+
+    // tiling -> work list
+    int nwork = 0;
+    auto set_tiling = [&](TileCfg c) -> asets_status {
+        std::vector<int2> w;
+        for (int v = 0; v < nvol; ++v) {
+            BatchVol<T> &b = hv[v];
+            b.ntile_y = (int)cdiv(b.ny, 64);
+            b.ntile_x = (int)cdiv(b.nx, c.tx);
+            b.zchunk = std::min(c.zchunk, b.nz);
+            b.nzc = (int)cdiv(b.nz, b.zchunk);
+            b.nblocks = b.ntile_y * b.ntile_x * b.nzc;
+            b.work0 = (int)w.size();
+            hm[v].work0 = b.work0;
+            hm[v].nblocks = b.nblocks;
+            for (int lb = 0; lb < b.nblocks; ++lb) w.push_back(make_int2(v, lb));
+        }
+        nwork = (int)w.size();
+        HIPCHK(hipMemcpyAsync(d_work, w.data(), w.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_vols, hv.data(), nvol * sizeof(BatchVol<T>), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_meta, hm.data(), nvol * sizeof(BatchMeta), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return ASETS_OK;
+    };
+    auto one_iter = [&](int tx) {
+        launch_batched<T>(tx, d_vols, d_work, nwork, vdone, viter, partials, ctl, p->cc, p->steps,
+                          fast_cc, s);
+        k_finalize_batched<<<nvol, 256, 0, s>>>(d_meta, vdone, viter, partials, p->err_bound,
+                                                max_iter, ctl);
+    };
+
+    // tile shape: options, tuning cache, or tune now
+    TileCfg cfg{16, 32};
+    const bool fixed = o && o->tile_x > 0;
+    if (fixed) {
+        cfg.tx = (o->tile_x == 8 || o->tile_x == 32) ? o->tile_x : 16;
+        cfg.zchunk = o->zchunk > 0 ? o->zchunk : 32;
+    } else if (max_iter > 0) {
+        const std::string key = tune_key("batched3d", nvol, total, (int)sizeof(T));
+        if (!tune_lookup(key, &cfg)) {
+            float best = 1e30f;
+            hipEvent_t e0, e1;
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            for (int tx : {8, 16, 32})
+                for (int zc : {8, 16, 32, 64}) {
+                    st = set_tiling(TileCfg{tx, zc});
+                    if (st) return st;
+                    st = init_all();
+                    if (st) return st;
+                    one_iter(tx);   // warm-up
+                    HIPCHK(hipEventRecord(e0, s));
+                    one_iter(tx);
+                    one_iter(tx);
+                    HIPCHK(hipEventRecord(e1, s));
+                    HIPCHK(hipEventSynchronize(e1));
+                    float ms = 0.0f;
+                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < best) {
+                        best = ms;
+                        cfg = TileCfg{tx, zc};
+                    }
+                }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            tune_store(key, cfg);
+        }
+    }
+    st = set_tiling(cfg);
+    if (st) return st;
+    st = init_all();
+    if (st) return st;
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (o && o->kernel_ms) {
+        HIPCHK(hipEventCreate(&ev0));
+        HIPCHK(hipEventCreate(&ev1));
+        HIPCHK(hipEventRecord(ev0, s));
+    }
+    // K iterations captured once as a graph, replayed until every volume stopped
+    int32_t done_iters = 0;
+    if (max_iter > 0) {
+        const int K = std::max(1, std::min(8, (int)max_iter));
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < K; ++k) one_iter(cfg.tx);
+        HIPCHK(hipStreamEndCapture(s, &graph));
+        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        asets_status lst = ASETS_OK;
+        while (done_iters < max_iter) {
+            if (hipGraphLaunch(exec, s) != hipSuccess) {
+                lst = fail(ASETS_EDEVICE, "hipGraphLaunch failed");
+                break;
+            }
+            done_iters += K;
+            if (hipMemcpyAsync(dc.host_flag, &ctl->remaining, 4, hipMemcpyDeviceToHost, s) !=
+                    hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                lst = fail(ASETS_EDEVICE, "stream error in the batched loop");
+                break;
+            }
+            if (*dc.host_flag <= 0) break;
+        }
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        if (lst) return lst;
+    }
+    if (ev1) HIPCHK(hipEventRecord(ev1, s));
+    // u of each volume lives in buffer (iterations executed) & 1
+    std::vector<int32_t> itn(nvol);
+    for (int v = 0; v < nvol; ++v)
+        HIPCHK(hipMemcpyAsync(&itn[v], vols[v].it_num, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int v = 0; v < nvol; ++v) {
+        const int32_t n_exec = (itn[v] < max_iter) ? itn[v] + 1 : max_iter;
+        const int64_t N = (int64_t)hv[v].ny * hv[v].nx * hv[v].nz;
+        HIPCHK(hipMemcpyAsync(vols[v].u, hv[v].u[n_exec & 1], N * 4, hipMemcpyDeviceToDevice, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    if (ev0) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        *o->kernel_ms = ms;   // whole batched loop (ms)
         (void)hipEventDestroy(ev0);
         (void)hipEventDestroy(ev1);
     }
@@ -719,12 +1113,10 @@ asets_status asets_potts_dev(int ndim, const float *Ct, const float *alpha, int6
 asets_status asets_binary3d_batched_dev(int32_t nvol, const asets_volume *vols,
                                         const asets_params *p, double *seconds,
                                         const asets_opts *o) {
-    (void)nvol;
-    (void)vols;
-    (void)p;
-    (void)seconds;
-    (void)o;
-    return fail(ASETS_EBADARG, "batched solve is not available in this build");
+    const int flow16 = o ? o->flow_dtype : ASETS_FLOW_F32;
+    if (flow16 == ASETS_FLOW_F16) return batched_dev<__half>(nvol, vols, p, seconds, o);
+    if (flow16 != ASETS_FLOW_F32) return fail(ASETS_EBADARG, "unknown flow_dtype %d", flow16);
+    return batched_dev<float>(nvol, vols, p, seconds, o);
 }
 
 asets_status asets_selftest_division(int mode, float d, uint64_t *mismatches) {

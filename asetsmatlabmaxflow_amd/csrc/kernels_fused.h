@@ -40,9 +40,12 @@
 
 namespace asets {
 
+// u is always fp32; the flows ps, pt, bx, by, bz are stored as T (float, or
+// __half for the fp16-storage variant; arithmetic stays fp32 either way).
 template <typename T>
 struct FusedState {
-    T *u, *ps, *pt, *bx, *by, *bz;
+    float *u;
+    T *ps, *pt, *bx, *by, *bz;
 };
 
 template <typename T>
@@ -91,6 +94,8 @@ template <> struct Io<float> {
     static __device__ __forceinline__ void store(__amdgpu_buffer_rsrc_t r, int off, float v) {
         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off * 4, 0, 0);
     }
+    static __device__ __forceinline__ float get(const float *p) { return *p; }
+    static __device__ __forceinline__ void put(float *p, float v) { *p = v; }
 };
 template <> struct Io<__half> {
     static constexpr int kBytes = 2;
@@ -102,6 +107,8 @@ template <> struct Io<__half> {
         __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(__float2half_rn(v)), r, off * 2,
                                               0, 0);
     }
+    static __device__ __forceinline__ float get(const __half *p) { return __half2float(*p); }
+    static __device__ __forceinline__ void put(__half *p, float v) { *p = __float2half_rn(v); }
 };
 
 __device__ __forceinline__ float ldf(__amdgpu_buffer_rsrc_t r, int off) {
@@ -205,8 +212,11 @@ __device__ __forceinline__ float rescale(float g_hi, float g_lo, float b, bool t
 
 // ---- the kernel -------------------------------------------------------------
 
-template <bool IS3D, int TX, int NT, int WAVES, typename T>
-__global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
+// One work item: tile lb (of a.ntile_y x a.ntile_x tiles x a.nzc z chunks) of
+// one volume, all its planes.  Returns this thread's share of sum |e|.  Ends
+// with a barrier, so a workgroup can run several items back to back.
+template <bool IS3D, int TX, int NT, typename T>
+__device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
     using IO = Io<T>;
     constexpr int TY = 64;
     constexpr int W2 = TY + 4, H2 = TX + 4;   // gk / b1 region (tile + radius 2)
@@ -223,19 +233,10 @@ __global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
     // only the last position slot of a thread can fall past the region
     auto in2 = [&](int k, int idx) { return k < K2 - 1 || (N2 % NT) == 0 || idx < N2; };
     auto in1 = [&](int k, int idx) { return k < K1 - 1 || (N1 % NT) == 0 || idx < N1; };
-    __shared__ double red[16];
 
-    if (a.flags->done) return;
-
-    // XCD-aware block order: hardware dispatches block b to XCD b % 8; remap
-    // so that consecutive logical tiles share an XCD and its L2.
-    const int nb = gridDim.x;
-    int lb = blockIdx.x;
-    if ((nb & 7) == 0) lb = (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
     const int tid = threadIdx.x;
     double e_acc = 0.0;
-
-    if (lb < a.nblocks) {
+    {
         const int tyi = lb % a.ntile_y;
         const int txi = (lb / a.ntile_y) % a.ntile_x;
         const int zci = lb / (a.ntile_y * a.ntile_x);
@@ -294,7 +295,7 @@ __global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
         // load raw plane q (u, ps, pt, bx own/high, by own/high), bz own of
         // plane q+1, and alpha of plane q-1 (for S2b one step later)
         auto load_raw = [&](int q, Raw &r) {
-            const auto ru = plane_q(a.in.u, q, nz, IO::kBytes, plane);
+            const auto ru = plane_q(a.in.u, q, nz, 4, plane);
             const auto rps = plane_q(a.in.ps, q, nz, IO::kBytes, plane);
             const auto rpt = plane_q(a.in.pt, q, nz, IO::kBytes, plane);
             const auto rbx = plane_q(a.in.bx, q, nz, IO::kBytes, plane);
@@ -302,7 +303,7 @@ __global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
 #pragma unroll
             for (int k = 0; k < K2; ++k) {
                 const int o = off2[k];
-                r.u[k] = IO::load(ru, o);
+                r.u[k] = ldf(ru, o);
                 r.ps[k] = IO::load(rps, o);
                 r.pt[k] = IO::load(rpt, o);
                 r.bxl[k] = IO::load(rbx, o);
@@ -401,7 +402,7 @@ __global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
         };
         auto load_out = [&](int s, Out &o) {
             const int sv = (s >= z0 && s < z1) ? s : -1;   // outside: empty descriptors
-            const auto ru = plane_q(a.in.u, sv, nz, IO::kBytes, plane);
+            const auto ru = plane_q(a.in.u, sv, nz, 4, plane);
             const auto rpt = plane_q(a.in.pt, sv, nz, IO::kBytes, plane);
             const auto rcs = plane_q(a.Cs, sv, nz, 4, plane);
             const auto rct = plane_q(a.Ct, sv, nz, 4, plane);
@@ -409,7 +410,7 @@ __global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
             for (int k = 0; k < KO; ++k) {
                 const int i = tid % TY, j = tid / TY + k * (NT / TY);
                 const int off = (x0 + j) * ny + (y0 + i);
-                o.u[k] = IO::load(ru, off);
+                o.u[k] = ldf(ru, off);
                 o.pt[k] = IO::load(rpt, off);
                 o.cs[k] = ldf(rcs, off);
                 o.ct[k] = ldf(rct, off);
@@ -447,7 +448,7 @@ __global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
             const float *g0 = G[s & 1];
             const float *g1 = G[(s + 1) & 1];
             const int64_t base = (int64_t)s * plane;
-            const auto wu = plane_rsrc(a.out.u, base, IO::kBytes, plane);
+            const auto wu = plane_rsrc(a.out.u, base, 4, plane);
             const auto wps = plane_rsrc(a.out.ps, base, IO::kBytes, plane);
             const auto wpt = plane_rsrc(a.out.pt, base, IO::kBytes, plane);
             const auto wbx = plane_rsrc(a.out.bx, base, IO::kBytes, plane);
@@ -494,7 +495,7 @@ __global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
                 if (y < ny && x < nx) {
                     e_acc += fabs((double)e);
                     const int off = x * ny + y;
-                    IO::store(wu, off, uo - e);
+                    Io<float>::store(wu, off, uo - e);
                     IO::store(wps, off, nps);
                     IO::store(wpt, off, npt);
                     IO::store(wbx, off, bxl);
@@ -536,8 +537,24 @@ __global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
         }
         for (int s = z0 - 4; s <= z1; ++s) step(s, r, o);
     }
-    const double s = block_sum(e_acc, red);
-    if (tid == 0) a.partials[blockIdx.x] = s;
+    return e_acc;
+}
+
+// XCD-aware block order: hardware dispatches block b to XCD b % 8; remap so
+// that consecutive logical tiles share an XCD and its L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    return ((nb & 7) == 0) ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+}
+
+// One volume: one workgroup per work item, partial |e| sums per workgroup.
+template <bool IS3D, int TX, int NT, int WAVES, typename T>
+__global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
+    __shared__ double red[16];
+    if (a.flags->done) return;
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const double e = (lb < a.nblocks) ? fused_tile<IS3D, TX, NT, T>(a, lb) : 0.0;
+    const double s = block_sum(e, red);
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
 }
 
 }  // namespace asets

@@ -141,3 +141,40 @@ def potts(Ct, alpha, shape, nlab: int, max_iter: int, err_bound: float, cc: floa
                kernel_ms=kms.value if kms is not None else None)
     r._executed = it + 1 if it < max_iter else max_iter
     return r
+
+
+def binary_batched(volumes, max_iter: int, err_bound: float, cc: float, steps: float, *,
+                   flow: str = "f32", tile_x: int = 0, zchunk: int = 0, stream=None,
+                   time_kernel: bool = False):
+    """Many binary 3D volumes of uneven size in one graph-captured launch per iteration.
+
+    volumes: list of (Cs, Ct, alpha, (ny, nx, nz)) with flat float32 CUDA tensors.
+    Returns (results, seconds, loop_ms): one Result per volume (u, cvg, it_num)."""
+    vols = (nat.Volume * len(volumes))()
+    keep = []
+    for i, (Cs, Ct, alpha, shape) in enumerate(volumes):
+        ny, nx, nz = shape
+        N = ny * nx * nz
+        for t in (Cs, Ct, alpha):
+            if t.numel() != N:
+                raise ValueError(f"volume {i}: array has {t.numel()} elements, shape implies {N}")
+        u = torch.empty(N, dtype=torch.float32, device=Cs.device)
+        cvg = torch.empty(max(max_iter, 1), dtype=torch.float32, device=Cs.device)
+        itn = torch.empty(1, dtype=torch.int32, device=Cs.device)
+        keep.append((u, cvg, itn))
+        vols[i] = nat.Volume(_ptr(Cs), _ptr(Ct), _ptr(alpha), ny, nx, nz, u.data_ptr(),
+                             cvg.data_ptr(), itn.data_ptr())
+    secs = C.c_double(0.0)
+    kms = C.c_double(0.0) if time_kernel else None
+    handle = stream.cuda_stream if stream is not None else None
+    opts = nat.make_opts(handle, _FLOWS[flow], nat.KERNEL_FUSED, 0, tile_x, zchunk, kms)
+    p = nat.make_params(max_iter, err_bound, cc, steps)
+    nat.check(nat.lib().asets_binary3d_batched_dev(len(volumes), vols, C.byref(p),
+                                                   C.byref(secs), C.byref(opts)))
+    out = []
+    for u, cvg, itn in keep:
+        it = int(itn.item())
+        r = Result(u=u, cvg=cvg[:max_iter], it_num=it, seconds=secs.value)
+        r._executed = it + 1 if it < max_iter else max_iter
+        out.append(r)
+    return out, secs.value, (kms.value if kms is not None else None)

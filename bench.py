@@ -39,6 +39,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=int(os.environ.get("ASETS_CPU_ITERS", 20)))
     ap.add_argument("--no-tol", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp16", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -111,6 +112,19 @@ def main():
                      "bytes_per_voxel_iter": BYTES_PER_VOXEL_ITER},
         "gen_s": gen_s,
     }
+
+    # fp16 flow storage (ps, pt, bx, by, bz in fp16; u and arithmetic fp32): 40 B/voxel-iter
+    if not args.no_fp16:
+        with torch.cuda.stream(stream):
+            solver.binary(dCs, dCt, dA, shape, 20, 0.0, cc, steps, flow="f16", stream=stream)
+            r16 = solver.binary(dCs, dCt, dA, shape, args.iters, 0.0, cc, steps, flow="f16",
+                                stream=stream, time_kernel=True)
+        torch.cuda.synchronize()
+        out["fp16_flows"] = {"kernel_ms_per_iter": r16.kernel_ms,
+                             "voxel_iter_per_s": N / (r16.kernel_ms * 1e-3),
+                             "bytes_per_voxel_iter": 40,
+                             "achieved_GBps": 40 * N / (r16.kernel_ms * 1e-3) / 1e9,
+                             "max_abs_u_diff_vs_fp32": float((r16.u - u_out).abs().max().item())}
 
     if not args.no_tol:
         torch.cuda.synchronize()

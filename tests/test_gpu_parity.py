@@ -248,3 +248,88 @@ def test_short_division_is_ieee(cuda, mode, d):
     bad = C.c_uint64(0)
     nat.check(nat.lib().asets_selftest_division(mode, float(np.float32(d)), C.byref(bad)))
     assert bad.value == 0
+
+
+# ---- fp16 flow storage (fp32 arithmetic) ----------------------------------------------
+
+@pytest.mark.parametrize("shape,iters", [((96, 80, 72), 150), ((130, 77), 300)])
+def test_fp16_flow_storage_error(cuda, shape, iters):
+    """ps, pt, bx, by, bz stored in fp16, u and all arithmetic in fp32.  Reported against
+    the FP64 oracle next to the fp32 kernel's own error; bar: labels (u > 0.5) agree on
+    >= 99.9 % of voxels and max |u - u_f64| <= 0.05."""
+    if len(shape) == 3:
+        Cs, Ct, a, _ = workloads.binary3d_ellipsoids(shape, seed=1002, count=6)
+        cc, st = workloads.BINARY3D_CC, workloads.BINARY3D_STEPS
+    else:
+        Cs, Ct, a, _, _ = workloads.binary2d_ellipses(n=shape[0], seed=1001)[:4] + (None,)
+        Cs, Ct, a = Cs[:, :shape[1]], Ct[:, :shape[1]], a[:, :shape[1]]
+        cc, st = 0.3, 0.16
+    ref64 = orc.binary(Cs, Ct, a, shape, iters, 0.0, cc, st, precision="f64")
+    r16 = _run_binary(Cs, Ct, a, shape, iters, 0.0, cc, st, kernel="fused", flow="f16")
+    r32 = _run_binary(Cs, Ct, a, shape, iters, 0.0, cc, st, kernel="fused")
+    u16, u32 = r16.u.cpu().numpy().astype(np.float64), r32.u.cpu().numpy().astype(np.float64)
+    err16 = np.abs(u16 - ref64.u).max()
+    err32 = np.abs(u32 - ref64.u).max()
+    agree = ((u16 > 0.5) == (ref64.u > 0.5)).mean()
+    print(f"fp16 storage {shape} x{iters}: max|u-u64| fp16 {err16:.3e} fp32 {err32:.3e}, "
+          f"label agreement {agree:.6f}")
+    assert err32 < 1e-4
+    assert err16 < 0.05 and agree >= 0.999
+
+
+def test_fp16_requires_fused(cuda):
+    from asetsmatlabmaxflow_amd import _native as nat
+
+    shape = (8, 8, 4)
+    Cs, Ct, a = workloads.random_smooth_binary(shape, seed=1)
+    with pytest.raises(nat.AsetsError, match="fused"):
+        _run_binary(Cs, Ct, a, shape, 3, 0.0, 0.25, 0.11, kernel="passes", flow="f16")
+
+
+# ---- batched uneven volumes --------------------------------------------------------------
+
+BATCH_SHAPES = [(40, 33, 21), (64, 64, 17), (17, 90, 33), (70, 45, 23), (33, 20, 66),
+                (130, 8, 9)]
+
+
+def _batch_inputs():
+    vols = []
+    for i, sh in enumerate(BATCH_SHAPES):
+        Cs, Ct, a, _ = workloads.binary3d_ellipsoids(sh, seed=1004000 + i, count=3)
+        vols.append((Cs, Ct, a, sh))
+    return vols
+
+
+@pytest.mark.parametrize("flow", ["f32", "f16"])
+def test_batched_matches_single(cuda, flow, tmp_path, monkeypatch):
+    """Each volume of a batched solve (autotuned tiling, graph replay, per-volume stop)
+    equals its own single-volume solve bit for bit, including the stop iteration."""
+    monkeypatch.setenv("ASETS_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    host = _batch_inputs()
+    dev = [tuple(solver.to_device(x) for x in v[:3]) + (v[3],) for v in host]
+    eb = 2e-4
+    res, secs, _ = solver.binary_batched(dev, 300, eb, 0.35, 0.11, flow=flow)
+    its = []
+    for (Cs, Ct, a, sh), r in zip(dev, res):
+        single = solver.binary(Cs, Ct, a, sh, 300, eb, 0.35, 0.11, kernel="fused", flow=flow)
+        assert r.it_num == single.it_num, sh
+        assert torch.equal(r.u.view(torch.int32), single.u.view(torch.int32)), sh
+        torch.testing.assert_close(r.cvg[:r.iterations], single.cvg[:single.iterations],
+                                   rtol=1e-6, atol=0)
+        its.append(r.it_num)
+    assert len(set(its)) > 1   # volumes stopped at different iterations
+    lines = (tmp_path / "tune.txt").read_text().splitlines()
+    assert len(lines) == 1 and lines[0].startswith("batched3d:")
+    # second call with the same shape bucket: cache hit, no new tuning entry
+    solver.binary_batched(dev, 5, 0.0, 0.35, 0.11, flow=flow)
+    assert len((tmp_path / "tune.txt").read_text().splitlines()) == 1
+
+
+def test_batched_vs_oracle(cuda):
+    """Fixed iteration count: batched u equals the CPU oracle bit for bit."""
+    host = _batch_inputs()[:3]
+    dev = [tuple(solver.to_device(x) for x in v[:3]) + (v[3],) for v in host]
+    res, _, _ = solver.binary_batched(dev, 25, 0.0, 0.35, 0.11, tile_x=8, zchunk=8)
+    for (Cs, Ct, a, sh), r in zip(host, res):
+        ref = orc.binary(Cs, Ct, a, sh, 25, 0.0, 0.35, 0.11)
+        assert _bits_equal(r.u.cpu().numpy(), ref.u), sh
