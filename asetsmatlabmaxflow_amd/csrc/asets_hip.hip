@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -369,6 +370,42 @@ asets_status run_fused(const FusedRun &r, Carver &cv, DeviceCache &dc) {
     return ASETS_OK;
 }
 
+template <bool IS3D, int TX, int NT>
+void launch_potts_flow_t(const PottsArgs &p, int grid, hipStream_t s) {
+    constexpr int W = (NT >= 512) ? 4 : 3;
+    kf_potts_flow<IS3D, TX, NT, W><<<grid, NT, 0, s>>>(p);
+}
+
+void launch_potts_flow(bool is3d, int tx, const PottsArgs &p, int grid, hipStream_t s) {
+    if (is3d) {
+        if (tx == 8) launch_potts_flow_t<true, 8, 256>(p, grid, s);
+        else if (tx == 32) launch_potts_flow_t<true, 32, 1024>(p, grid, s);
+        else launch_potts_flow_t<true, 16, 512>(p, grid, s);
+    } else {
+        if (tx == 8) launch_potts_flow_t<false, 8, 256>(p, grid, s);
+        else if (tx == 32) launch_potts_flow_t<false, 32, 1024>(p, grid, s);
+        else launch_potts_flow_t<false, 16, 512>(p, grid, s);
+    }
+}
+
+void launch_potts_couple(int L, const float *ft, float *u, float *ps, int64_t N, float cc,
+                         int fast_cc, double *partials, const Flags *flags, hipStream_t s) {
+    const unsigned nb = (unsigned)cdiv(N, kBlock);
+#define ASETS_COUPLE(LT) \
+    kp_potts_couple<LT><<<nb, kBlock, 0, s>>>(ft, u, ps, N, L, cc, fast_cc, partials, flags)
+    switch (L) {
+    case 2: ASETS_COUPLE(2); break;
+    case 3: ASETS_COUPLE(3); break;
+    case 4: ASETS_COUPLE(4); break;
+    case 5: ASETS_COUPLE(5); break;
+    case 6: ASETS_COUPLE(6); break;
+    case 8: ASETS_COUPLE(8); break;
+    case 10: ASETS_COUPLE(10); break;
+    default: ASETS_COUPLE(0); break;
+    }
+#undef ASETS_COUPLE
+}
+
 // ---------------------------------------------------------------------------
 // binary model on device memory
 // ---------------------------------------------------------------------------
@@ -540,16 +577,17 @@ asets_status potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny
     const Geo G{ny, nx, nz, N, L};
     const int32_t max_iter = p->max_iter;
     const int64_t nparts = cdiv(N, kBlock);
+    const int kern = o ? o->kernel : ASETS_KERNEL_AUTO;
+    const bool fused = (kern != ASETS_KERNEL_PASSES);
     size_t need = Carver::sz(1, sizeof(Flags)) + Carver::sz(nparts, sizeof(double)) +
-                  Carver::sz(1, sizeof(int32_t)) + 6 * Carver::sz(NL, 4) + Carver::sz(N, 4);
+                  Carver::sz(1, sizeof(int32_t)) + Carver::sz(N, 4) +
+                  (fused ? 9 : 6) * Carver::sz(NL, 4);
     st = ensure(&dc.dev, &dc.dev_bytes, need);
     if (st) return st;
     Carver cv(dc.dev);
     Flags *flags = cv.take<Flags>(1);
     double *partials = cv.take<double>(nparts);
     int32_t *scratch_i32 = cv.take<int32_t>(1);
-    float *bx = cv.take<float>(NL), *by = cv.take<float>(NL), *bz = cv.take<float>(NL);
-    float *dv = cv.take<float>(NL), *pt = cv.take<float>(NL), *gk = cv.take<float>(NL);
     float *ps = cv.take<float>(N);
 
     if (max_iter > 0) {
@@ -559,38 +597,106 @@ asets_status potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny
     k_fill_i32<<<1, 64, 0, s>>>((int32_t *)flags, 0, sizeof(Flags) / 4);
     k_fill_i32<<<1, 1, 0, s>>>(it_num, max_iter, 1);
     fill(cvg, 0.0f, max_iter, s);
-    fill(bx, 0.0f, NL, s);
-    fill(by, 0.0f, NL, s);
-    fill(bz, 0.0f, NL, s);
-    fill(dv, 0.0f, NL, s);
-    k_init_potts<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Ct, u, ps, pt, N, L);
-    HIPCHK(hipGetLastError());
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (o && o->kernel_ms) {
         HIPCHK(hipEventCreate(&ev0));
         HIPCHK(hipEventCreate(&ev1));
-        HIPCHK(hipEventRecord(ev0, s));
     }
     const unsigned nbl = (unsigned)cdiv(NL, kBlock), nbv = (unsigned)cdiv(N, kBlock);
-    auto iter = [&](int32_t it) -> asets_status {
-        kp_gk<<<nbl, kBlock, 0, s>>>(dv, ps, pt, u, gk, p->cc, G, flags);
-        if (is3d) {
-            kp_step<true><<<nbl, kBlock, 0, s>>>(gk, bx, by, bz, p->steps, G, flags);
-            kp_proj<true><<<nbl, kBlock, 0, s>>>(bx, by, bz, alpha, gk, G, flags);
-            kp_rescale<true><<<nbl, kBlock, 0, s>>>(gk, bx, by, bz, G, flags);
-            kp_ptdiv<true><<<nbl, kBlock, 0, s>>>(bx, by, bz, dv, ps, pt, u, Ct, p->cc, G, flags);
-        } else {
-            kp_step<false><<<nbl, kBlock, 0, s>>>(gk, bx, by, bz, p->steps, G, flags);
-            kp_proj<false><<<nbl, kBlock, 0, s>>>(bx, by, bz, alpha, gk, G, flags);
-            kp_rescale<false><<<nbl, kBlock, 0, s>>>(gk, bx, by, bz, G, flags);
-            kp_ptdiv<false><<<nbl, kBlock, 0, s>>>(bx, by, bz, dv, ps, pt, u, Ct, p->cc, G, flags);
+    std::function<asets_status(int32_t)> iter;
+    // per-pass family state
+    float *bx = nullptr, *by = nullptr, *bz = nullptr, *dv = nullptr, *pt = nullptr, *gk = nullptr;
+    // fused family state
+    PottsArgs pa{};
+    float *ptb[2] = {nullptr, nullptr}, *bb[2][3] = {};
+    TileCfg tc{};
+    int grid = 0;
+    if (!fused) {
+        bx = cv.take<float>(NL);
+        by = cv.take<float>(NL);
+        bz = cv.take<float>(NL);
+        dv = cv.take<float>(NL);
+        pt = cv.take<float>(NL);
+        gk = cv.take<float>(NL);
+        fill(bx, 0.0f, NL, s);
+        fill(by, 0.0f, NL, s);
+        fill(bz, 0.0f, NL, s);
+        fill(dv, 0.0f, NL, s);
+        k_init_potts<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Ct, u, ps, pt, N, L);
+        iter = [&](int32_t it) -> asets_status {
+            kp_gk<<<nbl, kBlock, 0, s>>>(dv, ps, pt, u, gk, p->cc, G, flags);
+            if (is3d) {
+                kp_step<true><<<nbl, kBlock, 0, s>>>(gk, bx, by, bz, p->steps, G, flags);
+                kp_proj<true><<<nbl, kBlock, 0, s>>>(bx, by, bz, alpha, gk, G, flags);
+                kp_rescale<true><<<nbl, kBlock, 0, s>>>(gk, bx, by, bz, G, flags);
+                kp_ptdiv<true><<<nbl, kBlock, 0, s>>>(bx, by, bz, dv, ps, pt, u, Ct, p->cc, G, flags);
+            } else {
+                kp_step<false><<<nbl, kBlock, 0, s>>>(gk, bx, by, bz, p->steps, G, flags);
+                kp_proj<false><<<nbl, kBlock, 0, s>>>(bx, by, bz, alpha, gk, G, flags);
+                kp_rescale<false><<<nbl, kBlock, 0, s>>>(gk, bx, by, bz, G, flags);
+                kp_ptdiv<false><<<nbl, kBlock, 0, s>>>(bx, by, bz, dv, ps, pt, u, Ct, p->cc, G, flags);
+            }
+            kp_psu<<<nbv, kBlock, 0, s>>>(dv, pt, u, ps, p->cc, G, partials, flags);
+            k_finalize<<<1, 1024, 0, s>>>(partials, nparts, (double)NL, it, p->err_bound, cvg,
+                                          it_num, flags);
+            return ASETS_OK;
+        };
+    } else {
+        // label-parallel fused flow kernel + per-voxel coupling kernel; u is
+        // updated in place in the caller's buffer, pt and faces ping-pong
+        for (int b = 0; b < 2; ++b) {
+            ptb[b] = cv.take<float>(NL);
+            for (int k = 0; k < 3; ++k) bb[b][k] = cv.take<float>(NL);
         }
-        kp_psu<<<nbv, kBlock, 0, s>>>(dv, pt, u, ps, p->cc, G, partials, flags);
-        k_finalize<<<1, 1024, 0, s>>>(partials, nparts, (double)NL, it, p->err_bound, cvg, it_num,
-                                      flags);
-        return ASETS_OK;
-    };
+        float *ft = cv.take<float>(NL);
+        for (int b = 0; b < 2; ++b)
+            for (int k = 0; k < 3; ++k) fill(bb[b][k], 0.0f, NL, s);
+        k_init_potts<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Ct, u, ps, ptb[0], N, L);
+        tc.tx = (o && o->tile_x > 0) ? o->tile_x : 16;
+        if (tc.tx != 8 && tc.tx != 32) tc.tx = 16;
+        const int64_t tiles = cdiv(ny, 64) * cdiv(nx, tc.tx) * L;
+        if (o && o->zchunk > 0) {
+            tc.zchunk = (int)std::min<int64_t>(o->zchunk, nz);
+        } else {
+            const int64_t nzc = std::max<int64_t>(1, (512 + tiles / 2) / tiles);
+            tc.zchunk = (int)std::min<int64_t>(nz, std::max<int64_t>(8, cdiv(nz, nzc)));
+        }
+        pa.Ct = Ct;
+        pa.alpha = alpha;
+        pa.u = u;
+        pa.ps = ps;
+        pa.ft = ft;
+        pa.ny = (int)ny;
+        pa.nx = (int)nx;
+        pa.nz = (int)nz;
+        pa.nlab = (int)L;
+        pa.cc = p->cc;
+        pa.steps = p->steps;
+        pa.zchunk = tc.zchunk;
+        pa.ntile_y = (int)cdiv(ny, 64);
+        pa.ntile_x = (int)cdiv(nx, tc.tx);
+        pa.nzc = (int)cdiv(nz, tc.zchunk);
+        pa.nblocks = pa.ntile_y * pa.ntile_x * pa.nzc;
+        pa.fast_cc = (std::fabs(p->cc) >= 0x1p-16f && std::fabs(p->cc) <= 0x1p16f) ? 1 : 0;
+        pa.flags = flags;
+        grid = (int)cdiv((int64_t)pa.nblocks * L, 8) * 8;
+        iter = [&](int32_t it) -> asets_status {
+            pa.pt_in = ptb[it & 1];
+            pa.pt_out = ptb[(it + 1) & 1];
+            for (int k = 0; k < 3; ++k) {
+                pa.b_in[k] = bb[it & 1][k];
+                pa.b_out[k] = bb[(it + 1) & 1][k];
+            }
+            launch_potts_flow(is3d, tc.tx, pa, grid, s);
+            launch_potts_couple((int)L, pa.ft, u, ps, N, p->cc, pa.fast_cc, partials, flags, s);
+            k_finalize<<<1, 1024, 0, s>>>(partials, nparts, (double)NL, it, p->err_bound, cvg,
+                                          it_num, flags);
+            return ASETS_OK;
+        };
+    }
+    HIPCHK(hipGetLastError());
+    if (ev0) HIPCHK(hipEventRecord(ev0, s));
     st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc, flags, iter);
     if (st) return st;
     if (ev1) HIPCHK(hipEventRecord(ev1, s));

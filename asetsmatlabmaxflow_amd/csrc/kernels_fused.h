@@ -215,7 +215,12 @@ __device__ __forceinline__ float rescale(float g_hi, float g_lo, float b, bool t
 // One work item: tile lb (of a.ntile_y x a.ntile_x tiles x a.nzc z chunks) of
 // one volume, all its planes.  Returns this thread's share of sum |e|.  Ends
 // with a barrier, so a workgroup can run several items back to back.
-template <bool IS3D, int TX, int NT, typename T>
+// POTTS: the same flow update for one label of the Potts model
+// (asetsPotts3D_mex.c:260-471, label k): in.ps is the label-shared source flow,
+// and the output stage instead updates the sink flow pt_k (:457-466) and writes
+// ft_k = dv_k + pt_k (the first term of updatePSU, :498) to out.ps; u is only
+// read.  The coupling over labels runs afterwards (kp_potts_couple).
+template <bool IS3D, int TX, int NT, typename T, bool POTTS = false>
 __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
     using IO = Io<T>;
     constexpr int TY = 64;
@@ -397,14 +402,15 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         };
 
         // S3 operands of plane s at the output positions
+        // (binary: u, pt, Cs, Ct; Potts: u, ps, -, Ct)
         struct Out {
             float u[KO], pt[KO], cs[KO], ct[KO];
         };
         auto load_out = [&](int s, Out &o) {
             const int sv = (s >= z0 && s < z1) ? s : -1;   // outside: empty descriptors
             const auto ru = plane_q(a.in.u, sv, nz, 4, plane);
-            const auto rpt = plane_q(a.in.pt, sv, nz, IO::kBytes, plane);
-            const auto rcs = plane_q(a.Cs, sv, nz, 4, plane);
+            const auto rpt = plane_q(POTTS ? a.in.ps : a.in.pt, sv, nz, IO::kBytes, plane);
+            const auto rcs = plane_q(a.Cs, POTTS ? -1 : sv, nz, 4, plane);
             const auto rct = plane_q(a.Ct, sv, nz, 4, plane);
 #pragma unroll
             for (int k = 0; k < KO; ++k) {
@@ -412,7 +418,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                 const int off = (x0 + j) * ny + (y0 + i);
                 o.u[k] = ldf(ru, off);
                 o.pt[k] = IO::load(rpt, off);
-                o.cs[k] = ldf(rcs, off);
+                if (!POTTS) o.cs[k] = ldf(rcs, off);
                 o.ct[k] = ldf(rct, off);
             }
         };
@@ -448,7 +454,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             const float *g0 = G[s & 1];
             const float *g1 = G[(s + 1) & 1];
             const int64_t base = (int64_t)s * plane;
-            const auto wu = plane_rsrc(a.out.u, base, 4, plane);
+            const auto wu = plane_rsrc(POTTS ? a.in.u : a.out.u, base, 4, plane);
             const auto wps = plane_rsrc(a.out.ps, base, IO::kBytes, plane);
             const auto wpt = plane_rsrc(a.out.pt, base, IO::kBytes, plane);
             const auto wbx = plane_rsrc(a.out.bx, base, IO::kBytes, plane);
@@ -484,23 +490,37 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     d = (d + bzr) - bzl;
                     carry[k] = bzr;
                 }
-                // source/sink flows and multiplier (asetsBinaryMF3D_mex.c:419-432)
                 const float uo = o.u[k];
                 const float uc = div_fast(uo, cc, rcc, fast_cc);
-                float f = ((o.pt[k] - uc) + d) + one_cc;
-                const float nps = fmin_ref(f, o.cs[k]);
-                f = (nps + uc) - d;
-                const float npt = fmin_ref(f, o.ct[k]);
-                const float e = cc * ((npt + d) - nps);
-                if (y < ny && x < nx) {
-                    e_acc += fabs((double)e);
-                    const int off = x * ny + y;
-                    Io<float>::store(wu, off, uo - e);
-                    IO::store(wps, off, nps);
-                    IO::store(wpt, off, npt);
-                    IO::store(wbx, off, bxl);
-                    IO::store(wby, off, byl);
-                    if (IS3D) IO::store(wbz, off, bzl);
+                if (POTTS) {
+                    // sink flow of this label (asetsPotts3D_mex.c:457-466), ft = dv + pt
+                    const float t = (o.pt[k] + uc) - d;   // o.pt holds ps here
+                    const float npt = fmin_ref(t, o.ct[k]);
+                    if (y < ny && x < nx) {
+                        const int off = x * ny + y;
+                        IO::store(wps, off, d + npt);
+                        IO::store(wpt, off, npt);
+                        IO::store(wbx, off, bxl);
+                        IO::store(wby, off, byl);
+                        if (IS3D) IO::store(wbz, off, bzl);
+                    }
+                } else {
+                    // source/sink flows and multiplier (asetsBinaryMF3D_mex.c:419-432)
+                    float f = ((o.pt[k] - uc) + d) + one_cc;
+                    const float nps = fmin_ref(f, o.cs[k]);
+                    f = (nps + uc) - d;
+                    const float npt = fmin_ref(f, o.ct[k]);
+                    const float e = cc * ((npt + d) - nps);
+                    if (y < ny && x < nx) {
+                        e_acc += fabs((double)e);
+                        const int off = x * ny + y;
+                        Io<float>::store(wu, off, uo - e);
+                        IO::store(wps, off, nps);
+                        IO::store(wpt, off, npt);
+                        IO::store(wbx, off, bxl);
+                        IO::store(wby, off, byl);
+                        if (IS3D) IO::store(wbz, off, bzl);
+                    }
                 }
             }
         };
@@ -555,6 +575,117 @@ __global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
     const double e = (lb < a.nblocks) ? fused_tile<IS3D, TX, NT, T>(a, lb) : 0.0;
     const double s = block_sum(e, red);
     if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
+}
+
+// ---- Potts model -----------------------------------------------------------
+
+struct PottsArgs {
+    const float *Ct, *alpha;   // [L*N]
+    const float *u;            // [L*N] (read)
+    const float *ps;           // [N]   label-shared source flow (read)
+    float *pt_in, *pt_out;     // [L*N]
+    float *b_in[3], *b_out[3]; // [L*N] faces x, y, z
+    float *ft;                 // [L*N] out: dv + pt per label
+    int ny, nx, nz, nlab;
+    float cc, steps;
+    int zchunk, ntile_y, ntile_x, nzc;
+    int nblocks;               // per label
+    int fast_cc;
+    const Flags *flags;
+};
+
+// All labels' flow updates of one iteration in one launch: workgroup b handles
+// tile b % nblocks of label b / nblocks (labels are independent inside an
+// iteration, asetsPotts3D_mex.c:172-191).
+template <bool IS3D, int TX, int NT, int WAVES>
+__global__ __launch_bounds__(NT, WAVES) void kf_potts_flow(PottsArgs p) {
+    if (p.flags->done) return;
+    const int total = p.nblocks * p.nlab;
+    const int gb = xcd_remap(blockIdx.x, gridDim.x);
+    if (gb >= total) return;
+    const int l = gb / p.nblocks, lb = gb % p.nblocks;
+    const int64_t off = (int64_t)l * p.ny * p.nx * p.nz;
+    FusedArgs<float> a;
+    a.Cs = nullptr;
+    a.Ct = p.Ct + off;
+    a.alpha = p.alpha + off;
+    a.in = FusedState<float>{(float *)p.u + off, (float *)p.ps, p.pt_in + off, p.b_in[0] + off,
+                             p.b_in[1] + off, p.b_in[2] + off};
+    a.out = FusedState<float>{nullptr, p.ft + off, p.pt_out + off, p.b_out[0] + off,
+                              p.b_out[1] + off, p.b_out[2] + off};
+    a.ny = p.ny;
+    a.nx = p.nx;
+    a.nz = p.nz;
+    a.plane = p.ny * p.nx;
+    a.cc = p.cc;
+    a.steps = p.steps;
+    a.zchunk = p.zchunk;
+    a.ntile_y = p.ntile_y;
+    a.ntile_x = p.ntile_x;
+    a.nzc = p.nzc;
+    a.nblocks = p.nblocks;
+    a.fast_cc = p.fast_cc;
+    a.partials = nullptr;
+    a.flags = nullptr;
+    (void)fused_tile<IS3D, TX, NT, float, true>(a, lb);
+}
+
+// Source flow and multipliers, labels coupled per voxel in label order
+// (asetsPotts3D_mex.c:495-514): s += ft_l - u_l/cc; ps = s/L + 1/(cc*L);
+// e_l = cc*(ft_l - ps); u_l -= e_l; sum |e_l| in FP64 per workgroup.
+// LT > 0: the label count is the compile-time LT and ft stays in registers;
+// LT == 0: any label count, ft is read again in the second pass.
+template <int LT>
+__global__ __launch_bounds__(256) void kp_potts_couple(const float *ft, float *u, float *ps,
+                                                       int64_t N, int nlab_rt, float cc,
+                                                       int fast_cc, double *partials,
+                                                       const Flags *flags) {
+    __shared__ double red[16];
+    if (flags->done) return;
+    const int nlab = LT > 0 ? LT : nlab_rt;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = g < N;
+    const int64_t gg = in ? g : 0;
+    const float rcc = rcp_refined(cc);
+    const float fl = (float)nlab;
+    const float tail = 1.0f / (cc * fl);
+    constexpr int R = LT > 0 ? LT : 1;
+    float ftr[R], ur[R];
+    float s = 0.0f;
+#pragma unroll
+    for (int l = 0; l < (LT > 0 ? LT : 1); ++l) {
+        if (LT > 0) {
+            ftr[l] = ft[gg + l * N];
+            ur[l] = u[gg + l * N];
+            s += (ftr[l] - div_fast(ur[l], cc, rcc, fast_cc != 0));
+        }
+    }
+    if (LT == 0)
+        for (int l = 0; l < nlab; ++l)
+            s += (ft[gg + l * N] - div_fast(u[gg + l * N], cc, rcc, fast_cc != 0));
+    const float nps = s / fl + tail;
+    double e_abs = 0.0;
+    if (LT > 0) {
+#pragma unroll
+        for (int l = 0; l < (LT > 0 ? LT : 1); ++l) {
+            const float e = cc * (ftr[l] - nps);
+            if (in) {
+                u[gg + l * N] = ur[l] - e;
+                e_abs += fabs((double)e);
+            }
+        }
+    } else {
+        for (int l = 0; l < nlab; ++l) {
+            const float e = cc * (ft[gg + l * N] - nps);
+            if (in) {
+                u[gg + l * N] = u[gg + l * N] - e;
+                e_abs += fabs((double)e);
+            }
+        }
+    }
+    if (in) ps[gg] = nps;
+    const double sum = block_sum(e_abs, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = sum;
 }
 
 }  // namespace asets

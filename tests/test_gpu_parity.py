@@ -136,24 +136,26 @@ def test_final_state_matches(cuda, kernel):
         assert _bits_equal(r.state[k].cpu().numpy(), ref.state[k]), k
 
 
+@pytest.mark.parametrize("kernel", ["fused", "passes"])
 @pytest.mark.parametrize("shape,nlab", [((1, 1, 1), 2), ((17, 13, 9), 3), ((33, 20, 11), 6),
-                                        ((70, 9, 5), 10)])
-def test_potts3d_bitexact(cuda, shape, nlab):
+                                        ((70, 9, 5), 10), ((65, 18, 12), 7)])
+def test_potts3d_bitexact(cuda, shape, nlab, kernel):
     Ct, a = workloads.random_smooth_potts(shape, nlab, seed=sum(shape) + nlab)
     ref = orc.potts(Ct, a, shape, nlab, 30, 0.0, 0.25, 0.11)
     d = [solver.to_device(x) for x in (Ct, a)]
-    r = solver.potts(*d, shape, nlab, 30, 0.0, 0.25, 0.11)
+    r = solver.potts(*d, shape, nlab, 30, 0.0, 0.25, 0.11, kernel=kernel)
     u = r.u.cpu().numpy()
     assert _bits_equal(u, ref.u), _report(u, ref.u)
     np.testing.assert_allclose(r.cvg.cpu().numpy(), ref.cvg64.astype(np.float32), rtol=1e-6)
 
 
+@pytest.mark.parametrize("kernel", ["fused", "passes"])
 @pytest.mark.parametrize("shape,nlab", [((17, 13), 3), ((64, 48), 4), ((100, 77), 10)])
-def test_potts2d_bitexact(cuda, shape, nlab):
+def test_potts2d_bitexact(cuda, shape, nlab, kernel):
     Ct, a = workloads.random_smooth_potts(shape, nlab, seed=sum(shape), alpha_step=0.05)
     ref = orc.potts(Ct, a, shape, nlab, 40, 0.0, 0.2, 0.16)
     d = [solver.to_device(x) for x in (Ct, a)]
-    r = solver.potts(*d, shape, nlab, 40, 0.0, 0.2, 0.16)
+    r = solver.potts(*d, shape, nlab, 40, 0.0, 0.2, 0.16, kernel=kernel)
     assert _bits_equal(r.u.cpu().numpy(), ref.u), _report(r.u.cpu().numpy(), ref.u)
 
 
@@ -333,3 +335,19 @@ def test_batched_vs_oracle(cuda):
     for (Cs, Ct, a, sh), r in zip(host, res):
         ref = orc.binary(Cs, Ct, a, sh, 25, 0.0, 0.35, 0.11)
         assert _bits_equal(r.u.cpu().numpy(), ref.u), sh
+
+
+def test_potts_voronoi_fused_vs_passes_and_stop(cuda):
+    """Potts workload generator at reduced size: fused == per-pass kernels bit for bit over
+    100 iterations, and the stop rule fires on the same iteration."""
+    shape, L = (96, 80, 40), 6
+    Ct, a, _ = workloads.potts3d_voronoi(shape, nlab=L, seed=1003)
+    d = [solver.to_device(x) for x in (Ct, a)]
+    rf = solver.potts(*d, shape, L, 100, 0.0, 0.25, 0.11, kernel="fused")
+    rp = solver.potts(*d, shape, L, 100, 0.0, 0.25, 0.11, kernel="passes")
+    assert torch.equal(rf.u.view(torch.int32), rp.u.view(torch.int32))
+    eb = float(rf.cvg[40].item())
+    sf = solver.potts(*d, shape, L, 100, eb, 0.25, 0.11, kernel="fused", poll_every=9)
+    sp = solver.potts(*d, shape, L, 100, eb, 0.25, 0.11, kernel="passes")
+    assert sf.it_num == sp.it_num <= 40
+    assert torch.equal(sf.u.view(torch.int32), sp.u.view(torch.int32))
