@@ -2,6 +2,7 @@
 
 Products (next to this file):
   libasets_hip.so            C-ABI HIP library (include/asets.h), gfx950 code objects
+  libasets_nn.so             warm-start network kernels (include/asets_nn.h)
   mex/libmexshim.so          project-written minimal mx/mex API (tests, Python driver)
   mex/asets*_mex.so          the four mex-ABI host files, linked against the two above
 
@@ -60,6 +61,15 @@ def build_hip_library() -> Path:
     return out
 
 
+def build_nn_library() -> Path:
+    out = PKG / "libasets_nn.so"
+    tmp = _atomic_out(out)
+    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}",
+          "-o", str(tmp), str(CSRC / "nn_kernels.hip")])
+    os.replace(tmp, out)
+    return out
+
+
 def build_mex() -> list[Path]:
     MEX_OUT.mkdir(exist_ok=True)
     shim_dir = CSRC / "mex" / "shim"
@@ -88,7 +98,7 @@ def build_oracle() -> Path:
 
 
 def build_all(oracle: bool = True) -> list[Path]:
-    outs = [build_hip_library()]
+    outs = [build_hip_library(), build_nn_library()]
     outs += build_mex()
     if oracle:
         outs.append(build_oracle())

@@ -26,6 +26,7 @@
 #include "asets.h"
 #include "common.h"
 #include "kernels_batched.h"
+#include "kernels_energy.h"
 #include "kernels_fused.h"
 #include "kernels_passes.h"
 
@@ -1149,6 +1150,47 @@ asets_status potts_host(int ndim, const float *Ct, const float *alpha, int64_t n
     return ASETS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// FP64 energy of a labelling (binary: Cs != nullptr, nlab == 1; Potts: Cs null)
+// ---------------------------------------------------------------------------
+asets_status energy_dev(int ndim, const float *Cs, const float *Ct, const float *alpha,
+                        int64_t ny, int64_t nx, int64_t nz, int64_t nlab, const float *u,
+                        double *out, const asets_opts *o) {
+    if (ndim != 2 && ndim != 3) return fail(ASETS_EBADARG, "ndim must be 2 or 3");
+    if (ndim == 2 && nz != 1) return fail(ASETS_EBADARG, "2D energy needs nz == 1");
+    asets_status st = check_dims(ny, nx, nz, nlab);
+    if (st) return st;
+    if (!Ct || !alpha || !u || !out) return fail(ASETS_EBADARG, "null array argument");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(ASETS_EDEVICE, "device id %d", dev);
+    DeviceCache &dc = g_cache[dev];
+    std::lock_guard<std::mutex> lock(dc.mu);
+    hipStream_t s = o ? (hipStream_t)o->stream : nullptr;
+    if (!dc.host_flag) HIPCHK(hipHostMalloc((void **)&dc.host_flag, 64));
+    const size_t need = Carver::sz(2 * kEnergyBlocks, 8) + Carver::sz(3, 8);
+    st = ensure(&dc.dev, &dc.dev_bytes, need);
+    if (st) return st;
+    Carver cv(dc.dev);
+    double *partials = cv.take<double>(2 * kEnergyBlocks);
+    double *res = cv.take<double>(3);
+    if (ndim == 3)
+        k_energy<true><<<kEnergyBlocks, kEnergyThreads, 0, s>>>(Cs, Ct, alpha, u, (int)ny, (int)nx,
+                                                                (int)nz, (int)nlab, partials);
+    else
+        k_energy<false><<<kEnergyBlocks, kEnergyThreads, 0, s>>>(Cs, Ct, alpha, u, (int)ny,
+                                                                 (int)nx, 1, (int)nlab, partials);
+    k_energy_finalize<<<1, 64, 0, s>>>(partials, kEnergyBlocks, res);
+    HIPCHK(hipGetLastError());
+    double *h = (double *)((char *)dc.host_flag + 32);   // 3 doubles of the pinned slot
+    HIPCHK(hipMemcpyAsync(h, res, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    out[0] = h[0];
+    out[1] = h[1];
+    out[2] = h[2];
+    return ASETS_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1223,6 +1265,19 @@ asets_status asets_binary3d_batched_dev(int32_t nvol, const asets_volume *vols,
     if (flow16 == ASETS_FLOW_F16) return batched_dev<__half>(nvol, vols, p, seconds, o);
     if (flow16 != ASETS_FLOW_F32) return fail(ASETS_EBADARG, "unknown flow_dtype %d", flow16);
     return batched_dev<float>(nvol, vols, p, seconds, o);
+}
+
+asets_status asets_binary_energy_dev(int ndim, const float *Cs, const float *Ct,
+                                     const float *alpha, int64_t ny, int64_t nx, int64_t nz,
+                                     const float *u, double *energy, const asets_opts *o) {
+    if (!Cs) return fail(ASETS_EBADARG, "null array argument");
+    return energy_dev(ndim, Cs, Ct, alpha, ny, nx, nz, 1, u, energy, o);
+}
+
+asets_status asets_potts_energy_dev(int ndim, const float *Ct, const float *alpha, int64_t ny,
+                                    int64_t nx, int64_t nz, int64_t nlab, const float *u,
+                                    double *energy, const asets_opts *o) {
+    return energy_dev(ndim, nullptr, Ct, alpha, ny, nx, nz, nlab, u, energy, o);
 }
 
 asets_status asets_selftest_division(int mode, float d, uint64_t *mismatches) {

@@ -1,0 +1,656 @@
+// nn_kernels.hip -- libasets_nn.so: kernels of the warm-start network, a small
+// 3D U-Net that maps the cost volumes (Cs, Ct, alpha) of a binary max-flow
+// problem to a starting point (u0, pt0) for the iteration.  No reference
+// counterpart: the reference always starts from the initial cut of
+// asetsBinaryMF3D_mex.c:205-224.
+//
+// Tensors are NDHWC (channels fastest) in bf16, accumulation in fp32.  A batch
+// of column-major volumes maps onto NDHWC with D = Nz, H = Nx, W = Ny: voxel
+// p = n*N + g has the same flat index as in the solver's layout, so no
+// transposition is needed.  3x3x3 convolutions (stride 1, zero padding 1) run
+// as implicit GEMMs on the gfx950 matrix cores (v_mfma_f32_16x16x32_bf16):
+// M = voxels, N = output channels, K = 27 taps x input channels.  The input
+// gradient is the same convolution with the taps mirrored and the weight
+// matrix transposed; the weight gradient reduces over fixed voxel chunks into
+// partials that one kernel adds in order (deterministic).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "asets_nn.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+thread_local std::string g_nn_err;
+
+asets_status nn_fail(asets_status s, const char *fmt, ...) {
+    char buf[256];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_nn_err = buf;
+    return s;
+}
+
+#define NNCHK()                                                                  \
+    do {                                                                         \
+        hipError_t e_ = hipGetLastError();                                       \
+        if (e_ != hipSuccess)                                                    \
+            return nn_fail(ASETS_EDEVICE, "launch: %s", hipGetErrorString(e_));  \
+    } while (0)
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+constexpr int64_t kMaxElems = (int64_t)1 << 31;   // 32-bit element indexing
+
+__device__ __forceinline__ float bf2f(u16 b) { return __uint_as_float((uint32_t)b << 16); }
+// fp32 -> bf16, round to nearest even; NaN stays NaN
+__device__ __forceinline__ u16 f2bf(float f) {
+    uint32_t x = __float_as_uint(f);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return (u16)((x >> 16) | 0x40);
+    x += 0x7fffu + ((x >> 16) & 1u);
+    return (u16)(x >> 16);
+}
+
+struct Geo5 {
+    int n, d, h, w;   // batch, depth (z), height (x), width (y)
+};
+
+// ---------------------------------------------------------------------------
+// 3x3x3 convolution: out[p, co] = act(b[co] + sum_{k, ci} in[p + o(k), ci] *
+// w[k, co, ci]), weights [27][COUT][cin] (ci contiguous), tap k = (dz+1)*9 +
+// (dy+1)*3 + (dx+1) with (dz, dy, dx) the offsets along (d, h, w).  A 256-thread
+// workgroup covers 256 consecutive voxels; each wave 64 voxels (4 MFMA row
+// tiles) x all COUT (COUT/16 column tiles).  MFMA 16x16x32 operand layout:
+// lane l supplies row l%16 and the 8 consecutive K indices 8*(l/16)..+7 of A
+// (and column l%16 of B); it receives rows 4*(l/16)+i, i < 4, of column l%16.
+// cin is a multiple of 8, so the 8 K indices of a lane share one tap and are
+// one 16-byte load.
+// ---------------------------------------------------------------------------
+template <int COUT>
+__global__ __launch_bounds__(256) void k_conv3_fwd(const u16 *__restrict__ in, Geo5 g, int cin,
+                                                   const u16 *__restrict__ w,
+                                                   const float *__restrict__ bias, int relu,
+                                                   u16 *__restrict__ out) {
+    constexpr int MT = 4, NT = COUT / 16;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int P = g.n * g.d * g.h * g.w;
+    const int p0 = ((int)blockIdx.x * 4 + wv) * 64;
+    const int r = lane & 15, kg = lane >> 4;
+    int pz[MT], py[MT], px[MT], pb[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+        int p = p0 + m * 16 + r;
+        const bool v = p < P;
+        if (!v) p = 0;
+        px[m] = p % g.w;
+        int t = p / g.w;
+        py[m] = t % g.h;
+        t /= g.h;
+        const int z = t % g.d;
+        pb[m] = p - (z * g.h + py[m]) * g.w - px[m];   // voxel index of (n, 0, 0, 0)
+        pz[m] = v ? z : -1000;   // an invalid row never passes the bounds test
+    }
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int ktot = 27 * cin;
+    for (int k0 = 0; k0 < ktot; k0 += 32) {
+        const int kk = k0 + 8 * kg;
+        const int tap = kk / cin, ci = kk - tap * cin;
+        const bool tv = tap < 27;
+        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+        bf16x8 a[MT], b[NT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int z = pz[m] + dz, y = py[m] + dy, x = px[m] + dx;
+            const bool ok = tv && (unsigned)z < (unsigned)g.d && (unsigned)y < (unsigned)g.h &&
+                            (unsigned)x < (unsigned)g.w;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (ok) {
+                const int q = pb[m] + (z * g.h + y) * g.w + x;
+                v = *(const uint4 *)(in + (size_t)q * cin + ci);
+            }
+            a[m] = __builtin_bit_cast(bf16x8, v);
+        }
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (tv) v = *(const uint4 *)(w + ((size_t)tap * COUT + n * 16 + r) * cin + ci);
+            b[n] = __builtin_bit_cast(bf16x8, v);
+        }
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int n = 0; n < NT; ++n)
+                acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[n], acc[m][n], 0, 0, 0);
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int p = p0 + m * 16 + 4 * kg + i;
+            if (p < P)
+#pragma unroll
+                for (int n = 0; n < NT; ++n) {
+                    const int co = n * 16 + r;
+                    float v = acc[m][n][i] + (bias ? bias[co] : 0.0f);
+                    if (relu) v = v > 0.0f ? v : 0.0f;
+                    out[(size_t)p * COUT + co] = f2bf(v);
+                }
+        }
+}
+
+// [27][cout][cin] -> [27][cin][cout] with the taps mirrored (k -> 26-k): the
+// weights of the input-gradient convolution.
+__global__ void k_flip_transpose(const u16 *w, u16 *wt, int cout, int cin) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 27 * cout * cin) return;
+    const int ci = i % cin;
+    const int t = i / cin;
+    const int co = t % cout;
+    const int k = t / cout;
+    wt[((26 - k) * cin + ci) * cout + co] = w[i];
+}
+
+// g *= (y > 0): gradient through a ReLU whose output is y
+__global__ void k_relu_mask(u16 *g, const u16 *y, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !(bf2f(y[i]) > 0.0f)) g[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// weight gradient: dw[k, co, ci] = sum_p g[p, co] * in[p + o(k), ci].  Workgroup
+// = (tap k, 16 x CIT (co, ci) tile, chunk of kWgChunk voxels), one (co, ci) per
+// thread (CIT = 16, or 8 for an 8-channel input); 64 voxels at a time are staged
+// in LDS (their shifted source voxel computed once per voxel).  Each chunk
+// writes its own partial.
+// ---------------------------------------------------------------------------
+constexpr int kWgChunk = 4096;
+
+template <int CIT>
+__global__ __launch_bounds__(16 * CIT) void k_conv3_wgrad(const u16 *__restrict__ in, Geo5 g,
+                                                          int cin, const u16 *__restrict__ go,
+                                                          int cout, float *__restrict__ part) {
+    constexpr int NTH = 16 * CIT;
+    __shared__ float sg[64][17], sx[64][CIT + 1];
+    __shared__ int src[64];
+    const int tap = blockIdx.x;
+    const int nci = cin / CIT;
+    const int cot = blockIdx.y / nci, cit = blockIdx.y % nci;
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const int P = g.n * g.d * g.h * g.w;
+    const int pb = blockIdx.z * kWgChunk;
+    const int pe = min(P, pb + kWgChunk);
+    const int tco = threadIdx.x / CIT, tci = threadIdx.x % CIT;
+    float acc = 0.0f;
+    for (int s = pb; s < pe; s += 64) {
+        if (threadIdx.x < 64) {
+            const int p = s + threadIdx.x;
+            int q = -1;
+            if (p < pe) {
+                const int x = p % g.w;
+                int t = p / g.w;
+                const int y = t % g.h;
+                t /= g.h;
+                const int z = t % g.d;
+                const int zz = z + dz, yy = y + dy, xx = x + dx;
+                if ((unsigned)zz < (unsigned)g.d && (unsigned)yy < (unsigned)g.h &&
+                    (unsigned)xx < (unsigned)g.w)
+                    q = p + (dz * g.h + dy) * g.w + dx;
+            }
+            src[threadIdx.x] = q;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 16; e += NTH) {
+            const int pi = e >> 4, c = e & 15;
+            const int p = s + pi;
+            sg[pi][c] = p < pe ? bf2f(go[(size_t)p * cout + cot * 16 + c]) : 0.0f;
+        }
+        for (int e = threadIdx.x; e < 64 * CIT; e += NTH) {
+            const int pi = e / CIT, c = e % CIT;
+            const int q = src[pi];
+            sx[pi][c] = q >= 0 ? bf2f(in[(size_t)q * cin + cit * CIT + c]) : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll 16
+        for (int pi = 0; pi < 64; ++pi) acc = __builtin_fmaf(sg[pi][tco], sx[pi][tci], acc);
+        __syncthreads();
+    }
+    const int co = cot * 16 + tco, ci = cit * CIT + tci;
+    part[(((size_t)blockIdx.z * 27 + tap) * cout + co) * cin + ci] = acc;
+}
+
+// out[i] (+)= sum_c part[c*n + i], chunks added in order
+__global__ void k_sum_chunks(const float *part, int nchunk, int n, float *out, int accumulate) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int c = 0; c < nchunk; ++c) s += part[(size_t)c * n + i];
+    out[i] = accumulate ? out[i] + s : s;
+}
+
+// per-chunk partial sums of g over the voxels: the bias gradient
+__global__ void k_bias_part(const u16 *go, int P, int cout, float *part) {
+    const int co = threadIdx.x;
+    const int pb = blockIdx.x * kWgChunk, pe = min(P, pb + kWgChunk);
+    if (co >= cout) return;
+    float s = 0.0f;
+    for (int p = pb; p < pe; ++p) s += bf2f(go[(size_t)p * cout + co]);
+    part[(size_t)blockIdx.x * cout + co] = s;
+}
+
+// ---------------------------------------------------------------------------
+// resampling, channel copies, features, head, optimiser
+// ---------------------------------------------------------------------------
+// 2x2x2 average pool, g the fine geometry (even extents)
+__global__ void k_pool2_fwd(const u16 *in, Geo5 g, int c, u16 *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int D = g.d / 2, H = g.h / 2, W = g.w / 2;
+    if (i >= g.n * D * H * W * c) return;
+    const int ch = i % c;
+    int t = i / c;
+    const int x = t % W;
+    t /= W;
+    const int y = t % H;
+    t /= H;
+    const int z = t % D;
+    const int n = t / D;
+    float s = 0.0f;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int e = 0; e < 2; ++e)
+                s += bf2f(in[((((size_t)n * g.d + 2 * z + a) * g.h + 2 * y + b) * g.w + 2 * x + e) * c + ch]);
+    out[i] = f2bf(s * 0.125f);
+}
+
+// pool gradient: each fine voxel gets 1/8 of its coarse gradient (added to gin
+// when accumulate != 0)
+__global__ void k_pool2_bwd(const u16 *gout, Geo5 g, int c, u16 *gin, int accumulate) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n * g.d * g.h * g.w * c) return;
+    const int ch = i % c;
+    int t = i / c;
+    const int x = t % g.w;
+    t /= g.w;
+    const int y = t % g.h;
+    t /= g.h;
+    const int z = t % g.d;
+    const int n = t / g.d;
+    const int D = g.d / 2, H = g.h / 2, W = g.w / 2;
+    float v = bf2f(gout[((((size_t)n * D + z / 2) * H + y / 2) * W + x / 2) * c + ch]) * 0.125f;
+    if (accumulate) v += bf2f(gin[i]);
+    gin[i] = f2bf(v);
+}
+
+// nearest-neighbour 2x upsample into channels [off, off+c) of out (cd channels);
+// g the fine geometry
+__global__ void k_up2_fwd(const u16 *in, Geo5 g, int c, u16 *out, int cd, int off) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n * g.d * g.h * g.w * c) return;
+    const int ch = i % c;
+    const int p = i / c;
+    int t = p;
+    const int x = t % g.w;
+    t /= g.w;
+    const int y = t % g.h;
+    t /= g.h;
+    const int z = t % g.d;
+    const int n = t / g.d;
+    const int D = g.d / 2, H = g.h / 2, W = g.w / 2;
+    out[(size_t)p * cd + off + ch] = in[((((size_t)n * D + z / 2) * H + y / 2) * W + x / 2) * c + ch];
+}
+
+// upsample gradient: sum of the 8 fine gradients read from channels [off,
+// off+c) of gout (cd channels); g the fine geometry
+__global__ void k_up2_bwd(const u16 *gout, int cd, int off, Geo5 g, int c, u16 *gin) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int D = g.d / 2, H = g.h / 2, W = g.w / 2;
+    if (i >= g.n * D * H * W * c) return;
+    const int ch = i % c;
+    int t = i / c;
+    const int x = t % W;
+    t /= W;
+    const int y = t % H;
+    t /= H;
+    const int z = t % D;
+    const int n = t / D;
+    float s = 0.0f;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int e = 0; e < 2; ++e)
+                s += bf2f(gout[((((size_t)n * g.d + 2 * z + a) * g.h + 2 * y + b) * g.w + 2 * x + e) * cd + off + ch]);
+    gin[i] = f2bf(s);
+}
+
+// channel block copy: dst[p, doff + c] = src[p, soff + c] for c < nc (or +=)
+__global__ void k_chan_copy(u16 *dst, int cd, int doff, const u16 *src, int cs, int soff, int nc,
+                            int nvox, int accumulate) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nvox * nc) return;
+    const int p = i / nc, c = i % nc;
+    const size_t di = (size_t)p * cd + doff + c;
+    float v = bf2f(src[(size_t)p * cs + soff + c]);
+    if (accumulate) v += bf2f(dst[di]);
+    dst[di] = f2bf(v);
+}
+
+// input features, 8 channels: Cs/s, Ct/s, alpha, (Ct-Cs)/s, min(Cs,Ct)/s, 0, 0, 0
+// with s = kCostScale
+constexpr float kCostScale = 10.0f;
+
+__global__ void k_features(const float *Cs, const float *Ct, const float *alpha, int P, u16 *out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const float s = Cs[p], t = Ct[p], a = alpha[p];
+    const float k = 1.0f / kCostScale;
+    uint4 v;
+    v.x = (uint32_t)f2bf(s * k) | ((uint32_t)f2bf(t * k) << 16);
+    v.y = (uint32_t)f2bf(a) | ((uint32_t)f2bf((t - s) * k) << 16);
+    v.z = (uint32_t)f2bf((s < t ? s : t) * k);
+    v.w = 0;
+    *(uint4 *)(out + (size_t)p * 8) = v;
+}
+
+// head: 1x1x1 convolution of the 16-channel x to (a, b);
+//   u0 = a,  pt0 = min(Cs, Ct) + kCostScale * b
+// w = [w_u(16), w_pt(16)], bias = [b_u, b_pt]
+__global__ void k_head_fwd(const u16 *x, int P, const float *w, const float *bias,
+                           const float *Cs, const float *Ct, float *u0, float *pt0) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float a = bias[0], b = bias[1];
+    for (int c = 0; c < 16; ++c) {
+        const float v = bf2f(x[(size_t)p * 16 + c]);
+        a += w[c] * v;
+        b += w[16 + c] * v;
+    }
+    const float cs = Cs[p], ct = Ct[p];
+    u0[p] = a;
+    pt0[p] = (cs < ct ? cs : ct) + kCostScale * b;
+}
+
+// loss = mean_p (u0 - tu)^2 + mean_p ((pt0 - tpt) / kCostScale)^2.
+// Writes dx (bf16) and per-block partials [35] = dw(32), db(2), loss sum.
+__global__ __launch_bounds__(256) void k_head_bwd(const u16 *x, int P, const float *w,
+                                                  const float *u0, const float *pt0,
+                                                  const float *tu, const float *tpt, u16 *dx,
+                                                  float *part) {
+    __shared__ float red[35][257];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    float gu = 0.0f, gp = 0.0f, l = 0.0f;
+    float xv[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) xv[c] = 0.0f;
+    if (p < P) {
+        const float eu = u0[p] - tu[p];
+        const float ep = (pt0[p] - tpt[p]) * (1.0f / kCostScale);
+        l = eu * eu + ep * ep;
+        const float inv = 2.0f / (float)P;
+        gu = eu * inv;   // d loss / d a
+        gp = ep * inv;   // d loss / d b  (pt0 = base + s*b, residual divided by s)
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            xv[c] = bf2f(x[(size_t)p * 16 + c]);
+            dx[(size_t)p * 16 + c] = f2bf(gu * w[c] + gp * w[16 + c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        red[c][threadIdx.x] = gu * xv[c];
+        red[16 + c][threadIdx.x] = gp * xv[c];
+    }
+    red[32][threadIdx.x] = gu;
+    red[33][threadIdx.x] = gp;
+    red[34][threadIdx.x] = l;
+    __syncthreads();
+    if (threadIdx.x < 35) {
+        float s = 0.0f;
+        for (int t = 0; t < 256; ++t) s += red[threadIdx.x][t];
+        part[(size_t)blockIdx.x * 35 + threadIdx.x] = s;
+    }
+}
+
+// AdamW on fp32 master weights; refreshes the bf16 copy when wb != null
+__global__ void k_adamw(float *w, u16 *wb, const float *g, float *m, float *v, int n, float lr,
+                        float b1, float b2, float eps, float wd, float bc1, float bc2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / bc1, vh = vi / bc2;
+    const float wi = w[i] * (1.0f - lr * wd) - lr * mh / (sqrtf(vh) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = f2bf(wi);
+}
+
+__global__ void k_f32_to_bf16(const float *a, u16 *b, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = f2bf(a[i]);
+}
+
+constexpr int kB = 256;
+
+inline unsigned nblk(int64_t n) { return (unsigned)cdiv(n, kB); }
+
+asets_status check_geo(int n, int d, int h, int w, int c) {
+    if (n <= 0 || d <= 0 || h <= 0 || w <= 0 || c <= 0)
+        return nn_fail(ASETS_EBADARG, "sizes must be positive");
+    if ((int64_t)n * d * h * w * c >= kMaxElems)
+        return nn_fail(ASETS_ETOOBIG, "tensor of more than 2^31 elements");
+    return ASETS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *asets_nn_last_error(void) { return g_nn_err.c_str(); }
+
+asets_status asets_nn_conv3(const void *in, int n, int d, int h, int w, int cin, const void *wt,
+                            const float *bias, int cout, int relu, void *out, void *stream) {
+    if (cin % 8 || cin > 512) return nn_fail(ASETS_EBADARG, "cin must be a multiple of 8, <= 512");
+    asets_status st = check_geo(n, d, h, w, cin > cout ? cin : cout);
+    if (st) return st;
+    if (!in || !wt || !out) return nn_fail(ASETS_EBADARG, "null pointer");
+    const Geo5 g{n, d, h, w};
+    const unsigned grid = (unsigned)cdiv((int64_t)n * d * h * w, 256);
+    hipStream_t s = (hipStream_t)stream;
+    const u16 *x = (const u16 *)in, *wb = (const u16 *)wt;
+    u16 *y = (u16 *)out;
+    switch (cout) {
+    case 16: k_conv3_fwd<16><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y); break;
+    case 32: k_conv3_fwd<32><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y); break;
+    case 48: k_conv3_fwd<48><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y); break;
+    case 64: k_conv3_fwd<64><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y); break;
+    case 96: k_conv3_fwd<96><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y); break;
+    case 128: k_conv3_fwd<128><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y); break;
+    default: return nn_fail(ASETS_EBADARG, "cout must be 16, 32, 48, 64, 96 or 128");
+    }
+    NNCHK();
+    return ASETS_OK;
+}
+
+asets_status asets_nn_flip_transpose(const void *wt, void *wt_bwd, int cout, int cin,
+                                     void *stream) {
+    if (cout <= 0 || cin <= 0) return nn_fail(ASETS_EBADARG, "sizes must be positive");
+    k_flip_transpose<<<nblk(27LL * cout * cin), kB, 0, (hipStream_t)stream>>>(
+        (const u16 *)wt, (u16 *)wt_bwd, cout, cin);
+    NNCHK();
+    return ASETS_OK;
+}
+
+asets_status asets_nn_relu_mask(void *g, const void *y, int64_t n, void *stream) {
+    if (n < 0 || n >= kMaxElems) return nn_fail(ASETS_EBADARG, "bad element count");
+    if (n == 0) return ASETS_OK;
+    k_relu_mask<<<nblk(n), kB, 0, (hipStream_t)stream>>>((u16 *)g, (const u16 *)y, (int)n);
+    NNCHK();
+    return ASETS_OK;
+}
+
+size_t asets_nn_wgrad_scratch_bytes(int n, int d, int h, int w, int cin, int cout) {
+    const int64_t P = (int64_t)n * d * h * w;
+    const int64_t nch = cdiv(P, kWgChunk);
+    return (size_t)(nch * 27LL * cout * cin + nch * cout) * sizeof(float);
+}
+
+asets_status asets_nn_conv3_wgrad(const void *in, int n, int d, int h, int w, int cin,
+                                  const void *gout, int cout, float *dw, float *dbias,
+                                  int accumulate, void *scratch, size_t scratch_bytes,
+                                  void *stream) {
+    if (cin % 8 || cout % 16 || (cin > 8 && cin % 16))
+        return nn_fail(ASETS_EBADARG, "cout must be a multiple of 16, cin 8 or a multiple of 16");
+    asets_status st = check_geo(n, d, h, w, cin > cout ? cin : cout);
+    if (st) return st;
+    if (!scratch || scratch_bytes < asets_nn_wgrad_scratch_bytes(n, d, h, w, cin, cout))
+        return nn_fail(ASETS_EBADARG, "scratch too small");
+    const Geo5 g{n, d, h, w};
+    const int P = n * d * h * w;
+    const int nch = (int)cdiv(P, kWgChunk);
+    hipStream_t s = (hipStream_t)stream;
+    float *part = (float *)scratch;
+    if (cin == 8) {
+        dim3 grid(27, cout / 16, nch);
+        k_conv3_wgrad<8><<<grid, 128, 0, s>>>((const u16 *)in, g, cin, (const u16 *)gout, cout,
+                                              part);
+    } else {
+        dim3 grid(27, (cout / 16) * (cin / 16), nch);
+        k_conv3_wgrad<16><<<grid, 256, 0, s>>>((const u16 *)in, g, cin, (const u16 *)gout, cout,
+                                               part);
+    }
+    const int nw = 27 * cout * cin;
+    k_sum_chunks<<<nblk(nw), kB, 0, s>>>(part, nch, nw, dw, accumulate);
+    if (dbias) {
+        float *bp = part + (size_t)nch * nw;
+        k_bias_part<<<nch, 128, 0, s>>>((const u16 *)gout, P, cout, bp);
+        k_sum_chunks<<<1, 128, 0, s>>>(bp, nch, cout, dbias, accumulate);
+    }
+    NNCHK();
+    return ASETS_OK;
+}
+
+asets_status asets_nn_pool2(const void *in, int n, int d, int h, int w, int c, void *out,
+                            void *stream) {
+    asets_status st = check_geo(n, d, h, w, c);
+    if (st) return st;
+    if (d % 2 || h % 2 || w % 2) return nn_fail(ASETS_EBADARG, "pool needs even extents");
+    k_pool2_fwd<<<nblk((int64_t)n * d * h * w * c / 8), kB, 0, (hipStream_t)stream>>>(
+        (const u16 *)in, Geo5{n, d, h, w}, c, (u16 *)out);
+    NNCHK();
+    return ASETS_OK;
+}
+
+asets_status asets_nn_pool2_bwd(const void *gout, int n, int d, int h, int w, int c, void *gin,
+                                int accumulate, void *stream) {
+    asets_status st = check_geo(n, d, h, w, c);
+    if (st) return st;
+    if (d % 2 || h % 2 || w % 2) return nn_fail(ASETS_EBADARG, "pool needs even extents");
+    k_pool2_bwd<<<nblk((int64_t)n * d * h * w * c), kB, 0, (hipStream_t)stream>>>(
+        (const u16 *)gout, Geo5{n, d, h, w}, c, (u16 *)gin, accumulate);
+    NNCHK();
+    return ASETS_OK;
+}
+
+asets_status asets_nn_up2(const void *in, int n, int d, int h, int w, int c, void *out, int cd,
+                          int off, void *stream) {
+    asets_status st = check_geo(n, d, h, w, cd);
+    if (st) return st;
+    if (d % 2 || h % 2 || w % 2) return nn_fail(ASETS_EBADARG, "upsample needs even extents");
+    if (off < 0 || off + c > cd) return nn_fail(ASETS_EBADARG, "channel range out of bounds");
+    k_up2_fwd<<<nblk((int64_t)n * d * h * w * c), kB, 0, (hipStream_t)stream>>>(
+        (const u16 *)in, Geo5{n, d, h, w}, c, (u16 *)out, cd, off);
+    NNCHK();
+    return ASETS_OK;
+}
+
+asets_status asets_nn_up2_bwd(const void *gout, int cd, int off, int n, int d, int h, int w,
+                              int c, void *gin, void *stream) {
+    asets_status st = check_geo(n, d, h, w, cd);
+    if (st) return st;
+    if (d % 2 || h % 2 || w % 2) return nn_fail(ASETS_EBADARG, "upsample needs even extents");
+    if (off < 0 || off + c > cd) return nn_fail(ASETS_EBADARG, "channel range out of bounds");
+    k_up2_bwd<<<nblk((int64_t)n * d * h * w * c / 8), kB, 0, (hipStream_t)stream>>>(
+        (const u16 *)gout, cd, off, Geo5{n, d, h, w}, c, (u16 *)gin);
+    NNCHK();
+    return ASETS_OK;
+}
+
+asets_status asets_nn_chan_copy(void *dst, int cd, int doff, const void *src, int cs, int soff,
+                                int nc, int64_t nvox, int accumulate, void *stream) {
+    if (nvox <= 0 || nc <= 0 || doff < 0 || soff < 0 || doff + nc > cd || soff + nc > cs)
+        return nn_fail(ASETS_EBADARG, "channel range out of bounds");
+    if (nvox * (cd > cs ? cd : cs) >= kMaxElems) return nn_fail(ASETS_ETOOBIG, "too large");
+    k_chan_copy<<<nblk(nvox * nc), kB, 0, (hipStream_t)stream>>>(
+        (u16 *)dst, cd, doff, (const u16 *)src, cs, soff, nc, (int)nvox, accumulate);
+    NNCHK();
+    return ASETS_OK;
+}
+
+asets_status asets_nn_features(const float *Cs, const float *Ct, const float *alpha, int64_t P,
+                               void *out, void *stream) {
+    if (P <= 0 || P * 8 >= kMaxElems) return nn_fail(ASETS_EBADARG, "bad voxel count");
+    k_features<<<nblk(P), kB, 0, (hipStream_t)stream>>>(Cs, Ct, alpha, (int)P, (u16 *)out);
+    NNCHK();
+    return ASETS_OK;
+}
+
+asets_status asets_nn_head(const void *x, int64_t P, const float *w, const float *b,
+                           const float *Cs, const float *Ct, float *u0, float *pt0,
+                           void *stream) {
+    if (P <= 0 || P * 16 >= kMaxElems) return nn_fail(ASETS_EBADARG, "bad voxel count");
+    k_head_fwd<<<nblk(P), kB, 0, (hipStream_t)stream>>>((const u16 *)x, (int)P, w, b, Cs, Ct,
+                                                        u0, pt0);
+    NNCHK();
+    return ASETS_OK;
+}
+
+size_t asets_nn_head_scratch_bytes(int64_t P) { return (size_t)cdiv(P, kB) * 35 * sizeof(float); }
+
+asets_status asets_nn_head_bwd(const void *x, int64_t P, const float *w, const float *u0,
+                               const float *pt0, const float *tu, const float *tpt, void *dx,
+                               float *grads, void *scratch, size_t scratch_bytes,
+                               void *stream) {
+    if (P <= 0 || P * 16 >= kMaxElems) return nn_fail(ASETS_EBADARG, "bad voxel count");
+    if (!scratch || scratch_bytes < asets_nn_head_scratch_bytes(P))
+        return nn_fail(ASETS_EBADARG, "scratch too small");
+    hipStream_t s = (hipStream_t)stream;
+    const int nb = (int)cdiv(P, kB);
+    float *part = (float *)scratch;
+    k_head_bwd<<<nb, kB, 0, s>>>((const u16 *)x, (int)P, w, u0, pt0, tu, tpt, (u16 *)dx, part);
+    k_sum_chunks<<<1, 64, 0, s>>>(part, nb, 35, grads, 0);
+    NNCHK();
+    return ASETS_OK;
+}
+
+asets_status asets_nn_adamw(float *w, void *wb, const float *g, float *m, float *v, int64_t n,
+                            float lr, float b1, float b2, float eps, float wd, int step,
+                            void *stream) {
+    if (n <= 0 || n >= kMaxElems || step < 1) return nn_fail(ASETS_EBADARG, "bad arguments");
+    const float bc1 = 1.0f - std::pow(b1, (float)step), bc2 = 1.0f - std::pow(b2, (float)step);
+    k_adamw<<<nblk(n), kB, 0, (hipStream_t)stream>>>(w, (u16 *)wb, g, m, v, (int)n, lr, b1, b2,
+                                                     eps, wd, bc1, bc2);
+    NNCHK();
+    return ASETS_OK;
+}
+
+asets_status asets_nn_to_bf16(const float *a, void *b, int64_t n, void *stream) {
+    if (n <= 0 || n >= kMaxElems) return nn_fail(ASETS_EBADARG, "bad element count");
+    k_f32_to_bf16<<<nblk(n), kB, 0, (hipStream_t)stream>>>(a, (u16 *)b, (int)n);
+    NNCHK();
+    return ASETS_OK;
+}
+
+}  // extern "C"

@@ -178,3 +178,33 @@ def binary_batched(volumes, max_iter: int, err_bound: float, cc: float, steps: f
         r._executed = it + 1 if it < max_iter else max_iter
         out.append(r)
     return out, secs.value, (kms.value if kms is not None else None)
+
+
+def _ndim_dims(shape):
+    if len(shape) == 2:
+        return 2, shape[0], shape[1], 1
+    if len(shape) == 3:
+        return 3, shape[0], shape[1], shape[2]
+    raise ValueError("shape must be (Ny, Nx) or (Ny, Nx, Nz)")
+
+
+def energy_binary(Cs, Ct, alpha, u, shape, *, stream=None) -> np.ndarray:
+    """FP64 energy {E, data, regularisation} of the binary labelling u (device tensors)."""
+    ndim, ny, nx, nz = _ndim_dims(shape)
+    out = np.zeros(3)
+    opts = nat.make_opts(_stream_handle(stream), nat.FLOW_F32, nat.KERNEL_AUTO, 0, 0, 0, None)
+    nat.check(nat.lib().asets_binary_energy_dev(
+        ndim, _ptr(Cs), _ptr(Ct), _ptr(alpha), ny, nx, nz, _ptr(u),
+        out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(opts)))
+    return out
+
+
+def energy_potts(Ct, alpha, u, shape, nlab: int, *, stream=None) -> np.ndarray:
+    """FP64 energy {E, data, regularisation} of the Potts label fields u (device tensors)."""
+    ndim, ny, nx, nz = _ndim_dims(shape)
+    out = np.zeros(3)
+    opts = nat.make_opts(_stream_handle(stream), nat.FLOW_F32, nat.KERNEL_AUTO, 0, 0, 0, None)
+    nat.check(nat.lib().asets_potts_energy_dev(
+        ndim, _ptr(Ct), _ptr(alpha), ny, nx, nz, nlab, _ptr(u),
+        out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(opts)))
+    return out

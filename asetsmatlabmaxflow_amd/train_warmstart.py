@@ -1,0 +1,130 @@
+"""Train the warm-start network on synthetic binary volumes solved by the plain GPU solver.
+
+    python -m asetsmatlabmaxflow_amd.train_warmstart --volumes 256 --size 128 --steps 2000 \
+        --batch 4 --out weights.npz
+
+Data: volumes of the 256^3 ellipsoid generator (workloads.binary3d_ellipsoids) at
+--size^3, seeds 2000, 2001, ...; targets (u, pt) are the plain solver's state after
+--target-iters fixed iterations (errBound 0).  Loss: mean (u0-u)^2 + mean ((pt0-pt)/10)^2.
+Optimiser: AdamW (lr --lr, betas 0.9/0.999, weight decay 0.01), cosine decay to 10 %.
+Every forward/backward/optimiser operation is a kernel of libasets_nn.so.
+Prints one progress line every --log-every steps and a JSON summary at the end.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import multiprocessing as mp
+import os
+import time
+from concurrent.futures import ProcessPoolExecutor
+
+import numpy as np
+
+
+def _gen(args):
+    size, seed = args
+    from asetsmatlabmaxflow_amd import workloads
+
+    Cs, Ct, a, _ = workloads.binary3d_ellipsoids((size, size, size), seed=seed)
+    return tuple(np.asarray(x, np.float32).ravel(order="F") for x in (Cs, Ct, a))
+
+
+def make_dataset(nvol: int, size: int, seed0: int, target_iters: int, device, workers: int,
+                 log=print):
+    """Generate nvol volumes on the CPU (process pool), solve each on the GPU, and return
+    device tensors Cs, Ct, alpha, tu, tpt of shape [nvol * size^3]."""
+    import torch
+
+    from asetsmatlabmaxflow_amd import solver, workloads
+
+    N = size ** 3
+    shape = (size, size, size)
+    bufs = {k: torch.empty(nvol * N, dtype=torch.float32, device=device)
+            for k in ("Cs", "Ct", "alpha", "tu", "tpt")}
+    cc, st = workloads.BINARY3D_CC, workloads.BINARY3D_STEPS
+    t0 = time.time()
+    # spawned workers (numpy only) never inherit this process's GPU context
+    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        for v, (Cs, Ct, a) in enumerate(ex.map(_gen, [(size, seed0 + i) for i in range(nvol)])):
+            sl = slice(v * N, (v + 1) * N)
+            for k, arr in (("Cs", Cs), ("Ct", Ct), ("alpha", a)):
+                bufs[k][sl].copy_(torch.from_numpy(arr))
+            r = solver.binary(bufs["Cs"][sl], bufs["Ct"][sl], bufs["alpha"][sl], shape,
+                              target_iters, 0.0, cc, st, want_state=True, out=bufs["tu"][sl])
+            bufs["tpt"][sl].copy_(r.state["pt"])
+            if (v + 1) % max(1, nvol // 8) == 0:
+                log(f"dataset {v + 1}/{nvol} volumes, {time.time() - t0:.1f} s")
+    torch.cuda.synchronize()
+    return bufs
+
+
+def train(net, data, size: int, nvol: int, steps: int, batch: int, lr: float, seed: int,
+          log_every: int = 50, log=print):
+    import torch
+
+    N = size ** 3
+    shape = (size, size, size)
+    ngroups = nvol // batch
+    rng = np.random.default_rng(seed)
+    hist = []
+    t0 = time.time()
+    for s in range(steps):
+        g = int(rng.integers(ngroups))
+        sl = slice(g * batch * N, (g + 1) * batch * N)
+        Cs, Ct, a = data["Cs"][sl], data["Ct"][sl], data["alpha"][sl]
+        u0, pt0 = net.forward(Cs, Ct, a, shape, batch, train=True)
+        loss = net.backward(Cs, Ct, u0, pt0, data["tu"][sl], data["tpt"][sl], shape, batch)
+        lr_s = lr * (0.1 + 0.9 * 0.5 * (1 + math.cos(math.pi * s / max(1, steps))))
+        net.adamw_step(lr=lr_s)
+        if (s + 1) % log_every == 0 or s == 0:
+            lv = float(loss)
+            hist.append((s + 1, lv))
+            log(f"step {s + 1}/{steps} loss {lv:.5f} lr {lr_s:.2e} "
+                f"{(time.time() - t0) / (s + 1) * 1e3:.1f} ms/step")
+    torch.cuda.synchronize()
+    return hist, time.time() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--volumes", type=int, default=64)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--seed0", type=int, default=2000)
+    ap.add_argument("--target-iters", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--workers", type=int, default=min(8, os.cpu_count() or 1))
+    ap.add_argument("--log-every", type=int, default=50)
+    ap.add_argument("--init", default=None, help="start from these weights (.npz)")
+    ap.add_argument("--out", required=True)
+    args = ap.parse_args()
+
+    import torch
+
+    from asetsmatlabmaxflow_amd.warmstart import WarmStartNet
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    data = make_dataset(args.volumes, args.size, args.seed0, args.target_iters, dev,
+                        args.workers, log=lambda m: print(m, flush=True))
+    data_s = time.time() - t0
+    net = WarmStartNet(dev, seed=args.seed)
+    if args.init:
+        net.load(args.init)
+    hist, train_s = train(net, data, args.size, args.volumes, args.steps, args.batch, args.lr,
+                          args.seed, args.log_every, log=lambda m: print(m, flush=True))
+    net.save(args.out)
+    print(json.dumps({"volumes": args.volumes, "size": args.size, "steps": args.steps,
+                      "batch": args.batch, "lr": args.lr, "target_iters": args.target_iters,
+                      "dataset_s": data_s, "train_s": train_s,
+                      "ms_per_step": train_s / max(1, args.steps) * 1e3,
+                      "loss_first": hist[0][1], "loss_last": hist[-1][1], "history": hist,
+                      "weights": args.out}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,14 +1,23 @@
-"""Benchmark: 256^3 binary continuous max-flow on one MI355X.
+"""Benchmark of the continuous max-flow solvers on one MI355X.  Prints one JSON line.
 
-Workload: the 256x256x256 binary volume of asetsmatlabmaxflow_amd.workloads
-.binary3d_ellipsoids (seed 1002; 12 ellipsoids, Gaussian NLL costs, contrast-sensitive
-alpha), cc = 0.35, steps = 0.11.  A step is one solve of 200 fixed iterations
-(errBound 0) on inputs already resident in HBM; value = voxel-iterations per second over
-the K timed steps.  After the timed steps the same volume is solved to errBound 1e-4
-(time-to-tolerance).  The CPU baseline is the single-threaded FP32 reference-order oracle
-timed on a bounded number of iterations of the same volume.
+Default workload (--workload binary3d): the 256x256x256 binary volume of
+asetsmatlabmaxflow_amd.workloads.binary3d_ellipsoids (seed 1002; 12 ellipsoids, Gaussian
+NLL costs, contrast-sensitive alpha), cc = 0.35, steps = 0.11.  A step is one solve of 200
+fixed iterations (errBound 0) on inputs already resident in HBM; value = voxel-iterations
+per second over the K timed steps.  After the timed steps the same volume is solved to
+errBound 1e-4 (time-to-tolerance), with fp16 flow storage, and warm-started by the
+learned network when its weights are present.  The CPU baseline is the single-threaded
+FP32 reference-order oracle timed on a bounded number of iterations of the same volume.
 
-Prints one JSON line.
+Other workloads (one JSON line each, same keys):
+  binary2d  256x256 ellipses (seed 1001), 300 fixed iterations, cc 0.3, steps 0.16
+  potts3d   192^3, L = 6 Voronoi labels (seed 1003), 300 fixed iterations, cc 0.25
+  batched   64 binary volumes with axes uniform in {64..192} (seeds 1004*1000+v), one
+            graph-replayed persistent launch per iteration, to errBound 1e-4, fp32 and
+            fp16 flow storage
+
+Algorithmic bytes per voxel-iteration: binary 3D 60 (fp32 flows) / 40 (fp16 flows),
+binary 2D 52, Potts 3D 48L+8.
 """
 from __future__ import annotations
 
@@ -23,28 +32,30 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_VOXEL_ITER = 60  # Cs, Ct, alpha read; u, ps, pt, bx, by, bz read + written (fp32)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--iters", type=int, default=200)
-    ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--kernel", default="fused", choices=["fused", "passes"])
-    ap.add_argument("--tile-x", type=int, default=0)
-    ap.add_argument("--zchunk", type=int, default=0)
-    ap.add_argument("--cpu-iters", type=int, default=int(os.environ.get("ASETS_CPU_ITERS", 20)))
-    ap.add_argument("--no-tol", action="store_true")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp16", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
-        sys.exit(2)
+def _oracle():
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
 
+    return orc
+
+
+def _base(args, metric, value, unit, ms_per_step, dtype, data, config):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": data,
+            "config": config}
+
+
+def _roofline(bytes_per_unit, units, k_ms, traffic=None):
+    achieved = bytes_per_unit * units / (k_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel_ms_per_iter": k_ms,
+            "bytes_per_voxel_iter": bytes_per_unit}
+
+
+def bench_binary3d(args):
     import numpy as np
     import torch
 
@@ -62,26 +73,23 @@ def main():
     stream = torch.cuda.Stream()
     u_out = torch.empty(N, dtype=torch.float32, device=dev)
 
-    def solve(max_iter, eb, time_kernel=False):
+    def solve(max_iter, eb, time_kernel=False, **kw):
         with torch.cuda.stream(stream):
             return solver.binary(dCs, dCt, dA, shape, max_iter, eb, cc, steps,
                                  kernel=args.kernel, tile_x=args.tile_x, zchunk=args.zchunk,
-                                 stream=stream, time_kernel=time_kernel, out=u_out)
+                                 stream=stream, time_kernel=time_kernel, **kw)
 
     for _ in range(args.warmup):
-        solve(args.iters, 0.0)
+        solve(args.iters, 0.0, out=u_out)
     torch.cuda.synchronize()
     kms = []
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        r = solve(args.iters, 0.0, time_kernel=True)
+        r = solve(args.iters, 0.0, time_kernel=True, out=u_out)
         kms.append(r.kernel_ms)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
-    ms_per_step = wall * 1e3 / args.steps
-    value = N * args.iters * args.steps / wall
     k_ms = float(np.mean(kms))
-    achieved = BYTES_PER_VOXEL_ITER * N / (k_ms * 1e-3) / 1e9
     # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs of
     # this kernel and workload (scaled by a known-size copy), committed under profiles/
     traffic = None
@@ -89,29 +97,15 @@ def main():
     if prof.exists() and n == 256 and args.kernel == "fused" and not args.tile_x \
             and not args.zchunk:
         traffic = json.loads(prof.read_text())["traffic_bytes_per_launch"]
-
-    out = {
-        "metric": "voxel-iters/s, 256^3 binary MF (plain, 200 fixed iterations); "
-                  "time-to-tol at errBound 1e-4; HBM GB/s",
-        "value": value,
-        "unit": "voxel-iter/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded ellipsoid volume, workloads.binary3d_ellipsoids seed 1002)",
-        "config": {"workload": f"binary3d {n}^3 ellipsoids, contrast alpha, cc 0.35 steps 0.11",
-                   "shape": list(shape), "iters_per_step": args.iters, "kernel": args.kernel},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel_ms_per_iter": k_ms,
-                     "bytes_per_voxel_iter": BYTES_PER_VOXEL_ITER},
-        "gen_s": gen_s,
-    }
+    out = _base(args, "voxel-iters/s, 256^3 binary MF (plain, 200 fixed iterations); "
+                      "time-to-tol at errBound 1e-4; HBM GB/s",
+                N * args.iters * args.steps / wall, "voxel-iter/s", wall * 1e3 / args.steps,
+                "f32", "synthetic (seeded ellipsoid volume, workloads.binary3d_ellipsoids "
+                       "seed 1002)",
+                {"workload": f"binary3d {n}^3 ellipsoids, contrast alpha, cc 0.35 steps 0.11",
+                 "shape": list(shape), "iters_per_step": args.iters, "kernel": args.kernel})
+    out["roofline"] = _roofline(60, N, k_ms, traffic)
+    out["gen_s"] = gen_s
 
     # fp16 flow storage (ps, pt, bx, by, bz in fp16; u and arithmetic fp32): 40 B/voxel-iter
     if not args.no_fp16:
@@ -133,15 +127,15 @@ def main():
         torch.cuda.synchronize()
         ttt = time.perf_counter() - t1
         lab = (rt.u > 0.5)
+        e = solver.energy_binary(dCs, dCt, dA, rt.u, shape)
         out["time_to_tol"] = {"err_bound": 1e-4, "seconds": ttt, "it_num": rt.it_num,
                               "iterations": rt.iterations,
                               "final_cvg": float(rt.cvg[rt.iterations - 1].item()),
-                              "fg_fraction": float(lab.float().mean().item())}
+                              "fg_fraction": float(lab.float().mean().item()),
+                              "energy": float(e[0])}
 
     if not args.no_cpu:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        import oracle as orc
-
+        orc = _oracle()
         k = max(1, args.cpu_iters)
         t2 = time.perf_counter()
         orc.binary(Cs, Ct, alpha, shape, k, 0.0, cc, steps)
@@ -151,7 +145,162 @@ def main():
                                "sample": f"{k} iterations of the same {n}^3 volume "
                                          f"(FP32 reference-order oracle, gcc -O2, "
                                          f"1 thread), {cpu_s:.1f} s incl. init"}
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def bench_binary2d(args):
+    import numpy as np
+    import torch
+
+    from asetsmatlabmaxflow_amd import solver, workloads
+
+    Cs, Ct, a, pars, _ = workloads.binary2d_ellipses(256, seed=1001)
+    shape, N, iters = (256, 256), 256 * 256, 300
+    d = [solver.to_device(x) for x in (Cs, Ct, a)]
+    for _ in range(args.warmup):
+        solver.binary(*d, shape, iters, 0.0, 0.3, 0.16)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        r = solver.binary(*d, shape, iters, 0.0, 0.3, 0.16, time_kernel=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    out = _base(args, "voxel-iters/s, 256^2 binary MF 2D (300 fixed iterations)",
+                N * iters * args.steps / wall, "voxel-iter/s", wall * 1e3 / args.steps, "f32",
+                "synthetic (workloads.binary2d_ellipses seed 1001)",
+                {"workload": "binary2d 256^2 ellipses, alpha 0.5, cc 0.3 steps 0.16",
+                 "iters_per_step": iters})
+    out["roofline"] = _roofline(52, N, r.kernel_ms)
+    out["roofline"]["note"] = "3.4 MB per iteration: cache-resident and launch-bound"
+    if not args.no_cpu:
+        orc = _oracle()
+        t = time.perf_counter()
+        ref = orc.binary(Cs, Ct, a, shape, iters, 0.0, 0.3, 0.16)
+        cpu_s = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": N * iters / cpu_s, "unit": "voxel-iter/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"the full 300-iteration solve, {cpu_s:.1f} s"}
+        out["u_bit_identical_to_cpu"] = bool(np.array_equal(
+            r.u.cpu().numpy().view(np.uint32), ref.u.view(np.uint32)))
+    return out
+
+
+def bench_potts3d(args):
+    import numpy as np
+    import torch
+
+    from asetsmatlabmaxflow_amd import solver, workloads
+
+    shape, L, iters = (192, 192, 192), 6, 300
+    Ct, a, _ = workloads.potts3d_voronoi(shape, nlab=L, seed=1003)
+    N = int(np.prod(shape))
+    d = [solver.to_device(x) for x in (Ct, a)]
+    cc, st = workloads.POTTS3D_CC, workloads.POTTS3D_STEPS
+    kern = "passes" if args.kernel == "passes" else "fused"
+    for _ in range(args.warmup):
+        solver.potts(*d, shape, L, 5, 0.0, cc, st, kernel=kern)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        r = solver.potts(*d, shape, L, iters, 0.0, cc, st, kernel=kern, time_kernel=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    out = _base(args, "voxel-iters/s, 192^3 Potts L=6 (300 fixed iterations)",
+                N * iters * args.steps / wall, "voxel-iter/s", wall * 1e3 / args.steps, "f32",
+                "synthetic (workloads.potts3d_voronoi seed 1003)",
+                {"workload": "potts3d 192^3 L=6 Voronoi, alpha 0.8, cc 0.25 steps 0.11",
+                 "iters_per_step": iters, "kernel": kern})
+    out["roofline"] = _roofline(48 * L + 8, N, r.kernel_ms)
+    lab = r.u.view(L, -1).argmax(0)
+    out["label_fractions"] = [float((lab == l).float().mean()) for l in range(L)]
+    if not args.no_cpu:
+        orc = _oracle()
+        k = max(1, args.cpu_iters // 4)
+        t = time.perf_counter()
+        orc.potts(Ct, a, shape, L, k, 0.0, cc, st)
+        cpu_s = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": N * k / cpu_s, "unit": "voxel-iter/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{k} iterations incl. init, FP32 reference-order "
+                                         f"oracle, {cpu_s:.1f} s"}
+    return out
+
+
+def bench_batched(args):
+    import numpy as np
+    import torch
+
+    from asetsmatlabmaxflow_amd import solver, workloads
+
+    rng = np.random.default_rng(1004)
+    sizes = [tuple(int(v) for v in rng.integers(64, 193, 3)) for _ in range(args.nvol)]
+    dev = []
+    t = time.perf_counter()
+    for v, sh in enumerate(sizes):
+        Cs, Ct, a, _ = workloads.binary3d_ellipsoids(sh, seed=1004 * 1000 + v)
+        dev.append(tuple(solver.to_device(x) for x in (Cs, Ct, a)) + (sh,))
+    gen_s = time.perf_counter() - t
+    cc, st = workloads.BINARY3D_CC, workloads.BINARY3D_STEPS
+    res, legs = {}, {}
+    for flow in ("f32", "f16"):
+        solver.binary_batched(dev, 3, 0.0, cc, st, flow=flow)   # autotuning + warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rs, secs, loop_ms = solver.binary_batched(dev, 2000, 1e-4, cc, st, flow=flow,
+                                                  time_kernel=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
+        vit = sum(int(np.prod(s)) * r.iterations for s, r in zip(sizes, rs))
+        bpv = 60 if flow == "f32" else 40
+        res[flow] = rs
+        legs[flow] = {"seconds": wall, "loop_ms": loop_ms, "voxel_iterations": vit,
+                      "voxel_iter_per_s": vit / wall,
+                      "achieved_GBps_loop": bpv * vit / (loop_ms * 1e-3) / 1e9,
+                      "iterations": [r.iterations for r in rs]}
+    diff = max(float((a.u - b.u).abs().max().item()) for a, b in zip(res["f32"], res["f16"]))
+    agree = np.mean([float(((a.u > 0.5) == (b.u > 0.5)).float().mean().item())
+                     for a, b in zip(res["f32"], res["f16"])])
+    out = _base(args, f"voxel-iters/s, {args.nvol} uneven binary volumes batched to errBound "
+                      "1e-4 (fp32 flows)", legs["f32"]["voxel_iter_per_s"], "voxel-iter/s",
+                legs["f32"]["seconds"] * 1e3, "f32",
+                "synthetic (workloads.binary3d_ellipsoids, seeds 1004*1000+v)",
+                {"workload": f"batched binary3d, {args.nvol} volumes, axes U{{64..192}}",
+                 "total_voxels": int(sum(np.prod(s) for s in sizes))})
+    out["roofline"] = {"bound": "hbm", "achieved": legs["f32"]["achieved_GBps_loop"],
+                       "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": legs["f32"]["achieved_GBps_loop"] / HBM_PEAK_GBS,
+                       "traffic": None}
+    out["fp32"] = legs["f32"]
+    out["fp16"] = legs["f16"]
+    out["fp16_vs_fp32"] = {"max_abs_u_diff": diff, "label_agreement": agree}
+    out["gen_s"] = gen_s
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="binary3d",
+                    choices=["binary3d", "binary2d", "potts3d", "batched"])
+    ap.add_argument("--iters", type=int, default=200)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--nvol", type=int, default=64)
+    ap.add_argument("--kernel", default="fused", choices=["fused", "passes"])
+    ap.add_argument("--tile-x", type=int, default=0)
+    ap.add_argument("--zchunk", type=int, default=0)
+    ap.add_argument("--cpu-iters", type=int, default=int(os.environ.get("ASETS_CPU_ITERS", 20)))
+    ap.add_argument("--no-tol", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp16", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        sys.exit(2)
+    fn = {"binary3d": bench_binary3d, "binary2d": bench_binary2d, "potts3d": bench_potts3d,
+          "batched": bench_batched}[args.workload]
+    print(json.dumps(fn(args)), flush=True)
 
 
 if __name__ == "__main__":

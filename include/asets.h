@@ -162,6 +162,22 @@ asets_status asets_binary3d_batched_dev(int32_t nvol, const asets_volume *vols,
                                         const asets_params *p, double *seconds,
                                         const asets_opts *o);
 
+/* ---- energy of a labelling (FP64) ----------------------------------------
+ * No reference counterpart: asetsBinaryMF3D_mex.c:183 and asetsPotts3D_mex.c:203
+ * report only the mean |du| trace.  Binary:
+ *   E(u) = sum (1-u) Cs + u Ct + alpha |grad u|;  Potts: sum_l u_l Ct_l + alpha_l |grad u_l|,
+ * where |grad u| at a voxel is sqrt(0.5 * sum of the squared differences across its
+ * low and high face on each axis) -- the face set and 0.5 factor of the projection
+ * at asetsBinaryMF3D_mex.c:321-324; faces outside the volume count 0.  Device
+ * arrays, column-major; energy (host) receives {E, data term, regularisation term}.
+ * Accumulated in FP64 with a fixed reduction order (deterministic). */
+asets_status asets_binary_energy_dev(int ndim, const float *Cs, const float *Ct,
+                                     const float *alpha, int64_t ny, int64_t nx, int64_t nz,
+                                     const float *u, double *energy, const asets_opts *o);
+asets_status asets_potts_energy_dev(int ndim, const float *Ct, const float *alpha, int64_t ny,
+                                    int64_t nx, int64_t nz, int64_t nlab, const float *u,
+                                    double *energy, const asets_opts *o);
+
 /* ---- misc ---------------------------------------------------------------- */
 /* Self-check of the short correctly-rounded division form the kernels use for
  * x/cc, n/alpha and 1/n (no reference counterpart): compares it with IEEE

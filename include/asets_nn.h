@@ -1,0 +1,115 @@
+/*
+ * asets_nn.h -- C ABI of libasets_nn.so, the kernels of the learned warm start
+ * for the binary max-flow solver: a small 3D U-Net (3x3x3 convolutions on the
+ * gfx950 matrix cores, bf16 activations and weights, fp32 accumulation) that
+ * maps (Cs, Ct, alpha) to a starting point (u0, pt0) for asets_binary_warm /
+ * asets_binary_dev.
+ *
+ * None of these entry points has a counterpart in the reference: its solvers
+ * always start from the initial cut of asetsBinaryMF3D_mex.c:205-224
+ * (u = [Cs >= Ct], ps = pt = min(Cs, Ct)).  The warm start replaces exactly that
+ * initialisation and leaves the iteration (and its stop rule) unchanged.
+ *
+ * Conventions: every pointer is device memory; tensors are NDHWC (channels
+ * fastest) bf16 (16-bit storage passed as void *) unless typed float.  A batch
+ * of n column-major volumes (Ny fastest) is NDHWC with D = Nz, H = Nx, W = Ny,
+ * so voxel p = v*N + g keeps its solver index.  Tensors hold fewer than 2^31
+ * elements.  Conv weights are [27][cout][cin] (tap-major, cin contiguous), tap
+ * k = (dz+1)*9 + (dh+1)*3 + (dw+1).  Work is enqueued on `stream` (NULL = the
+ * default stream); nothing synchronises.  Errors return a status;
+ * asets_nn_last_error() holds the message.
+ */
+#ifndef ASETS_NN_H
+#define ASETS_NN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "asets.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* out = act(conv3x3x3(in) + bias); in [n,d,h,w,cin], out [n,d,h,w,cout];
+ * cin a multiple of 8; cout in {16,32,48,64,96,128}; bias may be NULL;
+ * relu != 0 applies max(0, .).  Zero padding 1, stride 1. */
+asets_status asets_nn_conv3(const void *in, int n, int d, int h, int w, int cin,
+                            const void *wt, const float *bias, int cout, int relu, void *out,
+                            void *stream);
+
+/* Weights of the input-gradient convolution: wt [27][cout][cin] ->
+ * wt_bwd [27][cin][cout] with taps mirrored (k -> 26-k).  Then
+ * asets_nn_conv3(gout, ..., cout, wt_bwd, NULL, cin, 0, gin) is d loss / d in. */
+asets_status asets_nn_flip_transpose(const void *wt, void *wt_bwd, int cout, int cin,
+                                     void *stream);
+
+/* g[i] = 0 where y[i] <= 0 (gradient through a ReLU with output y). */
+asets_status asets_nn_relu_mask(void *g, const void *y, int64_t n, void *stream);
+
+/* Weight and bias gradient of asets_nn_conv3: dw [27][cout][cin] fp32,
+ * dbias [cout] fp32 (may be NULL); accumulate != 0 adds to them.  cout a
+ * multiple of 16; cin 8 or a multiple of 16.  scratch: device memory of asets_nn_wgrad_scratch_bytes().
+ * Fixed reduction order (deterministic). */
+size_t asets_nn_wgrad_scratch_bytes(int n, int d, int h, int w, int cin, int cout);
+asets_status asets_nn_conv3_wgrad(const void *in, int n, int d, int h, int w, int cin,
+                                  const void *gout, int cout, float *dw, float *dbias,
+                                  int accumulate, void *scratch, size_t scratch_bytes,
+                                  void *stream);
+
+/* 2x2x2 average pool, (d, h, w) the fine extents (even): out [n,d/2,h/2,w/2,c];
+ * backward: gin [n,d,h,w,c] = gout/8 broadcast (added when accumulate != 0). */
+asets_status asets_nn_pool2(const void *in, int n, int d, int h, int w, int c, void *out,
+                            void *stream);
+asets_status asets_nn_pool2_bwd(const void *gout, int n, int d, int h, int w, int c, void *gin,
+                                int accumulate, void *stream);
+
+/* Nearest 2x upsample of in [n,d/2,h/2,w/2,c] into channels [off, off+c) of
+ * out [n,d,h,w,cd] (a concatenation buffer); backward sums the 8 fine gradients
+ * of that channel range of gout [n,d,h,w,cd] into gin [n,d/2,h/2,w/2,c]. */
+asets_status asets_nn_up2(const void *in, int n, int d, int h, int w, int c, void *out, int cd,
+                          int off, void *stream);
+asets_status asets_nn_up2_bwd(const void *gout, int cd, int off, int n, int d, int h, int w,
+                              int c, void *gin, void *stream);
+
+/* dst[p, doff+c] (+)= src[p, soff+c], c < nc, over nvox voxels (skip
+ * connections into a concatenation buffer and their gradients back). */
+asets_status asets_nn_chan_copy(void *dst, int cd, int doff, const void *src, int cs, int soff,
+                                int nc, int64_t nvox, int accumulate, void *stream);
+
+/* Input features [P, 8] of P voxels: Cs/10, Ct/10, alpha, (Ct-Cs)/10,
+ * min(Cs,Ct)/10, 0, 0, 0. */
+asets_status asets_nn_features(const float *Cs, const float *Ct, const float *alpha, int64_t P,
+                               void *out, void *stream);
+
+/* Head: 1x1x1 convolution of x [P,16] with w = [w_u(16), w_pt(16)], b = [b_u, b_pt]:
+ *   u0 = w_u.x + b_u,   pt0 = min(Cs, Ct) + 10 * (w_pt.x + b_pt)
+ * (pt0 is predicted relative to the reference's initial pt). */
+asets_status asets_nn_head(const void *x, int64_t P, const float *w, const float *b,
+                           const float *Cs, const float *Ct, float *u0, float *pt0,
+                           void *stream);
+
+/* Loss mean((u0-tu)^2) + mean(((pt0-tpt)/10)^2) and its gradients: dx [P,16]
+ * bf16; grads (fp32, 35 values) = d/dw (32), d/db (2), then the loss * P.
+ * scratch: device memory of asets_nn_head_scratch_bytes(P). */
+size_t asets_nn_head_scratch_bytes(int64_t P);
+asets_status asets_nn_head_bwd(const void *x, int64_t P, const float *w, const float *u0,
+                               const float *pt0, const float *tu, const float *tpt, void *dx,
+                               float *grads, void *scratch, size_t scratch_bytes,
+                               void *stream);
+
+/* AdamW step `step` (1-based) on fp32 weights w with moments m, v; refreshes the
+ * bf16 copy wb when it is not NULL. */
+asets_status asets_nn_adamw(float *w, void *wb, const float *g, float *m, float *v, int64_t n,
+                            float lr, float b1, float b2, float eps, float wd, int step,
+                            void *stream);
+
+/* b = bf16(a), round to nearest even. */
+asets_status asets_nn_to_bf16(const float *a, void *b, int64_t n, void *stream);
+
+const char *asets_nn_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ASETS_NN_H */

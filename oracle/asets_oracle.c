@@ -95,3 +95,70 @@ EXPORT int orc_potts_f64(const double *Ct, const double *alpha,
     return potts_f64(Ct, alpha, ny, nx, nz, L, is3d, max_it, err_bound, cc, steps,
                      stop_rule, u, cvg_ref, cvg64, it_num, ps, pt, bx, by, bz);
 }
+
+/* ---- energy evaluator (FP64) ------------------------------------------------
+ * No reference counterpart: the C-MEX solvers report only the mean |du| trace.
+ * The primal energy of the binary model,
+ *     E(u) = sum_g (1 - u_g) Cs_g + u_g Ct_g  +  sum_g alpha_g |grad u|_g,
+ * with the voxel-centred norm built from the same face differences as the
+ * projection of asetsBinaryMF3D_mex.c:321-324 (the high and the low face of each
+ * axis, squared, summed, times 0.5, square root; a face outside the volume
+ * contributes 0).  An axis-aligned unit edge therefore costs sqrt(2)*alpha, which
+ * is the capacity the projection gives a single face (|q| <= sqrt(2) alpha).
+ * Potts: E = sum_l sum_g u_lg Ct_lg + alpha_lg |grad u_l|_g.
+ * out[0] = E, out[1] = data term, out[2] = regularisation term.  is3d == 0 uses
+ * the 4-face 2D norm (nz must be 1).
+ */
+static void energy_field(const double *Cs, const double *Ct, const double *alpha,
+                         const double *u, int64_t ny, int64_t nx, int64_t nz, int is3d,
+                         double *data, double *reg)
+{
+    const int64_t sx = ny, sz = nx * ny;
+    double d = 0.0, r = 0.0;
+    for (int64_t z = 0; z < nz; ++z)
+        for (int64_t x = 0; x < nx; ++x)
+            for (int64_t y = 0; y < ny; ++y) {
+                const int64_t g = z * sz + x * sx + y;
+                const double ug = u[g];
+                d += (Cs ? (1.0 - ug) * Cs[g] : 0.0) + ug * Ct[g];
+                double t = 0.0, q;
+                if (x > 0)      { q = ug - u[g - sx]; t += q * q; }
+                if (x < nx - 1) { q = u[g + sx] - ug; t += q * q; }
+                if (y > 0)      { q = ug - u[g - 1];  t += q * q; }
+                if (y < ny - 1) { q = u[g + 1] - ug;  t += q * q; }
+                if (is3d) {
+                    if (z > 0)      { q = ug - u[g - sz]; t += q * q; }
+                    if (z < nz - 1) { q = u[g + sz] - ug; t += q * q; }
+                }
+                r += alpha[g] * sqrt(0.5 * t);
+            }
+    *data += d;
+    *reg += r;
+}
+
+EXPORT int orc_energy_binary(const double *Cs, const double *Ct, const double *alpha,
+                             int64_t ny, int64_t nx, int64_t nz, int is3d, const double *u,
+                             double *out)
+{
+    if (ny <= 0 || nx <= 0 || nz <= 0 || (!is3d && nz != 1)) return ORC_EBADARG;
+    double d = 0.0, r = 0.0;
+    energy_field(Cs, Ct, alpha, u, ny, nx, nz, is3d, &d, &r);
+    out[0] = d + r;
+    out[1] = d;
+    out[2] = r;
+    return ORC_OK;
+}
+
+EXPORT int orc_energy_potts(const double *Ct, const double *alpha, int64_t ny, int64_t nx,
+                            int64_t nz, int64_t L, int is3d, const double *u, double *out)
+{
+    if (ny <= 0 || nx <= 0 || nz <= 0 || L <= 0 || (!is3d && nz != 1)) return ORC_EBADARG;
+    const int64_t N = ny * nx * nz;
+    double d = 0.0, r = 0.0;
+    for (int64_t l = 0; l < L; ++l)
+        energy_field(NULL, Ct + l * N, alpha + l * N, u + l * N, ny, nx, nz, is3d, &d, &r);
+    out[0] = d + r;
+    out[1] = d;
+    out[2] = r;
+    return ORC_OK;
+}

@@ -1,0 +1,121 @@
+"""CPU oracle of the warm-start network (TEST INFRASTRUCTURE ONLY: the checker for
+libasets_nn.so; the product never imports it).
+
+The network has no reference counterpart (the reference always starts from the initial cut
+of asetsBinaryMF3D_mex.c:205-224).  This module restates each operation of
+include/asets_nn.h with float64 torch operations on the CPU, and the whole network as
+asetsmatlabmaxflow_amd/warmstart.py composes them, so autograd gives reference gradients.
+Where the kernels store bf16 (every activation, every activation gradient), `rb` rounds
+the forward value to bf16 as well; the backward pass stays float64, so the network-level
+gradient comparison carries a tolerance (the per-operation checks are tight).
+
+Layout conversions: kernel tensors are NDHWC flat; conv weights [27][cout][cin] with tap
+k = kd*9 + kh*3 + kw map to torch's [cout, cin, kd, kh, kw] (cross-correlation, padding 1).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+COST_SCALE = 10.0
+
+
+def bf16_round(a: np.ndarray) -> np.ndarray:
+    """Round float32 values to bf16 (nearest even) and return them as float32."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    x = a.view(np.uint32).astype(np.uint64)
+    r = ((x + 0x7FFF + ((x >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def rb(t: torch.Tensor) -> torch.Tensor:
+    """bf16 rounding in the forward pass, identity in the backward pass."""
+    return t + (t.to(torch.bfloat16).to(t.dtype) - t).detach()
+
+
+def to_ncdhw(x: np.ndarray | torch.Tensor, n, d, h, w, c) -> torch.Tensor:
+    t = torch.as_tensor(np.asarray(x, dtype=np.float64)) if isinstance(x, np.ndarray) else x
+    return t.reshape(n, d, h, w, c).permute(0, 4, 1, 2, 3)
+
+
+def from_ncdhw(t: torch.Tensor) -> torch.Tensor:
+    return t.permute(0, 2, 3, 4, 1).reshape(-1)
+
+
+def weight_to_torch(w: np.ndarray | torch.Tensor, cin, cout) -> torch.Tensor:
+    t = torch.as_tensor(np.asarray(w, dtype=np.float64)) if isinstance(w, np.ndarray) else w
+    return t.reshape(3, 3, 3, cout, cin).permute(3, 4, 0, 1, 2)
+
+
+def weight_from_torch(t: torch.Tensor) -> torch.Tensor:
+    return t.permute(2, 3, 4, 0, 1).reshape(-1)
+
+
+def conv3(x, w, b, relu):
+    y = F.conv3d(x, w, b, padding=1)
+    return F.relu(y) if relu else y
+
+
+def pool2(x):
+    return F.avg_pool3d(x, 2)
+
+
+def up2(x):
+    return x.repeat_interleave(2, 2).repeat_interleave(2, 3).repeat_interleave(2, 4)
+
+
+def features(Cs, Ct, alpha):
+    """[P] float arrays -> [P, 8] features of asets_nn_features (float64 math, then bf16)."""
+    Cs, Ct, alpha = (np.asarray(a, dtype=np.float32) for a in (Cs, Ct, alpha))
+    k = np.float32(1.0 / COST_SCALE)
+    f = np.zeros((Cs.size, 8), dtype=np.float32)
+    f[:, 0] = Cs * k
+    f[:, 1] = Ct * k
+    f[:, 2] = alpha
+    f[:, 3] = (Ct - Cs) * k
+    f[:, 4] = np.minimum(Cs, Ct) * k
+    return bf16_round(f)
+
+
+CONVS = (("enc0a", 8, 16), ("enc0b", 16, 16), ("enc1a", 16, 32), ("enc1b", 32, 32),
+         ("enc2a", 32, 64), ("enc2b", 64, 64), ("enc3a", 64, 64), ("enc3b", 64, 64),
+         ("dec2", 128, 64), ("dec1", 96, 32), ("dec0", 48, 16))
+
+
+def network(params: dict, Cs, Ct, alpha, n, shape, tu=None, tpt=None):
+    """Forward pass of the U-Net (and the loss if targets are given).
+
+    params: name -> float64 leaf tensors ('<conv>.w' flat [27][cout][cin] bf16-exact values,
+    '<conv>.b', 'head.w' [32], 'head.b' [2]).  Returns (u0, pt0, loss)."""
+    ny, nx, nz = shape
+    P = n * ny * nx * nz
+    x = to_ncdhw(features(Cs, Ct, alpha), n, nz, nx, ny, 8)
+
+    def C(name, inp, relu=True):
+        cin, cout = next((ci, co) for nm, ci, co in CONVS if nm == name)
+        return rb(conv3(inp, weight_to_torch(params[name + ".w"], cin, cout),
+                        params[name + ".b"], relu))
+
+    e0 = C("enc0b", C("enc0a", x))
+    e1 = C("enc1b", C("enc1a", rb(pool2(e0))))
+    e2 = C("enc2b", C("enc2a", rb(pool2(e1))))
+    e3 = C("enc3b", C("enc3a", rb(pool2(e2))))
+    d2 = C("dec2", torch.cat([up2(e3), e2], 1))
+    d1 = C("dec1", torch.cat([up2(d2), e1], 1))
+    d0 = C("dec0", torch.cat([up2(d1), e0], 1))
+    xf = from_ncdhw(d0).reshape(P, 16)
+    hw = params["head.w"]
+    hb = params["head.b"]
+    a = xf @ hw[:16] + hb[0]
+    b = xf @ hw[16:] + hb[1]
+    base = torch.as_tensor(np.minimum(np.asarray(Cs, np.float32), np.asarray(Ct, np.float32)),
+                           dtype=torch.float64)
+    u0 = a
+    pt0 = base + COST_SCALE * b
+    loss = None
+    if tu is not None:
+        tu = torch.as_tensor(np.asarray(tu, np.float64))
+        tpt = torch.as_tensor(np.asarray(tpt, np.float64))
+        loss = ((u0 - tu) ** 2).mean() + (((pt0 - tpt) / COST_SCALE) ** 2).mean()
+    return u0, pt0, loss

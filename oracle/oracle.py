@@ -45,6 +45,10 @@ def lib():
             f.argtypes = [P, P, I64, I64, I64, I64, C.c_int, C.c_int, real, real, real, C.c_int,
                           P, P, P, P, P, P, P, P, P]
             f.restype = C.c_int
+        L.orc_energy_binary.argtypes = [P, P, P, I64, I64, I64, C.c_int, P, P]
+        L.orc_energy_binary.restype = C.c_int
+        L.orc_energy_potts.argtypes = [P, P, I64, I64, I64, I64, C.c_int, P, P]
+        L.orc_energy_potts.restype = C.c_int
         _lib = L
     return _lib
 
@@ -130,3 +134,33 @@ def potts(Ct, alpha, shape, nlab, max_iter, err_bound, cc, steps, *, precision="
     if rc != ORC_OK:
         raise OracleError(rc)
     return OracleResult(u, cr[:max_iter], c64[:max_iter], itn.value, st)
+
+
+def _dims(shape, is3d):
+    if len(shape) == 2:
+        return shape[0], shape[1], 1, (False if is3d is None else is3d)
+    return shape[0], shape[1], shape[2], (True if is3d is None else is3d)
+
+
+def energy_binary(Cs, Ct, alpha, u, shape, *, is3d=None):
+    """FP64 primal energy (total, data, regularisation) of u; see orc_energy_binary."""
+    ny, nx, nz, d3 = _dims(shape, is3d)
+    Cs, Ct, alpha, u = (_flat(x, np.float64) for x in (Cs, Ct, alpha, u))
+    out = np.zeros(3)
+    rc = lib().orc_energy_binary(Cs.ctypes.data, Ct.ctypes.data, alpha.ctypes.data, ny, nx,
+                                 nz, int(d3), u.ctypes.data, out.ctypes.data)
+    if rc != ORC_OK:
+        raise OracleError(rc)
+    return out
+
+
+def energy_potts(Ct, alpha, u, shape, nlab, *, is3d=None):
+    """FP64 Potts energy (total, data, regularisation) of the L label fields u."""
+    ny, nx, nz, d3 = _dims(shape, is3d)
+    Ct, alpha, u = (_flat(x, np.float64) for x in (Ct, alpha, u))
+    out = np.zeros(3)
+    rc = lib().orc_energy_potts(Ct.ctypes.data, alpha.ctypes.data, ny, nx, nz, nlab, int(d3),
+                                u.ctypes.data, out.ctypes.data)
+    if rc != ORC_OK:
+        raise OracleError(rc)
+    return out

@@ -29,6 +29,32 @@ def test_library_exports_every_declared_symbol():
     assert lib.asets_version().decode().startswith("asets-mi355x")
 
 
+def test_nn_library_exports_every_declared_symbol():
+    from asetsmatlabmaxflow_amd import warmstart
+
+    lib = warmstart.nnlib()
+    declared = _declared(ROOT / "include" / "asets_nn.h")
+    assert declared, "no declarations parsed"
+    for name in declared:
+        assert hasattr(lib, name), f"libasets_nn.so does not export {name}"
+    assert set(declared) == set(warmstart.NN_EXPORTED)
+
+
+def test_nn_bad_arguments_rejected_before_device_work():
+    from asetsmatlabmaxflow_amd import warmstart
+
+    L = warmstart.nnlib()
+    # cin not a multiple of 8, unsupported cout, odd pool extent, scratch too small
+    assert L.asets_nn_conv3(1, 1, 4, 4, 4, 12, 1, None, 16, 1, 1, None) == nat.ASETS_EBADARG
+    assert L.asets_nn_conv3(1, 1, 4, 4, 4, 16, 1, None, 24, 1, 1, None) == nat.ASETS_EBADARG
+    assert L.asets_nn_pool2(1, 1, 4, 5, 4, 16, 1, None) == nat.ASETS_EBADARG
+    assert L.asets_nn_conv3_wgrad(1, 1, 4, 4, 4, 16, 1, 16, 1, None, 0, 1, 16,
+                                  None) == nat.ASETS_EBADARG
+    assert b"scratch" in L.asets_nn_last_error()
+    assert L.asets_nn_conv3(1, 1 << 12, 1 << 12, 1 << 12, 8, 16, 1, None, 16, 1, 1,
+                            None) == nat.ASETS_ETOOBIG
+
+
 def test_mex_modules_export_mexFunction():
     for name in mex.MEX_DIR.glob("asets*_mex.so"):
         assert hasattr(C.CDLL(str(name)), "mexFunction"), name
