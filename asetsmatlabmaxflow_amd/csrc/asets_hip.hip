@@ -249,28 +249,57 @@ __global__ void k_copy_f32(float *dst, const float *src, int64_t n) {
 }
 
 template <typename T>
-__global__ void k_init_binary_t(const float *Cs, const float *Ct, const float *u0,
-                                const float *pt0, float *u, T *ps, T *pt, int64_t N) {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= N) return;
-    float uu, m;
-    if (u0) {
-        uu = u0[g];
-        m = pt0[g];
-    } else {
-        const float cs = Cs[g], ct = Ct[g];
-        uu = (cs - ct >= 0.0f) ? 1.0f : 0.0f;   // asetsBinaryMF3D_mex.c:215-219
-        m = fmin_ref(cs, ct);
-    }
-    u[g] = uu;
-    Io<T>::put(ps + g, m);
-    Io<T>::put(pt + g, m);
-}
-
-template <typename T>
 __global__ void k_fill_t(T *p, float v, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) Io<T>::put(p + i, v);
+}
+
+template <typename T>
+__global__ void k_init_binary_t(const float *Cs, const float *Ct, const float *u0,
+                                const float *ps0, const float *pt0, float *u, T *ps, T *pt,
+                                int64_t N) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    float uu, ms, mt;
+    if (u0) {
+        uu = u0[g];
+        mt = pt0[g];
+        ms = ps0 ? ps0[g] : mt;
+    } else {
+        const float cs = Cs[g], ct = Ct[g];
+        uu = (cs - ct >= 0.0f) ? 1.0f : 0.0f;   // asetsBinaryMF3D_mex.c:215-219
+        ms = mt = fmin_ref(cs, ct);
+    }
+    u[g] = uu;
+    Io<T>::put(ps + g, ms);
+    Io<T>::put(pt + g, mt);
+}
+
+// Initial flow faces of a resumed solve: dst = src, except the faces with axis
+// index 0, which are the zero-flux boundary (never updated; asetsBinaryMF3D_mex.c
+// leaves index 0 of bx, by, bz at 0) and are forced to 0.  axis: 0 = y (by),
+// 1 = x (bx), 2 = z (bz).
+template <typename T>
+__global__ void k_init_faces(T *dst, const float *src, int64_t ny, int64_t nx, int64_t N,
+                             int axis) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    int64_t c = axis == 0 ? g % ny : (axis == 1 ? (g / ny) % nx : g / (ny * nx));
+    Io<T>::put(dst + g, c == 0 ? 0.0f : src[g]);
+}
+
+template <typename T>
+void init_faces(T *const dst[3], const float *const src[3], int64_t ny, int64_t nx, int64_t nz,
+                hipStream_t s) {
+    const int64_t N = ny * nx * nz;
+    const unsigned nb = (unsigned)cdiv(N, kBlock);
+    const int axis[3] = {1, 0, 2};   // bx, by, bz
+    for (int k = 0; k < 3; ++k) {
+        if (src[k])
+            k_init_faces<T><<<nb, kBlock, 0, s>>>(dst[k], src[k], ny, nx, N, axis[k]);
+        else
+            k_fill_t<T><<<nb, kBlock, 0, s>>>(dst[k], 0.0f, N);
+    }
 }
 
 template <typename T>
@@ -285,7 +314,8 @@ struct FusedRun {
     const float *Cs, *Ct, *alpha;
     int64_t ny, nx, nz;
     const asets_params *p;
-    const float *u0, *pt0;
+    const float *u0, *ps0, *pt0;
+    const float *b0[3];   // initial bx, by, bz (nullptr: 0)
     float *u, *cvg;
     int32_t *it_num;
     float *o_ps, *o_pt, *o_bx, *o_by, *o_bz;
@@ -319,10 +349,11 @@ asets_status run_fused(const FusedRun &r, Carver &cv, DeviceCache &dc) {
         buf[b].by = cv.take<T>(N);
         buf[b].bz = cv.take<T>(N);
     }
-    k_init_binary_t<T><<<nb, kBlock, 0, s>>>(r.Cs, r.Ct, r.u0, r.pt0, buf[0].u, buf[0].ps,
-                                            buf[0].pt, N);
-    for (int b = 0; b < 2; ++b)
-        for (T *f : {buf[b].bx, buf[b].by, buf[b].bz}) k_fill_t<T><<<nb, kBlock, 0, s>>>(f, 0.0f, N);
+    k_init_binary_t<T><<<nb, kBlock, 0, s>>>(r.Cs, r.Ct, r.u0, r.ps0, r.pt0, buf[0].u,
+                                            buf[0].ps, buf[0].pt, N);
+    T *const f0[3] = {buf[0].bx, buf[0].by, buf[0].bz};
+    init_faces<T>(f0, r.b0, r.ny, r.nx, r.nz, s);
+    for (T *f : {buf[1].bx, buf[1].by, buf[1].bz}) k_fill_t<T><<<nb, kBlock, 0, s>>>(f, 0.0f, N);
     HIPCHK(hipGetLastError());
 
     FusedArgs<T> fa{};
@@ -410,9 +441,16 @@ void launch_potts_couple(int L, const float *ft, float *u, float *ps, int64_t N,
 // ---------------------------------------------------------------------------
 // binary model on device memory
 // ---------------------------------------------------------------------------
+// Initial state of a solve: all null = the reference's initial cut; u0 and pt0 =
+// warm start (ps0 = pt0, flows 0 unless given); every field = resume.
+struct WarmIn {
+    const float *u0 = nullptr, *ps0 = nullptr, *pt0 = nullptr;
+    const float *b0[3] = {nullptr, nullptr, nullptr};
+};
+
 asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float *alpha,
                         int64_t ny, int64_t nx, int64_t nz, const asets_params *p,
-                        const float *u0, const float *pt0, float *u, float *cvg,
+                        const WarmIn &w, float *u, float *cvg,
                         int32_t *it_num, double *seconds, float *o_ps, float *o_pt,
                         float *o_bx, float *o_by, float *o_bz, const asets_opts *o) {
     auto t_start = std::chrono::steady_clock::now();
@@ -424,8 +462,12 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
     if (st) return st;
     if (!Cs || !Ct || !alpha || !u || !it_num || (p->max_iter > 0 && !cvg))
         return fail(ASETS_EBADARG, "null array argument");
+    const float *u0 = w.u0, *pt0 = w.pt0;
     if ((u0 == nullptr) != (pt0 == nullptr))
         return fail(ASETS_EBADARG, "warm start needs both u0 and pt0");
+    if (!u0 && (w.ps0 || w.b0[0] || w.b0[1] || w.b0[2]))
+        return fail(ASETS_EBADARG, "a resumed state needs u0 and pt0");
+    if (ndim == 2 && w.b0[2]) return fail(ASETS_EBADARG, "a 2D state has no bz");
     const int flow16 = o ? o->flow_dtype : ASETS_FLOW_F32;
     if (flow16 != ASETS_FLOW_F32 && flow16 != ASETS_FLOW_F16)
         return fail(ASETS_EBADARG, "unknown flow_dtype %d", flow16);
@@ -488,11 +530,15 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
         float *ps = cv.take<float>(N), *pt = cv.take<float>(N), *bx = cv.take<float>(N);
         float *by = cv.take<float>(N), *bz = cv.take<float>(N), *dv = cv.take<float>(N);
         float *gk = cv.take<float>(N);
-        fill(bx, 0.0f, N, s);
-        fill(by, 0.0f, N, s);
-        fill(bz, 0.0f, N, s);
+        float *const f0[3] = {bx, by, bz};
+        init_faces<float>(f0, w.b0, ny, nx, nz, s);
         fill(dv, 0.0f, N, s);
-        k_init_binary<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Cs, Ct, u0, pt0, u, ps, pt, N);
+        k_init_binary<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Cs, Ct, u0, w.ps0, pt0, u, ps,
+                                                                   pt, N);
+        if (w.b0[0] || w.b0[1] || w.b0[2]) {
+            // dv of the resumed flows: the divergence the previous iteration left
+            kp_div<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(bx, by, bz, dv, G, is3d);
+        }
         HIPCHK(hipGetLastError());
         const unsigned nb = (unsigned)cdiv(N, kBlock);
         if (ev0) HIPCHK(hipEventRecord(ev0, s));
@@ -524,7 +570,8 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
         if (o_by) HIPCHK(hipMemcpyAsync(o_by, by, N * 4, hipMemcpyDeviceToDevice, s));
         if (o_bz) HIPCHK(hipMemcpyAsync(o_bz, bz, N * 4, hipMemcpyDeviceToDevice, s));
     } else {
-        FusedRun fr{is3d, Cs, Ct, alpha, ny, nx, nz, p, u0, pt0, u, cvg, it_num,
+        FusedRun fr{is3d, Cs, Ct, alpha, ny, nx, nz, p, u0, w.ps0, pt0,
+                    {w.b0[0], w.b0[1], w.b0[2]}, u, cvg, it_num,
                     o_ps, o_pt, o_bx, o_by, o_bz, tc, ntile_y, ntile_x, nzc, nblocks,
                     grid_fused, nparts, partials, flags, s, ev0, ev1,
                     o ? o->poll_every : 0};
@@ -907,8 +954,8 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
             const BatchVol<T> &b = hv[v];
             const int64_t N = (int64_t)b.ny * b.nx * b.nz;
             const unsigned nb = (unsigned)cdiv(N, kBlock);
-            k_init_binary_t<T><<<nb, kBlock, 0, s>>>(b.Cs, b.Ct, nullptr, nullptr, b.u[0], b.ps[0],
-                                                    b.pt[0], N);
+            k_init_binary_t<T><<<nb, kBlock, 0, s>>>(b.Cs, b.Ct, nullptr, nullptr, nullptr,
+                                                    b.u[0], b.ps[0], b.pt[0], N);
             for (int k = 0; k < 2; ++k)
                 for (T *f : {b.bx[k], b.by[k], b.bz[k]}) k_fill_t<T><<<nb, kBlock, 0, s>>>(f, 0.0f, N);
             fill(vols[v].cvg, 0.0f, max_iter, s);
@@ -1112,9 +1159,11 @@ asets_status binary_host(int ndim, const float *Cs, const float *Ct, const float
     HostIo io;
     st = stage(u0 ? 5 : 3, in, N, N, p->max_iter, io, s);
     if (st) return st;
-    st = binary_dev(ndim, io.d_in[0], io.d_in[1], io.d_in[2], ny, nx, nz, p, io.d_in[3], io.d_in[4],
-                    io.d_u, io.d_cvg, io.d_it, nullptr, nullptr, nullptr, nullptr, nullptr,
-                    nullptr, o);
+    WarmIn w;
+    w.u0 = io.d_in[3];
+    w.pt0 = io.d_in[4];
+    st = binary_dev(ndim, io.d_in[0], io.d_in[1], io.d_in[2], ny, nx, nz, p, w, io.d_u,
+                    io.d_cvg, io.d_it, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, o);
     if (st) return st;
     st = unstage(io, N, p->max_iter, u, cvg, it_num, s);
     if (st) return st;
@@ -1237,8 +1286,11 @@ asets_status asets_binary_dev(int ndim, const float *Cs, const float *Ct, const 
                               int64_t ny, int64_t nx, int64_t nz, const asets_params *p,
                               const float *u0, const float *pt0, float *u, float *cvg,
                               int32_t *it_num, double *seconds, const asets_opts *o) {
-    return binary_dev(ndim, Cs, Ct, alpha, ny, nx, nz, p, u0, pt0, u, cvg, it_num, seconds,
-                      nullptr, nullptr, nullptr, nullptr, nullptr, o);
+    WarmIn w;
+    w.u0 = u0;
+    w.pt0 = pt0;
+    return binary_dev(ndim, Cs, Ct, alpha, ny, nx, nz, p, w, u, cvg, it_num, seconds, nullptr,
+                      nullptr, nullptr, nullptr, nullptr, o);
 }
 
 asets_status asets_binary_dev_state(int ndim, const float *Cs, const float *Ct,
@@ -1247,8 +1299,31 @@ asets_status asets_binary_dev_state(int ndim, const float *Cs, const float *Ct,
                                     float *u, float *cvg, int32_t *it_num, double *seconds,
                                     float *ps, float *pt, float *bx, float *by, float *bz,
                                     const asets_opts *o) {
-    return binary_dev(ndim, Cs, Ct, alpha, ny, nx, nz, p, u0, pt0, u, cvg, it_num, seconds, ps,
-                      pt, bx, by, bz, o);
+    WarmIn w;
+    w.u0 = u0;
+    w.pt0 = pt0;
+    return binary_dev(ndim, Cs, Ct, alpha, ny, nx, nz, p, w, u, cvg, it_num, seconds, ps, pt, bx,
+                      by, bz, o);
+}
+
+asets_status asets_binary_dev_resume(int ndim, const float *Cs, const float *Ct,
+                                     const float *alpha, int64_t ny, int64_t nx, int64_t nz,
+                                     const asets_params *p, const float *u0, const float *ps0,
+                                     const float *pt0, const float *bx0, const float *by0,
+                                     const float *bz0, float *u, float *cvg, int32_t *it_num,
+                                     double *seconds, float *ps, float *pt, float *bx, float *by,
+                                     float *bz, const asets_opts *o) {
+    if (!u0 || !ps0 || !pt0 || !bx0 || !by0 || (ndim == 3 && !bz0))
+        return fail(ASETS_EBADARG, "resume needs u0, ps0, pt0 and the flow faces");
+    WarmIn w;
+    w.u0 = u0;
+    w.ps0 = ps0;
+    w.pt0 = pt0;
+    w.b0[0] = bx0;
+    w.b0[1] = by0;
+    w.b0[2] = ndim == 3 ? bz0 : nullptr;
+    return binary_dev(ndim, Cs, Ct, alpha, ny, nx, nz, p, w, u, cvg, it_num, seconds, ps, pt, bx,
+                      by, bz, o);
 }
 
 asets_status asets_potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny,

@@ -51,14 +51,16 @@ __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
     if (i < n) p[i] = v;
 }
 
-// asetsBinaryMF3D_mex.c:205-224; warm start (u0, pt0) has no reference counterpart.
+// asetsBinaryMF3D_mex.c:205-224; warm start (u0, pt0 and optionally ps0) has no
+// reference counterpart.
 __global__ void k_init_binary(const float *Cs, const float *Ct, const float *u0,
-                              const float *pt0, float *u, float *ps, float *pt, int64_t N) {
+                              const float *ps0, const float *pt0, float *u, float *ps,
+                              float *pt, int64_t N) {
     int64_t g = gid();
     if (g >= N) return;
     if (u0) {
         u[g] = u0[g];
-        ps[g] = pt0[g];
+        ps[g] = ps0 ? ps0[g] : pt0[g];
         pt[g] = pt0[g];
     } else {
         float cs = Cs[g], ct = Ct[g];
@@ -154,6 +156,16 @@ __device__ __forceinline__ float divergence(const float *bx, const float *by, co
         d = (d + zr) - bz[gl];
     }
     return d;
+}
+
+// dv = div b of a given flow field (the dv a resumed solve starts from; the same
+// expression as the update below, asetsBinaryMF3D_mex.c:414-415)
+__global__ void kp_div(const float *bx, const float *by, const float *bz, float *dv, Geo G,
+                       bool is3d) {
+    int64_t g = gid();
+    if (g >= G.N) return;
+    Coord c = decode(g, G);
+    dv[g] = is3d ? divergence<true>(bx, by, bz, g, c, G) : divergence<false>(bx, by, bz, g, c, G);
 }
 
 // divergence, ps, pt, u and |e| (asetsBinaryMF3D_mex.c:414-432)

@@ -1,0 +1,136 @@
+"""Evaluate the learned warm start against the cold start on 256^3 binary volumes.
+
+    python -m asetsmatlabmaxflow_amd.eval_warmstart --weights w.npz --seeds 5000 5001
+
+For each volume (workloads.binary3d_ellipsoids, the given seeds), with inputs resident on
+the GPU:
+  reference  plain solver, --ref-iters fixed iterations: E* and the reference labels
+  cold       plain solver from the initial cut to errBound (the reference's rule)
+  warm       network inference + plain solver from (u0, pt0) to errBound
+  ideal      plain solver from the reference's own (u, pt) to errBound: the bound on what
+             any (u0, pt0) predictor can gain (the flows restart from 0 either way)
+Reported per run: wall seconds (warm includes inference), iterations, relative energy gap
+(E - E*) / |E*| and label agreement with the reference; for the network output alone its
+energy gap and label agreement.  Also the iterations each start needs to reach a relative
+energy gap of 1e-5 (searched over a doubling schedule of fixed iteration counts).
+One JSON line per volume and a summary line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import numpy as np
+
+
+def _sync():
+    import torch
+
+    torch.cuda.synchronize()
+
+
+def evaluate_volume(net, seed: int, size: int, err_bound: float, ref_iters: int,
+                    gap_target: float = 1e-5):
+    import torch
+
+    from asetsmatlabmaxflow_amd import solver, workloads
+
+    shape = (size, size, size)
+    Cs, Ct, a, _ = workloads.binary3d_ellipsoids(shape, seed=seed)
+    d = [solver.to_device(x) for x in (Cs, Ct, a)]
+    cc, st = workloads.BINARY3D_CC, workloads.BINARY3D_STEPS
+    ref = solver.binary(*d, shape, ref_iters, 0.0, cc, st, want_state=True)
+    Estar = solver.energy_binary(*d, ref.u, shape)[0]
+    lab_ref = ref.u > 0.5
+
+    def quality(u):
+        e = solver.energy_binary(*d, u, shape)[0]
+        return {"gap": (e - Estar) / abs(Estar),
+                "label_agreement": float(((u > 0.5) == lab_ref).float().mean())}
+
+    def timed(fn):
+        _sync()
+        t = time.perf_counter()
+        r = fn()
+        _sync()
+        return r, time.perf_counter() - t
+
+    out = {"seed": seed, "shape": list(shape), "E_star": Estar, "ref_iters": ref_iters}
+    # warm-up of both paths (kernel loading, workspace allocation)
+    solver.binary(*d, shape, 3, 0.0, cc, st)
+    if net is not None:
+        net.forward(d[0], d[1], d[2], shape, 1)
+    r, s = timed(lambda: solver.binary(*d, shape, 20000, err_bound, cc, st))
+    out["cold"] = {"seconds": s, "iterations": r.iterations, **quality(r.u)}
+    u_id, pt_id = ref.u.clone(), ref.state["pt"].clone()
+    r, s = timed(lambda: solver.binary(*d, shape, 20000, err_bound, cc, st, u0=u_id, pt0=pt_id))
+    out["ideal_warm"] = {"seconds": s, "iterations": r.iterations, **quality(r.u)}
+    if net is not None:
+        (u0, pt0), s_inf = timed(lambda: net.forward(d[0], d[1], d[2], shape, 1))
+        out["network_output"] = {"inference_seconds": s_inf, **quality(u0)}
+
+        def warm():
+            w0, w1 = net.forward(d[0], d[1], d[2], shape, 1)
+            return solver.binary(*d, shape, 20000, err_bound, cc, st, u0=w0, pt0=w1)
+
+        r, s = timed(warm)
+        out["warm"] = {"seconds": s, "iterations": r.iterations, **quality(r.u)}
+
+    # iterations to a relative energy gap of gap_target (doubling schedule)
+    def iters_to_gap(**kw):
+        k = 8
+        while k <= ref_iters // 2:
+            r = solver.binary(*d, shape, k, 0.0, cc, st, **kw)
+            if quality(r.u)["gap"] <= gap_target:
+                return k
+            k *= 2
+        return None
+
+    out[f"iters_to_gap_{gap_target:g}"] = {"cold": iters_to_gap(),
+                                           "ideal_warm": iters_to_gap(u0=u_id, pt0=pt_id)}
+    if net is not None:
+        u0, pt0 = net.forward(d[0], d[1], d[2], shape, 1)
+        out[f"iters_to_gap_{gap_target:g}"]["warm"] = iters_to_gap(u0=u0.clone(),
+                                                                   pt0=pt0.clone())
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--weights", default=None)
+    ap.add_argument("--seeds", type=int, nargs="+", default=[5000, 5001])
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--err-bound", type=float, default=1e-4)
+    ap.add_argument("--ref-iters", type=int, default=2000)
+    args = ap.parse_args()
+
+    import torch
+
+    from asetsmatlabmaxflow_amd.warmstart import WarmStartNet
+
+    net = None
+    if args.weights:
+        net = WarmStartNet(torch.device("cuda:0"))
+        net.load(args.weights)
+    rows = []
+    for seed in args.seeds:
+        row = evaluate_volume(net, seed, args.size, args.err_bound, args.ref_iters)
+        rows.append(row)
+        print(json.dumps(row), flush=True)
+    keys = ["cold", "ideal_warm"] + (["warm"] if net is not None else [])
+    summary = {k: {"mean_seconds": float(np.mean([r[k]["seconds"] for r in rows])),
+                   "mean_iterations": float(np.mean([r[k]["iterations"] for r in rows])),
+                   "mean_gap": float(np.mean([r[k]["gap"] for r in rows])),
+                   "mean_label_agreement": float(np.mean([r[k]["label_agreement"]
+                                                          for r in rows]))}
+               for k in keys}
+    if net is not None:
+        summary["speedup_warm_vs_cold"] = summary["cold"]["mean_seconds"] / \
+            summary["warm"]["mean_seconds"]
+    print(json.dumps({"summary": summary, "volumes": len(rows), "size": args.size,
+                      "err_bound": args.err_bound}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

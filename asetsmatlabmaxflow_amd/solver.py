@@ -109,6 +109,39 @@ def binary(Cs, Ct, alpha, shape, max_iter: int, err_bound: float, cc: float, ste
     return r
 
 
+def binary_resume(Cs, Ct, alpha, shape, state: dict, max_iter: int, err_bound: float,
+                  cc: float, steps: float, *, kernel: str = "auto", flow: str = "f32",
+                  tile_x: int = 0, zchunk: int = 0, poll_every: int = 0, stream=None,
+                  want_state: bool = False, time_kernel: bool = False, out=None) -> Result:
+    """Continue a binary solve from a full state {u, ps, pt, bx, by, bz} (device tensors,
+    the convention of Result.state); k iterations then a resume of m equal k+m iterations."""
+    ndim, ny, nx, nz = _ndim_dims(shape)
+    N = ny * nx * nz
+    dev = Cs.device
+    u = out if out is not None else torch.empty(N, dtype=torch.float32, device=dev)
+    cvg = torch.empty(max(max_iter, 1), dtype=torch.float32, device=dev)
+    itn = torch.empty(1, dtype=torch.int32, device=dev)
+    secs = C.c_double(0.0)
+    kms = C.c_double(0.0) if time_kernel else None
+    opts = nat.make_opts(_stream_handle(stream), _FLOWS[flow], _KERNELS[kernel], poll_every,
+                         tile_x, zchunk, kms)
+    p = nat.make_params(max_iter, err_bound, cc, steps)
+    st = {k: torch.empty(N, dtype=torch.float32, device=dev)
+          for k in ("ps", "pt", "bx", "by", "bz")} if want_state else {}
+    rc = nat.lib().asets_binary_dev_resume(
+        ndim, _ptr(Cs), _ptr(Ct), _ptr(alpha), ny, nx, nz, C.byref(p), _ptr(state["u"]),
+        _ptr(state["ps"]), _ptr(state["pt"]), _ptr(state["bx"]), _ptr(state["by"]),
+        _ptr(state.get("bz") if ndim == 3 else None), _ptr(u), _ptr(cvg), itn.data_ptr(),
+        C.byref(secs), *(_ptr(st.get(k)) for k in ("ps", "pt", "bx", "by", "bz")),
+        C.byref(opts))
+    nat.check(rc)
+    it = int(itn.item())
+    r = Result(u=u, cvg=cvg[:max_iter], it_num=it, seconds=secs.value,
+               kernel_ms=kms.value if kms is not None else None, state=st)
+    r._executed = it + 1 if it < max_iter else max_iter
+    return r
+
+
 def potts(Ct, alpha, shape, nlab: int, max_iter: int, err_bound: float, cc: float,
           steps: float, *, kernel: str = "auto", poll_every: int = 0, stream=None,
           time_kernel: bool = False) -> Result:

@@ -139,6 +139,22 @@ asets_status asets_binary_dev_state(int ndim, const float *Cs, const float *Ct,
                                     float *ps, float *pt, float *bx, float *by, float *bz,
                                     const asets_opts *o);
 
+/* Resume a binary solve from a full iteration state (no reference counterpart: the
+ * reference always starts from the initial cut, asetsBinaryMF3D_mex.c:205-224).  The
+ * state is (u, ps, pt, bx, by, bz) exactly as asets_binary_dev_state returns it:
+ * bx[g] is the flow on the face between voxels x-1 and x (by: y-1|y, bz: z-1|z);
+ * the faces with axis index 0 are the zero-flux boundary and are taken as 0 whatever
+ * the input holds.  A solve of k iterations followed by a resume of m iterations from
+ * its state gives the same u (bit for bit) as one solve of k+m iterations.  bz0 is
+ * ignored (may be NULL) when ndim == 2; ps..bz out may be NULL. */
+asets_status asets_binary_dev_resume(int ndim, const float *Cs, const float *Ct,
+                                     const float *alpha, int64_t ny, int64_t nx, int64_t nz,
+                                     const asets_params *p, const float *u0, const float *ps0,
+                                     const float *pt0, const float *bx0, const float *by0,
+                                     const float *bz0, float *u, float *cvg, int32_t *it_num,
+                                     double *seconds, float *ps, float *pt, float *bx, float *by,
+                                     float *bz, const asets_opts *o);
+
 /* Potts solve on device memory; ndim 2 (nz == 1) or 3. */
 asets_status asets_potts_dev(int ndim, const float *Ct, const float *alpha,
                              int64_t ny, int64_t nx, int64_t nz, int64_t nlab,
