@@ -8,7 +8,8 @@
  * Two instantiations of oracle_impl.h:
  *   orc_*_f32 : single precision in the reference's operation order
  *               (maxflow/asetsBinaryMF2D_mex.c, asetsBinaryMF3D_mex.c,
- *               asetsPotts2D_mex.c, asetsPotts3D_mex.c).  Built with
+ *               asetsPotts2D_mex.c, asetsPotts3D_mex.c, asetsIshikawa2D_mex.c,
+ *               asetsIshikawa3D_mex.c).  Built with
  *               -ffp-contract=off so no multiply-add is fused, as in the
  *               reference's x86-64 SSE build.
  *   orc_*_f64 : the same algorithm in double precision.
@@ -94,6 +95,26 @@ EXPORT int orc_potts_f64(const double *Ct, const double *alpha,
 {
     return potts_f64(Ct, alpha, ny, nx, nz, L, is3d, max_it, err_bound, cc, steps,
                      stop_rule, u, cvg_ref, cvg64, it_num, ps, pt, bx, by, bz);
+}
+
+EXPORT int orc_ishikawa_f32(const float *Ct, const float *alpha,
+                            int64_t ny, int64_t nx, int64_t nz, int64_t L, int is3d,
+                            int max_it, float err_bound, float cc, float steps, int stop_rule,
+                            float *u, float *cvg_ref, double *cvg64, int32_t *it_num,
+                            float *pt, float *bx, float *by, float *bz)
+{
+    return ishikawa_f32(Ct, alpha, ny, nx, nz, L, is3d, max_it, err_bound, cc, steps,
+                        stop_rule, u, cvg_ref, cvg64, it_num, pt, bx, by, bz);
+}
+
+EXPORT int orc_ishikawa_f64(const double *Ct, const double *alpha,
+                            int64_t ny, int64_t nx, int64_t nz, int64_t L, int is3d,
+                            int max_it, double err_bound, double cc, double steps, int stop_rule,
+                            double *u, float *cvg_ref, double *cvg64, int32_t *it_num,
+                            double *pt, double *bx, double *by, double *bz)
+{
+    return ishikawa_f64(Ct, alpha, ny, nx, nz, L, is3d, max_it, err_bound, cc, steps,
+                        stop_rule, u, cvg_ref, cvg64, it_num, pt, bx, by, bz);
 }
 
 /* ---- energy evaluator (FP64) ------------------------------------------------

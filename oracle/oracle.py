@@ -45,6 +45,11 @@ def lib():
             f.argtypes = [P, P, I64, I64, I64, I64, C.c_int, C.c_int, real, real, real, C.c_int,
                           P, P, P, P, P, P, P, P, P]
             f.restype = C.c_int
+        for name, real in (("orc_ishikawa_f32", C.c_float), ("orc_ishikawa_f64", C.c_double)):
+            f = getattr(L, name)
+            f.argtypes = [P, P, I64, I64, I64, I64, C.c_int, C.c_int, real, real, real, C.c_int,
+                          P, P, P, P, P, P, P, P]
+            f.restype = C.c_int
         L.orc_energy_binary.argtypes = [P, P, P, I64, I64, I64, C.c_int, P, P]
         L.orc_energy_binary.restype = C.c_int
         L.orc_energy_potts.argtypes = [P, P, I64, I64, I64, I64, C.c_int, P, P]
@@ -131,6 +136,35 @@ def potts(Ct, alpha, shape, nlab, max_iter, err_bound, cc, steps, *, precision="
     rc = f(Ct.ctypes.data, alpha.ctypes.data, ny, nx, nz, nlab, int(d3), int(max_iter),
            err_bound, cc, steps, stop_rule, u.ctypes.data, cr.ctypes.data, c64.ctypes.data,
            C.byref(itn), *(_p(st.get(k)) for k in ("ps", "pt", "bx", "by", "bz")))
+    if rc != ORC_OK:
+        raise OracleError(rc)
+    return OracleResult(u, cr[:max_iter], c64[:max_iter], itn.value, st)
+
+
+def ishikawa(Ct, alpha, shape, nlab, max_iter, err_bound, cc, steps, *, precision="f32",
+             stop_rule=0, want_state=False, is3d=None) -> OracleResult:
+    """Ishikawa model: Ct has nlab label fields, alpha (at least) nlab-1; u has nlab-1.
+    2D stops on cvg < errBound, 3D on <= (as the two reference files)."""
+    dt = np.float32 if precision == "f32" else np.float64
+    ny, nx, nz, d3 = _dims(shape, is3d)
+    N = ny * nx * nz
+    M = nlab - 1
+    Ct = _flat(Ct, dt)
+    alpha = _flat(alpha, dt)
+    if alpha.size < N * M or Ct.size != N * nlab:
+        raise ValueError("Ct needs nlab fields and alpha at least nlab-1")
+    u = np.zeros(N * M, dtype=dt)
+    cr = np.zeros(max(max_iter, 1), dtype=np.float32)
+    c64 = np.zeros(max(max_iter, 1), dtype=np.float64)
+    itn = C.c_int32(0)
+    st = {}
+    if want_state:
+        st = {"pt": np.zeros(N * nlab, dtype=dt)}
+        st.update({k: np.zeros(N * M, dtype=dt) for k in ("bx", "by", "bz")})
+    f = lib().orc_ishikawa_f32 if dt == np.float32 else lib().orc_ishikawa_f64
+    rc = f(Ct.ctypes.data, alpha.ctypes.data, ny, nx, nz, nlab, int(d3), int(max_iter),
+           err_bound, cc, steps, stop_rule, u.ctypes.data, cr.ctypes.data, c64.ctypes.data,
+           C.byref(itn), *(_p(st.get(k)) for k in ("pt", "bx", "by", "bz")))
     if rc != ORC_OK:
         raise OracleError(rc)
     return OracleResult(u, cr[:max_iter], c64[:max_iter], itn.value, st)

@@ -335,3 +335,172 @@ static int SFX(potts)(const REAL *Ct, const REAL *alpha,
     free(bx); free(by); free(bz); free(dv); free(pt); free(gk); free(ps); free(ft);
     return ORC_OK;
 }
+
+/* ---- Ishikawa model (linearly ordered labels), 2D and 3D ------------------
+ * Follows asetsIshikawa3D_mex.c:141-228 (runMaxFlow), :230-270 (init) and the
+ * passes :272-610; asetsIshikawa2D_mex.c:135-481 for 2D (4-face forms, stop rule
+ * cvg < errBound instead of <=, :205).  L labels of Ct; L-1 cut functions u_l,
+ * flow fields and alpha_l (alpha holds at least L-1 fields).  pt has L fields;
+ * the top one (l = L-1) starts at 0 (init never writes it, :253-265).
+ */
+static int SFX(ishikawa)(const REAL *Ct, const REAL *alpha,
+                         int64_t ny, int64_t nx, int64_t nz, int64_t L, int is3d,
+                         int max_it, REAL err_bound, REAL cc, REAL steps, int stop_rule,
+                         REAL *u, float *cvg_ref, double *cvg64, int32_t *it_num,
+                         REAL *out_pt, REAL *out_bx, REAL *out_by, REAL *out_bz)
+{
+    const int64_t N = nx * ny * nz;
+    const int64_t M = L - 1;            /* cut functions / flow fields */
+    const int64_t sx = ny, sz = nx * ny;
+    int64_t g, x, y, z, l;
+    int it;
+    REAL *bx, *by, *bz, *dv, *gk, *pt;
+
+    if (N <= 0 || L < 2) return ORC_EBADARG;
+    if (!is3d && nz != 1) return ORC_EBADARG;
+    if (max_it > 0)
+        for (g = 0; g < N * M; ++g)
+            if (alpha[g] <= 0) return ORC_EALPHA;   /* asetsIshikawa3D_mex.c:377-379 */
+
+    bx = (REAL *)calloc((size_t)(N * M), sizeof(REAL));
+    by = (REAL *)calloc((size_t)(N * M), sizeof(REAL));
+    bz = (REAL *)calloc((size_t)(N * M), sizeof(REAL));
+    dv = (REAL *)calloc((size_t)(N * M), sizeof(REAL));
+    gk = (REAL *)calloc((size_t)N, sizeof(REAL));
+    pt = (REAL *)calloc((size_t)(N * L), sizeof(REAL));
+    if (!bx || !by || !bz || !dv || !gk || !pt) {
+        free(bx); free(by); free(bz); free(dv); free(gk); free(pt);
+        return ORC_ENOMEM;
+    }
+
+    /* init (:230-270): first strict arg-min of Ct from 1e30; pt_l = min for
+       l < L-1; u_l = 1 below the arg-min label, 0 from it on */
+    for (g = 0; g < N; ++g) {
+        REAL mv = (REAL)1e30;
+        int64_t mi = 1000000000;
+        for (l = 0; l < L; ++l)
+            if (mv > Ct[g + l * N]) { mv = Ct[g + l * N]; mi = l; }
+        for (l = 0; l < M; ++l) {
+            pt[g + l * N] = mv;
+            u[g + l * N] = (l >= mi) ? (REAL)0 : (REAL)1;
+        }
+    }
+
+    for (it = 0; it < max_it; ++it) {
+        REAL err = 0;
+        double err_d = 0.0;
+        int64_t k;
+        for (k = 0; k < M; ++k) {
+            REAL *bxk = bx + k * N, *byk = by + k * N, *bzk = bz + k * N;
+            REAL *dvk = dv + k * N, *uk = u + k * N;
+            const REAL *ptk = pt + k * N, *ptk1 = pt + (k + 1) * N;
+            const REAL *ak = alpha + k * N;
+
+            for (g = 0; g < N; ++g)                                   /* :287-288 */
+                gk[g] = dvk[g] - ((ptk[g] - ptk1[g]) + uk[g] / cc);
+
+            for (z = 0; z < nz; ++z)                                  /* :296-359 */
+                for (x = 0; x < nx; ++x)
+                    for (y = 0; y < ny; ++y) {
+                        g = z * sz + x * sx + y;
+                        if (x > 0) bxk[g] = steps * (gk[g] - gk[g - sx]) + bxk[g];
+                        if (y > 0) byk[g] = steps * (gk[g] - gk[g - 1]) + byk[g];
+                        if (is3d && z > 0) bzk[g] = steps * (gk[g] - gk[g - sz]) + bzk[g];
+                    }
+
+            for (z = 0; z < nz; ++z)                                  /* :361-397 */
+                for (x = 0; x < nx; ++x)
+                    for (y = 0; y < ny; ++y) {
+                        double t, xr, yr;
+                        REAL n;
+                        g = z * sz + x * sx + y;
+                        xr = (x + 1 < nx) ? (double)bxk[g + sx] : 0.0;
+                        yr = (y + 1 < ny) ? (double)byk[g + 1] : 0.0;
+                        t = xr * xr + (double)bxk[g] * (double)bxk[g];
+                        t = t + yr * yr;
+                        t = t + (double)byk[g] * (double)byk[g];
+                        if (is3d) {
+                            double zr = (z + 1 < nz) ? (double)bzk[g + sz] : 0.0;
+                            t = t + zr * zr;
+                            t = t + (double)bzk[g] * (double)bzk[g];
+                        }
+                        n = (REAL)sqrt(t * 0.5);
+                        if (n > ak[g]) n = n / ak[g];
+                        else n = 1;
+                        gk[g] = (REAL)1 / n;
+                    }
+
+            for (z = 0; z < nz; ++z)                                  /* :399-468 */
+                for (x = 0; x < nx; ++x)
+                    for (y = 0; y < ny; ++y) {
+                        g = z * sz + x * sx + y;
+                        if (x > 0)
+                            bxk[g] = (REAL)((double)(gk[g] + gk[g - sx]) * 0.5 * (double)bxk[g]);
+                        if (y > 0)
+                            byk[g] = (REAL)(0.5 * (double)(gk[g - 1] + gk[g]) * (double)byk[g]);
+                        if (is3d && z > 0)
+                            bzk[g] = (REAL)(0.5 * (double)(gk[g - sz] + gk[g]) * (double)bzk[g]);
+                    }
+
+            for (z = 0; z < nz; ++z)                                  /* :470-496 */
+                for (x = 0; x < nx; ++x)
+                    for (y = 0; y < ny; ++y) {
+                        REAL d, xr, yr;
+                        g = z * sz + x * sx + y;
+                        xr = (x + 1 < nx) ? bxk[g + sx] : (REAL)0;
+                        yr = (y + 1 < ny) ? byk[g + 1] : (REAL)0;
+                        d = ((yr - byk[g]) + xr) - bxk[g];
+                        if (is3d) {
+                            REAL zr = (z + 1 < nz) ? bzk[g + sz] : (REAL)0;
+                            d = (d + zr) - bzk[g];
+                        }
+                        dvk[g] = d;
+                    }
+        }
+
+        /* sink flows, labels in order; layer l reads the already updated
+           pt_{l-1} and the previous pt_{l+1} (:189-201, :498-576) */
+        for (l = 0; l < L; ++l)
+            for (g = 0; g < N; ++g) {
+                const int64_t i = g + l * N;
+                REAL f;
+                if (l == 0) {
+                    f = ((dv[i] + pt[i + N]) - u[i] / cc) + (REAL)1 / cc;
+                } else if (l < L - 1) {
+                    f = (-dv[i - N] + pt[i - N]) + u[i - N] / cc;
+                    f += (dv[i] + pt[i + N]) - u[i] / cc;
+                    f /= (REAL)2;
+                } else {
+                    f = (-dv[i - N] + pt[i - N]) + u[i - N] / cc;
+                }
+                pt[i] = ORC_MIN(f, Ct[i]);
+            }
+
+        /* multipliers (updateU, :578-612): voxel-major, labels inner */
+        for (g = 0; g < N; ++g)
+            for (l = 0; l < M; ++l) {
+                const int64_t i = g + l * N;
+                const REAL e = cc * ((dv[i] + pt[i + N]) - pt[i]);
+                u[i] -= e;
+                err += ORC_ABS(e);
+                err_d += fabs((double)e);
+            }
+
+        cvg_ref[it] = (float)(err / (REAL)(N * M));
+        if (cvg64) cvg64[it] = err_d / (double)(N * M);
+        {
+            const float c = (stop_rule == 0) ? (float)(err / (REAL)(N * M))
+                                             : (float)(err_d / (double)(N * M));
+            /* 3D stops on <= (asetsIshikawa3D_mex.c:213), 2D on < (2D :205) */
+            if (is3d ? (c <= (float)err_bound) : (c < (float)err_bound)) break;
+        }
+    }
+    *it_num = it;
+
+    if (out_pt) memcpy(out_pt, pt, (size_t)(N * L) * sizeof(REAL));
+    if (out_bx) memcpy(out_bx, bx, (size_t)(N * M) * sizeof(REAL));
+    if (out_by) memcpy(out_by, by, (size_t)(N * M) * sizeof(REAL));
+    if (out_bz) memcpy(out_bz, bz, (size_t)(N * M) * sizeof(REAL));
+    free(bx); free(by); free(bz); free(dv); free(gk); free(pt);
+    return ORC_OK;
+}
