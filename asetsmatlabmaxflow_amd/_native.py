@@ -18,6 +18,7 @@ KERNEL_AUTO, KERNEL_FUSED, KERNEL_PASSES = 0, 1, 2
 # every extern "C" symbol declared in include/asets.h
 EXPORTED = (
     "asets_binary2d", "asets_binary3d", "asets_binary_warm", "asets_potts2d", "asets_potts3d",
+    "asets_ishikawa2d", "asets_ishikawa3d", "asets_ishikawa_dev",
     "asets_binary_dev", "asets_binary_dev_state", "asets_binary_dev_resume",
     "asets_potts_dev",
     "asets_binary3d_batched_dev", "asets_binary_energy_dev", "asets_potts_energy_dev",
@@ -80,6 +81,9 @@ def lib() -> C.CDLL:
         L.asets_binary_dev_resume.argtypes = [C.c_int, P, P, P, I64, I64, I64, PP, P, P, P, P,
                                               P, P, P, P, P, PD, P, P, P, P, P, PO]
         L.asets_potts_dev.argtypes = [C.c_int, P, P, I64, I64, I64, I64, PP, P, P, P, PD, PO]
+        L.asets_ishikawa2d.argtypes = [P, P, I64, I64, I64, PP, P, P, P, PD, PO]
+        L.asets_ishikawa3d.argtypes = [P, P, I64, I64, I64, I64, PP, P, P, P, PD, PO]
+        L.asets_ishikawa_dev.argtypes = [C.c_int, P, P, I64, I64, I64, I64, PP, P, P, P, PD, PO]
         L.asets_binary3d_batched_dev.argtypes = [C.c_int32, C.POINTER(Volume), PP, PD, PO]
         L.asets_binary_energy_dev.argtypes = [C.c_int, P, P, P, I64, I64, I64, P, PD, PO]
         L.asets_potts_energy_dev.argtypes = [C.c_int, P, P, I64, I64, I64, I64, P, PD, PO]

@@ -4,7 +4,7 @@ Products (next to this file):
   libasets_hip.so            C-ABI HIP library (include/asets.h), gfx950 code objects
   libasets_nn.so             warm-start network kernels (include/asets_nn.h)
   mex/libmexshim.so          project-written minimal mx/mex API (tests, Python driver)
-  mex/asets*_mex.so          the four mex-ABI host files, linked against the two above
+  mex/asets*_mex.so          the six mex-ABI host files, linked against the two above
 
 `python -m asetsmatlabmaxflow_amd.build` builds everything; `build_oracle()` builds the
 CPU checker under oracle/ (test infrastructure, not part of the product).
@@ -37,7 +37,7 @@ C_FLAGS = ["-O2", "-fPIC", "-std=c99", "-Wall", "-Wextra", "-Wno-unused-paramete
            "-Wno-unused-function"]
 
 MEX_SOURCES = ["asetsBinaryMF2D_mex", "asetsBinaryMF3D_mex", "asetsPotts2D_mex",
-               "asetsPotts3D_mex"]
+               "asetsPotts3D_mex", "asetsIshikawa2D_mex", "asetsIshikawa3D_mex"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:

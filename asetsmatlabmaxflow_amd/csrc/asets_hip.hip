@@ -438,6 +438,40 @@ void launch_potts_couple(int L, const float *ft, float *u, float *ps, int64_t N,
 #undef ASETS_COUPLE
 }
 
+template <bool IS3D, int TX, int NT>
+void launch_ishikawa_flow_t(const IshiArgs &p, int grid, hipStream_t s) {
+    constexpr int W = (NT >= 512) ? 4 : 3;
+    kf_ishikawa_flow<IS3D, TX, NT, W><<<grid, NT, 0, s>>>(p);
+}
+void launch_ishikawa_flow(bool is3d, int tx, const IshiArgs &p, int grid, hipStream_t s) {
+    if (is3d) {
+        if (tx == 8) launch_ishikawa_flow_t<true, 8, 256>(p, grid, s);
+        else if (tx == 32) launch_ishikawa_flow_t<true, 32, 1024>(p, grid, s);
+        else launch_ishikawa_flow_t<true, 16, 512>(p, grid, s);
+    } else {
+        if (tx == 8) launch_ishikawa_flow_t<false, 8, 256>(p, grid, s);
+        else if (tx == 32) launch_ishikawa_flow_t<false, 32, 1024>(p, grid, s);
+        else launch_ishikawa_flow_t<false, 16, 512>(p, grid, s);
+    }
+}
+void launch_ishikawa_couple(int L, const float *dv, float *pt, float *u, const float *Ct,
+                            int64_t N, float cc, int fast_cc, double *partials,
+                            const Flags *flags, hipStream_t s) {
+    const unsigned nb = (unsigned)cdiv(N, kBlock);
+#define ASETS_ICOUPLE(LT) \
+    k_ishikawa_couple<LT><<<nb, kBlock, 0, s>>>(dv, pt, u, Ct, N, L, cc, fast_cc, partials, flags)
+    switch (L) {
+    case 2: ASETS_ICOUPLE(2); break;
+    case 3: ASETS_ICOUPLE(3); break;
+    case 4: ASETS_ICOUPLE(4); break;
+    case 5: ASETS_ICOUPLE(5); break;
+    case 6: ASETS_ICOUPLE(6); break;
+    case 8: ASETS_ICOUPLE(8); break;
+    default: ASETS_ICOUPLE(0); break;
+    }
+#undef ASETS_ICOUPLE
+}
+
 // ---------------------------------------------------------------------------
 // binary model on device memory
 // ---------------------------------------------------------------------------
@@ -1107,19 +1141,20 @@ struct HostIo {
 };
 
 // Device staging for host arrays; a separate cache from the solver workspace.
-asets_status stage(int nin, const float *const *h_in, int64_t n_in, int64_t n_u,
+// n_in[i]: element count of input i.
+asets_status stage(int nin, const float *const *h_in, const int64_t *n_in, int64_t n_u,
                    int32_t max_iter, HostIo &io, hipStream_t s) {
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     DeviceCache &dc = g_cache[dev];
-    size_t need = nin * Carver::sz(n_in, 4) + Carver::sz(n_u, 4) +
-                  Carver::sz(std::max(max_iter, 1), 4) + Carver::sz(1, 4);
+    size_t need = Carver::sz(n_u, 4) + Carver::sz(std::max(max_iter, 1), 4) + Carver::sz(1, 4);
+    for (int i = 0; i < nin; ++i) need += Carver::sz(n_in[i], 4);
     asets_status st = ensure(&dc.io, &dc.io_bytes, need);
     if (st) return st;
     Carver cv(dc.io);
     for (int i = 0; i < nin; ++i) {
-        io.d_in[i] = cv.take<float>(n_in);
-        HIPCHK(hipMemcpyAsync(io.d_in[i], h_in[i], n_in * 4, hipMemcpyHostToDevice, s));
+        io.d_in[i] = cv.take<float>(n_in[i]);
+        HIPCHK(hipMemcpyAsync(io.d_in[i], h_in[i], n_in[i] * 4, hipMemcpyHostToDevice, s));
     }
     io.d_u = cv.take<float>(n_u);
     io.d_cvg = cv.take<float>(std::max(max_iter, 1));
@@ -1157,7 +1192,8 @@ asets_status binary_host(int ndim, const float *Cs, const float *Ct, const float
     HIPCHK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> io_lock(g_cache[dev].io_mu);
     HostIo io;
-    st = stage(u0 ? 5 : 3, in, N, N, p->max_iter, io, s);
+    const int64_t sizes[5] = {N, N, N, N, N};
+    st = stage(u0 ? 5 : 3, in, sizes, N, p->max_iter, io, s);
     if (st) return st;
     WarmIn w;
     w.u0 = io.d_in[3];
@@ -1188,7 +1224,8 @@ asets_status potts_host(int ndim, const float *Ct, const float *alpha, int64_t n
     HIPCHK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> io_lock(g_cache[dev].io_mu);
     HostIo io;
-    st = stage(2, in, NL, NL, p->max_iter, io, s);
+    const int64_t sizes[2] = {NL, NL};
+    st = stage(2, in, sizes, NL, p->max_iter, io, s);
     if (st) return st;
     st = potts_dev(ndim, io.d_in[0], io.d_in[1], ny, nx, nz, L, p, io.d_u, io.d_cvg, io.d_it,
                    nullptr, o);
@@ -1196,6 +1233,171 @@ asets_status potts_host(int ndim, const float *Ct, const float *alpha, int64_t n
     st = unstage(io, NL, p->max_iter, u, cvg, it_num, s);
     if (st) return st;
     if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return ASETS_OK;
+}
+
+asets_status ishikawa_dev(int ndim, const float *Ct, const float *alpha, int64_t ny,
+                          int64_t nx, int64_t nz, int64_t L, const asets_params *p, float *u,
+                          float *cvg, int32_t *it_num, double *seconds, const asets_opts *o);
+
+asets_status ishikawa_host(int ndim, const float *Ct, const float *alpha, int64_t ny,
+                           int64_t nx, int64_t nz, int64_t L, const asets_params *p, float *u,
+                           float *cvg, int32_t *it_num, double *seconds, const asets_opts *o) {
+    auto t0 = std::chrono::steady_clock::now();
+    asets_status st = check_params(p);
+    if (st) return st;
+    st = check_dims(ny, nx, nz, L);
+    if (st) return st;
+    if (L < 2) return fail(ASETS_EBADARG, "the Ishikawa model needs at least 2 labels");
+    if (!Ct || !alpha || !u || !it_num || (p->max_iter > 0 && !cvg))
+        return fail(ASETS_EBADARG, "null array argument");
+    const int64_t N = ny * nx * nz;
+    hipStream_t s = o ? (hipStream_t)o->stream : nullptr;
+    const float *in[2] = {Ct, alpha};
+    const int64_t sizes[2] = {N * L, N * (L - 1)};
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> io_lock(g_cache[dev].io_mu);
+    HostIo io;
+    st = stage(2, in, sizes, N * (L - 1), p->max_iter, io, s);
+    if (st) return st;
+    st = ishikawa_dev(ndim, io.d_in[0], io.d_in[1], ny, nx, nz, L, p, io.d_u, io.d_cvg, io.d_it,
+                      nullptr, o);
+    if (st) return st;
+    st = unstage(io, N * (L - 1), p->max_iter, u, cvg, it_num, s);
+    if (st) return st;
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return ASETS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ishikawa model (linearly ordered labels) on device memory
+// ---------------------------------------------------------------------------
+asets_status ishikawa_dev(int ndim, const float *Ct, const float *alpha, int64_t ny,
+                          int64_t nx, int64_t nz, int64_t L, const asets_params *p, float *u,
+                          float *cvg, int32_t *it_num, double *seconds, const asets_opts *o) {
+    auto t_start = std::chrono::steady_clock::now();
+    if (ndim != 2 && ndim != 3) return fail(ASETS_EBADARG, "ndim must be 2 or 3");
+    if (ndim == 2 && nz != 1) return fail(ASETS_EBADARG, "2D solve needs nz == 1");
+    asets_status st = check_params(p);
+    if (st) return st;
+    st = check_dims(ny, nx, nz, L);
+    if (st) return st;
+    if (L < 2) return fail(ASETS_EBADARG, "the Ishikawa model needs at least 2 labels");
+    if (!Ct || !alpha || !u || !it_num || (p->max_iter > 0 && !cvg))
+        return fail(ASETS_EBADARG, "null array argument");
+    if (o && o->flow_dtype != ASETS_FLOW_F32)
+        return fail(ASETS_EBADARG, "fp16 flow storage is available for the binary model only");
+    if (o && o->kernel == ASETS_KERNEL_PASSES)
+        return fail(ASETS_EBADARG, "the Ishikawa model runs on the fused kernel only");
+
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(ASETS_EDEVICE, "device id %d", dev);
+    DeviceCache &dc = g_cache[dev];
+    std::lock_guard<std::mutex> lock(dc.mu);
+    hipStream_t s = o ? (hipStream_t)o->stream : nullptr;
+    if (!dc.host_flag) HIPCHK(hipHostMalloc((void **)&dc.host_flag, 64));
+
+    const bool is3d = (ndim == 3);
+    const int64_t N = ny * nx * nz, M = L - 1, NM = N * M;
+    const int32_t max_iter = p->max_iter;
+    const int64_t nparts = cdiv(N, kBlock);
+    const size_t need = Carver::sz(1, sizeof(Flags)) + Carver::sz(nparts, sizeof(double)) +
+                        Carver::sz(1, sizeof(int32_t)) + Carver::sz(N * L, 4) +
+                        7 * Carver::sz(NM, 4);
+    st = ensure(&dc.dev, &dc.dev_bytes, need);
+    if (st) return st;
+    Carver cv(dc.dev);
+    Flags *flags = cv.take<Flags>(1);
+    double *partials = cv.take<double>(nparts);
+    int32_t *scratch_i32 = cv.take<int32_t>(1);
+    float *pt = cv.take<float>(N * L);
+    float *dv = cv.take<float>(NM);
+    float *bb[2][3];
+    for (int b = 0; b < 2; ++b)
+        for (int k = 0; k < 3; ++k) bb[b][k] = cv.take<float>(NM);
+
+    if (max_iter > 0) {
+        // only the first L-1 alpha fields are used (asetsIshikawa3D_mex.c:377-379)
+        st = alpha_ok(alpha, NM, s, dc, scratch_i32);
+        if (st) return st;
+    }
+    k_fill_i32<<<1, 64, 0, s>>>((int32_t *)flags, 0, sizeof(Flags) / 4);
+    k_fill_i32<<<1, 1, 0, s>>>(it_num, max_iter, 1);
+    fill(cvg, 0.0f, max_iter, s);
+    for (int b = 0; b < 2; ++b)
+        for (int k = 0; k < 3; ++k) fill(bb[b][k], 0.0f, NM, s);
+    k_init_ishikawa<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Ct, u, pt, N, (int)L);
+    HIPCHK(hipGetLastError());
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (o && o->kernel_ms) {
+        HIPCHK(hipEventCreate(&ev0));
+        HIPCHK(hipEventCreate(&ev1));
+    }
+    TileCfg tc{};
+    tc.tx = (o && o->tile_x > 0) ? o->tile_x : 16;
+    if (tc.tx != 8 && tc.tx != 32) tc.tx = 16;
+    const int64_t tiles = cdiv(ny, 64) * cdiv(nx, tc.tx) * M;
+    if (o && o->zchunk > 0) {
+        tc.zchunk = (int)std::min<int64_t>(o->zchunk, nz);
+    } else {
+        const int64_t nzc = std::max<int64_t>(1, (512 + tiles / 2) / tiles);
+        tc.zchunk = (int)std::min<int64_t>(nz, std::max<int64_t>(8, cdiv(nz, nzc)));
+    }
+    IshiArgs ia{};
+    ia.alpha = alpha;
+    ia.u = u;
+    ia.pt = pt;
+    ia.dv = dv;
+    ia.ny = (int)ny;
+    ia.nx = (int)nx;
+    ia.nz = (int)nz;
+    ia.nflow = (int)M;
+    ia.cc = p->cc;
+    ia.steps = p->steps;
+    ia.zchunk = tc.zchunk;
+    ia.ntile_y = (int)cdiv(ny, 64);
+    ia.ntile_x = (int)cdiv(nx, tc.tx);
+    ia.nzc = (int)cdiv(nz, tc.zchunk);
+    ia.nblocks = ia.ntile_y * ia.ntile_x * ia.nzc;
+    ia.fast_cc = (std::fabs(p->cc) >= 0x1p-16f && std::fabs(p->cc) <= 0x1p16f) ? 1 : 0;
+    ia.flags = flags;
+    const int grid = (int)cdiv((int64_t)ia.nblocks * M, 8) * 8;
+    auto iter = [&](int32_t it) -> asets_status {
+        for (int k = 0; k < 3; ++k) {
+            ia.b_in[k] = bb[it & 1][k];
+            ia.b_out[k] = bb[(it + 1) & 1][k];
+        }
+        launch_ishikawa_flow(is3d, tc.tx, ia, grid, s);
+        launch_ishikawa_couple((int)L, dv, pt, u, Ct, N, p->cc, ia.fast_cc, partials, flags, s);
+        // 3D stops on cvg <= errBound (asetsIshikawa3D_mex.c:213), 2D on < (2D :205)
+        if (is3d)
+            k_finalize<false><<<1, 1024, 0, s>>>(partials, nparts, (double)NM, it,
+                                                 p->err_bound, cvg, it_num, flags);
+        else
+            k_finalize<true><<<1, 1024, 0, s>>>(partials, nparts, (double)NM, it,
+                                                p->err_bound, cvg, it_num, flags);
+        return ASETS_OK;
+    };
+    if (ev0) HIPCHK(hipEventRecord(ev0, s));
+    st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc, flags, iter);
+    if (st) return st;
+    if (ev1) HIPCHK(hipEventRecord(ev1, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (ev0) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        int32_t itn = 0;
+        HIPCHK(hipMemcpy(&itn, it_num, sizeof(int32_t), hipMemcpyDeviceToHost));
+        const int32_t n_exec = (itn < max_iter) ? itn + 1 : max_iter;
+        *o->kernel_ms = n_exec > 0 ? (double)ms / n_exec : 0.0;
+        (void)hipEventDestroy(ev0);
+        (void)hipEventDestroy(ev1);
+    }
+    if (seconds)
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     return ASETS_OK;
 }
 
@@ -1280,6 +1482,25 @@ asets_status asets_potts3d(const float *Ct, const float *alpha, int64_t ny, int6
                            int64_t nz, int64_t nlab, const asets_params *p, float *u,
                            float *cvg, int32_t *it_num, double *seconds, const asets_opts *o) {
     return potts_host(3, Ct, alpha, ny, nx, nz, nlab, p, u, cvg, it_num, seconds, o);
+}
+
+asets_status asets_ishikawa2d(const float *Ct, const float *alpha, int64_t ny, int64_t nx,
+                              int64_t nlab, const asets_params *p, float *u, float *cvg,
+                              int32_t *it_num, double *seconds, const asets_opts *o) {
+    return ishikawa_host(2, Ct, alpha, ny, nx, 1, nlab, p, u, cvg, it_num, seconds, o);
+}
+
+asets_status asets_ishikawa3d(const float *Ct, const float *alpha, int64_t ny, int64_t nx,
+                              int64_t nz, int64_t nlab, const asets_params *p, float *u,
+                              float *cvg, int32_t *it_num, double *seconds, const asets_opts *o) {
+    return ishikawa_host(3, Ct, alpha, ny, nx, nz, nlab, p, u, cvg, it_num, seconds, o);
+}
+
+asets_status asets_ishikawa_dev(int ndim, const float *Ct, const float *alpha, int64_t ny,
+                                int64_t nx, int64_t nz, int64_t nlab, const asets_params *p,
+                                float *u, float *cvg, int32_t *it_num, double *seconds,
+                                const asets_opts *o) {
+    return ishikawa_dev(ndim, Ct, alpha, ny, nx, nz, nlab, p, u, cvg, it_num, seconds, o);
 }
 
 asets_status asets_binary_dev(int ndim, const float *Cs, const float *Ct, const float *alpha,

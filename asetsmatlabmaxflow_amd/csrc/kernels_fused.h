@@ -220,9 +220,14 @@ __device__ __forceinline__ float rescale(float g_hi, float g_lo, float b, bool t
 // and the output stage instead updates the sink flow pt_k (:457-466) and writes
 // ft_k = dv_k + pt_k (the first term of updatePSU, :498) to out.ps; u is only
 // read.  The coupling over labels runs afterwards (kp_potts_couple).
-template <bool IS3D, int TX, int NT, typename T, bool POTTS = false>
+// MODE 2 (Ishikawa, asetsIshikawa3D_mex.c:184-199, label k): in.ps holds pt_k and
+// in.pt holds pt_{k+1} (gk = dv - ((pt_k - pt_{k+1}) + u_k/cc), :287-288); the
+// output stage stores only the faces and dv_k (:470-496) to out.ps; the sink-flow
+// chain and the multipliers run afterwards (k_ishikawa_couple).
+template <bool IS3D, int TX, int NT, typename T, int MODE = 0>
 __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
     using IO = Io<T>;
+    constexpr bool POTTS = (MODE == 1), ISHI = (MODE == 2);
     constexpr int TY = 64;
     constexpr int W2 = TY + 4, H2 = TX + 4;   // gk / b1 region (tile + radius 2)
     constexpr int W1 = TY + 2, H1 = TX + 2;   // g region (tile + radius 1)
@@ -407,6 +412,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             float u[KO], pt[KO], cs[KO], ct[KO];
         };
         auto load_out = [&](int s, Out &o) {
+            if constexpr (ISHI) return;   // the Ishikawa output stage needs none of them
             const int sv = (s >= z0 && s < z1) ? s : -1;   // outside: empty descriptors
             const auto ru = plane_q(a.in.u, sv, nz, 4, plane);
             const auto rpt = plane_q(POTTS ? a.in.ps : a.in.pt, sv, nz, IO::kBytes, plane);
@@ -454,7 +460,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             const float *g0 = G[s & 1];
             const float *g1 = G[(s + 1) & 1];
             const int64_t base = (int64_t)s * plane;
-            const auto wu = plane_rsrc(POTTS ? a.in.u : a.out.u, base, 4, plane);
+            const auto wu = plane_rsrc((POTTS || ISHI) ? a.in.u : a.out.u, base, 4, plane);
             const auto wps = plane_rsrc(a.out.ps, base, IO::kBytes, plane);
             const auto wpt = plane_rsrc(a.out.pt, base, IO::kBytes, plane);
             const auto wbx = plane_rsrc(a.out.bx, base, IO::kBytes, plane);
@@ -489,6 +495,16 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     bzl = carry[k];
                     d = (d + bzr) - bzl;
                     carry[k] = bzr;
+                }
+                if constexpr (ISHI) {
+                    if (y < ny && x < nx) {
+                        const int off = x * ny + y;
+                        IO::store(wps, off, d);
+                        IO::store(wbx, off, bxl);
+                        IO::store(wby, off, byl);
+                        if (IS3D) IO::store(wbz, off, bzl);
+                    }
+                    continue;
                 }
                 const float uo = o.u[k];
                 const float uc = div_fast(uo, cc, rcc, fast_cc);
@@ -627,7 +643,7 @@ __global__ __launch_bounds__(NT, WAVES) void kf_potts_flow(PottsArgs p) {
     a.fast_cc = p.fast_cc;
     a.partials = nullptr;
     a.flags = nullptr;
-    (void)fused_tile<IS3D, TX, NT, float, true>(a, lb);
+    (void)fused_tile<IS3D, TX, NT, float, 1>(a, lb);
 }
 
 // Source flow and multipliers, labels coupled per voxel in label order
@@ -686,6 +702,173 @@ __global__ __launch_bounds__(256) void kp_potts_couple(const float *ft, float *u
     if (in) ps[gg] = nps;
     const double sum = block_sum(e_abs, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = sum;
+}
+
+// ---- Ishikawa model (linearly ordered labels) --------------------------------
+
+struct IshiArgs {
+    const float *alpha;        // [M*N], M = L-1
+    const float *u;            // [M*N] (read)
+    const float *pt;           // [L*N] (read)
+    float *b_in[3], *b_out[3]; // [M*N] faces x, y, z
+    float *dv;                 // [M*N] out
+    int ny, nx, nz, nflow;     // nflow = M
+    float cc, steps;
+    int zchunk, ntile_y, ntile_x, nzc;
+    int nblocks;               // per flow field
+    int fast_cc;
+    const Flags *flags;
+};
+
+// All M flow updates of one iteration in one launch (the label loop of
+// asetsIshikawa3D_mex.c:162-187 has no dependence between labels).
+template <bool IS3D, int TX, int NT, int WAVES>
+__global__ __launch_bounds__(NT, WAVES) void kf_ishikawa_flow(IshiArgs p) {
+    if (p.flags->done) return;
+    const int total = p.nblocks * p.nflow;
+    const int gb = xcd_remap(blockIdx.x, gridDim.x);
+    if (gb >= total) return;
+    const int l = gb / p.nblocks, lb = gb % p.nblocks;
+    const int64_t N = (int64_t)p.ny * p.nx * p.nz;
+    const int64_t off = (int64_t)l * N;
+    FusedArgs<float> a;
+    a.Cs = nullptr;
+    a.Ct = nullptr;
+    a.alpha = p.alpha + off;
+    a.in = FusedState<float>{(float *)p.u + off, (float *)p.pt + off, (float *)p.pt + off + N,
+                             p.b_in[0] + off, p.b_in[1] + off, p.b_in[2] + off};
+    a.out = FusedState<float>{nullptr, p.dv + off, nullptr, p.b_out[0] + off, p.b_out[1] + off,
+                              p.b_out[2] + off};
+    a.ny = p.ny;
+    a.nx = p.nx;
+    a.nz = p.nz;
+    a.plane = p.ny * p.nx;
+    a.cc = p.cc;
+    a.steps = p.steps;
+    a.zchunk = p.zchunk;
+    a.ntile_y = p.ntile_y;
+    a.ntile_x = p.ntile_x;
+    a.nzc = p.nzc;
+    a.nblocks = p.nblocks;
+    a.fast_cc = p.fast_cc;
+    a.partials = nullptr;
+    a.flags = nullptr;
+    (void)fused_tile<IS3D, TX, NT, float, 2>(a, lb);
+}
+
+// Sink flows (asetsIshikawa3D_mex.c:498-576: bottom, middle and top layer, in
+// label order, each middle layer reading the already updated layer below) and
+// multipliers (updateU, :578-612) per voxel; sum |e| in FP64 per workgroup.
+// LT > 0: compile-time label count (values stay in registers); LT == 0: any.
+template <int LT>
+__global__ __launch_bounds__(256) void k_ishikawa_couple(const float *dv, float *pt, float *u,
+                                                         const float *Ct, int64_t N,
+                                                         int nlab_rt, float cc, int fast_cc,
+                                                         double *partials,
+                                                         const Flags *flags) {
+    __shared__ double red[16];
+    if (flags->done) return;
+    const int L = LT > 0 ? LT : nlab_rt;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = g < N;
+    const int64_t gg = in ? g : 0;
+    const float rcc = rcp_refined(cc);
+    const bool fcc = fast_cc != 0;
+    const float one_cc = 1.0f / cc;
+    double e_abs = 0.0;
+    if constexpr (LT > 0) {
+        float d[LT - 1], uu[LT - 1], uc[LT - 1], q[LT];
+#pragma unroll
+        for (int l = 0; l < LT - 1; ++l) {
+            d[l] = dv[gg + l * N];
+            uu[l] = u[gg + l * N];
+            uc[l] = div_fast(uu[l], cc, rcc, fcc);
+        }
+#pragma unroll
+        for (int l = 0; l < LT; ++l) q[l] = pt[gg + l * N];
+#pragma unroll
+        for (int l = 0; l < LT; ++l) {
+            float f;
+            if (l == 0) {
+                f = ((d[0] + q[1]) - uc[0]) + one_cc;
+            } else if (l < LT - 1) {
+                f = (-d[l - 1] + q[l - 1]) + uc[l - 1];
+                f += (d[l] + q[l + 1]) - uc[l];
+                f /= 2.0f;
+            } else {
+                f = (-d[l - 1] + q[l - 1]) + uc[l - 1];
+            }
+            q[l] = fmin_ref(f, Ct[gg + l * N]);
+        }
+#pragma unroll
+        for (int l = 0; l < LT - 1; ++l) {
+            const float e = cc * ((d[l] + q[l + 1]) - q[l]);
+            if (in) {
+                u[gg + l * N] = uu[l] - e;
+                e_abs += fabs((double)e);
+            }
+        }
+        if (in)
+#pragma unroll
+            for (int l = 0; l < LT; ++l) pt[gg + l * N] = q[l];
+    } else {
+        // chain with the neighbours in registers: layer l needs the new pt_{l-1}
+        // and the old pt_{l+1}; pt is rewritten in place behind the chain
+        float qprev = 0.0f, dprev = 0.0f, ucprev = 0.0f;
+        float qcur = pt[gg];
+        for (int l = 0; l < L; ++l) {
+            const float qnext = (l + 1 < L) ? pt[gg + (l + 1) * N] : 0.0f;
+            float dl = 0.0f, ucl = 0.0f;
+            if (l < L - 1) {
+                dl = dv[gg + l * N];
+                ucl = div_fast(u[gg + l * N], cc, rcc, fcc);
+            }
+            float f;
+            if (l == 0) {
+                f = ((dl + qnext) - ucl) + one_cc;
+            } else if (l < L - 1) {
+                f = (-dprev + qprev) + ucprev;
+                f += (dl + qnext) - ucl;
+                f /= 2.0f;
+            } else {
+                f = (-dprev + qprev) + ucprev;
+            }
+            const float qn = fmin_ref(f, Ct[gg + l * N]);
+            if (in) pt[gg + l * N] = qn;
+            qprev = qn;
+            dprev = dl;
+            ucprev = ucl;
+            qcur = qnext;
+        }
+        (void)qcur;
+        for (int l = 0; l < L - 1; ++l) {
+            const float e = cc * ((dv[gg + l * N] + pt[gg + (l + 1) * N]) - pt[gg + l * N]);
+            if (in) {
+                u[gg + l * N] = u[gg + l * N] - e;
+                e_abs += fabs((double)e);
+            }
+        }
+    }
+    const double sum = block_sum(e_abs, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = sum;
+}
+
+// asetsIshikawa3D_mex.c:230-270: first strict arg-min of Ct from 1e30; pt_l =
+// min for l < L-1, the top layer 0; u_l = [l < arg-min]
+__global__ void k_init_ishikawa(const float *Ct, float *u, float *pt, int64_t N, int L) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    float mv = 1e30f;
+    int mi = 1000000000;
+    for (int l = 0; l < L; ++l) {
+        const float c = Ct[g + l * N];
+        if (mv > c) { mv = c; mi = l; }
+    }
+    for (int l = 0; l < L - 1; ++l) {
+        pt[g + l * N] = mv;
+        u[g + l * N] = (l >= mi) ? 0.0f : 1.0f;
+    }
+    pt[g + (int64_t)(L - 1) * N] = 0.0f;
 }
 
 }  // namespace asets

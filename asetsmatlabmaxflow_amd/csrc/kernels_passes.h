@@ -246,6 +246,8 @@ __global__ void kp_psu(const float *dv, const float *pt, float *u, float *ps, fl
 
 // Sum the per-block partials in a fixed order, store cvg[it] and raise the stop
 // flag when cvg <= err_bound (asetsBinaryMF3D_mex.c:183-188).
+// STRICT: stop on cvg < err_bound (asetsIshikawa2D_mex.c:205) instead of <=.
+template <bool STRICT = false>
 __global__ void k_finalize(const double *partials, int64_t nparts, double denom, int32_t it,
                            float err_bound, float *cvg, int32_t *it_num, Flags *f) {
     __shared__ double red[16];
@@ -256,7 +258,7 @@ __global__ void k_finalize(const double *partials, int64_t nparts, double denom,
     if (threadIdx.x == 0) {
         const float c = (float)(s / denom);
         cvg[it] = c;
-        if (c <= err_bound) {
+        if (STRICT ? (c < err_bound) : (c <= err_bound)) {
             f->done = 1;
             *it_num = it;
         }

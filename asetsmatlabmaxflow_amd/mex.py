@@ -165,3 +165,16 @@ def asetsPotts3D_mex(Ct, alpha, pars):
     """[u, erriter, numIter, time] = asetsPotts3D_mex(single(Ct), single(alpha),
     single([Ny; Nx; Nz; L; maxIt; errBound; cc; steps]))  (asetsPotts3D_mex.c:44-141)."""
     return tuple(call("asetsPotts3D_mex", _single(Ct), _single(alpha), _single(pars)))
+
+
+def asetsIshikawa2D_mex(Ct, alpha, pars):
+    """[u, erriter, numIter, time] = asetsIshikawa2D_mex(single(Ct), single(alpha),
+    single([Ny; Nx; L; maxIt; errBound; cc; steps]))  (asetsIshikawa2D_mex.c:40-133);
+    u has L-1 cut functions."""
+    return tuple(call("asetsIshikawa2D_mex", _single(Ct), _single(alpha), _single(pars)))
+
+
+def asetsIshikawa3D_mex(Ct, alpha, pars):
+    """[u, erriter, numIter, time] = asetsIshikawa3D_mex(single(Ct), single(alpha),
+    single([Ny; Nx; Nz; L; maxIt; errBound; cc; steps]))  (asetsIshikawa3D_mex.c:42-139)."""
+    return tuple(call("asetsIshikawa3D_mex", _single(Ct), _single(alpha), _single(pars)))

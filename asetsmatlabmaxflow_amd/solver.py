@@ -176,6 +176,36 @@ def potts(Ct, alpha, shape, nlab: int, max_iter: int, err_bound: float, cc: floa
     return r
 
 
+def ishikawa(Ct, alpha, shape, nlab: int, max_iter: int, err_bound: float, cc: float,
+             steps: float, *, tile_x: int = 0, zchunk: int = 0, poll_every: int = 0,
+             stream=None, time_kernel: bool = False) -> Result:
+    """Ishikawa continuous max-flow (asetsIshikawa2D/3D_mex semantics) on device tensors:
+    Ct holds nlab label fields, alpha nlab-1 (a longer alpha is cut); u has nlab-1."""
+    ndim, ny, nx, nz = _ndim_dims(shape)
+    N = ny * nx * nz
+    if Ct.numel() != N * nlab:
+        raise ValueError(f"Ct has {Ct.numel()} elements, shape implies {N * nlab}")
+    if alpha.numel() < N * (nlab - 1):
+        raise ValueError("alpha needs nlab-1 label fields")
+    dev = Ct.device
+    u = torch.empty(N * (nlab - 1), dtype=torch.float32, device=dev)
+    cvg = torch.empty(max(max_iter, 1), dtype=torch.float32, device=dev)
+    itn = torch.empty(1, dtype=torch.int32, device=dev)
+    secs = C.c_double(0.0)
+    kms = C.c_double(0.0) if time_kernel else None
+    opts = nat.make_opts(_stream_handle(stream), nat.FLOW_F32, nat.KERNEL_FUSED, poll_every,
+                         tile_x, zchunk, kms)
+    p = nat.make_params(max_iter, err_bound, cc, steps)
+    nat.check(nat.lib().asets_ishikawa_dev(ndim, _ptr(Ct), _ptr(alpha), ny, nx, nz, nlab,
+                                           C.byref(p), _ptr(u), _ptr(cvg), itn.data_ptr(),
+                                           C.byref(secs), C.byref(opts)))
+    it = int(itn.item())
+    r = Result(u=u, cvg=cvg[:max_iter], it_num=it, seconds=secs.value,
+               kernel_ms=kms.value if kms is not None else None)
+    r._executed = it + 1 if it < max_iter else max_iter
+    return r
+
+
 def binary_batched(volumes, max_iter: int, err_bound: float, cc: float, steps: float, *,
                    flow: str = "f32", tile_x: int = 0, zchunk: int = 0, stream=None,
                    time_kernel: bool = False):

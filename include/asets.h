@@ -116,6 +116,25 @@ asets_status asets_potts3d(const float *Ct, const float *alpha,
                            float *u, float *cvg, int32_t *it_num, double *seconds,
                            const asets_opts *o);
 
+/* asetsIshikawa2D_mex(Ct, alpha, [Ny Nx L maxIt errBound cc steps])
+ * replaces maxflow/asetsIshikawa2D_mex.c:40-133 (mexFunction) and :136-481.
+ * Ct has L label fields, alpha L-1 (asetsIshikawa2D_mex.c reads only those);
+ * u receives the L-1 cut functions [Ny Nx L-1].  Stops on cvg < errBound
+ * (asetsIshikawa2D_mex.c:205; the 3D solver uses <=). */
+asets_status asets_ishikawa2d(const float *Ct, const float *alpha,
+                              int64_t ny, int64_t nx, int64_t nlab, const asets_params *p,
+                              float *u, float *cvg, int32_t *it_num, double *seconds,
+                              const asets_opts *o);
+
+/* asetsIshikawa3D_mex(Ct, alpha, [Ny Nx Nz L maxIt errBound cc steps])
+ * replaces maxflow/asetsIshikawa3D_mex.c:42-139 (mexFunction) and :141-612;
+ * u receives [Ny Nx Nz L-1]; stops on cvg <= errBound (:213). */
+asets_status asets_ishikawa3d(const float *Ct, const float *alpha,
+                              int64_t ny, int64_t nx, int64_t nz, int64_t nlab,
+                              const asets_params *p,
+                              float *u, float *cvg, int32_t *it_num, double *seconds,
+                              const asets_opts *o);
+
 /* ---- device-pointer entry points (inputs already resident in HBM) -------- */
 
 /* Binary solve on device memory.  ndim is 2 (nz must be 1; the 2D operation
@@ -161,6 +180,13 @@ asets_status asets_potts_dev(int ndim, const float *Ct, const float *alpha,
                              const asets_params *p,
                              float *u, float *cvg, int32_t *it_num, double *seconds,
                              const asets_opts *o);
+
+/* Ishikawa solve on device memory; ndim 2 (nz == 1, stop on <) or 3 (<=). */
+asets_status asets_ishikawa_dev(int ndim, const float *Ct, const float *alpha,
+                                int64_t ny, int64_t nx, int64_t nz, int64_t nlab,
+                                const asets_params *p,
+                                float *u, float *cvg, int32_t *it_num, double *seconds,
+                                const asets_opts *o);
 
 /* ---- batched binary 3D: many volumes of uneven size, one launch per iteration
  * No reference counterpart (the reference solves one volume per call).  Each

@@ -58,7 +58,7 @@ def test_nn_bad_arguments_rejected_before_device_work():
 def test_mex_modules_export_mexFunction():
     for name in mex.MEX_DIR.glob("asets*_mex.so"):
         assert hasattr(C.CDLL(str(name)), "mexFunction"), name
-    assert len(list(mex.MEX_DIR.glob("asets*_mex.so"))) == 4
+    assert len(list(mex.MEX_DIR.glob("asets*_mex.so"))) == 6
     shim = C.CDLL(str(mex.MEX_DIR / "libmexshim.so"))
     for sym in ("mxCreateNumericArray", "mxGetData", "mxGetNumberOfElements", "mexErrMsgTxt",
                 "mexPrintf", "mexshim_call"):
