@@ -16,6 +16,10 @@ potts3d_voronoi    L labels from L uniform random seeds: Voronoi indicator per l
                    Gaussian-smoothed (sigma = 3 voxels), arg-max label map;
                    I ~ N(l/5, 0.08); Ct_l = (I - l/5)^2/(2*0.08^2); alpha = 0.8.
                    cc = 0.25, steps = 0.11.
+ishikawa3d_voronoi the Potts volume above (labels ordered by intensity l/5, so the
+                   linearly ordered model applies) with alpha_l = 0.3/(l+1) for the L-1 cut
+                   functions, as tests/runTestIshikawa.m:119-121; cc = 0.75, steps = 0.1
+                   (runTestIshikawa.m:127).
 """
 from __future__ import annotations
 
@@ -156,3 +160,15 @@ def random_smooth_potts(shape, nlab, seed, alpha_step=0.025):
         Ct[..., l] = gaussian_filter(rng.random(shape), sig)
         alpha[..., l] = alpha_step * (l + 1)
     return Ct, alpha
+
+
+def ishikawa3d_voronoi(shape=(192, 192, 192), nlab: int = 6, seed: int = 1003):
+    """Returns (Ct, alpha, labels): Ct (Ny, Nx, Nz, L), alpha (Ny, Nx, Nz, L-1)."""
+    Ct, _, lab = potts3d_voronoi(shape, nlab, seed)
+    alpha = np.empty(tuple(shape) + (nlab - 1,), dtype=np.float32, order="F")
+    for l in range(nlab - 1):
+        alpha[..., l] = np.float32(0.3 / (l + 1))
+    return Ct, alpha, lab
+
+
+ISHIKAWA3D_CC, ISHIKAWA3D_STEPS = 0.75, 0.1

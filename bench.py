@@ -6,8 +6,9 @@ NLL costs, contrast-sensitive alpha), cc = 0.35, steps = 0.11.  A step is one so
 fixed iterations (errBound 0) on inputs already resident in HBM; value = voxel-iterations
 per second over the K timed steps.  After the timed steps the same volume is solved to
 errBound 1e-4 (time-to-tolerance), with fp16 flow storage, and warm-started by the
-learned network when its weights are present.  The CPU baseline is the single-threaded
-FP32 reference-order oracle timed on a bounded number of iterations of the same volume.
+learned network when its weights are present (asetsmatlabmaxflow_amd/weights).  The CPU
+baseline is the single-threaded FP32 reference-order oracle timed on a bounded number of
+iterations of the same volume.
 
 Other workloads (one JSON line each, same keys):
   binary2d  256x256 ellipses (seed 1001), 300 fixed iterations, cc 0.3, steps 0.16
@@ -15,9 +16,11 @@ Other workloads (one JSON line each, same keys):
   batched   64 binary volumes with axes uniform in {64..192} (seeds 1004*1000+v), one
             graph-replayed persistent launch per iteration, to errBound 1e-4, fp32 and
             fp16 flow storage
+  ishikawa3d  192^3, L = 6 ordered labels (seed 1003), 300 fixed iterations, cc 0.75
 
 Algorithmic bytes per voxel-iteration: binary 3D 60 (fp32 flows) / 40 (fp16 flows),
-binary 2D 52, Potts 3D 48L+8.
+binary 2D 52, Potts 3D 48L+8, Ishikawa 3D 12L+36(L-1) (read Ct, alpha; read and write u,
+pt and the three face fields).
 """
 from __future__ import annotations
 
@@ -134,6 +137,42 @@ def bench_binary3d(args):
                               "fg_fraction": float(lab.float().mean().item()),
                               "energy": float(e[0])}
 
+    # learned warm start (network inference + solve from (u0, pt0)) vs the cold start,
+    # both to errBound 1e-4; energies against a 2000-iteration reference solve
+    wpath = ROOT / "asetsmatlabmaxflow_amd" / "weights" / "warmstart_unet.npz"
+    if not args.no_tol and not args.no_warm and wpath.exists() and n % 8 == 0:
+        from asetsmatlabmaxflow_amd.warmstart import WarmStartNet
+
+        net = WarmStartNet(dev)
+        net.load(wpath)
+        with torch.cuda.stream(stream):
+            ref = solver.binary(dCs, dCt, dA, shape, 2000, 0.0, cc, steps, stream=stream)
+            e_star = solver.energy_binary(dCs, dCt, dA, ref.u, shape, stream=stream)[0]
+            net.forward(dCs, dCt, dA, shape, 1)   # warm-up
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            u0, pt0 = net.forward(dCs, dCt, dA, shape, 1)
+            torch.cuda.synchronize()
+            t_inf = time.perf_counter() - t3
+            rw = solver.binary(dCs, dCt, dA, shape, 20000, 1e-4, cc, steps, u0=u0, pt0=pt0,
+                               stream=stream)
+            torch.cuda.synchronize()
+            t_warm = time.perf_counter() - t3
+            e_warm = solver.energy_binary(dCs, dCt, dA, rw.u, shape, stream=stream)[0]
+            e_net = solver.energy_binary(dCs, dCt, dA, u0, shape, stream=stream)[0]
+            lab_ref = ref.u > 0.5
+        out["warm_start"] = {
+            "err_bound": 1e-4, "inference_seconds": t_inf, "seconds": t_warm,
+            "iterations": rw.iterations, "cold_seconds": out["time_to_tol"]["seconds"],
+            "cold_iterations": out["time_to_tol"]["iterations"],
+            "speedup_vs_cold": out["time_to_tol"]["seconds"] / t_warm,
+            "energy_gap_warm": (e_warm - e_star) / abs(e_star),
+            "energy_gap_cold": (out["time_to_tol"]["energy"] - e_star) / abs(e_star),
+            "energy_gap_network_output": (e_net - e_star) / abs(e_star),
+            "label_agreement_warm": float(((rw.u > 0.5) == lab_ref).float().mean().item()),
+            "label_agreement_cold": float(((rt.u > 0.5) == lab_ref).float().mean().item()),
+            "weights": "asetsmatlabmaxflow_amd/weights/warmstart_unet.npz"}
+
     if not args.no_cpu:
         orc = _oracle()
         k = max(1, args.cpu_iters)
@@ -226,6 +265,46 @@ def bench_potts3d(args):
     return out
 
 
+def bench_ishikawa3d(args):
+    import numpy as np
+    import torch
+
+    from asetsmatlabmaxflow_amd import solver, workloads
+
+    shape, L, iters = (192, 192, 192), 6, 300
+    Ct, a, _ = workloads.ishikawa3d_voronoi(shape, nlab=L, seed=1003)
+    N = int(np.prod(shape))
+    d = [solver.to_device(x) for x in (Ct, a)]
+    cc, st = workloads.ISHIKAWA3D_CC, workloads.ISHIKAWA3D_STEPS
+    for _ in range(args.warmup):
+        solver.ishikawa(*d, shape, L, 5, 0.0, cc, st)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        r = solver.ishikawa(*d, shape, L, iters, 0.0, cc, st, time_kernel=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    out = _base(args, "voxel-iters/s, 192^3 Ishikawa L=6 (300 fixed iterations)",
+                N * iters * args.steps / wall, "voxel-iter/s", wall * 1e3 / args.steps, "f32",
+                "synthetic (workloads.ishikawa3d_voronoi seed 1003)",
+                {"workload": "ishikawa3d 192^3 L=6 ordered Voronoi labels, alpha 0.3/(l+1), "
+                             "cc 0.75 steps 0.1", "iters_per_step": iters})
+    out["roofline"] = _roofline(12 * L + 36 * (L - 1), N, r.kernel_ms)
+    lab = (r.u.view(L - 1, -1) > 0.5).sum(0)
+    out["label_fractions"] = [float((lab == l).float().mean()) for l in range(L)]
+    if not args.no_cpu:
+        orc = _oracle()
+        k = max(1, args.cpu_iters // 4)
+        t = time.perf_counter()
+        orc.ishikawa(Ct, a, shape, L, k, 0.0, cc, st)
+        cpu_s = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": N * k / cpu_s, "unit": "voxel-iter/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{k} iterations incl. init, FP32 reference-order "
+                                         f"oracle, {cpu_s:.1f} s"}
+    return out
+
+
 def bench_batched(args):
     import numpy as np
     import torch
@@ -283,7 +362,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="binary3d",
-                    choices=["binary3d", "binary2d", "potts3d", "batched"])
+                    choices=["binary3d", "binary2d", "potts3d", "batched", "ishikawa3d"])
     ap.add_argument("--iters", type=int, default=200)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--nvol", type=int, default=64)
@@ -294,12 +373,13 @@ def main():
     ap.add_argument("--no-tol", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--no-warm", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
     fn = {"binary3d": bench_binary3d, "binary2d": bench_binary2d, "potts3d": bench_potts3d,
-          "batched": bench_batched}[args.workload]
+          "batched": bench_batched, "ishikawa3d": bench_ishikawa3d}[args.workload]
     print(json.dumps(fn(args)), flush=True)
 
 
