@@ -121,6 +121,13 @@ asets_status check_dims(int64_t ny, int64_t nx, int64_t nz, int64_t L) {
     return ASETS_OK;
 }
 
+// The fused kernels address one field with 32-bit byte offsets, planes -2 to
+// nz+1 included (out-of-range planes must fall past the field and read 0): the
+// field plus two planes must stay below 2^31 bytes (about 800^3 voxels).
+inline bool fused_fits(int64_t ny, int64_t nx, int64_t nz) {
+    return ny * nx * (nz + 2) * 4 < ((int64_t)1 << 31);
+}
+
 // alpha <= 0 anywhere is an error when at least one iteration runs
 // (asetsBinaryMF3D_mex.c:317-319 raises it inside the first projection pass).
 __global__ void k_alpha_check(const float *alpha, int64_t n, int32_t *bad) {
@@ -506,9 +513,15 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
     if (flow16 != ASETS_FLOW_F32 && flow16 != ASETS_FLOW_F16)
         return fail(ASETS_EBADARG, "unknown flow_dtype %d", flow16);
     const int kern = o ? o->kernel : ASETS_KERNEL_AUTO;
-    const bool fused = (kern != ASETS_KERNEL_PASSES);
+    bool fused = (kern != ASETS_KERNEL_PASSES);
     if (flow16 == ASETS_FLOW_F16 && !fused)
         return fail(ASETS_EBADARG, "fp16 flow storage needs the fused kernel");
+    if (fused && !fused_fits(ny, nx, nz)) {
+        if (kern == ASETS_KERNEL_FUSED || flow16 == ASETS_FLOW_F16)
+            return fail(ASETS_ETOOBIG, "volume too large for the fused kernel (fields of "
+                                       "2^31 bytes or more); use the per-pass kernels");
+        fused = false;   // auto: the per-pass kernels have no such limit
+    }
 
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
@@ -645,6 +658,14 @@ asets_status potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny
         return fail(ASETS_EBADARG, "null array argument");
     if (o && o->flow_dtype != ASETS_FLOW_F32)
         return fail(ASETS_EBADARG, "fp16 flow storage is available for the binary model only");
+    const int kern = o ? o->kernel : ASETS_KERNEL_AUTO;
+    bool fused = (kern != ASETS_KERNEL_PASSES);
+    if (fused && !fused_fits(ny, nx, nz)) {
+        if (kern == ASETS_KERNEL_FUSED)
+            return fail(ASETS_ETOOBIG, "volume too large for the fused kernel (fields of "
+                                       "2^31 bytes or more); use the per-pass kernels");
+        fused = false;
+    }
 
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
@@ -659,8 +680,6 @@ asets_status potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny
     const Geo G{ny, nx, nz, N, L};
     const int32_t max_iter = p->max_iter;
     const int64_t nparts = cdiv(N, kBlock);
-    const int kern = o ? o->kernel : ASETS_KERNEL_AUTO;
-    const bool fused = (kern != ASETS_KERNEL_PASSES);
     size_t need = Carver::sz(1, sizeof(Flags)) + Carver::sz(nparts, sizeof(double)) +
                   Carver::sz(1, sizeof(int32_t)) + Carver::sz(N, 4) +
                   (fused ? 9 : 6) * Carver::sz(NL, 4);
@@ -909,6 +928,8 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
         if (st) return fail(st, "volume %d: %s", v, asets_last_error());
         if (!V.Cs || !V.Ct || !V.alpha || !V.u || !V.it_num || (p->max_iter > 0 && !V.cvg))
             return fail(ASETS_EBADARG, "volume %d: null array", v);
+        if (!fused_fits(V.ny, V.nx, V.nz))
+            return fail(ASETS_ETOOBIG, "volume %d too large for the fused kernel", v);
         total += V.ny * V.nx * V.nz;
     }
     int dev = 0;
@@ -1290,6 +1311,9 @@ asets_status ishikawa_dev(int ndim, const float *Ct, const float *alpha, int64_t
         return fail(ASETS_EBADARG, "fp16 flow storage is available for the binary model only");
     if (o && o->kernel == ASETS_KERNEL_PASSES)
         return fail(ASETS_EBADARG, "the Ishikawa model runs on the fused kernel only");
+    if (!fused_fits(ny, nx, nz))
+        return fail(ASETS_ETOOBIG, "volume too large for the fused kernel (fields of 2^31 "
+                                   "bytes or more)");
 
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));

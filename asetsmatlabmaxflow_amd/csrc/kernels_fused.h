@@ -28,10 +28,10 @@
 // rescaled z face below the current output plane is carried in registers.  Halo values are recomputed by neighbouring
 // workgroups from identical inputs in the same operation order (same bits).
 //
-// Global reads go through buffer descriptors built per plane (base = field +
-// plane offset, size = one plane): 32-bit in-plane offsets, and reads past the
-// plane return 0 -- which is exactly the zero-flux value of a high face past
-// the last voxel.  The state is double-buffered (in -> out) because
+// Global accesses go through one buffer descriptor per field (size = the
+// field) with the plane offset in the bounds-checked vector offset: reads of a
+// plane outside the volume, or past the end of the field, return 0 -- exactly
+// the zero-flux value of a high face past the last voxel.  The state is double-buffered (in -> out) because
 // neighbouring workgroups read the halo of the previous iterate.
 #pragma once
 #include <hip/hip_fp16.h>
@@ -65,23 +65,22 @@ struct FusedArgs {
 
 // ---- buffer access --------------------------------------------------------
 
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *base, int64_t elem_off,
-                                                             int bytes_per_elem, int plane) {
-    const char *p = (const char *)base + elem_off * bytes_per_elem;
-    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, plane * bytes_per_elem,
-                                             0x00020000);
-}
+// A plane of a field: a descriptor over the whole field (num_records = the
+// field's bytes) and the element offset of plane q.  Reads go through the
+// vector offset, which the hardware bounds-checks: a plane outside [0, nz)
+// (negative offsets wrap to above 2^31) and a position past the field read 0 --
+// the zero-flux value of a face past the volume.  Valid while
+// (N + 2*plane) * bytes < 2^31 (fused_fits on the host).
+struct PRef {
+    __amdgpu_buffer_rsrc_t r;
+    int b;   // element offset of the plane
+};
 
-// Descriptor of plane q of a field, or an empty one (every load returns 0)
-// when q is outside [0, nz): loads never need a branch, which keeps the
-// prefetched loads countable by the compiler's wait-counter insertion.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_q(const void *base, int q, int nz,
-                                                          int bytes_per_elem, int plane) {
-    const bool in = q >= 0 && q < nz;
-    const int qq = in ? q : 0;
-    return __builtin_amdgcn_make_buffer_rsrc(
-        (void *)((const char *)base + (int64_t)qq * plane * bytes_per_elem), (short)0,
-        in ? plane * bytes_per_elem : 0, 0x00020000);
+__device__ __forceinline__ PRef pref(const void *base, int q, int nz, int bytes_per_elem,
+                                     int plane) {
+    return PRef{__builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0,
+                                                  nz * plane * bytes_per_elem, 0x00020000),
+                q * plane};
 }
 
 template <typename T> struct Io;
@@ -114,6 +113,11 @@ template <> struct Io<__half> {
 __device__ __forceinline__ float ldf(__amdgpu_buffer_rsrc_t r, int off) {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off * 4, 0, 0));
 }
+__device__ __forceinline__ float ldf(const PRef &p, int off) { return ldf(p.r, p.b + off); }
+template <typename IO>
+__device__ __forceinline__ float ld(const PRef &p, int off) { return IO::load(p.r, p.b + off); }
+template <typename IO>
+__device__ __forceinline__ void st(const PRef &p, int off, float v) { IO::store(p.r, p.b + off, v); }
 
 // ---- correctly rounded division, short form ------------------------------
 // The compiler's IEEE fp32 division is: refine a hardware reciprocal of the
@@ -164,9 +168,32 @@ __device__ __forceinline__ float div_fast(float x, float d, float r, bool d_ok =
 // divisions are needed only when some lane of the wave is above that bound.
 // Lanes that do not use a division feed it benign operands (1) so they never
 // trigger the exact fallback.
+//
+// Before any FP64 work an fp32 screen proves that bound for most waves: with
+// s = the fp32 sum of the six squares (fused multiply-adds, 6 roundings), the
+// real sum is at most s*(1 + 8*2^-24) when no square underflows, and the
+// double-rounded sum at most that times (1 + 2^-50); an underflowing square
+// (|b| < 2^-63) adds at most 2^-126 each.  The test
+//     s * 0.5*(1 + 2^-18)  <=  (a*a) * (1 - 2^-20)          (fp32, both sides rounded)
+// therefore implies t*0.5 <= alpha^2 whenever alpha is in [2^-50, 2^50]
+// (the slack alpha^2 * 2^-19 covers 3*2^-126).  An inf or NaN square fails
+// the test and takes the exact path.
 template <bool IS3D>
 __device__ __forceinline__ float proj_factor_fast(float xr, float xl, float yr, float yl,
                                                   float zr, float zl, float a) {
+    {
+        float s = xr * xr;
+        s = __builtin_fmaf(xl, xl, s);
+        s = __builtin_fmaf(yr, yr, s);
+        s = __builtin_fmaf(yl, yl, s);
+        if (IS3D) {
+            s = __builtin_fmaf(zr, zr, s);
+            s = __builtin_fmaf(zl, zl, s);
+        }
+        const bool a_ok = a >= 0x1p-50f && a <= 0x1p50f;
+        const bool one = a_ok && (s * 0x1.00004p-1f <= (a * a) * 0x1.ffffep-1f);
+        if (wave_all(one)) return 1.0f;
+    }
     double t = (double)xr * (double)xr;
     t = __builtin_fma((double)xl, (double)xl, t);
     t = __builtin_fma((double)yr, (double)yr, t);
@@ -305,28 +332,31 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         // load raw plane q (u, ps, pt, bx own/high, by own/high), bz own of
         // plane q+1, and alpha of plane q-1 (for S2b one step later)
         auto load_raw = [&](int q, Raw &r) {
-            const auto ru = plane_q(a.in.u, q, nz, 4, plane);
-            const auto rps = plane_q(a.in.ps, q, nz, IO::kBytes, plane);
-            const auto rpt = plane_q(a.in.pt, q, nz, IO::kBytes, plane);
-            const auto rbx = plane_q(a.in.bx, q, nz, IO::kBytes, plane);
-            const auto rby = plane_q(a.in.by, q, nz, IO::kBytes, plane);
+            const PRef ru = pref(a.in.u, q, nz, 4, plane);
+            const PRef rps = pref(a.in.ps, q, nz, IO::kBytes, plane);
+            const PRef rpt = pref(a.in.pt, q, nz, IO::kBytes, plane);
+            const PRef rbx = pref(a.in.bx, q, nz, IO::kBytes, plane);
+            const PRef rby = pref(a.in.by, q, nz, IO::kBytes, plane);
 #pragma unroll
             for (int k = 0; k < K2; ++k) {
                 const int o = off2[k];
                 r.u[k] = ldf(ru, o);
-                r.ps[k] = IO::load(rps, o);
-                r.pt[k] = IO::load(rpt, o);
-                r.bxl[k] = IO::load(rbx, o);
-                r.bxr[k] = IO::load(rbx, o + ny);   // past the plane -> 0
-                r.byl[k] = IO::load(rby, o);
-                r.byr[k] = IO::load(rby, o + 1);    // next column's y=0 face is 0
+                r.ps[k] = ld<IO>(rps, o);
+                r.pt[k] = ld<IO>(rpt, o);
+                r.bxl[k] = ld<IO>(rbx, o);
+                // the high x face of the last column is the next plane's (or past
+                // the field's) x = 0 face, and the high y face of the last row the
+                // next column's y = 0 face: both are 0
+                r.bxr[k] = ld<IO>(rbx, o + ny);
+                r.byl[k] = ld<IO>(rby, o);
+                r.byr[k] = ld<IO>(rby, o + 1);
             }
             if (IS3D) {
-                const auto rbz = plane_q(a.in.bz, q + 1, nz, IO::kBytes, plane);
+                const PRef rbz = pref(a.in.bz, q + 1, nz, IO::kBytes, plane);
 #pragma unroll
-                for (int k = 0; k < K2; ++k) r.bzn[k] = IO::load(rbz, off2[k]);
+                for (int k = 0; k < K2; ++k) r.bzn[k] = ld<IO>(rbz, off2[k]);
             }
-            const auto ra = plane_q(a.alpha, q - 1, nz, 4, plane);
+            const PRef ra = pref(a.alpha, q - 1, nz, 4, plane);
 #pragma unroll
             for (int k = 0; k < K1; ++k) r.al[k] = ldf(ra, off1[k]);
         };
@@ -414,16 +444,16 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         auto load_out = [&](int s, Out &o) {
             if constexpr (ISHI) return;   // the Ishikawa output stage needs none of them
             const int sv = (s >= z0 && s < z1) ? s : -1;   // outside: empty descriptors
-            const auto ru = plane_q(a.in.u, sv, nz, 4, plane);
-            const auto rpt = plane_q(POTTS ? a.in.ps : a.in.pt, sv, nz, IO::kBytes, plane);
-            const auto rcs = plane_q(a.Cs, POTTS ? -1 : sv, nz, 4, plane);
-            const auto rct = plane_q(a.Ct, sv, nz, 4, plane);
+            const PRef ru = pref(a.in.u, sv, nz, 4, plane);
+            const PRef rpt = pref(POTTS ? a.in.ps : a.in.pt, sv, nz, IO::kBytes, plane);
+            const PRef rcs = pref(a.Cs, POTTS ? -1 : sv, nz, 4, plane);
+            const PRef rct = pref(a.Ct, sv, nz, 4, plane);
 #pragma unroll
             for (int k = 0; k < KO; ++k) {
                 const int i = tid % TY, j = tid / TY + k * (NT / TY);
                 const int off = (x0 + j) * ny + (y0 + i);
                 o.u[k] = ldf(ru, off);
-                o.pt[k] = IO::load(rpt, off);
+                o.pt[k] = ld<IO>(rpt, off);
                 if (!POTTS) o.cs[k] = ldf(rcs, off);
                 o.ct[k] = ldf(rct, off);
             }
@@ -459,13 +489,12 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             const int sl = (s + 3) % 3;
             const float *g0 = G[s & 1];
             const float *g1 = G[(s + 1) & 1];
-            const int64_t base = (int64_t)s * plane;
-            const auto wu = plane_rsrc((POTTS || ISHI) ? a.in.u : a.out.u, base, 4, plane);
-            const auto wps = plane_rsrc(a.out.ps, base, IO::kBytes, plane);
-            const auto wpt = plane_rsrc(a.out.pt, base, IO::kBytes, plane);
-            const auto wbx = plane_rsrc(a.out.bx, base, IO::kBytes, plane);
-            const auto wby = plane_rsrc(a.out.by, base, IO::kBytes, plane);
-            const auto wbz = plane_rsrc(a.out.bz, base, IO::kBytes, plane);
+            const PRef wu = pref((POTTS || ISHI) ? a.in.u : a.out.u, s, nz, 4, plane);
+            const PRef wps = pref(a.out.ps, s, nz, IO::kBytes, plane);
+            const PRef wpt = pref(a.out.pt, s, nz, IO::kBytes, plane);
+            const PRef wbx = pref(a.out.bx, s, nz, IO::kBytes, plane);
+            const PRef wby = pref(a.out.by, s, nz, IO::kBytes, plane);
+            const PRef wbz = pref(a.out.bz, s, nz, IO::kBytes, plane);
 #pragma unroll
             for (int k = 0; k < KO; ++k) {
                 const int i = tid % TY, j = tid / TY + k * (NT / TY);
@@ -499,10 +528,10 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                 if constexpr (ISHI) {
                     if (y < ny && x < nx) {
                         const int off = x * ny + y;
-                        IO::store(wps, off, d);
-                        IO::store(wbx, off, bxl);
-                        IO::store(wby, off, byl);
-                        if (IS3D) IO::store(wbz, off, bzl);
+                        st<IO>(wps, off, d);
+                        st<IO>(wbx, off, bxl);
+                        st<IO>(wby, off, byl);
+                        if (IS3D) st<IO>(wbz, off, bzl);
                     }
                     continue;
                 }
@@ -514,11 +543,11 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     const float npt = fmin_ref(t, o.ct[k]);
                     if (y < ny && x < nx) {
                         const int off = x * ny + y;
-                        IO::store(wps, off, d + npt);
-                        IO::store(wpt, off, npt);
-                        IO::store(wbx, off, bxl);
-                        IO::store(wby, off, byl);
-                        if (IS3D) IO::store(wbz, off, bzl);
+                        st<IO>(wps, off, d + npt);
+                        st<IO>(wpt, off, npt);
+                        st<IO>(wbx, off, bxl);
+                        st<IO>(wby, off, byl);
+                        if (IS3D) st<IO>(wbz, off, bzl);
                     }
                 } else {
                     // source/sink flows and multiplier (asetsBinaryMF3D_mex.c:419-432)
@@ -530,12 +559,12 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     if (y < ny && x < nx) {
                         e_acc += fabs((double)e);
                         const int off = x * ny + y;
-                        Io<float>::store(wu, off, uo - e);
-                        IO::store(wps, off, nps);
-                        IO::store(wpt, off, npt);
-                        IO::store(wbx, off, bxl);
-                        IO::store(wby, off, byl);
-                        if (IS3D) IO::store(wbz, off, bzl);
+                        st<Io<float>>(wu, off, uo - e);
+                        st<IO>(wps, off, nps);
+                        st<IO>(wpt, off, npt);
+                        st<IO>(wbx, off, bxl);
+                        st<IO>(wby, off, byl);
+                        if (IS3D) st<IO>(wbz, off, bzl);
                     }
                 }
             }
@@ -567,9 +596,9 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         Out o;
         // bz own of plane z0-2 (raw plane z0-3 would carry it as its "next" bz)
         if (IS3D) {
-            const auto rbz = plane_q(a.in.bz, z0 - 2, nz, IO::kBytes, plane);
+            const PRef rbz = pref(a.in.bz, z0 - 2, nz, IO::kBytes, plane);
 #pragma unroll
-            for (int k = 0; k < K2; ++k) bz_cur[k] = IO::load(rbz, off2[k]);
+            for (int k = 0; k < K2; ++k) bz_cur[k] = ld<IO>(rbz, off2[k]);
         }
         for (int s = z0 - 4; s <= z1; ++s) step(s, r, o);
     }

@@ -374,13 +374,27 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-warm", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the workload, run three device-to-device copies of MIB MiB "
+                         "(a kernel of known traffic to scale rocprofv3 byte counters by)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
     fn = {"binary3d": bench_binary3d, "binary2d": bench_binary2d, "potts3d": bench_potts3d,
           "batched": bench_batched, "ishikawa3d": bench_ishikawa3d}[args.workload]
-    print(json.dumps(fn(args)), flush=True)
+    out = fn(args)
+    if args.copy_calibration > 0:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        out["copy_calibration_mib"] = args.copy_calibration
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -142,3 +142,24 @@ def test_product_does_not_reference_the_oracle():
             continue
         text = f.read_text()
         assert "import oracle" not in text and "liboracle" not in text, f
+
+
+def test_fused_size_limit_rejected_before_device_work():
+    """The fused kernels need every field (plus two planes) below 2^31 bytes; an explicit
+    fused request beyond that is refused before any device call (the per-pass kernels
+    take over under ASETS_KERNEL_AUTO)."""
+    L = nat.lib()
+    p = nat.make_params(10, 0.0, 0.35, 0.11)
+    buf = np.zeros(64, np.float32)
+    ptr = buf.ctypes.data
+    opts = nat.make_opts(None, nat.FLOW_F32, nat.KERNEL_FUSED, 0, 0, 0, None)
+    rc = L.asets_binary_dev(3, ptr, ptr, ptr, 1024, 1024, 1024, C.byref(p), None, None, ptr,
+                            ptr, ptr, None, C.byref(opts))
+    assert rc == nat.ASETS_ETOOBIG and b"fused" in L.asets_last_error()
+    opts16 = nat.make_opts(None, nat.FLOW_F16, nat.KERNEL_AUTO, 0, 0, 0, None)
+    rc = L.asets_binary_dev(3, ptr, ptr, ptr, 1024, 1024, 1024, C.byref(p), None, None, ptr,
+                            ptr, ptr, None, C.byref(opts16))
+    assert rc == nat.ASETS_ETOOBIG
+    rc = L.asets_ishikawa_dev(3, ptr, ptr, 1024, 1024, 1024, 3, C.byref(p), ptr, ptr, ptr, None,
+                              None)
+    assert rc == nat.ASETS_ETOOBIG
