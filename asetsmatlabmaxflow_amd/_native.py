@@ -22,6 +22,8 @@ EXPORTED = (
     "asets_binary_dev", "asets_binary_dev_state", "asets_binary_dev_resume",
     "asets_potts_dev",
     "asets_binary3d_batched_dev", "asets_binary_energy_dev", "asets_potts_energy_dev",
+    "asets_cost_loglik_dev", "asets_cost_neglog_dev", "asets_alpha_contrast_dev",
+    "asets_labels_dev",
     "asets_selftest_division", "asets_last_error",
     "asets_version", "asets_release",
 )
@@ -87,6 +89,11 @@ def lib() -> C.CDLL:
         L.asets_binary3d_batched_dev.argtypes = [C.c_int32, C.POINTER(Volume), PP, PD, PO]
         L.asets_binary_energy_dev.argtypes = [C.c_int, P, P, P, I64, I64, I64, P, PD, PO]
         L.asets_potts_energy_dev.argtypes = [C.c_int, P, P, I64, I64, I64, I64, P, PD, PO]
+        L.asets_cost_loglik_dev.argtypes = [P, P, I64, C.c_int32, C.c_double, P, PO]
+        L.asets_cost_neglog_dev.argtypes = [P, I64, C.c_double, P, PO]
+        L.asets_alpha_contrast_dev.argtypes = [C.c_int, P, I64, I64, I64, C.c_double, C.c_double,
+                                               C.c_double, C.c_int, C.c_int, P, PO]
+        L.asets_labels_dev.argtypes = [C.c_int, P, I64, C.c_int32, P, PO]
         L.asets_selftest_division.argtypes = [C.c_int, C.c_float, C.POINTER(C.c_uint64)]
         for name in EXPORTED:
             if name in ("asets_last_error", "asets_version"):

@@ -27,6 +27,7 @@
 #include "common.h"
 #include "kernels_batched.h"
 #include "kernels_energy.h"
+#include "kernels_prepost.h"
 #include "kernels_fused.h"
 #include "kernels_passes.h"
 
@@ -1466,6 +1467,114 @@ asets_status energy_dev(int ndim, const float *Cs, const float *Ct, const float 
     return ASETS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// costs, weights and labels on device memory
+// ---------------------------------------------------------------------------
+struct PrepCtx {
+    DeviceCache *dc = nullptr;
+    hipStream_t s = nullptr;
+};
+
+asets_status prep_begin(const asets_opts *o, size_t need, PrepCtx &c,
+                        std::unique_lock<std::mutex> &lock) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(ASETS_EDEVICE, "device id %d", dev);
+    c.dc = &g_cache[dev];
+    lock = std::unique_lock<std::mutex>(c.dc->mu);
+    c.s = o ? (hipStream_t)o->stream : nullptr;
+    if (!c.dc->host_flag) HIPCHK(hipHostMalloc((void **)&c.dc->host_flag, 64));
+    return ensure(&c.dc->dev, &c.dc->dev_bytes, need);
+}
+
+// min and max of x into mm[2] (device); the values are copied to host[2]
+asets_status minmax(const float *x, int64_t n, float *part, float *mm, float *host, hipStream_t s,
+                    DeviceCache &dc) {
+    const int nb = (int)std::min<int64_t>(kPrepBlocks, cdiv(n, 256));
+    k_minmax_part<<<nb, 256, 0, s>>>(x, n, part);
+    k_minmax_final<<<1, 64, 0, s>>>(part, nb, mm);
+    HIPCHK(hipGetLastError());
+    float *h = (float *)((char *)dc.host_flag + 32);
+    HIPCHK(hipMemcpyAsync(h, mm, 2 * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    host[0] = h[0];
+    host[1] = h[1];
+    return ASETS_OK;
+}
+
+asets_status cost_loglik(const float *img, const int32_t *scrib, int64_t n, int32_t nlab,
+                         double eps, float *Ct, const asets_opts *o) {
+    if (!img || !scrib || !Ct) return fail(ASETS_EBADARG, "null array argument");
+    if (n <= 0 || nlab <= 0 || nlab > 64)
+        return fail(ASETS_EBADARG, "need n > 0 and 1 <= nlab <= 64");
+    if (n > ((int64_t)1 << 40)) return fail(ASETS_ETOOBIG, "too many voxels");
+    PrepCtx c;
+    std::unique_lock<std::mutex> lock;
+    const size_t need = Carver::sz(2 * kPrepBlocks, 4) + Carver::sz(2, 4) +
+                        Carver::sz(256 * nlab, 4) + Carver::sz(nlab, 8);
+    asets_status st = prep_begin(o, need, c, lock);
+    if (st) return st;
+    Carver cv(c.dc->dev);
+    float *part = cv.take<float>(2 * kPrepBlocks);
+    float *mm = cv.take<float>(2);
+    uint32_t *hist = cv.take<uint32_t>(256 * nlab);
+    uint64_t *tot = cv.take<uint64_t>(nlab);
+    float h[2];
+    st = minmax(img, n, part, mm, h, c.s, *c.dc);
+    if (st) return st;
+    if (!(h[1] > h[0]))
+        return fail(ASETS_EBADARG, "the image is constant (or holds NaN): no normalisation");
+    HIPCHK(hipMemsetAsync(hist, 0, 256 * nlab * sizeof(uint32_t), c.s));
+    const int nb = (int)std::min<int64_t>(kPrepBlocks, cdiv(n, 256));
+    k_loglik_hist<<<nb, 256, 256 * nlab * sizeof(uint32_t), c.s>>>(img, scrib, n, nlab, mm, hist);
+    k_loglik_totals<<<1, 64, 0, c.s>>>(hist, nlab, tot);
+    HIPCHK(hipGetLastError());
+    uint64_t *ht = (uint64_t *)c.dc->host_flag;   // 64 pinned bytes: 4 totals at a time
+    for (int l0 = 0; l0 < nlab; l0 += 4) {
+        const int k = std::min(4, nlab - l0);
+        HIPCHK(hipMemcpyAsync(ht, tot + l0, k * sizeof(uint64_t), hipMemcpyDeviceToHost, c.s));
+        HIPCHK(hipStreamSynchronize(c.s));
+        for (int j = 0; j < k; ++j)
+            if (ht[j] == 0)
+                return fail(ASETS_EBADARG, "label %d has no scribbled voxel", l0 + j + 1);
+    }
+    k_loglik_cost<<<(unsigned)cdiv(n, 256), 256, 0, c.s>>>(img, n, nlab, mm, hist, tot, eps, Ct);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c.s));
+    return ASETS_OK;
+}
+
+asets_status alpha_contrast(int ndim, const float *img, int64_t ny, int64_t nx, int64_t nz,
+                            double a0, double a1, double beta, int power, int normalize,
+                            float *alpha, const asets_opts *o) {
+    if (ndim != 2 && ndim != 3) return fail(ASETS_EBADARG, "ndim must be 2 or 3");
+    if (ndim == 2 && nz != 1) return fail(ASETS_EBADARG, "2D needs nz == 1");
+    asets_status st = check_dims(ny, nx, nz, 1);
+    if (st) return st;
+    if (!img || !alpha) return fail(ASETS_EBADARG, "null array argument");
+    if (power != 1 && power != 2) return fail(ASETS_EBADARG, "power must be 1 or 2");
+    PrepCtx c;
+    std::unique_lock<std::mutex> lock;
+    st = prep_begin(o, Carver::sz(2 * kPrepBlocks, 4) + Carver::sz(2, 4), c, lock);
+    if (st) return st;
+    Carver cv(c.dc->dev);
+    float *part = cv.take<float>(2 * kPrepBlocks);
+    float *mm = cv.take<float>(2);
+    const int64_t N = ny * nx * nz;
+    if (normalize) {
+        float h[2];
+        st = minmax(img, N, part, mm, h, c.s, *c.dc);
+        if (st) return st;
+        if (!(h[1] > h[0])) return fail(ASETS_EBADARG, "the image is constant: no normalisation");
+    }
+    k_alpha_contrast<<<(unsigned)cdiv(N, 256), 256, 0, c.s>>>(img, ny, nx, nz, ndim == 3, mm,
+                                                             normalize, a0, a1, beta, power,
+                                                             alpha);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c.s));
+    return ASETS_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1598,6 +1707,44 @@ asets_status asets_potts_energy_dev(int ndim, const float *Ct, const float *alph
                                     int64_t nx, int64_t nz, int64_t nlab, const float *u,
                                     double *energy, const asets_opts *o) {
     return energy_dev(ndim, nullptr, Ct, alpha, ny, nx, nz, nlab, u, energy, o);
+}
+
+asets_status asets_cost_loglik_dev(const float *img, const int32_t *scribbles, int64_t n,
+                                   int32_t nlab, double epsilon, float *Ct,
+                                   const asets_opts *o) {
+    return cost_loglik(img, scribbles, n, nlab, epsilon, Ct, o);
+}
+
+asets_status asets_cost_neglog_dev(const float *p, int64_t n, double epsilon, float *Ct,
+                                   const asets_opts *o) {
+    if (!p || !Ct) return fail(ASETS_EBADARG, "null array argument");
+    if (n <= 0 || n > ((int64_t)1 << 40)) return fail(ASETS_EBADARG, "bad element count");
+    hipStream_t s = o ? (hipStream_t)o->stream : nullptr;
+    k_neglog<<<(unsigned)cdiv(n, 256), 256, 0, s>>>(p, n, epsilon, Ct);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return ASETS_OK;
+}
+
+asets_status asets_alpha_contrast_dev(int ndim, const float *img, int64_t ny, int64_t nx,
+                                      int64_t nz, double a0, double a1, double beta, int power,
+                                      int normalize, float *alpha, const asets_opts *o) {
+    return alpha_contrast(ndim, img, ny, nx, nz, a0, a1, beta, power, normalize, alpha, o);
+}
+
+asets_status asets_labels_dev(int model, const float *u, int64_t n, int32_t nlab,
+                              uint8_t *labels, const asets_opts *o) {
+    if (!u || !labels) return fail(ASETS_EBADARG, "null array argument");
+    if (n <= 0 || n > ((int64_t)1 << 40)) return fail(ASETS_EBADARG, "bad element count");
+    if (model < 0 || model > 2) return fail(ASETS_EBADARG, "model must be 0, 1 or 2");
+    if (model == 0) nlab = 1;
+    if (nlab < 1 || nlab > 255 || (model == 2 && nlab < 2))
+        return fail(ASETS_EBADARG, "bad label count %d", nlab);
+    hipStream_t s = o ? (hipStream_t)o->stream : nullptr;
+    k_labels<<<(unsigned)cdiv(n, 256), 256, 0, s>>>(model, u, n, nlab, labels);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return ASETS_OK;
 }
 
 asets_status asets_selftest_division(int mode, float d, uint64_t *mismatches) {

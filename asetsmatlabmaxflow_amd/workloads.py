@@ -91,8 +91,10 @@ def _grad_sq(I):
     return g
 
 
-def binary3d_ellipsoids(shape=(256, 256, 256), seed: int = 1002, count: int = 12):
-    """Returns (Cs, Ct, alpha, fg) for the 3D binary workload."""
+def binary3d_ellipsoids(shape=(256, 256, 256), seed: int = 1002, count: int = 12,
+                        return_image: bool = False):
+    """Returns (Cs, Ct, alpha, fg) for the 3D binary workload (and the image I last when
+    return_image)."""
     rng = np.random.default_rng(seed)
     fg = _ellipsoid_mask(tuple(shape), rng, count=count)
     I = rng.normal(0.0, 0.12, size=shape).astype(np.float32)
@@ -102,7 +104,8 @@ def binary3d_ellipsoids(shape=(256, 256, 256), seed: int = 1002, count: int = 12
     Cs = ((I - np.float32(0.35)) ** 2 / s2).astype(np.float32)
     alpha = (np.float32(0.2) + np.float32(0.8) * np.exp(np.float32(-20.0) * _grad_sq(I)))
     alpha = alpha.astype(np.float32)
-    return (np.asfortranarray(Cs), np.asfortranarray(Ct), np.asfortranarray(alpha), fg)
+    out = ((np.asfortranarray(Cs), np.asfortranarray(Ct), np.asfortranarray(alpha), fg))
+    return out + (I,) if return_image else out
 
 
 BINARY3D_CC, BINARY3D_STEPS = 0.35, 0.11

@@ -220,6 +220,44 @@ asets_status asets_potts_energy_dev(int ndim, const float *Ct, const float *alph
                                     int64_t nx, int64_t nz, int64_t nlab, const float *u,
                                     double *energy, const asets_opts *o);
 
+/* ---- before and after the solve: data costs, weights, labels (device arrays) --
+ * The steps the reference's applications and tests run around the solvers, on the
+ * GPU.  No reference C counterpart (they are MATLAB code); semantics as cited. */
+
+/* Log-likelihood data costs, applications/interactiveGraphCutSegmentation3D.m:96-118
+ * (computeLogLikelihoodCost) for labels 1..nlab: the image is normalised to
+ * x = ((img - min) / (max - min)) * 255 in fp32; label l's 256-bin histogram
+ * counts its scribbled voxels (scribbles == l) in bin floor(x) (histc with edges
+ * 0..255, x == 255 in the last bin); Ct[(l-1)*n + i] = -log10(P_l(uint16(x_i)) +
+ * epsilon) with P the normalised histogram and uint16() rounding half away from
+ * zero, in FP64 stored as fp32.  A constant image or a label without scribbles is
+ * an error (MATLAB would produce NaN costs).  nlab <= 64. */
+asets_status asets_cost_loglik_dev(const float *img, const int32_t *scribbles, int64_t n,
+                                   int32_t nlab, double epsilon, float *Ct,
+                                   const asets_opts *o);
+
+/* Ct[i] = -log(p[i] + epsilon) (natural logarithm, FP64 stored as fp32), the data term
+ * from probability maps of applications/atlasProbabilityMapsRegularization3D.m:47-53. */
+asets_status asets_cost_neglog_dev(const float *p, int64_t n, double epsilon, float *Ct,
+                                   const asets_opts *o);
+
+/* Contrast-sensitive regularisation weight, applications/smoothManualSegmentations2D.m:
+ * 73-82: I normalised to [0, 1] when normalize != 0, the gradient of MATLAB's gradient()
+ * (central differences inside, one-sided at the borders; ndim 2 or 3), and
+ *   alpha = a0 + a1 * exp(-beta * m),  m = |grad I| (power 1) or |grad I|^2 (power 2),
+ * in FP64 stored as fp32.  The reference application uses a0 = 0, a1 = 0.2, beta = 10,
+ * power 1. */
+asets_status asets_alpha_contrast_dev(int ndim, const float *img, int64_t ny, int64_t nx,
+                                      int64_t nz, double a0, double a1, double beta, int power,
+                                      int normalize, float *alpha, const asets_opts *o);
+
+/* Discrete labels from a solution u of n voxels: model 0 (binary) u > 0.5
+ * (tests/runTestBinaryMF.m:55); model 1 (Potts, nlab fields) the 1-based index of the
+ * first maximum over labels (tests/runTestPotts.m:58, MATLAB max); model 2 (Ishikawa,
+ * nlab-1 cut functions) the number of cut functions > 0.5 (tests/runTestIshikawa.m:57-60). */
+asets_status asets_labels_dev(int model, const float *u, int64_t n, int32_t nlab,
+                              uint8_t *labels, const asets_opts *o);
+
 /* ---- misc ---------------------------------------------------------------- */
 /* Self-check of the short correctly-rounded division form the kernels use for
  * x/cc, n/alpha and 1/n (no reference counterpart): compares it with IEEE
