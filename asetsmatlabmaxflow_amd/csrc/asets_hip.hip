@@ -279,8 +279,8 @@ __global__ void k_init_binary_t(const float *Cs, const float *Ct, const float *u
         ms = mt = fmin_ref(cs, ct);
     }
     u[g] = uu;
-    Io<T>::put(ps + g, ms);
-    Io<T>::put(pt + g, mt);
+    Io<T>::put(ps + kPair * g, ms);   // paired binary state: pt == ps + 1
+    Io<T>::put(pt + kPair * g, mt);
 }
 
 // Initial flow faces of a resumed solve: dst = src, except the faces with axis
@@ -289,31 +289,43 @@ __global__ void k_init_binary_t(const float *Cs, const float *Ct, const float *u
 // 1 = x (bx), 2 = z (bz).
 template <typename T>
 __global__ void k_init_faces(T *dst, const float *src, int64_t ny, int64_t nx, int64_t N,
-                             int axis) {
+                             int axis, int stride) {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= N) return;
     int64_t c = axis == 0 ? g % ny : (axis == 1 ? (g / ny) % nx : g / (ny * nx));
-    Io<T>::put(dst + g, c == 0 ? 0.0f : src[g]);
+    Io<T>::put(dst + stride * g, c == 0 ? 0.0f : src[g]);
+}
+
+// Zero the three face fields (pair == 2: bx and by interleaved, by == bx + 1).
+template <typename T>
+void zero_faces(T *bx, T *by, T *bz, int64_t N, int pair, hipStream_t s) {
+    const unsigned nb = (unsigned)cdiv(N, kBlock);
+    if (pair == 2) {
+        k_fill_t<T><<<(unsigned)cdiv(2 * N, kBlock), kBlock, 0, s>>>(bx, 0.0f, 2 * N);
+    } else {
+        k_fill_t<T><<<nb, kBlock, 0, s>>>(bx, 0.0f, N);
+        k_fill_t<T><<<nb, kBlock, 0, s>>>(by, 0.0f, N);
+    }
+    k_fill_t<T><<<nb, kBlock, 0, s>>>(bz, 0.0f, N);
 }
 
 template <typename T>
 void init_faces(T *const dst[3], const float *const src[3], int64_t ny, int64_t nx, int64_t nz,
-                hipStream_t s) {
+                int pair, hipStream_t s) {
     const int64_t N = ny * nx * nz;
     const unsigned nb = (unsigned)cdiv(N, kBlock);
     const int axis[3] = {1, 0, 2};   // bx, by, bz
-    for (int k = 0; k < 3; ++k) {
+    const int stride[3] = {pair, pair, 1};
+    zero_faces<T>(dst[0], dst[1], dst[2], N, pair, s);
+    for (int k = 0; k < 3; ++k)
         if (src[k])
-            k_init_faces<T><<<nb, kBlock, 0, s>>>(dst[k], src[k], ny, nx, N, axis[k]);
-        else
-            k_fill_t<T><<<nb, kBlock, 0, s>>>(dst[k], 0.0f, N);
-    }
+            k_init_faces<T><<<nb, kBlock, 0, s>>>(dst[k], src[k], ny, nx, N, axis[k], stride[k]);
 }
 
 template <typename T>
-__global__ void k_to_f32(float *dst, const T *src, int64_t n) {
+__global__ void k_to_f32(float *dst, const T *src, int64_t n, int stride) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = Io<T>::get(src + i);
+    if (i < n) dst[i] = Io<T>::get(src + stride * i);
 }
 
 // Everything the fused solve loop needs (one volume).
@@ -351,17 +363,17 @@ asets_status run_fused(const FusedRun &r, Carver &cv, DeviceCache &dc) {
     FusedState<T> buf[2];
     for (int b = 0; b < 2; ++b) {
         buf[b].u = cv.take<float>(N);
-        buf[b].ps = cv.take<T>(N);
-        buf[b].pt = cv.take<T>(N);
-        buf[b].bx = cv.take<T>(N);
-        buf[b].by = cv.take<T>(N);
+        buf[b].ps = cv.take<T>(kPair * N);
+        buf[b].pt = buf[b].ps + 1;
+        buf[b].bx = cv.take<T>(kPair * N);
+        buf[b].by = buf[b].bx + 1;
         buf[b].bz = cv.take<T>(N);
     }
     k_init_binary_t<T><<<nb, kBlock, 0, s>>>(r.Cs, r.Ct, r.u0, r.ps0, r.pt0, buf[0].u,
                                             buf[0].ps, buf[0].pt, N);
     T *const f0[3] = {buf[0].bx, buf[0].by, buf[0].bz};
-    init_faces<T>(f0, r.b0, r.ny, r.nx, r.nz, s);
-    for (T *f : {buf[1].bx, buf[1].by, buf[1].bz}) k_fill_t<T><<<nb, kBlock, 0, s>>>(f, 0.0f, N);
+    init_faces<T>(f0, r.b0, r.ny, r.nx, r.nz, kPair, s);
+    zero_faces<T>(buf[1].bx, buf[1].by, buf[1].bz, N, kPair, s);
     HIPCHK(hipGetLastError());
 
     FusedArgs<T> fa{};
@@ -405,7 +417,7 @@ asets_status run_fused(const FusedRun &r, Carver &cv, DeviceCache &dc) {
     const T *srcs[5] = {fin.ps, fin.pt, fin.bx, fin.by, fin.bz};
     float *dsts[5] = {r.o_ps, r.o_pt, r.o_bx, r.o_by, r.o_bz};
     for (int k = 0; k < 5; ++k)
-        if (dsts[k]) k_to_f32<T><<<nb, kBlock, 0, s>>>(dsts[k], srcs[k], N);
+        if (dsts[k]) k_to_f32<T><<<nb, kBlock, 0, s>>>(dsts[k], srcs[k], N, k < 4 ? kPair : 1);
     HIPCHK(hipGetLastError());
     return ASETS_OK;
 }
@@ -579,7 +591,7 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
         float *by = cv.take<float>(N), *bz = cv.take<float>(N), *dv = cv.take<float>(N);
         float *gk = cv.take<float>(N);
         float *const f0[3] = {bx, by, bz};
-        init_faces<float>(f0, w.b0, ny, nx, nz, s);
+        init_faces<float>(f0, w.b0, ny, nx, nz, 1, s);
         fill(dv, 0.0f, N, s);
         k_init_binary<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Cs, Ct, u0, w.ps0, pt0, u, ps,
                                                                    pt, N);
@@ -986,10 +998,10 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
         b.alpha = V.alpha;
         for (int k = 0; k < 2; ++k) {
             b.u[k] = cv.take<float>(N);
-            b.ps[k] = cv.take<T>(N);
-            b.pt[k] = cv.take<T>(N);
-            b.bx[k] = cv.take<T>(N);
-            b.by[k] = cv.take<T>(N);
+            b.ps[k] = cv.take<T>(kPair * N);
+            b.pt[k] = b.ps[k] + 1;
+            b.bx[k] = cv.take<T>(kPair * N);
+            b.by[k] = b.bx[k] + 1;
             b.bz[k] = cv.take<T>(N);
         }
         b.ny = (int)V.ny;
@@ -1013,7 +1025,7 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
             k_init_binary_t<T><<<nb, kBlock, 0, s>>>(b.Cs, b.Ct, nullptr, nullptr, nullptr,
                                                     b.u[0], b.ps[0], b.pt[0], N);
             for (int k = 0; k < 2; ++k)
-                for (T *f : {b.bx[k], b.by[k], b.bz[k]}) k_fill_t<T><<<nb, kBlock, 0, s>>>(f, 0.0f, N);
+                zero_faces<T>(b.bx[k], b.by[k], b.bz[k], N, kPair, s);
             fill(vols[v].cvg, 0.0f, max_iter, s);
             k_fill_i32<<<1, 1, 0, s>>>(vols[v].it_num, max_iter, 1);
         }

@@ -119,6 +119,44 @@ __device__ __forceinline__ float ld(const PRef &p, int off) { return IO::load(p.
 template <typename IO>
 __device__ __forceinline__ void st(const PRef &p, int off, float v) { IO::store(p.r, p.b + off, v); }
 
+// The binary model's state keeps (ps, pt) and (bx, by) interleaved, one pair per
+// voxel (the first of the two at the lower address): a pair is one 64-bit (fp32)
+// or 32-bit (fp16) access, so a plane needs 6 loads and 4 stores per position
+// instead of 8 and 6.  Element stride of ps, pt, bx, by in binary state: kPair;
+// bz, u and the Potts and Ishikawa fields are not paired.
+constexpr int kPair = 2;
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ __forceinline__ float2 ld_pair(const PRef &p, int off);
+template <>
+__device__ __forceinline__ float2 ld_pair<float>(const PRef &p, int off) {
+    const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(p.r, (p.b + off) * 8, 0, 0);
+    return make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
+}
+template <>
+__device__ __forceinline__ float2 ld_pair<__half>(const PRef &p, int off) {
+    const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(p.r, (p.b + off) * 4, 0, 0);
+    return make_float2(__half2float(__ushort_as_half((unsigned short)(w & 0xffffu))),
+                       __half2float(__ushort_as_half((unsigned short)(w >> 16))));
+}
+template <typename T>
+__device__ __forceinline__ void st_pair(const PRef &p, int off, float lo, float hi);
+template <>
+__device__ __forceinline__ void st_pair<float>(const PRef &p, int off, float lo, float hi) {
+    u32x2 w;
+    w.x = __float_as_uint(lo);
+    w.y = __float_as_uint(hi);
+    __builtin_amdgcn_raw_buffer_store_b64(w, p.r, (p.b + off) * 8, 0, 0);
+}
+template <>
+__device__ __forceinline__ void st_pair<__half>(const PRef &p, int off, float lo, float hi) {
+    const uint32_t w = (uint32_t)__half_as_ushort(__float2half_rn(lo)) |
+                       ((uint32_t)__half_as_ushort(__float2half_rn(hi)) << 16);
+    __builtin_amdgcn_raw_buffer_store_b32(w, p.r, (p.b + off) * 4, 0, 0);
+}
+
 // ---- correctly rounded division, short form ------------------------------
 // The compiler's IEEE fp32 division is: refine a hardware reciprocal of the
 // (scaled) divisor with two FMAs, multiply, then two remainder/correction FMA
@@ -255,6 +293,9 @@ template <bool IS3D, int TX, int NT, typename T, int MODE = 0>
 __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
     using IO = Io<T>;
     constexpr bool POTTS = (MODE == 1), ISHI = (MODE == 2);
+    constexpr bool PAIR = (MODE == 0);   // binary state: (ps, pt), (bx, by) interleaved
+    constexpr int PB = 2 * IO::kBytes;   // bytes of a pair
+    static_assert(PAIR || IO::kBytes == 4, "fp16 flow storage is for the binary model");
     constexpr int TY = 64;
     constexpr int W2 = TY + 4, H2 = TX + 4;   // gk / b1 region (tile + radius 2)
     constexpr int W1 = TY + 2, H1 = TX + 2;   // g region (tile + radius 1)
@@ -333,23 +374,41 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         // plane q+1, and alpha of plane q-1 (for S2b one step later)
         auto load_raw = [&](int q, Raw &r) {
             const PRef ru = pref(a.in.u, q, nz, 4, plane);
-            const PRef rps = pref(a.in.ps, q, nz, IO::kBytes, plane);
-            const PRef rpt = pref(a.in.pt, q, nz, IO::kBytes, plane);
-            const PRef rbx = pref(a.in.bx, q, nz, IO::kBytes, plane);
-            const PRef rby = pref(a.in.by, q, nz, IO::kBytes, plane);
+            // the high x face of the last column is the next plane's (or past the
+            // field's) x = 0 face, and the high y face of the last row the next
+            // column's y = 0 face: both are 0
+            if constexpr (PAIR) {
+                const PRef rpp = pref(a.in.ps, q, nz, PB, plane);   // (ps, pt) pairs
+                const PRef rbb = pref(a.in.bx, q, nz, PB, plane);   // (bx, by) pairs
 #pragma unroll
-            for (int k = 0; k < K2; ++k) {
-                const int o = off2[k];
-                r.u[k] = ldf(ru, o);
-                r.ps[k] = ld<IO>(rps, o);
-                r.pt[k] = ld<IO>(rpt, o);
-                r.bxl[k] = ld<IO>(rbx, o);
-                // the high x face of the last column is the next plane's (or past
-                // the field's) x = 0 face, and the high y face of the last row the
-                // next column's y = 0 face: both are 0
-                r.bxr[k] = ld<IO>(rbx, o + ny);
-                r.byl[k] = ld<IO>(rby, o);
-                r.byr[k] = ld<IO>(rby, o + 1);
+                for (int k = 0; k < K2; ++k) {
+                    const int o = off2[k];
+                    r.u[k] = ldf(ru, o);
+                    const float2 pp = ld_pair<T>(rpp, o);
+                    r.ps[k] = pp.x;
+                    r.pt[k] = pp.y;
+                    const float2 bb = ld_pair<T>(rbb, o);
+                    r.bxl[k] = bb.x;
+                    r.byl[k] = bb.y;
+                    r.bxr[k] = ld_pair<T>(rbb, o + ny).x;
+                    r.byr[k] = ld_pair<T>(rbb, o + 1).y;
+                }
+            } else {
+                const PRef rps = pref(a.in.ps, q, nz, IO::kBytes, plane);
+                const PRef rpt = pref(a.in.pt, q, nz, IO::kBytes, plane);
+                const PRef rbx = pref(a.in.bx, q, nz, IO::kBytes, plane);
+                const PRef rby = pref(a.in.by, q, nz, IO::kBytes, plane);
+#pragma unroll
+                for (int k = 0; k < K2; ++k) {
+                    const int o = off2[k];
+                    r.u[k] = ldf(ru, o);
+                    r.ps[k] = ld<IO>(rps, o);
+                    r.pt[k] = ld<IO>(rpt, o);
+                    r.bxl[k] = ld<IO>(rbx, o);
+                    r.bxr[k] = ld<IO>(rbx, o + ny);
+                    r.byl[k] = ld<IO>(rby, o);
+                    r.byr[k] = ld<IO>(rby, o + 1);
+                }
             }
             if (IS3D) {
                 const PRef rbz = pref(a.in.bz, q + 1, nz, IO::kBytes, plane);
@@ -445,7 +504,8 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             if constexpr (ISHI) return;   // the Ishikawa output stage needs none of them
             const int sv = (s >= z0 && s < z1) ? s : -1;   // outside: empty descriptors
             const PRef ru = pref(a.in.u, sv, nz, 4, plane);
-            const PRef rpt = pref(POTTS ? a.in.ps : a.in.pt, sv, nz, IO::kBytes, plane);
+            const PRef rpt = PAIR ? pref(a.in.ps, sv, nz, PB, plane)
+                                  : pref(POTTS ? a.in.ps : a.in.pt, sv, nz, IO::kBytes, plane);
             const PRef rcs = pref(a.Cs, POTTS ? -1 : sv, nz, 4, plane);
             const PRef rct = pref(a.Ct, sv, nz, 4, plane);
 #pragma unroll
@@ -453,7 +513,8 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                 const int i = tid % TY, j = tid / TY + k * (NT / TY);
                 const int off = (x0 + j) * ny + (y0 + i);
                 o.u[k] = ldf(ru, off);
-                o.pt[k] = ld<IO>(rpt, off);
+                if constexpr (PAIR) o.pt[k] = ld_pair<T>(rpt, off).y;
+                else o.pt[k] = ld<IO>(rpt, off);
                 if (!POTTS) o.cs[k] = ldf(rcs, off);
                 o.ct[k] = ldf(rct, off);
             }
@@ -490,9 +551,9 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             const float *g0 = G[s & 1];
             const float *g1 = G[(s + 1) & 1];
             const PRef wu = pref((POTTS || ISHI) ? a.in.u : a.out.u, s, nz, 4, plane);
-            const PRef wps = pref(a.out.ps, s, nz, IO::kBytes, plane);
+            const PRef wps = pref(a.out.ps, s, nz, PAIR ? PB : IO::kBytes, plane);
             const PRef wpt = pref(a.out.pt, s, nz, IO::kBytes, plane);
-            const PRef wbx = pref(a.out.bx, s, nz, IO::kBytes, plane);
+            const PRef wbx = pref(a.out.bx, s, nz, PAIR ? PB : IO::kBytes, plane);
             const PRef wby = pref(a.out.by, s, nz, IO::kBytes, plane);
             const PRef wbz = pref(a.out.bz, s, nz, IO::kBytes, plane);
 #pragma unroll
@@ -560,10 +621,15 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                         e_acc += fabs((double)e);
                         const int off = x * ny + y;
                         st<Io<float>>(wu, off, uo - e);
-                        st<IO>(wps, off, nps);
-                        st<IO>(wpt, off, npt);
-                        st<IO>(wbx, off, bxl);
-                        st<IO>(wby, off, byl);
+                        if constexpr (PAIR) {
+                            st_pair<T>(wps, off, nps, npt);
+                            st_pair<T>(wbx, off, bxl, byl);
+                        } else {
+                            st<IO>(wps, off, nps);
+                            st<IO>(wpt, off, npt);
+                            st<IO>(wbx, off, bxl);
+                            st<IO>(wby, off, byl);
+                        }
                         if (IS3D) st<IO>(wbz, off, bzl);
                     }
                 }
