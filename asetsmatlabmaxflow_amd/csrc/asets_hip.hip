@@ -1468,7 +1468,7 @@ asets_status energy_dev(int ndim, const float *Cs, const float *Ct, const float 
     else
         k_energy<false><<<kEnergyBlocks, kEnergyThreads, 0, s>>>(Cs, Ct, alpha, u, (int)ny,
                                                                  (int)nx, 1, (int)nlab, partials);
-    k_energy_finalize<<<1, 64, 0, s>>>(partials, kEnergyBlocks, res);
+    k_energy_finalize<<<1, 256, 0, s>>>(partials, kEnergyBlocks, res);
     HIPCHK(hipGetLastError());
     double *h = (double *)((char *)dc.host_flag + 32);   // 3 doubles of the pinned slot
     HIPCHK(hipMemcpyAsync(h, res, 3 * sizeof(double), hipMemcpyDeviceToHost, s));

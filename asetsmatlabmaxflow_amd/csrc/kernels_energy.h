@@ -58,17 +58,23 @@ __global__ __launch_bounds__(kEnergyThreads) void k_energy(
     }
 }
 
-// out[0] = E, out[1] = data term, out[2] = regularisation term
+// out[0] = E, out[1] = data term, out[2] = regularisation term.  One workgroup;
+// thread t sums partials t, t + blockDim, ... and the block sum is fixed-order.
 __global__ void k_energy_finalize(const double *partials, int nblocks, double *out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double red[16];
     double d = 0.0, r = 0.0;
-    for (int b = 0; b < nblocks; ++b) {
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
         d += partials[2 * b];
         r += partials[2 * b + 1];
     }
-    out[0] = d + r;
-    out[1] = d;
-    out[2] = r;
+    const double sd = block_sum(d, red);
+    __syncthreads();
+    const double sr = block_sum(r, red);
+    if (threadIdx.x == 0) {
+        out[0] = sd + sr;
+        out[1] = sd;
+        out[2] = sr;
+    }
 }
 
 }  // namespace asets
