@@ -50,6 +50,11 @@ def lib():
             f.argtypes = [P, P, I64, I64, I64, I64, C.c_int, C.c_int, real, real, real, C.c_int,
                           P, P, P, P, P, P, P, P]
             f.restype = C.c_int
+        for name, real in (("orc_hmf_f32", C.c_float), ("orc_hmf_f64", C.c_double)):
+            f = getattr(L, name)
+            f.argtypes = [I64, I64, I64, C.c_int, C.c_int, P, P, P, P, P, C.c_int, real, real,
+                          P, P, P, P, P, P, P]
+            f.restype = C.c_int
         L.orc_energy_binary.argtypes = [P, P, P, I64, I64, I64, C.c_int, P, P]
         L.orc_energy_binary.restype = C.c_int
         L.orc_energy_potts.argtypes = [P, P, I64, I64, I64, I64, C.c_int, P, P]
@@ -198,3 +203,65 @@ def energy_potts(Ct, alpha, u, shape, nlab, *, is3d=None):
     if rc != ORC_OK:
         raise OracleError(rc)
     return out
+
+
+@dataclass
+class HMFResult:
+    """Per-node state after MaxFullFlow; arrays are (nnodes, *shape) in MATLAB layout.
+    faces[n, k] is node n's own-face flow along dim k+1 (the reference's px, py, pz
+    are faces[n, 0][1:, ...], faces[n, 1][:, 1:, ...], faces[n, 2][:, :, 1:])."""
+    u: np.ndarray
+    pt: np.ndarray
+    div: np.ndarray
+    faces: np.ndarray
+    l: np.ndarray
+    data_cost: np.ndarray
+    regu_cost: np.ndarray
+
+
+def hmf(parent, Ct, alpha, shape, num_it, steps, cc, *, l=None, precision="f32"):
+    """Hierarchical max-flow, full-flow variant (asetsHMF2D.m / asetsHMF3D.m MaxFullFlow).
+
+    parent[n]: parent index of node n (-1 for the root); children in index order.
+    Ct[n]: data cost of leaf n (None elsewhere).  alpha[n]: scalar or array.
+    l[n]: None or (lx, ly[, lz]) shape-constraint directions (asetsHMF3D.m:217-238).
+    """
+    shape = tuple(int(d) for d in shape)
+    is3d = len(shape) == 3
+    ny, nx = shape[0], shape[1]
+    nz = shape[2] if is3d else 1
+    N = ny * nx * nz
+    M = len(parent)
+    dt = np.float32 if precision == "f32" else np.float64
+    par = np.ascontiguousarray(np.asarray(parent, dtype=np.int32))
+    ct = np.zeros((M, N), dt)
+    al = np.zeros((M, N), dt)
+    li = np.zeros((M, 3, N), dt)
+    hl = np.zeros(M, np.int32)
+    for n in range(M):
+        if Ct[n] is not None:
+            ct[n] = _flat(np.broadcast_to(np.asarray(Ct[n]), shape), dt)
+        al[n] = _flat(np.broadcast_to(np.asarray(alpha[n], dtype=dt), shape), dt)
+        if l is not None and l[n] is not None:
+            hl[n] = 1
+            for k, q in enumerate(l[n]):
+                li[n, k] = _flat(np.broadcast_to(np.asarray(q), shape), dt)
+    u = np.zeros((M, N), dt)
+    pt = np.zeros((M, N), dt)
+    dv = np.zeros((M, N), dt)
+    fc = np.zeros((M, 3, N), dt)
+    lo = np.zeros((M, 3, N), dt)
+    dcost = np.zeros(M)
+    rcost = np.zeros(M)
+    f = lib().orc_hmf_f32 if precision == "f32" else lib().orc_hmf_f64
+    rc = f(ny, nx, nz, int(is3d), M, par.ctypes.data, ct.ctypes.data, al.ctypes.data,
+           li.ctypes.data, hl.ctypes.data, int(num_it), steps, cc, u.ctypes.data,
+           pt.ctypes.data, dv.ctypes.data, fc.ctypes.data, lo.ctypes.data, dcost.ctypes.data,
+           rcost.ctypes.data)
+    if rc != ORC_OK:
+        raise OracleError(rc)
+
+    def sh(a):
+        return a.reshape(a.shape[:-1] + shape[::-1]).transpose(
+            tuple(range(a.ndim - 1)) + tuple(range(a.ndim - 1 + len(shape) - 1, a.ndim - 2, -1)))
+    return HMFResult(sh(u), sh(pt), sh(dv), sh(fc), sh(lo), dcost, rcost)

@@ -504,3 +504,294 @@ static int SFX(ishikawa)(const REAL *Ct, const REAL *alpha,
     free(bx); free(by); free(bz); free(dv); free(gk); free(pt);
     return ORC_OK;
 }
+
+/* ---- hierarchical max-flow (HMF), full-flow variant -------------------------
+ * Follows maxflow/asetsHMF2D.m and asetsHMF3D.m: MaxFullFlow (2D :78-88, 3D
+ * :77-87), InitializeFullFlow (2D :107-127, 3D :105-130), UpdateSinkFlows (2D
+ * :309-338, 3D :307-336), UpdateLabels (2D :206-217, 3D :191-202),
+ * UpdateSpatialFlows (2D :220-306, 3D :205-304), CalculateCostsFullFlow (2D
+ * :184-203).  Every statement of the reference is a whole-array expression in the
+ * class of Ct; here each element is computed in REAL in the same operation order.
+ *
+ * Tree: parent[n] is the index of node n's parent, -1 for the single root; the
+ * children of a node are visited in increasing index order (the order of the
+ * reference's cell array h.C).  Node rows of N elements: Ct (read for leaves),
+ * alpha (per voxel; a scalar alpha is a constant row), l[n][3] (the shape-
+ * constraint directions lx, ly, lz along dims 1, 2, 3, used when has_l[n]).
+ *
+ * Faces (own-face, as above): f[n][0] along dim 1 (the reference's px, stride 1),
+ * f[n][1] along dim 2 (py, stride Ny), f[n][2] along dim 3 (pz, stride Nx*Ny).
+ * The reference's px(i,j,k), the face between rows i and i+1 (1-based), is
+ * f[n][0] at row i (0-based), i.e. px == f[n][0][1:, :, :].
+ *
+ * Choices the reference leaves to MATLAB's runtime, fixed here:
+ *   .^2 is x*x, .^0.5 is the correctly rounded square root;
+ *   min(Ct, x) is (Ct < x) ? Ct : x;  max(x, 0) is (x > 0) ? x : +0;
+ *   the masks (px>0 & l>0).*l.*a select one product or +0;
+ *   1/cc of the root is (REAL)1 / cc; the 0.001 of the normalisation is (REAL)0.001;
+ *   the costs are sums, in double, of the REAL products u*Ct and u*div.
+ */
+#define HMF_MAXN 4096
+
+static void SFX(hmf_sink)(int n, const int32_t *parent, const int32_t *first_child,
+                          const int32_t *next_sib, int64_t N, REAL cc, const REAL *Ct,
+                          REAL *u, REAL *pt, REAL *div)
+{
+    /* children first (bottom-up), then this node (asetsHMF3D.m:307-336) */
+    int c, nc = 0;
+    int64_t g;
+    REAL *ptn = pt + (int64_t)n * N;
+    const REAL *dvn = div + (int64_t)n * N, *un = u + (int64_t)n * N;
+    for (c = first_child[n]; c >= 0; c = next_sib[c]) {
+        SFX(hmf_sink)(c, parent, first_child, next_sib, N, cc, Ct, u, pt, div);
+        ++nc;
+    }
+    if (nc == 0) {
+        /* leaf: pt = min(Ct, P.pt - div + u/cc) */
+        const REAL *ppt = pt + (int64_t)parent[n] * N, *ct = Ct + (int64_t)n * N;
+        for (g = 0; g < N; ++g) {
+            const REAL f = (ppt[g] - dvn[g]) + un[g] / cc;
+            ptn[g] = (ct[g] < f) ? ct[g] : f;
+        }
+        return;
+    }
+    for (g = 0; g < N; ++g) {
+        REAL acc = (parent[n] < 0) ? (REAL)1 / cc
+                                   : (pt[(int64_t)parent[n] * N + g] - dvn[g]) + un[g] / cc;
+        for (c = first_child[n]; c >= 0; c = next_sib[c]) {
+            const int64_t i = (int64_t)c * N + g;
+            acc = ((acc + pt[i]) + div[i]) - u[i] / cc;
+        }
+        ptn[g] = acc / (REAL)((parent[n] < 0) ? nc : nc + 1);
+    }
+}
+
+static int SFX(hmf)(int64_t ny, int64_t nx, int64_t nz, int is3d, int nnodes,
+                    const int32_t *parent, const REAL *Ct, const REAL *alpha,
+                    const REAL *l_in, const int32_t *has_l, int num_it, REAL steps, REAL cc,
+                    REAL *u, REAL *pt, REAL *div, REAL *f, REAL *l_out,
+                    double *data_cost, double *regu_cost)
+{
+    const int64_t N = nx * ny * nz;
+    const int64_t st[3] = {1, ny, nx * ny};
+    const int nd = is3d ? 3 : 2;
+    int32_t first_child[HMF_MAXN], next_sib[HMF_MAXN], last[HMF_MAXN], order[HMF_MAXN];
+    int n, k, it, root = -1, norder = 0;
+    int64_t g;
+    REAL *G, *A, *EX[3], *L;
+
+    if (N <= 0 || nnodes < 2 || nnodes > HMF_MAXN) return ORC_EBADARG;
+    if (!is3d && nz != 1) return ORC_EBADARG;
+    for (n = 0; n < nnodes; ++n) first_child[n] = next_sib[n] = last[n] = -1;
+    for (n = 0; n < nnodes; ++n) {
+        if (parent[n] < 0) {
+            if (root >= 0) return ORC_EBADARG;
+            root = n;
+        } else {
+            if (parent[n] >= nnodes || parent[n] == n) return ORC_EBADARG;
+            if (last[parent[n]] < 0) first_child[parent[n]] = n;
+            else next_sib[last[parent[n]]] = n;
+            last[parent[n]] = n;
+        }
+    }
+    if (root < 0 || first_child[root] < 0) return ORC_EBADARG;
+    /* every node reachable from the root (no cycles): breadth-first order */
+    order[norder++] = root;
+    for (k = 0; k < norder; ++k)
+        for (n = first_child[order[k]]; n >= 0; n = next_sib[n]) {
+            if (norder >= nnodes) return ORC_EBADARG;
+            order[norder++] = n;
+        }
+    if (norder != nnodes) return ORC_EBADARG;
+
+    G = (REAL *)calloc((size_t)N, sizeof(REAL));
+    A = (REAL *)calloc((size_t)N, sizeof(REAL));
+    L = (REAL *)calloc((size_t)(nnodes * 3) * (size_t)N, sizeof(REAL));
+    EX[0] = (REAL *)calloc((size_t)N, sizeof(REAL));
+    EX[1] = (REAL *)calloc((size_t)N, sizeof(REAL));
+    EX[2] = (REAL *)calloc((size_t)N, sizeof(REAL));
+    if (!G || !A || !L || !EX[0] || !EX[1] || !EX[2]) {
+        free(G); free(A); free(L); free(EX[0]); free(EX[1]); free(EX[2]);
+        return ORC_ENOMEM;
+    }
+    /* InitializeFullFlow: zero state; shape directions normalised where
+       |l| > 0.001, else 0 */
+    memset(u, 0, (size_t)(nnodes * N) * sizeof(REAL));
+    memset(pt, 0, (size_t)(nnodes * N) * sizeof(REAL));
+    memset(div, 0, (size_t)(nnodes * N) * sizeof(REAL));
+    memset(f, 0, (size_t)(nnodes * 3 * N) * sizeof(REAL));
+    for (n = 0; n < nnodes; ++n) {
+        if (!has_l || !has_l[n]) continue;
+        for (g = 0; g < N; ++g) {
+            REAL q[3] = {0, 0, 0}, s2, den;
+            for (k = 0; k < nd; ++k) q[k] = l_in[((int64_t)n * 3 + k) * N + g];
+            s2 = q[0] * q[0] + q[1] * q[1];
+            if (is3d) s2 = s2 + q[2] * q[2];
+            den = (REAL)sqrt((double)s2);
+            for (k = 0; k < nd; ++k)
+                L[((int64_t)n * 3 + k) * N + g] = (den > (REAL)0.001) ? q[k] / den : (REAL)0;
+        }
+    }
+
+    for (it = 0; it < num_it; ++it) {
+        SFX(hmf_sink)(root, parent, first_child, next_sib, N, cc, Ct, u, pt, div);
+
+        /* labels (top-down; order-free): u = u + cc*(P.pt - div - pt) */
+        for (n = 0; n < nnodes; ++n) {
+            if (n == root) continue;
+            for (g = 0; g < N; ++g) {
+                const int64_t i = (int64_t)n * N + g;
+                u[i] = u[i] + cc * ((pt[(int64_t)parent[n] * N + g] - div[i]) - pt[i]);
+            }
+        }
+
+        /* spatial flows of every non-root node */
+        for (n = 0; n < nnodes; ++n) {
+            REAL *fn[3], *un, *dvn;
+            const REAL *ptn, *ppt, *al, *ln[3];
+            int shape;
+            int64_t x, y, z;
+            if (n == root) continue;
+            un = u + (int64_t)n * N;
+            dvn = div + (int64_t)n * N;
+            ptn = pt + (int64_t)n * N;
+            ppt = pt + (int64_t)parent[n] * N;
+            al = alpha + (int64_t)n * N;
+            shape = has_l && has_l[n];
+            for (k = 0; k < 3; ++k) {
+                fn[k] = f + ((int64_t)n * 3 + k) * N;
+                ln[k] = L + ((int64_t)n * 3 + k) * N;
+            }
+            /* g = steps*(div + pt - P.pt - u/cc) */
+            for (g = 0; g < N; ++g) G[g] = steps * (((dvn[g] + ptn[g]) - ppt[g]) - un[g] / cc);
+            /* faces with axis index >= 1 step by the difference of g across them:
+               2D p = p + (g_hi - g_lo) (asetsHMF2D.m:225-226), 3D p = p + g_hi - g_lo
+               (asetsHMF3D.m:210-212) */
+            for (z = 0; z < nz; ++z)
+                for (x = 0; x < nx; ++x)
+                    for (y = 0; y < ny; ++y) {
+                        const int64_t c3[3] = {y, x, z};
+                        g = z * st[2] + x * st[1] + y;
+                        for (k = 0; k < nd; ++k) {
+                            if (c3[k] == 0) continue;
+                            if (is3d) fn[k][g] = (fn[k][g] + G[g]) - G[g - st[k]];
+                            else fn[k][g] = fn[k][g] + (G[g] - G[g - st[k]]);
+                        }
+                    }
+            if (shape) {
+                /* exemption amount per voxel (asetsHMF3D.m:217-223): dim-1 low face
+                   if negative, dim-1 high face if positive, then dims 2 and 3 high
+                   face if positive, low face if negative, each times l, clipped at 0 */
+                for (z = 0; z < nz; ++z)
+                    for (x = 0; x < nx; ++x)
+                        for (y = 0; y < ny; ++y) {
+                            const int64_t c3[3] = {y, x, z}, n3[3] = {ny, nx, nz};
+                            REAL a = 0, t;
+                            g = z * st[2] + x * st[1] + y;
+                            for (k = 0; k < nd; ++k) {
+                                const REAL lo = (c3[k] > 0) ? fn[k][g] : (REAL)0;
+                                const REAL hi = (c3[k] + 1 < n3[k]) ? fn[k][g + st[k]] : (REAL)0;
+                                const REAL lk = ln[k][g];
+                                if (k == 0) {
+                                    t = (lo < 0) ? lo * lk : (REAL)0;
+                                    a = (t > 0) ? t : (REAL)0;
+                                    t = (hi > 0) ? hi * lk : (REAL)0;
+                                    a = a + ((t > 0) ? t : (REAL)0);
+                                } else {
+                                    t = (hi > 0) ? hi * lk : (REAL)0;
+                                    a = a + ((t > 0) ? t : (REAL)0);
+                                    t = (lo < 0) ? lo * lk : (REAL)0;
+                                    a = a + ((t > 0) ? t : (REAL)0);
+                                }
+                            }
+                            A[g] = a;
+                        }
+                /* exemption per face (:226-233) and its removal (:236-238) */
+                for (k = 0; k < nd; ++k) memset(EX[k], 0, (size_t)N * sizeof(REAL));
+                for (z = 0; z < nz; ++z)
+                    for (x = 0; x < nx; ++x)
+                        for (y = 0; y < ny; ++y) {
+                            const int64_t c3[3] = {y, x, z};
+                            g = z * st[2] + x * st[1] + y;
+                            for (k = 0; k < nd; ++k) {
+                                const int64_t w = g - st[k];   /* the low voxel of the face */
+                                REAL p, e = 0;
+                                if (c3[k] == 0) continue;
+                                p = fn[k][g];
+                                if (p > 0 && ln[k][w] > 0) e = ln[k][w] * A[w];
+                                else if (p < 0 && ln[k][g] < 0) e = ln[k][g] * A[g];
+                                EX[k][g] = e;
+                                fn[k][g] = p - e;
+                            }
+                        }
+            }
+            /* flow magnitude and capacity factor (:241-258 / :266-282) */
+            for (z = 0; z < nz; ++z)
+                for (x = 0; x < nx; ++x)
+                    for (y = 0; y < ny; ++y) {
+                        const int64_t c3[3] = {y, x, z}, n3[3] = {ny, nx, nz};
+                        REAL t = 0, m;
+                        g = z * st[2] + x * st[1] + y;
+                        for (k = 0; k < nd; ++k) {
+                            const REAL lo = (c3[k] > 0) ? fn[k][g] : (REAL)0;
+                            const REAL hi = (c3[k] + 1 < n3[k]) ? fn[k][g + st[k]] : (REAL)0;
+                            if (k == 0) t = lo * lo + hi * hi;
+                            else t = (t + hi * hi) + lo * lo;
+                        }
+                        m = (REAL)sqrt((double)t);
+                        G[g] = (m <= al[g]) ? (REAL)1 : al[g] / m;
+                    }
+            /* rescale: p = [ex +] 0.5*p.*(g_hi + g_lo) (:259-261 / :283-285) */
+            for (z = 0; z < nz; ++z)
+                for (x = 0; x < nx; ++x)
+                    for (y = 0; y < ny; ++y) {
+                        const int64_t c3[3] = {y, x, z};
+                        g = z * st[2] + x * st[1] + y;
+                        for (k = 0; k < nd; ++k) {
+                            REAL r;
+                            if (c3[k] == 0) continue;
+                            r = ((REAL)0.5 * fn[k][g]) * (G[g] + G[g - st[k]]);
+                            fn[k][g] = shape ? EX[k][g] + r : r;
+                        }
+                    }
+            /* divergence (:288-295) */
+            for (z = 0; z < nz; ++z)
+                for (x = 0; x < nx; ++x)
+                    for (y = 0; y < ny; ++y) {
+                        const int64_t c3[3] = {y, x, z}, n3[3] = {ny, nx, nz};
+                        REAL d = 0;
+                        g = z * st[2] + x * st[1] + y;
+                        for (k = 0; k < nd; ++k) {
+                            const REAL lo = (c3[k] > 0) ? fn[k][g] : (REAL)0;
+                            const REAL hi = (c3[k] + 1 < n3[k]) ? fn[k][g + st[k]] : (REAL)0;
+                            d = (k == 0) ? hi - lo : (d + hi) - lo;
+                        }
+                        dvn[g] = d;
+                    }
+        }
+    }
+
+    /* CalculateCostsFullFlow: own terms, then subtree totals in reverse
+       breadth-first order (children before parents; children added in order) */
+    for (n = 0; n < nnodes; ++n) {
+        double dc = 0.0, rc = 0.0;
+        const int64_t o = (int64_t)n * N;
+        if (first_child[n] < 0)
+            for (g = 0; g < N; ++g) dc += (double)(u[o + g] * Ct[o + g]);
+        if (n != root)
+            for (g = 0; g < N; ++g) rc += (double)(u[o + g] * div[o + g]);
+        data_cost[n] = dc;
+        regu_cost[n] = rc;
+    }
+    for (k = norder - 1; k >= 0; --k) {
+        int c;
+        n = order[k];
+        for (c = first_child[n]; c >= 0; c = next_sib[c]) {
+            data_cost[n] += data_cost[c];
+            regu_cost[n] += regu_cost[c];
+        }
+    }
+    if (l_out) memcpy(l_out, L, (size_t)(nnodes * 3 * N) * sizeof(REAL));
+    free(G); free(A); free(L); free(EX[0]); free(EX[1]); free(EX[2]);
+    return ORC_OK;
+}
