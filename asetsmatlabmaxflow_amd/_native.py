@@ -23,7 +23,7 @@ EXPORTED = (
     "asets_potts_dev",
     "asets_binary3d_batched_dev", "asets_binary_energy_dev", "asets_potts_energy_dev",
     "asets_cost_loglik_dev", "asets_cost_neglog_dev", "asets_alpha_contrast_dev",
-    "asets_labels_dev",
+    "asets_labels_dev", "asets_hmf_dev",
     "asets_selftest_division", "asets_last_error",
     "asets_version", "asets_release",
 )
@@ -44,6 +44,14 @@ class Volume(C.Structure):
     _fields_ = [("Cs", C.c_void_p), ("Ct", C.c_void_p), ("alpha", C.c_void_p),
                 ("ny", C.c_int64), ("nx", C.c_int64), ("nz", C.c_int64),
                 ("u", C.c_void_p), ("cvg", C.c_void_p), ("it_num", C.c_void_p)]
+
+
+class HmfNode(C.Structure):
+    _fields_ = [("parent", C.c_int32), ("alpha", C.c_float), ("alpha_map", C.c_void_p),
+                ("Ct", C.c_void_p), ("l", C.c_void_p * 3), ("u", C.c_void_p),
+                ("pt", C.c_void_p), ("div", C.c_void_p), ("faces", C.c_void_p * 3),
+                ("l_out", C.c_void_p * 3), ("data_cost", C.c_double),
+                ("regu_cost", C.c_double)]
 
 
 class AsetsError(RuntimeError):
@@ -94,6 +102,8 @@ def lib() -> C.CDLL:
         L.asets_alpha_contrast_dev.argtypes = [C.c_int, P, I64, I64, I64, C.c_double, C.c_double,
                                                C.c_double, C.c_int, C.c_int, P, PO]
         L.asets_labels_dev.argtypes = [C.c_int, P, I64, C.c_int32, P, PO]
+        L.asets_hmf_dev.argtypes = [C.c_int, I64, I64, I64, C.POINTER(HmfNode), C.c_int32,
+                                    C.c_int32, C.c_float, C.c_float, PD, PO]
         L.asets_selftest_division.argtypes = [C.c_int, C.c_float, C.POINTER(C.c_uint64)]
         for name in EXPORTED:
             if name in ("asets_last_error", "asets_version"):

@@ -29,6 +29,7 @@
 #include "kernels_energy.h"
 #include "kernels_prepost.h"
 #include "kernels_fused.h"
+#include "kernels_hmf.h"
 #include "kernels_passes.h"
 
 using namespace asets;
@@ -1587,6 +1588,8 @@ asets_status alpha_contrast(int ndim, const float *img, int64_t ny, int64_t nx, 
     return ASETS_OK;
 }
 
+#include "hmf_host.h"
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1776,6 +1779,12 @@ asets_status asets_selftest_division(int mode, float d, uint64_t *mismatches) {
     if (e != hipSuccess) return fail(ASETS_EDEVICE, "selftest: %s", hipGetErrorString(e));
     *mismatches = h;
     return ASETS_OK;
+}
+
+asets_status asets_hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz, asets_hmf_node *nodes,
+                           int32_t nnodes, int32_t num_iter, float steps, float cc,
+                           double *seconds, const asets_opts *o) {
+    return hmf_dev(ndim, ny, nx, nz, nodes, nnodes, num_iter, steps, cc, seconds, o);
 }
 
 const char *asets_last_error(void) { return g_last_error.c_str(); }

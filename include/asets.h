@@ -258,6 +258,42 @@ asets_status asets_alpha_contrast_dev(int ndim, const float *img, int64_t ny, in
 asets_status asets_labels_dev(int model, const float *u, int64_t n, int32_t nlab,
                               uint8_t *labels, const asets_opts *o);
 
+/* ---- hierarchical max-flow (HMF), full-flow variant -----------------------
+ * Replaces the MATLAB classes maxflow/asetsHMF2D.m and asetsHMF3D.m: the tree
+ * built by the constructor asetsHMF2D(children, alpha, Ct) (:57-76, one node per
+ * object), the public properties lx, ly(, lz) (shape constraint) and Name, and
+ * MaxFullFlow(numIts, steps, cc) (2D :78-88, 3D :77-87: InitializeFullFlow, then
+ * numIts times UpdateSinkFlows, UpdateLabels, UpdateSpatialFlows, then
+ * CalculateCostsFullFlow).  The pseudo-flow variant (MaxPseudoFlow) and the
+ * plotting of named nodes are not provided.
+ *
+ * One entry per object of the tree.  parent is the index of the parent entry (-1
+ * for the single root); the children of a node are taken in increasing index
+ * order, the order of the reference's cell array.  A node without children is a
+ * leaf and needs Ct.  Arithmetic is fp32 (the reference computes in the class of
+ * Ct; single-precision Ct gives these results element for element).  Device
+ * arrays of N = ny*nx*nz floats, column-major; faces in own-face layout: faces[0]
+ * along dim 1 (px), faces[1] along dim 2 (py), faces[2] along dim 3 (pz), each
+ * voxel holding the face to its lower neighbour, so px(i,j,k) = faces[0][g] at
+ * row i (0-based) and the index-0 faces are 0.  Output pointers may be NULL.
+ * ndim == 2 needs nz == 1.  At most 4096 nodes and 16 levels of internal nodes. */
+typedef struct asets_hmf_node {
+    int32_t parent;          /* parent entry, -1 for the root                              */
+    float alpha;             /* scalar alpha, used when alpha_map is NULL                  */
+    const float *alpha_map;  /* per-voxel alpha [N], or NULL                               */
+    const float *Ct;         /* data cost [N] of a leaf                                    */
+    const float *l[3];       /* shape-constraint directions lx, ly(, lz) [N]: all or none  */
+    float *u, *pt, *div;     /* out [N]: label, sink flow, divergence of the spatial flow  */
+    float *faces[3];         /* out [N]: spatial flows px, py(, pz) in own-face layout     */
+    float *l_out[3];         /* out [N]: the normalised directions (InitializeFullFlow)    */
+    double data_cost;        /* out: dataCost of the subtree (CalculateCostsFullFlow)      */
+    double regu_cost;        /* out: reguCost of the subtree                               */
+} asets_hmf_node;
+
+asets_status asets_hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz,
+                           asets_hmf_node *nodes, int32_t nnodes, int32_t num_iter,
+                           float steps, float cc, double *seconds, const asets_opts *o);
+
 /* ---- misc ---------------------------------------------------------------- */
 /* Self-check of the short correctly-rounded division form the kernels use for
  * x/cc, n/alpha and 1/n (no reference counterpart): compares it with IEEE

@@ -1,0 +1,191 @@
+"""GPU parity of the hierarchical max-flow solve (asets_hmf_dev) against the oracle.
+
+Bit for bit in fp32 (u, pt, div and every face, all nodes), on the fused kernel and on
+the per-pass kernels, 2D and 3D, flat and nested trees, scalar and per-voxel alpha,
+shape constraints, volumes that do not fill a tile; the costs within 1e-9 (FP64 sums in
+another order).  The class mirror (asetsHMF2D / asetsHMF3D) is checked against the
+same oracle.  The fp32 result's distance from the FP64 oracle is bounded.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _costs(shape, nlab, seed, shift=0.0):
+    rng = np.random.default_rng(seed)
+    grids = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+    lab = np.zeros(shape, int)
+    for k in range(1, nlab):
+        c = [rng.uniform(0.2, 0.8) * s for s in shape]
+        lab[sum((g - ci) ** 2 for g, ci in zip(grids, c)) < (0.25 * min(shape)) ** 2] = k
+    img = lab + 0.45 * rng.standard_normal(shape)
+    return [np.abs(img - k).astype(np.float32) - np.float32(shift) for k in range(nlab)]
+
+
+FLAT4 = [-1, 0, 0, 0, 0]
+NESTED = [-1, 0, 0, 1, 1, 2, 2, 2]   # root -> [B1 -> (a, b), B2 -> (c, d, e)]
+
+
+def _problem(shape, parent, seed, alpha_map=False, shape_nodes=()):
+    rng = np.random.default_rng(seed)
+    M = len(parent)
+    leaves = [n for n in range(M) if n not in parent]
+    costs = _costs(shape, len(leaves), seed, shift=0.3)
+    Ct = [None] * M
+    for k, n in enumerate(leaves):
+        Ct[n] = costs[k]
+    alpha = [0.0] + [float(rng.uniform(0.1, 0.3)) for _ in range(M - 1)]
+    if alpha_map:
+        for n in range(1, M, 2):
+            alpha[n] = rng.uniform(0.05, 0.4, shape).astype(np.float32)
+    l = None
+    if shape_nodes:
+        l = [None] * M
+        for n in shape_nodes:
+            l[n] = [rng.standard_normal(shape).astype(np.float32) for _ in shape]
+            l[n][0][rng.random(shape) < 0.1] = 0   # some short directions
+            for q in l[n][1:]:
+                q[rng.random(shape) < 0.1] = 0
+    return Ct, alpha, l
+
+
+def _check(res, ref, shape, M):
+    from asetsmatlabmaxflow_amd.hmf import _matlab_view
+    nd = len(shape)
+    for n in range(M):
+        for name, got, want in (("u", res.u[n], ref.u[n]), ("pt", res.pt[n], ref.pt[n]),
+                                ("div", res.div[n], ref.div[n])):
+            g = _matlab_view(got, shape).cpu().numpy()
+            np.testing.assert_array_equal(g, want, err_msg=f"node {n} {name}")
+        for k in range(nd):
+            g = _matlab_view(res.faces[n][k], shape).cpu().numpy()
+            np.testing.assert_array_equal(g, ref.faces[n, k], err_msg=f"node {n} face {k}")
+    np.testing.assert_allclose(res.data_cost, ref.data_cost, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(res.regu_cost, ref.regu_cost, rtol=1e-9, atol=1e-7)
+
+
+CASES = [
+    # shape, tree, alpha maps, kernel, iterations
+    ((64, 48, 20), FLAT4, False, "fused", 25),
+    ((70, 37, 11), NESTED, True, "fused", 20),
+    ((130, 90), NESTED, True, "fused", 30),
+    ((50, 33, 9), NESTED, True, "passes", 15),
+    ((61, 45), FLAT4, False, "passes", 30),
+    ((3, 5, 4), FLAT4, True, "fused", 12),
+    ((1, 7, 3), [-1, 0, 0], False, "fused", 8),
+]
+
+
+@pytest.mark.parametrize("shape,parent,amap,kernel,its", CASES)
+def test_hmf_matches_oracle_bitwise(cuda, shape, parent, amap, kernel, its):
+    from asetsmatlabmaxflow_amd.hmf import solve_tree
+    print(f"hmf case shape={shape} nodes={len(parent)} kernel={kernel}", flush=True)
+    Ct, alpha, _ = _problem(shape, parent, seed=sum(shape) + len(parent), alpha_map=amap)
+    steps, cc = 0.12, 0.3
+    res = solve_tree(parent, Ct, alpha, shape, its, steps, cc, kernel=kernel)
+    ref = O.hmf(parent, Ct, alpha, shape, its, steps, cc)
+    _check(res, ref, shape, len(parent))
+    assert np.abs(ref.faces).max() > 1e-3   # flows moved
+
+
+@pytest.mark.parametrize("shape", [(40, 30), (24, 20, 9)])
+def test_hmf_shape_constraint_matches_oracle_bitwise(cuda, shape):
+    from asetsmatlabmaxflow_amd.hmf import _matlab_view, solve_tree
+    print(f"hmf shape-constraint case {shape}", flush=True)
+    Ct, alpha, l = _problem(shape, NESTED, seed=7, alpha_map=True, shape_nodes=(1, 3, 6))
+    res = solve_tree(NESTED, Ct, alpha, shape, 20, 0.15, 0.3, l=l)
+    ref = O.hmf(NESTED, Ct, alpha, shape, 20, 0.15, 0.3, l=l)
+    _check(res, ref, shape, len(NESTED))
+    for n in (1, 3, 6):
+        for k in range(len(shape)):
+            np.testing.assert_array_equal(_matlab_view(res.l[n][k], shape).cpu().numpy(),
+                                          ref.l[n][k])
+
+
+def test_hmf_fused_equals_passes(cuda):
+    from asetsmatlabmaxflow_amd.hmf import solve_tree
+    shape = (96, 40, 24)
+    Ct, alpha, _ = _problem(shape, NESTED, seed=3, alpha_map=True)
+    a = solve_tree(NESTED, Ct, alpha, shape, 40, 0.1, 0.25, kernel="fused")
+    b = solve_tree(NESTED, Ct, alpha, shape, 40, 0.1, 0.25, kernel="passes")
+    for n in range(len(NESTED)):
+        assert torch_equal(a.u[n], b.u[n]) and torch_equal(a.div[n], b.div[n])
+        for k in range(3):
+            assert torch_equal(a.faces[n][k], b.faces[n][k])
+
+
+def torch_equal(x, y):
+    import torch
+    return bool(torch.equal(x, y))
+
+
+def test_hmf_zero_iterations_and_errors(cuda):
+    from asetsmatlabmaxflow_amd import _native as nat
+    from asetsmatlabmaxflow_amd.hmf import solve_tree
+    shape = (16, 12)
+    Ct, alpha, l = _problem(shape, FLAT4, seed=1, shape_nodes=(2,))
+    r = solve_tree(FLAT4, Ct, alpha, shape, 0, 0.1, 0.3)
+    assert all(not bool(t.any()) for t in r.u + r.pt + r.div)
+    assert r.data_cost[0] == 0.0
+    bad = [([-1, -1, 0], "more than one root"), ([-1, 0, 3, 2], "not reachable"),
+           ([-1, 0, 7], "bad parent"), ([-1, 1, 0], "bad parent"), ([-1, 2, 1], "no children")]
+    for parent, msg in bad:
+        with pytest.raises(nat.AsetsError, match=msg):
+            solve_tree(parent, [None] + [Ct[1]] * (len(parent) - 1), [0.1] * len(parent),
+                       shape, 1, 0.1, 0.3)
+    with pytest.raises(nat.AsetsError, match="has no Ct"):
+        solve_tree([-1, 0, 0], [None, Ct[1], None], [0, 0.1, 0.1], shape, 1, 0.1, 0.3)
+    with pytest.raises(nat.AsetsError, match="radius 3"):
+        solve_tree(FLAT4, Ct, alpha, shape, 1, 0.1, 0.3, l=l, kernel="fused")
+
+
+def test_hmf_classes_mirror_the_reference(cuda):
+    from asetsmatlabmaxflow_amd.hmf import asetsHMF2D, asetsHMF3D
+    for cls, shape in ((asetsHMF2D, (45, 38)), (asetsHMF3D, (30, 26, 12))):
+        costs = _costs(shape, 3, seed=11)
+        a = cls([], 0.2, costs[0])
+        b = cls([], 0.25, costs[1])
+        c = cls([], 0.15, costs[2])
+        branch = cls([a, b], 0.1)
+        root = cls([branch, c], 0)
+        rng = np.random.default_rng(2)
+        dirs = [rng.standard_normal(shape).astype(np.float32) for _ in shape]
+        branch.lx, branch.ly = dirs[0], dirs[1]
+        if len(shape) == 3:
+            branch.lz = dirs[2]
+        root.MaxFullFlow(30, 0.1, 1 / 3)
+        # the same tree as parent indices, breadth-first: root, branch, c, a, b
+        parent = [-1, 0, 0, 1, 1]
+        Ct = [None, None, costs[2], costs[0], costs[1]]
+        alpha = [0, 0.1, 0.15, 0.2, 0.25]
+        l = [None, dirs, None, None, None]
+        ref = O.hmf(parent, Ct, alpha, shape, 30, 0.1, 1 / 3, l=l)
+        for n, h in enumerate([root, branch, c, a, b]):
+            np.testing.assert_array_equal(h.u.cpu().numpy(), ref.u[n])
+            np.testing.assert_array_equal(h.pt.cpu().numpy(), ref.pt[n])
+            np.testing.assert_array_equal(h.div.cpu().numpy(), ref.div[n])
+            assert tuple(h.px.shape) == (shape[0] - 1,) + shape[1:]
+            np.testing.assert_array_equal(h.px.cpu().numpy(), ref.faces[n, 0][1:])
+            np.testing.assert_array_equal(h.py.cpu().numpy(), ref.faces[n, 1][:, 1:])
+            if len(shape) == 3:
+                np.testing.assert_array_equal(h.pz.cpu().numpy(), ref.faces[n, 2][:, :, 1:])
+            assert h.dataCost == pytest.approx(ref.data_cost[n], rel=1e-9, abs=1e-9)
+            assert h.reguCost == pytest.approx(ref.regu_cost[n], rel=1e-9, abs=1e-7)
+        np.testing.assert_array_equal(branch.lx.cpu().numpy(), ref.l[1][0])
+        with pytest.raises(NotImplementedError):
+            a.MaxFullFlow(1, 0.1, 0.3)
+
+
+def test_hmf_fp32_error_against_fp64(cuda):
+    from asetsmatlabmaxflow_amd.hmf import _matlab_view, solve_tree
+    shape = (48, 40, 16)
+    Ct, alpha, _ = _problem(shape, NESTED, seed=5)
+    res = solve_tree(NESTED, Ct, alpha, shape, 150, 0.1, 0.3)
+    ref = O.hmf(NESTED, Ct, alpha, shape, 150, 0.1, 0.3, precision="f64")
+    err = max(np.abs(_matlab_view(res.u[n], shape).cpu().numpy() - ref.u[n]).max()
+              for n in range(1, len(NESTED)))
+    print(f"hmf fp32 vs fp64 max |du| after 150 iterations: {err:.3e}")
+    assert err < 1e-3
