@@ -163,3 +163,44 @@ def test_fused_size_limit_rejected_before_device_work():
     rc = L.asets_ishikawa_dev(3, ptr, ptr, 1024, 1024, 1024, 3, C.byref(p), ptr, ptr, ptr, None,
                               None)
     assert rc == nat.ASETS_ETOOBIG
+
+
+def test_hmf_tree_validation_before_device_work():
+    L = nat.lib()
+    buf = np.zeros(64, np.float32)
+    ptr = buf.ctypes.data
+
+    def call(parent, ct=None, l=None, ndim=2, nz=1, its=3, cc=0.3, kernel=nat.KERNEL_AUTO):
+        M = len(parent)
+        nodes = (nat.HmfNode * M)()
+        for n, p in enumerate(parent):
+            nodes[n].parent = p
+            nodes[n].alpha = 0.1
+            nodes[n].Ct = ptr if (ct is None or ct[n]) else None
+            if l is not None and l[n]:
+                for k in range(l[n]):
+                    nodes[n].l[k] = ptr
+        o = nat.make_opts(kernel=kernel)
+        return L.asets_hmf_dev(ndim, 4, 4, nz, nodes, M, its, 0.1, cc, None, C.byref(o))
+
+    cases = [
+        (dict(parent=[-1, -1, 0]), "more than one root"),
+        (dict(parent=[0, 1]), "bad parent"),
+        (dict(parent=[-1, 0, 4]), "bad parent"),
+        (dict(parent=[-1, 2, 1]), "no children"),
+        (dict(parent=[-1, 0, 3, 2]), "not reachable"),
+        (dict(parent=[-1, 0, 0], ct=[0, 1, 0]), "has no Ct"),
+        (dict(parent=[-1, 0, 0], l=[0, 1, 0]), "shape directions"),
+        (dict(parent=[-1, 0, 0], l=[0, 2, 0], kernel=nat.KERNEL_FUSED), "radius 3"),
+        (dict(parent=[-1]), "2 to 4096"),
+        (dict(parent=[-1, 0], its=-1), "numIts"),
+        (dict(parent=[-1, 0], cc=0.0), "cc"),
+        (dict(parent=[-1, 0], nz=2), "nz == 1"),
+    ]
+    for kw, msg in cases:
+        assert call(**kw) == nat.ASETS_EBADARG, kw
+        assert msg.encode() in L.asets_last_error(), (kw, L.asets_last_error())
+    # 17 levels of internal nodes: a chain root -> ... -> leaf
+    chain = [-1] + list(range(17))
+    assert call(parent=chain) == nat.ASETS_ETOOBIG
+    assert b"deeper" in L.asets_last_error()
