@@ -93,6 +93,52 @@ asets_status hmf_plan(int ndim, const asets_hmf_node *in, int32_t M, HmfPlan &pl
     return ASETS_OK;
 }
 
+// z-chunk length of the fused flow kernel: a workgroup streams chunk + 4 planes
+// (the 4-plane prologue), and the grid runs in rounds of `slots` resident
+// workgroups, so the time is about rounds(tiles * chunks) * (chunk + 4).  The
+// smallest such cost wins; ties go to the longer chunk (fewer halo re-reads).
+int hmf_zchunk(int64_t tiles, int64_t nz, int64_t slots) {
+    static const int cand[] = {8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512};
+    int64_t best = -1, best_zc = nz;
+    auto consider = [&](int64_t zc) {
+        const int64_t nb = tiles * cdiv(nz, zc);
+        const int64_t cost = cdiv(nb, slots) * (zc + 4);
+        if (best < 0 || cost < best || (cost == best && zc > best_zc)) {
+            best = cost;
+            best_zc = zc;
+        }
+    };
+    for (int zc : cand)
+        if (zc < nz) consider(zc);
+    consider(nz);
+    return (int)best_zc;
+}
+
+template <bool IS3D, int TX, int NT>
+int hmf_resident_t() {
+    constexpr int W = (NT >= 512) ? 4 : 3;
+    int per_cu = 0, cus = 0, dev = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf_hmf_flow<IS3D, TX, NT, W>, NT,
+                                                     0) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return 512;
+    }
+    return std::max(1, per_cu) * std::max(1, cus);
+}
+
+int hmf_resident(bool is3d, int tx) {
+    if (is3d) {
+        if (tx == 8) return hmf_resident_t<true, 8, 256>();
+        if (tx == 32) return hmf_resident_t<true, 32, 1024>();
+        return hmf_resident_t<true, 16, 512>();
+    }
+    if (tx == 8) return hmf_resident_t<false, 8, 256>();
+    if (tx == 32) return hmf_resident_t<false, 32, 1024>();
+    return hmf_resident_t<false, 16, 512>();
+}
+
 template <bool IS3D, int TX, int NT>
 void launch_hmf_flow_t(const HmfArgs &a, int grid, hipStream_t s) {
     constexpr int W = (NT >= 512) ? 4 : 3;
@@ -230,13 +276,8 @@ asets_status hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz, asets_hmf_nod
         tx = (o && o->tile_x > 0) ? o->tile_x : 16;
         if (tx != 8 && tx != 32) tx = 16;
         const int64_t tiles = cdiv(ny, 64) * cdiv(nx, tx) * nflow;
-        int zc;
-        if (o && o->zchunk > 0) {
-            zc = (int)std::min<int64_t>(o->zchunk, nz);
-        } else {
-            const int64_t nzc = std::max<int64_t>(1, (512 + tiles / 2) / tiles);
-            zc = (int)std::min<int64_t>(nz, std::max<int64_t>(8, cdiv(nz, nzc)));
-        }
+        const int zc = (o && o->zchunk > 0) ? (int)std::min<int64_t>(o->zchunk, nz)
+                                            : hmf_zchunk(tiles, nz, hmf_resident(is3d, tx));
         fa.nodes = d_nodes;
         fa.flow = d_flow;
         fa.nflow = nflow;

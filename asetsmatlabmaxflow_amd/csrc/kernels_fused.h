@@ -666,7 +666,24 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
 #pragma unroll
             for (int k = 0; k < K2; ++k) bz_cur[k] = ld<IO>(rbz, off2[k]);
         }
-        for (int s = z0 - 4; s <= z1; ++s) step(s, r, o);
+        if constexpr (IS3D) {
+            for (int s = z0 - 4; s <= z1; ++s) step(s, r, o);
+        } else {
+            // one plane: its stages in dependency order, no z pipeline; alpha of
+            // plane 0 (load_raw fetches the previous plane's for the pipeline)
+            load_raw(0, r);
+            const PRef ra = pref(a.alpha, 0, nz, 4, plane);
+#pragma unroll
+            for (int k = 0; k < K1; ++k) r.al[k] = ldf(ra, off1[k]);
+            load_out(0, o);
+            s1(0, r);
+            __syncthreads();
+            s2a(0, r);
+            __syncthreads();
+            s2b(0, r);
+            __syncthreads();
+            s3(0, o);
+        }
     }
     return e_acc;
 }

@@ -399,7 +399,24 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
 #pragma unroll
         for (int k = 0; k < K2; ++k) bz_cur[k] = ldf(rz, off2[k]);
     }
-    for (int s = z0 - 4; s <= z1; ++s) step(s, r);
+    if constexpr (IS3D) {
+        for (int s = z0 - 4; s <= z1; ++s) step(s, r);
+    } else {
+        // one plane: its stages in dependency order, no z pipeline
+        load_raw(0, r);
+        if (amap) {
+            const PRef ra = pref(amap, 0, nz, 4, plane);
+#pragma unroll
+            for (int k = 0; k < K1; ++k) r.al[k] = ldf(ra, off1[k]);
+        }
+        s1(0, r);
+        __syncthreads();
+        s2a(0, r);
+        __syncthreads();
+        s2b(0, r);
+        __syncthreads();
+        s3(0);
+    }
 }
 
 template <bool IS3D, int TX, int NT, int WAVES>

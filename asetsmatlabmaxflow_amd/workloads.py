@@ -175,3 +175,35 @@ def ishikawa3d_voronoi(shape=(192, 192, 192), nlab: int = 6, seed: int = 1003):
 
 
 ISHIKAWA3D_CC, ISHIKAWA3D_STEPS = 0.75, 0.1
+
+
+def hmf_linear2d(shape=(256, 256), levels: int = 40, seed: int = 1005, noise: float = 0.15):
+    """The linear (Ishikawa-like) reconstruction model of tests/runTestHMFLinearModel.m:11-27
+    on a synthetic image (the test's cell.tif is not in the repository): a smooth image in
+    [0, 1] plus Gaussian noise, leaf i of a flat tree with Ct_i = sqrt(|image - i/(L-1)|).
+    Returns (costs [L arrays], noisy image, clean image)."""
+    from scipy.ndimage import gaussian_filter
+
+    rng = np.random.default_rng(seed)
+    clean = gaussian_filter(rng.random(shape), 6.0)
+    clean = (clean - clean.min()) / (clean.max() - clean.min())
+    img = clean + noise * rng.standard_normal(shape)
+    costs = [np.sqrt(np.abs(img - i / (levels - 1))).astype(np.float32) for i in range(levels)]
+    return costs, img.astype(np.float32), clean.astype(np.float32)
+
+
+HMF2D_ALPHA, HMF2D_ITERS, HMF2D_STEPS = 0.15, 200, 0.1
+
+
+def hmf3d_tree(shape=(192, 192, 192), seed: int = 1006):
+    """A two-level label tree on the Voronoi intensity volume of potts3d_voronoi (6 labels,
+    intensities l/5): root -> [dark -> (0, 1, 2), bright -> (3, 4, 5)].  Returns
+    (parent, Ct per node (None for internal nodes), alpha per node, labels)."""
+    Ct, _, lab = potts3d_voronoi(shape, nlab=6, seed=seed)
+    parent = [-1, 0, 0, 1, 1, 1, 2, 2, 2]
+    costs = [None, None, None] + [np.asfortranarray(Ct[..., k]) for k in range(6)]
+    alpha = [0.0, 0.5, 0.5] + [1.0] * 6
+    return parent, costs, alpha, lab
+
+
+HMF3D_ITERS, HMF3D_CC, HMF3D_STEPS = 100, 0.25, 0.11

@@ -17,10 +17,14 @@ Other workloads (one JSON line each, same keys):
             graph-replayed persistent launch per iteration, to errBound 1e-4, fp32 and
             fp16 flow storage
   ishikawa3d  192^3, L = 6 ordered labels (seed 1003), 300 fixed iterations, cc 0.75
+  hmf2d     256x256 hierarchical max-flow, flat tree of 40 leaves (the linear model of
+            tests/runTestHMFLinearModel.m, seed 1005), 200 fixed iterations
+  hmf3d     192^3 hierarchical max-flow, root -> 2 branches -> 3 leaves (seed 1006), 100
+            fixed iterations
 
 Algorithmic bytes per voxel-iteration: binary 3D 60 (fp32 flows) / 40 (fp16 flows),
 binary 2D 52, Potts 3D 48L+8, Ishikawa 3D 12L+36(L-1) (read Ct, alpha; read and write u,
-pt and the three face fields).
+pt and the three face fields), HMF 8 + 60 (3D) or 52 (2D) per non-root node.
 """
 from __future__ import annotations
 
@@ -305,6 +309,82 @@ def bench_ishikawa3d(args):
     return out
 
 
+def _hmf_bytes(parent, nd):
+    """Algorithmic bytes per voxel-iteration of the fused HMF path: sink (read div, u and
+    Ct or the previous pt, write pt: 16 B per non-root node, 8 B at the root) plus the
+    fused flow kernel per non-root node (read u, pt, parent pt and nd faces, write u, nd
+    faces and div: 44 B in 3D, 36 B in 2D; scalar alpha)."""
+    nflow = sum(1 for p in parent if p >= 0)
+    return 8 + nflow * (16 + (44 if nd == 3 else 36))
+
+
+def _bench_hmf(args, parent, Ct, alpha, shape, iters, steps, cc, metric, data, workload):
+    import numpy as np
+    import torch
+
+    from asetsmatlabmaxflow_amd import hmf
+
+    N = int(np.prod(shape))
+    kern = "passes" if args.kernel == "passes" else "fused"
+    for _ in range(args.warmup):
+        hmf.solve_tree(parent, Ct, alpha, shape, 3, steps, cc, kernel=kern)
+    torch.cuda.synchronize()
+    # inputs resident on the device before the timed region
+    dev = [None if c is None else hmf._flat_dev(c, shape).view(*shape[::-1]).permute(
+        *range(len(shape) - 1, -1, -1)) for c in Ct]
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        r = hmf.solve_tree(parent, dev, alpha, shape, iters, steps, cc, kernel=kern,
+                           timing=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    out = _base(args, metric, N * iters * args.steps / wall, "voxel-iter/s",
+                wall * 1e3 / args.steps, "f32", data,
+                {"workload": workload, "nodes": len(parent), "iters_per_step": iters,
+                 "kernel": kern})
+    out["roofline"] = _roofline(_hmf_bytes(parent, len(shape)), N, r.kernel_ms)
+    out["data_cost"] = float(r.data_cost[0])
+    out["regu_cost"] = float(r.regu_cost[0])
+    if not args.no_cpu:
+        orc = _oracle()
+        k = max(1, args.cpu_iters // 10)
+        t = time.perf_counter()
+        orc.hmf(parent, Ct, alpha, shape, k, steps, cc)
+        cpu_s = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": N * k / cpu_s, "unit": "voxel-iter/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{k} iterations incl. init, FP32 reference-order "
+                                         f"oracle, {cpu_s:.1f} s"}
+    return out
+
+
+def bench_hmf2d(args):
+    from asetsmatlabmaxflow_amd import workloads as W
+
+    L = 40
+    costs, _, _ = W.hmf_linear2d((256, 256), L, seed=1005)
+    parent = [-1] + [0] * L
+    return _bench_hmf(args, parent, [None] + costs, [0.0] + [W.HMF2D_ALPHA] * L, (256, 256),
+                      W.HMF2D_ITERS, W.HMF2D_STEPS, 1.0 / L,
+                      "voxel-iters/s, 256^2 HMF linear model, 40 leaves (200 fixed iterations)",
+                      "synthetic (workloads.hmf_linear2d seed 1005)",
+                      "hmf2d 256^2 flat tree of 40 leaves, Ct_i = sqrt(|img - i/39|), alpha "
+                      "0.15, cc 1/40 steps 0.1 (runTestHMFLinearModel.m)")
+
+
+def bench_hmf3d(args):
+    from asetsmatlabmaxflow_amd import workloads as W
+
+    shape = (192, 192, 192)
+    parent, Ct, alpha, _ = W.hmf3d_tree(shape, seed=1006)
+    return _bench_hmf(args, parent, Ct, alpha, shape, W.HMF3D_ITERS, W.HMF3D_STEPS, W.HMF3D_CC,
+                      "voxel-iters/s, 192^3 HMF two-level tree, 6 leaves (100 fixed iterations)",
+                      "synthetic (workloads.hmf3d_tree seed 1006)",
+                      "hmf3d 192^3 root -> 2 branches -> 3 leaves each, Voronoi intensities, "
+                      "alpha 0.5 branches 1.0 leaves, cc 0.25 steps 0.11")
+
+
 def bench_batched(args):
     import numpy as np
     import torch
@@ -362,7 +442,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="binary3d",
-                    choices=["binary3d", "binary2d", "potts3d", "batched", "ishikawa3d"])
+                    choices=["binary3d", "binary2d", "potts3d", "batched", "ishikawa3d", "hmf2d",
+                             "hmf3d"])
     ap.add_argument("--iters", type=int, default=200)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--nvol", type=int, default=64)
@@ -382,7 +463,8 @@ def main():
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
     fn = {"binary3d": bench_binary3d, "binary2d": bench_binary2d, "potts3d": bench_potts3d,
-          "batched": bench_batched, "ishikawa3d": bench_ishikawa3d}[args.workload]
+          "batched": bench_batched, "ishikawa3d": bench_ishikawa3d, "hmf2d": bench_hmf2d,
+          "hmf3d": bench_hmf3d}[args.workload]
     out = fn(args)
     if args.copy_calibration > 0:
         import torch
