@@ -17,22 +17,24 @@
 // Work decomposition.  A workgroup (NT threads) owns an in-plane tile of
 // TY (=64, the contiguous y axis) x TX voxels and a range of z planes, and
 // streams through the planes.  Every raw plane is loaded from HBM once per
-// workgroup; its loads are issued at the start of a step and consumed after
-// the step's output stage.  Per plane step:
+// workgroup; its loads are issued at the end of the previous step and consumed
+// after the step's output stage.  Per plane step:
 //   S1 (plane s+2)  gk over the tile + 2-voxel halo -> LDS GK ring (2), and
 //                   the stepped z face below plane s+2 -> LDS B1Z ring (3)
 //   S2a(plane s+2)  stepped in-plane faces b1 from gk -> LDS B1X/B1Y ring (3)
 //   S2b(plane s+1)  projection factor g over tile + 1 halo -> LDS G ring (2)
 //   S3 (plane s)    rescale, divergence, source/sink/multiplier update, store
 // in two barrier phases: {S3(s-1), S1(s+2)} | {S2a(s+2), S2b(s+1)}.  The
-// rescaled z face below the current output plane is carried in registers.  Halo values are recomputed by neighbouring
-// workgroups from identical inputs in the same operation order (same bits).
+// rescaled z face below the current output plane is carried in registers.
+// Halo values are recomputed by neighbouring workgroups from identical inputs in
+// the same operation order (same bits).
 //
 // Global accesses go through one buffer descriptor per field (size = the
 // field) with the plane offset in the bounds-checked vector offset: reads of a
 // plane outside the volume, or past the end of the field, return 0 -- exactly
-// the zero-flux value of a high face past the last voxel.  The state is double-buffered (in -> out) because
-// neighbouring workgroups read the halo of the previous iterate.
+// the zero-flux value of a high face past the last voxel.  The state is
+// double-buffered (in -> out) because neighbouring workgroups read the halo of
+// the previous iterate.
 #pragma once
 #include <hip/hip_fp16.h>
 
@@ -83,15 +85,25 @@ __device__ __forceinline__ PRef pref(const void *base, int q, int nz, int bytes_
                 q * plane};
 }
 
+// Byte offset of element `e` as the 32-bit vector offset of a buffer access,
+// computed modulo 2^32 (a negative element wraps above the field and reads 0).
+__device__ __forceinline__ int boff(int e, unsigned bytes) { return (int)((unsigned)e * bytes); }
+
+// Element index whose byte offset (0xfffffff0) is past every field the fused
+// kernels address: a store there is dropped by the buffer bounds check.  Used to
+// keep stores of lanes outside the volume unconditional (no wave-level branch
+// around them).
+__device__ __forceinline__ int drop_elem(unsigned bytes) { return (int)(0xfffffff0u / bytes); }
+
 template <typename T> struct Io;
 template <> struct Io<float> {
     static constexpr int kBytes = 4;
     // the b32 buffer builtins move raw 32-bit words: bit-cast, never convert
     static __device__ __forceinline__ float load(__amdgpu_buffer_rsrc_t r, int off) {
-        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off * 4, 0, 0));
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, boff(off, 4), 0, 0));
     }
     static __device__ __forceinline__ void store(__amdgpu_buffer_rsrc_t r, int off, float v) {
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off * 4, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, boff(off, 4), 0, 0);
     }
     static __device__ __forceinline__ float get(const float *p) { return *p; }
     static __device__ __forceinline__ void put(float *p, float v) { *p = v; }
@@ -99,25 +111,37 @@ template <> struct Io<float> {
 template <> struct Io<__half> {
     static constexpr int kBytes = 2;
     static __device__ __forceinline__ float load(__amdgpu_buffer_rsrc_t r, int off) {
-        unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, off * 2, 0, 0);
+        unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, boff(off, 2), 0, 0);
         return __half2float(__ushort_as_half(b));
     }
     static __device__ __forceinline__ void store(__amdgpu_buffer_rsrc_t r, int off, float v) {
-        __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(__float2half_rn(v)), r, off * 2,
-                                              0, 0);
+        __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(__float2half_rn(v)), r,
+                                              boff(off, 2), 0, 0);
     }
     static __device__ __forceinline__ float get(const __half *p) { return __half2float(*p); }
     static __device__ __forceinline__ void put(__half *p, float v) { *p = __float2half_rn(v); }
 };
 
 __device__ __forceinline__ float ldf(__amdgpu_buffer_rsrc_t r, int off) {
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off * 4, 0, 0));
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, boff(off, 4), 0, 0));
 }
 __device__ __forceinline__ float ldf(const PRef &p, int off) { return ldf(p.r, p.b + off); }
 template <typename IO>
 __device__ __forceinline__ float ld(const PRef &p, int off) { return IO::load(p.r, p.b + off); }
 template <typename IO>
 __device__ __forceinline__ void st(const PRef &p, int off, float v) { IO::store(p.r, p.b + off, v); }
+// store of lane-dependent validity: invalid lanes store past the field (dropped)
+template <typename IO>
+__device__ __forceinline__ void stm(const PRef &p, int off, float v, bool ok) {
+    IO::store(p.r, ok ? p.b + off : drop_elem(IO::kBytes), v);
+}
+// One in-plane offset for all stores of an output position to fields of one plane
+// (plane offset b): invalid lanes get the element 2^29-1 relative to the field,
+// past the end of every field of 2-, 4- or 8-byte elements that fused_fits allows
+// (N < 2^29), so all their stores are dropped with one select per position.
+__device__ __forceinline__ int masked_off(bool ok, int off, int b) {
+    return ok ? off : 0x1fffffff - b;
+}
 
 // The binary model's state keeps (ps, pt) and (bx, by) interleaved, one pair per
 // voxel (the first of the two at the lower address): a pair is one 64-bit (fp32)
@@ -132,29 +156,33 @@ template <typename T>
 __device__ __forceinline__ float2 ld_pair(const PRef &p, int off);
 template <>
 __device__ __forceinline__ float2 ld_pair<float>(const PRef &p, int off) {
-    const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(p.r, (p.b + off) * 8, 0, 0);
+    const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(p.r, boff(p.b + off, 8), 0, 0);
     return make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
 }
 template <>
 __device__ __forceinline__ float2 ld_pair<__half>(const PRef &p, int off) {
-    const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(p.r, (p.b + off) * 4, 0, 0);
+    const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(p.r, boff(p.b + off, 4), 0, 0);
     return make_float2(__half2float(__ushort_as_half((unsigned short)(w & 0xffffu))),
                        __half2float(__ushort_as_half((unsigned short)(w >> 16))));
 }
+// pair store; ok == false stores past the field (dropped)
 template <typename T>
-__device__ __forceinline__ void st_pair(const PRef &p, int off, float lo, float hi);
+__device__ __forceinline__ void st_pair(const PRef &p, int off, float lo, float hi,
+                                        bool ok = true);
 template <>
-__device__ __forceinline__ void st_pair<float>(const PRef &p, int off, float lo, float hi) {
+__device__ __forceinline__ void st_pair<float>(const PRef &p, int off, float lo, float hi,
+                                               bool ok) {
     u32x2 w;
     w.x = __float_as_uint(lo);
     w.y = __float_as_uint(hi);
-    __builtin_amdgcn_raw_buffer_store_b64(w, p.r, (p.b + off) * 8, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(w, p.r, boff(ok ? p.b + off : drop_elem(8), 8), 0, 0);
 }
 template <>
-__device__ __forceinline__ void st_pair<__half>(const PRef &p, int off, float lo, float hi) {
+__device__ __forceinline__ void st_pair<__half>(const PRef &p, int off, float lo, float hi,
+                                                bool ok) {
     const uint32_t w = (uint32_t)__half_as_ushort(__float2half_rn(lo)) |
                        ((uint32_t)__half_as_ushort(__float2half_rn(hi)) << 16);
-    __builtin_amdgcn_raw_buffer_store_b32(w, p.r, (p.b + off) * 4, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(w, p.r, boff(ok ? p.b + off : drop_elem(4), 4), 0, 0);
 }
 
 // ---- correctly rounded division, short form ------------------------------
@@ -447,6 +475,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     }
                 }
                 gkp[k] = v;
+                if (IS3D) bz_cur[k] = r.bzn[k];   // the next plane's own z face is current
             }
         };
 
@@ -532,7 +561,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             return rescale(gup, gc, B1Z[sn][l2], tiny, !wave_all(!tiny));
         };
 
-        // S3 light: only the carried z face for plane s+1 (prologue)
+        // S3 light: only the carried z face for plane s+1 (step z0)
         auto s3_carry = [&](int s) {
             const float *g0 = G[s & 1];
             const float *g1 = G[(s + 1) & 1];
@@ -545,7 +574,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             }
         };
 
-        // S3: output plane s
+        // S3: output plane s (z0 <= s < z1)
         auto s3 = [&](int s, const Out &o) {
             const int sl = (s + 3) % 3;
             const float *g0 = G[s & 1];
@@ -586,14 +615,14 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     d = (d + bzr) - bzl;
                     carry[k] = bzr;
                 }
+                // lanes outside the volume store past the field (dropped)
+                const bool ok = y < ny && x < nx;
+                const int off = masked_off(ok, x * ny + y, wu.b);
                 if constexpr (ISHI) {
-                    if (y < ny && x < nx) {
-                        const int off = x * ny + y;
-                        st<IO>(wps, off, d);
-                        st<IO>(wbx, off, bxl);
-                        st<IO>(wby, off, byl);
-                        if (IS3D) st<IO>(wbz, off, bzl);
-                    }
+                    st<IO>(wps, off, d);
+                    st<IO>(wbx, off, bxl);
+                    st<IO>(wby, off, byl);
+                    if (IS3D) st<IO>(wbz, off, bzl);
                     continue;
                 }
                 const float uo = o.u[k];
@@ -602,14 +631,11 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     // sink flow of this label (asetsPotts3D_mex.c:457-466), ft = dv + pt
                     const float t = (o.pt[k] + uc) - d;   // o.pt holds ps here
                     const float npt = fmin_ref(t, o.ct[k]);
-                    if (y < ny && x < nx) {
-                        const int off = x * ny + y;
-                        st<IO>(wps, off, d + npt);
-                        st<IO>(wpt, off, npt);
-                        st<IO>(wbx, off, bxl);
-                        st<IO>(wby, off, byl);
-                        if (IS3D) st<IO>(wbz, off, bzl);
-                    }
+                    st<IO>(wps, off, d + npt);
+                    st<IO>(wpt, off, npt);
+                    st<IO>(wbx, off, bxl);
+                    st<IO>(wby, off, byl);
+                    if (IS3D) st<IO>(wbz, off, bzl);
                 } else {
                     // source/sink flows and multiplier (asetsBinaryMF3D_mex.c:419-432)
                     float f = ((o.pt[k] - uc) + d) + one_cc;
@@ -617,57 +643,71 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     f = (nps + uc) - d;
                     const float npt = fmin_ref(f, o.ct[k]);
                     const float e = cc * ((npt + d) - nps);
-                    if (y < ny && x < nx) {
-                        e_acc += fabs((double)e);
-                        const int off = x * ny + y;
-                        st<Io<float>>(wu, off, uo - e);
-                        if constexpr (PAIR) {
-                            st_pair<T>(wps, off, nps, npt);
-                            st_pair<T>(wbx, off, bxl, byl);
-                        } else {
-                            st<IO>(wps, off, nps);
-                            st<IO>(wpt, off, npt);
-                            st<IO>(wbx, off, bxl);
-                            st<IO>(wby, off, byl);
-                        }
-                        if (IS3D) st<IO>(wbz, off, bzl);
+                    e_acc += ok ? fabs((double)e) : 0.0;
+                    st<Io<float>>(wu, off, uo - e);
+                    if constexpr (PAIR) {
+                        st_pair<T>(wps, off, nps, npt);
+                        st_pair<T>(wbx, off, bxl, byl);
+                    } else {
+                        st<IO>(wps, off, nps);
+                        st<IO>(wpt, off, npt);
+                        st<IO>(wbx, off, bxl);
+                        st<IO>(wby, off, byl);
                     }
+                    if (IS3D) st<IO>(wbz, off, bzl);
                 }
             }
         };
 
         // ---- pipeline ----------------------------------------------------
         // Step s, phase X: S3(s-1) and S1(s+2); barrier; phase Y: S2a(s+2) and
-        // S2b(s+1); barrier.  Steps run from z0-4 (prologue: gk, b1 and g of
-        // the planes below the chunk) to z1; the carried z face of plane z0 is
-        // set at step z0.  A raw plane's loads are issued at the start of the
-        // step that consumes them (S3 of the previous plane covers their
-        // latency, together with the other resident waves); the S3 operands
-        // are loaded one step ahead.
-        auto step = [&](int s, Raw &r, Out &o) {
-            if (s + 2 <= z1 + 1) load_raw(s + 2, r);   // consumed after S3 below
-            if (s - 1 >= z0) s3(s - 1, o);
-            else if (s - 1 == z0 - 1) s3_carry(s - 1);
-            if (s + 2 <= z1 + 1) s1(s + 2, r);
+        // S2b(s+1), then the loads of the S3 operands of plane s and of raw
+        // plane s+3; barrier.  Steps z0-4 .. z0-1 are the prologue (gk, b1 and
+        // g of the planes below the chunk, no output), step z0 sets the carried
+        // z face of plane z0, steps z0+1 .. z1 output planes z0 .. z1-1.  Every
+        // load is issued right after the last use of its registers, so it stays
+        // in flight across the barrier and S3.  The hardware counts loads and
+        // stores together in issue order, and the compiler's wait counts are
+        // exact only when every path issues the same memory operations after a
+        // load: so the three phases are separate loops whose steps all end with
+        // the same loads in the same order, past the chunk's last raw plane the
+        // raw loads read plane -3 (never used), and lanes outside the volume
+        // store past the field instead of branching around their stores.
+        auto tail = [&](int s, Raw &r, Out &o, bool out) {
             __syncthreads();
             if (s + 2 <= z1 + 1) s2a(s + 2, r);
             if (s + 1 >= z0 - 1 && s + 1 <= z1) s2b(s + 1, r);
-            load_out(s, o);   // consumed by S3(s) at the next step
+            // the loads below reuse the registers S2a and S2b just read: keep the
+            // scheduler from hoisting them (that would need two raw planes live)
+            __builtin_amdgcn_sched_barrier(0);
+            if (out) load_out(s, o);   // consumed by S3(s) at the next step
+            // issued after the S3 operands, so that S3's waits do not cover them
+            __builtin_amdgcn_sched_barrier(0);
+            load_raw(s + 3 <= z1 + 1 ? s + 3 : -3, r);   // S1 at the next step
             __syncthreads();
-            // the next plane's own bz face becomes current
-#pragma unroll
-            for (int k = 0; k < K2; ++k) bz_cur[k] = r.bzn[k];
         };
         Raw r;
         Out o;
-        // bz own of plane z0-2 (raw plane z0-3 would carry it as its "next" bz)
-        if (IS3D) {
+        if constexpr (IS3D) {
+            // bz own of plane z0-2 (raw plane z0-3 would carry it as its "next" bz)
             const PRef rbz = pref(a.in.bz, z0 - 2, nz, IO::kBytes, plane);
 #pragma unroll
             for (int k = 0; k < K2; ++k) bz_cur[k] = ld<IO>(rbz, off2[k]);
-        }
-        if constexpr (IS3D) {
-            for (int s = z0 - 4; s <= z1; ++s) step(s, r, o);
+            load_raw(z0 - 2, r);
+            for (int s = z0 - 4; s < z0; ++s) {   // prologue (s + 2 <= z0 + 1 <= z1 + 1)
+                s1(s + 2, r);
+                tail(s, r, o, false);
+            }
+            s3_carry(z0 - 1);                     // step z0
+            __builtin_amdgcn_sched_barrier(0);
+            s1(z0 + 2, r);
+            tail(z0, r, o, true);
+            for (int s = z0 + 1; s <= z1; ++s) {
+                s3(s - 1, o);
+                __builtin_amdgcn_sched_barrier(0);   // S3 and S1 not interleaved (registers)
+                if (s + 2 <= z1 + 1) s1(s + 2, r);
+                tail(s, r, o, true);
+            }
         } else {
             // one plane: its stages in dependency order, no z pipeline; alpha of
             // plane 0 (load_raw fetches the previous plane's for the pipeline)
