@@ -81,15 +81,22 @@ asets_status hmf_plan(int ndim, const asets_hmf_node *in, int32_t M, HmfPlan &pl
     // depth-first program, children in index order
     std::function<void(int32_t)> visit = [&](int32_t n) {
         if (pl.kids[n].empty()) {
-            pl.events.push_back({kHmfLeaf, n, depth[n], 0.0f});
+            pl.events.push_back({kHmfLeaf, n, depth[n], 0.0f, 1});
             return;
         }
-        pl.events.push_back({kHmfEnter, n, depth[n], 0.0f});
+        pl.events.push_back({kHmfEnter, n, depth[n], 0.0f, 0});
         for (int32_t c : pl.kids[n]) visit(c);
         const float cnt = (float)(pl.kids[n].size() + (n == pl.root ? 0 : 1));
-        pl.events.push_back({kHmfExit, n, depth[n], cnt});
+        pl.events.push_back({kHmfExit, n, depth[n], cnt, 0});
     };
     visit(pl.root);
+    // runs of consecutive sibling leaves (the kernel loads four at a time)
+    for (int e = (int)pl.events.size() - 2; e >= 0; --e) {
+        HmfEvent &a = pl.events[e];
+        const HmfEvent &b = pl.events[e + 1];
+        if (a.type == kHmfLeaf && b.type == kHmfLeaf && in[a.node].parent == in[b.node].parent)
+            a.run = b.run + 1;
+    }
     return ASETS_OK;
 }
 

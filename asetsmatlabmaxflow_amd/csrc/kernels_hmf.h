@@ -41,6 +41,7 @@ enum : int32_t { kHmfEnter = 0, kHmfLeaf = 1, kHmfExit = 2 };
 struct HmfEvent {
     int32_t type, node, depth;
     float count;   // divisor of an internal node's pt: children (+1 below the root)
+    int32_t run;   // leaf: consecutive leaf events of the same parent from this one
 };
 
 // Sink flows of every node at one voxel.  The reference recurses children first;
@@ -76,6 +77,31 @@ __global__ __launch_bounds__(256) void k_hmf_sink(const HmfNode *__restrict__ no
             S(E.depth, 1) = po;
             S(E.depth, 2) = d;
             S(E.depth, 3) = uc;
+        } else if (E.type == kHmfLeaf && E.run >= 4) {
+            // four sibling leaves: their loads issued together, then the sink
+            // flows and the parent's running sum in child order
+            float d[4], uu[4], ct[4];
+            int64_t ii[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int n = ev[e + j].node;
+                ii[j] = (int64_t)n * N + g;
+                d[j] = div[ii[j]];
+                uu[j] = u[ii[j]];
+                ct[j] = nodes[n].Ct[g];
+            }
+            const float po = S(E.depth - 1, 1);
+            float acc = S(E.depth - 1, 0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float uc = div_fast(uu[j], cc, rcc, fast_cc != 0);
+                const float f = (po - d[j]) + uc;
+                const float p = (ct[j] < f) ? ct[j] : f;
+                pt[ii[j]] = p;
+                acc = ((acc + p) + d[j]) - uc;
+            }
+            S(E.depth - 1, 0) = acc;
+            e += 3;
         } else if (E.type == kHmfLeaf) {
             const float d = div[i];
             const float uc = div_fast(u[i], cc, rcc, fast_cc != 0);

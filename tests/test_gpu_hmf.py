@@ -76,6 +76,8 @@ CASES = [
     ((61, 45), FLAT4, False, "passes", 30),
     ((3, 5, 4), FLAT4, True, "fused", 12),
     ((1, 7, 3), [-1, 0, 0], False, "fused", 8),
+    ((40, 33), [-1] + [0] * 9, True, "fused", 10),            # leaf runs of 4, 4, 1
+    ((20, 18, 6), [-1, 0, 0, 1, 1, 1, 1, 1, 2, 2, 2, 2], False, "fused", 10),
 ]
 
 
