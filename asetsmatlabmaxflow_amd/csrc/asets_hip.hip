@@ -1588,6 +1588,57 @@ asets_status alpha_contrast(int ndim, const float *img, int64_t ny, int64_t nx, 
     return ASETS_OK;
 }
 
+asets_status cost_l1(const float *img, int64_t n, const float *models, int32_t nlab,
+                     float *Ct, const asets_opts *o) {
+    if (!img || !models || !Ct) return fail(ASETS_EBADARG, "null array argument");
+    if (n <= 0 || nlab <= 0 || nlab > kMaxCostLabels)
+        return fail(ASETS_EBADARG, "need n > 0 and 1 <= nlab <= %d", kMaxCostLabels);
+    if (n > ((int64_t)1 << 40)) return fail(ASETS_ETOOBIG, "too many voxels");
+    PrepCtx c;
+    std::unique_lock<std::mutex> lock;
+    asets_status st = prep_begin(o, Carver::sz(2 * kPrepBlocks, 4) + Carver::sz(2, 4), c, lock);
+    if (st) return st;
+    Carver cv(c.dc->dev);
+    float *part = cv.take<float>(2 * kPrepBlocks);
+    float *mm = cv.take<float>(2);
+    float h[2];
+    st = minmax(img, n, part, mm, h, c.s, *c.dc);
+    if (st) return st;
+    if (!(h[1] > h[0])) return fail(ASETS_EBADARG, "the image is constant: no normalisation");
+    CostModels ms{};
+    for (int l = 0; l < nlab; ++l) ms.m[l] = models[l];
+    k_l1_costs<<<(unsigned)cdiv(n, 256), 256, 0, c.s>>>(img, n, mm, ms, nlab, Ct);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c.s));
+    return ASETS_OK;
+}
+
+asets_status cost_smooth_labels(const int32_t *lab, int64_t ny, int64_t nx, const int32_t *ids,
+                                int32_t nlab, const double *h, int32_t K, float *Ct,
+                                const asets_opts *o) {
+    if (!lab || !ids || !h || !Ct) return fail(ASETS_EBADARG, "null array argument");
+    asets_status st = check_dims(ny, nx, 1, 1);
+    if (st) return st;
+    if (nlab <= 0 || nlab > kMaxCostLabels)
+        return fail(ASETS_EBADARG, "need 1 <= nlab <= %d", kMaxCostLabels);
+    if (K < 1 || K > 15 || K % 2 == 0)
+        return fail(ASETS_EBADARG, "the kernel size must be odd, 1 to 15 (got %d)", K);
+    PrepCtx c;
+    std::unique_lock<std::mutex> lock;
+    st = prep_begin(o, Carver::sz(K * K, 8), c, lock);
+    if (st) return st;
+    Carver cv(c.dc->dev);
+    double *dh = cv.take<double>(K * K);
+    HIPCHK(hipMemcpyAsync(dh, h, K * K * sizeof(double), hipMemcpyHostToDevice, c.s));
+    CostIds cid{};
+    for (int l = 0; l < nlab; ++l) cid.id[l] = ids[l];
+    k_smooth_labels<<<(unsigned)cdiv(ny * nx, 256), 256, 0, c.s>>>(lab, ny, nx, cid, nlab, dh, K,
+                                                                  Ct);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c.s));
+    return ASETS_OK;
+}
+
 #include "hmf_host.h"
 
 }  // namespace
@@ -1779,6 +1830,17 @@ asets_status asets_selftest_division(int mode, float d, uint64_t *mismatches) {
     if (e != hipSuccess) return fail(ASETS_EDEVICE, "selftest: %s", hipGetErrorString(e));
     *mismatches = h;
     return ASETS_OK;
+}
+
+asets_status asets_cost_l1_dev(const float *img, int64_t n, const float *models, int32_t nlab,
+                               float *Ct, const asets_opts *o) {
+    return cost_l1(img, n, models, nlab, Ct, o);
+}
+
+asets_status asets_cost_smooth_labels_dev(const int32_t *labels, int64_t ny, int64_t nx,
+                                          const int32_t *ids, int32_t nlab, const double *h,
+                                          int32_t K, float *Ct, const asets_opts *o) {
+    return cost_smooth_labels(labels, ny, nx, ids, nlab, h, K, Ct, o);
 }
 
 asets_status asets_hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz, asets_hmf_node *nodes,

@@ -13,6 +13,12 @@
 //                    normalised to [0, 1], MATLAB gradient() (central differences
 //                    inside, one-sided at the borders), alpha = a0 + a1*exp(-beta*m)
 //                    with m = |grad I| (power 1) or |grad I|^2 (power 2)
+//   L1 costs         applications/L1IntensitySegmentation2D.m:36-44: image normalised
+//                    to [0, 1] in fp32, Ct_l = |x - m_l|
+//   smoothed labels  applications/smoothManualSegmentations2D.m:48-55: Ct_l =
+//                    imfilter(1 - (man == id_l), h, 'replicate') for a caller's K x K
+//                    kernel h (fspecial('gaussian', [5 5], 1.5) there); FP64 sums of the
+//                    weights in h's column-major order, stored as fp32
 //   labels           tests/runTestBinaryMF.m:55 (u > 0.5), tests/runTestPotts.m:58
 //                    (1-based index of the first maximum over labels),
 //                    tests/runTestIshikawa.m:57-60 (number of cut functions > 0.5)
@@ -144,6 +150,45 @@ __global__ void k_alpha_contrast(const float *I, int64_t ny, int64_t nx, int64_t
     const double q = gx * gx + gy * gy + gz * gz;
     const double m = (power == 1) ? sqrt(q) : q;
     alpha[g] = (float)(a0 + a1 * exp(-beta * m));
+}
+
+constexpr int kMaxCostLabels = 64;
+struct CostModels {
+    float m[kMaxCostLabels];
+};
+struct CostIds {
+    int32_t id[kMaxCostLabels];
+};
+
+// Ct[l*n + i] = |x_i - m_l|, x = (img - min) / (max - min) in fp32
+__global__ void k_l1_costs(const float *img, int64_t n, const float *mm, CostModels ms, int nlab,
+                           float *Ct) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float x = (img[i] - mm[0]) / (mm[1] - mm[0]);
+    for (int l = 0; l < nlab; ++l) Ct[l * n + i] = fabsf(x - ms.m[l]);
+}
+
+// Ct[l*n + g] = sum over the K x K window around g (borders replicated) of
+// h(a, b) * (lab != id_l), h column-major (a along dim 1), summed in that order in FP64
+__global__ void k_smooth_labels(const int32_t *lab, int64_t ny, int64_t nx, CostIds ids,
+                                int nlab, const double *h, int K, float *Ct) {
+    const int64_t n = ny * nx;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int64_t y = g % ny, x = g / ny;
+    const int r = K / 2;
+    for (int l = 0; l < nlab; ++l) {
+        double sacc = 0.0;
+        for (int b = 0; b < K; ++b) {
+            const int64_t xx = min(max(x + b - r, (int64_t)0), nx - 1);
+            for (int a = 0; a < K; ++a) {
+                const int64_t yy = min(max(y + a - r, (int64_t)0), ny - 1);
+                if (lab[xx * ny + yy] != ids.id[l]) sacc += h[b * K + a];
+            }
+        }
+        Ct[l * n + g] = (float)sacc;
+    }
 }
 
 // model 0: u > 0.5; 1: 1-based first arg-max over nlab fields; 2: number of the

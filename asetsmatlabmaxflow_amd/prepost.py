@@ -5,6 +5,8 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
+
 from . import _native as nat
 from .solver import _ndim_dims, _ptr, _stream_handle
 
@@ -37,6 +39,47 @@ def cost_neglog(p, epsilon: float = 1e-10, *, stream=None):
     Ct = torch.empty_like(p)
     nat.check(nat.lib().asets_cost_neglog_dev(_ptr(p), p.numel(), epsilon, _ptr(Ct),
                                               C.byref(_opts(stream))))
+    return Ct
+
+
+def cost_l1(img, models, *, stream=None):
+    """Ct [len(models) * n]: |x - m_l| with x the image normalised to [0, 1]
+    (L1IntensitySegmentation2D.m:36-44)."""
+    m = (C.c_float * len(models))(*[float(v) for v in models])
+    n = img.numel()
+    Ct = torch.empty(n * len(models), dtype=torch.float32, device=img.device)
+    nat.check(nat.lib().asets_cost_l1_dev(_ptr(img), n, m, len(models), _ptr(Ct),
+                                          C.byref(_opts(stream))))
+    return Ct
+
+
+def fspecial_gaussian(size: int = 3, sigma: float = 0.5):
+    """MATLAB fspecial('gaussian', [size size], sigma): exp(-(x^2+y^2)/(2 sigma^2)) on the
+    centred grid, entries below eps*max set to 0, normalised to sum 1 (FP64)."""
+    r = (size - 1) / 2.0
+    x = np.arange(size) - r
+    X, Y = np.meshgrid(x, x)
+    h = np.exp(-(X * X + Y * Y) / (2.0 * sigma * sigma))
+    h[h < np.finfo(np.float64).eps * h.max()] = 0.0
+    s = h.ravel(order="F").sum()
+    return h / s if s != 0 else h
+
+
+def cost_smooth_labels(labels, shape, ids, h, *, stream=None):
+    """Ct [len(ids) * n]: imfilter(1 - (labels == id), h, 'replicate') on a 2D int32 label
+    image of shape (Ny, Nx) (smoothManualSegmentations2D.m:48-55)."""
+    ny, nx = int(shape[0]), int(shape[1])
+    if labels.dtype != torch.int32 or labels.numel() != ny * nx:
+        raise ValueError("labels must be int32 of shape (Ny, Nx)")
+    h = np.asarray(h, np.float64)
+    if h.ndim != 2 or h.shape[0] != h.shape[1]:
+        raise ValueError("h must be a square kernel")
+    hf = np.ascontiguousarray(h.ravel(order="F"))
+    idv = (C.c_int32 * len(ids))(*[int(v) for v in ids])
+    Ct = torch.empty(ny * nx * len(ids), dtype=torch.float32, device=labels.device)
+    nat.check(nat.lib().asets_cost_smooth_labels_dev(labels.data_ptr(), ny, nx, idv, len(ids),
+                                                     hf.ctypes.data, h.shape[0], _ptr(Ct),
+                                                     C.byref(_opts(stream))))
     return Ct
 
 

@@ -258,6 +258,21 @@ asets_status asets_alpha_contrast_dev(int ndim, const float *img, int64_t ny, in
 asets_status asets_labels_dev(int model, const float *u, int64_t n, int32_t nlab,
                               uint8_t *labels, const asets_opts *o);
 
+/* L1 intensity costs, applications/L1IntensitySegmentation2D.m:36-44: the image is
+ * normalised to [0, 1] in fp32, x = (img - min) / (max - min), and
+ * Ct[l*n + i] = |x_i - models[l]| for nlab <= 64 model intensities (host array). */
+asets_status asets_cost_l1_dev(const float *img, int64_t n, const float *models, int32_t nlab,
+                               float *Ct, const asets_opts *o);
+
+/* Smoothed inverse label indicators, applications/smoothManualSegmentations2D.m:48-55:
+ * Ct[l*n + g] = imfilter(1 - (labels == ids[l]), h, 'replicate') on an ny x nx label
+ * image (int32, column-major), h a host K x K kernel in double, column-major (K odd,
+ * <= 15; the script uses fspecial('gaussian', [5 5], 1.5)).  The window's weights are
+ * summed in FP64 in h's element order and the sum stored as fp32.  nlab <= 64. */
+asets_status asets_cost_smooth_labels_dev(const int32_t *labels, int64_t ny, int64_t nx,
+                                          const int32_t *ids, int32_t nlab, const double *h,
+                                          int32_t K, float *Ct, const asets_opts *o);
+
 /* ---- hierarchical max-flow (HMF), full-flow variant -----------------------
  * Replaces the MATLAB classes maxflow/asetsHMF2D.m and asetsHMF3D.m: the tree
  * built by the constructor asetsHMF2D(children, alpha, Ct) (:57-76, one node per

@@ -1,9 +1,12 @@
 """CPU oracle of the cost, weight and label steps (TEST INFRASTRUCTURE ONLY: the checker
-for asets_cost_loglik_dev, asets_cost_neglog_dev, asets_alpha_contrast_dev and
-asets_labels_dev).  numpy restatements of the reference's MATLAB:
+for asets_cost_loglik_dev, asets_cost_neglog_dev, asets_cost_l1_dev,
+asets_cost_smooth_labels_dev, asets_alpha_contrast_dev and asets_labels_dev).  numpy
+restatements of the reference's MATLAB:
 
   loglik   applications/interactiveGraphCutSegmentation3D.m:96-118
   neglog   applications/atlasProbabilityMapsRegularization3D.m:47-53
+  l1       applications/L1IntensitySegmentation2D.m:36-44
+  smooth   applications/smoothManualSegmentations2D.m:48-55 (imfilter, 'replicate')
   alpha    applications/smoothManualSegmentations2D.m:73-82 (MATLAB gradient())
   labels   tests/runTestBinaryMF.m:55, tests/runTestPotts.m:58,
            tests/runTestIshikawa.m:57-60
@@ -32,6 +35,32 @@ def cost_loglik(img, scribbles, nlab, epsilon=1e-10):
 
 def cost_neglog(p, epsilon=1e-10):
     return (-np.log(np.asarray(p, np.float64) + epsilon)).astype(np.float32)
+
+
+def cost_l1(img, models):
+    """|x - m_l| with x = (img - min) / (max - min), fp32 operations."""
+    img = np.asarray(img, np.float32).ravel(order="F")
+    x = (img - img.min()) / (img.max() - img.min())
+    return np.concatenate([np.abs(x - np.float32(m)) for m in models]).astype(np.float32)
+
+
+def cost_smooth_labels(lab, ids, h):
+    """imfilter(1 - (lab == id), h, 'replicate') per id, on a 2D label image: the
+    correlation sum over the K x K window written out, FP64, h's column-major order."""
+    lab = np.asarray(lab)
+    h = np.asarray(h, np.float64)
+    K = h.shape[0]
+    r = K // 2
+    ny, nx = lab.shape
+    pad = np.pad(lab, r, mode="edge")                 # 'replicate' border
+    out = []
+    for idv in ids:
+        acc = np.zeros((ny, nx))
+        for b in range(K):                            # column-major: a (dim 1) fastest
+            for a in range(K):
+                acc = acc + h[a, b] * (pad[a:a + ny, b:b + nx] != idv)
+        out.append(acc.astype(np.float32).ravel(order="F"))
+    return np.concatenate(out)
 
 
 def matlab_gradient(I, axis):
