@@ -204,3 +204,24 @@ def test_hmf_tree_validation_before_device_work():
     chain = [-1] + list(range(17))
     assert call(parent=chain) == nat.ASETS_ETOOBIG
     assert b"deeper" in L.asets_last_error()
+
+
+def test_cost_builder_validation_before_device_work():
+    L = nat.lib()
+    buf = np.zeros(64, np.float32)
+    ptr = buf.ctypes.data
+    ids = (C.c_int32 * 2)(0, 1)
+    h = np.ones(9)
+    models = (C.c_float * 2)(0.0, 1.0)
+    assert L.asets_cost_l1_dev(None, 8, models, 2, ptr, None) == nat.ASETS_EBADARG
+    assert L.asets_cost_l1_dev(ptr, 8, models, 65, ptr, None) == nat.ASETS_EBADARG
+    assert L.asets_cost_l1_dev(ptr, 0, models, 2, ptr, None) == nat.ASETS_EBADARG
+    assert L.asets_cost_smooth_labels_dev(ptr, 4, 4, ids, 2, h.ctypes.data, 4, ptr,
+                                          None) == nat.ASETS_EBADARG
+    assert b"odd" in L.asets_last_error()
+    assert L.asets_cost_smooth_labels_dev(ptr, 4, 4, ids, 2, None, 3, ptr,
+                                          None) == nat.ASETS_EBADARG
+    assert L.asets_cost_smooth_labels_dev(ptr, 4, 0, ids, 2, h.ctypes.data, 3, ptr,
+                                          None) == nat.ASETS_EBADARG
+    assert L.asets_cost_smooth_labels_dev(ptr, 4, 4, ids, 0, h.ctypes.data, 3, ptr,
+                                          None) == nat.ASETS_EBADARG
