@@ -48,6 +48,44 @@ def _oracle():
     return orc
 
 
+def _host_cpu():
+    """(model name, logical CPUs visible to this process)."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count()
+    return model, n
+
+
+def _cpu_time(fn, repeats):
+    """Wall seconds of fn() on one core (this process pinned to its first allowed CPU
+    for the duration), median of `repeats` runs."""
+    prev = None
+    try:
+        prev = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(prev)})
+    except (AttributeError, OSError):
+        prev = None
+    try:
+        ts = []
+        for _ in range(repeats):
+            t = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t)
+    finally:
+        if prev is not None:
+            os.sched_setaffinity(0, prev)
+    return sorted(ts)[len(ts) // 2], ts
+
+
 def _base(args, metric, value, unit, ms_per_step, dtype, data, config):
     return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
@@ -180,14 +218,21 @@ def bench_binary3d(args):
     if not args.no_cpu:
         orc = _oracle()
         k = max(1, args.cpu_iters)
-        t2 = time.perf_counter()
-        orc.binary(Cs, Ct, alpha, shape, k, 0.0, cc, steps)
-        cpu_s = time.perf_counter() - t2
-        out["cpu_baseline"] = {"value": N * k / cpu_s, "unit": "voxel-iter/s", "cores": 1,
+        cpu_s, runs = _cpu_time(lambda: orc.binary(Cs, Ct, alpha, shape, k, 0.0, cc, steps),
+                                args.cpu_repeats)
+        model, ncpu = _host_cpu()
+        rate = N * k / cpu_s
+        out["cpu_baseline"] = {"value": rate, "unit": "voxel-iter/s", "cores": 1,
                                "kind": "port",
                                "sample": f"{k} iterations of the same {n}^3 volume "
-                                         f"(FP32 reference-order oracle, gcc -O2, "
-                                         f"1 thread), {cpu_s:.1f} s incl. init"}
+                                         f"(FP32 reference-order oracle, gcc -O2, 1 thread "
+                                         f"pinned to one core), median of "
+                                         f"{len(runs)} runs of "
+                                         f"{', '.join(f'{t:.2f}' for t in runs)} s incl. init",
+                               "host_cpu": model, "host_logical_cpus": ncpu}
+        if "time_to_tol" in out:
+            it = out["time_to_tol"]["iterations"]
+            out["cpu_baseline"]["time_to_tol_extrapolated_s"] = N * it / rate
     return out
 
 
@@ -451,6 +496,7 @@ def main():
     ap.add_argument("--tile-x", type=int, default=0)
     ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--cpu-iters", type=int, default=int(os.environ.get("ASETS_CPU_ITERS", 20)))
+    ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-tol", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
