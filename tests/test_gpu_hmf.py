@@ -191,3 +191,32 @@ def test_hmf_fp32_error_against_fp64(cuda):
               for n in range(1, len(NESTED)))
     print(f"hmf fp32 vs fp64 max |du| after 150 iterations: {err:.3e}")
     assert err < 1e-3
+
+
+def test_hmf_bench_tree_fused_equals_passes_full_size(cuda):
+    """At the benchmark's full size (192^3 two-level tree) the fused kernel and the
+    per-pass kernels -- two independent implementations -- agree bit for bit, and the
+    first iteration gives the root its closed form (1/cc + sum of its children's sink
+    flows)/2 with the children's labels cc*(P.pt - pt)."""
+    import torch
+
+    from asetsmatlabmaxflow_amd import workloads
+    from asetsmatlabmaxflow_amd.hmf import solve_tree
+    shape = (192, 192, 192)
+    parent, Ct, alpha, _ = workloads.hmf3d_tree(shape, seed=1006)
+    steps, cc = workloads.HMF3D_STEPS, workloads.HMF3D_CC
+    a = solve_tree(parent, Ct, alpha, shape, 10, steps, cc, kernel="fused")
+    b = solve_tree(parent, Ct, alpha, shape, 10, steps, cc, kernel="passes")
+    for n in range(len(parent)):
+        assert torch.equal(a.u[n], b.u[n]) and torch.equal(a.pt[n], b.pt[n])
+        assert torch.equal(a.div[n], b.div[n])
+        for k in range(3):
+            assert torch.equal(a.faces[n][k], b.faces[n][k])
+    np.testing.assert_allclose(a.data_cost, b.data_cost, rtol=1e-12)
+    one = solve_tree(parent, Ct, alpha, shape, 1, steps, cc)
+    pt1, pt2 = one.pt[1].double(), one.pt[2].double()
+    np.testing.assert_allclose(one.pt[0].double().cpu().numpy(),
+                               ((1.0 / cc + pt1 + pt2) / 2).cpu().numpy(), rtol=2e-6)
+    np.testing.assert_allclose(one.u[1].double().cpu().numpy(),
+                               (cc * (one.pt[0].double() - pt1)).cpu().numpy(), rtol=1e-5,
+                               atol=1e-7)
