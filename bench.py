@@ -478,6 +478,52 @@ def bench_batched(args):
     out["fp16"] = legs["f16"]
     out["fp16_vs_fp32"] = {"max_abs_u_diff": diff, "label_agreement": agree}
     out["gen_s"] = gen_s
+    if not args.no_cpu:
+        # CPU baseline: the FP32 reference-order oracle on the volumes in index order
+        # until about 10 s of one pinned core have been spent, each run for exactly the
+        # iterations its GPU solve executed (and its u compared bit for bit); the FP64
+        # oracle on the three smallest volumes is the error yardstick of both storages
+        orc = _oracle()
+        cpu_s, vit_cpu, nv, bitexact = 0.0, 0, 0, True
+        for v, sh in enumerate(sizes):
+            if cpu_s >= 10.0:
+                break
+            Cs, Ct, a, _ = workloads.binary3d_ellipsoids(sh, seed=1004 * 1000 + v)
+            it32 = res["f32"][v].iterations
+            box = {}
+            t1, _ = _cpu_time(lambda: box.update(
+                r=orc.binary(Cs, Ct, a, sh, it32, 0.0, cc, st)), 1)
+            cpu_s += t1
+            vit_cpu += int(np.prod(sh)) * it32
+            nv += 1
+            bitexact &= bool(np.array_equal(res["f32"][v].u.cpu().numpy().view(np.uint32),
+                                            box["r"].u.view(np.uint32)))
+        model, ncpu = _host_cpu()
+        out["cpu_baseline"] = {
+            "value": vit_cpu / cpu_s, "unit": "voxel-iter/s", "cores": 1, "kind": "port",
+            "sample": f"volumes 0..{nv - 1} for the iterations their GPU solves executed, "
+                      f"FP32 reference-order oracle, 1 pinned core, {cpu_s:.1f} s",
+            "host_cpu": model, "host_logical_cpus": ncpu,
+            "u_fp32_bit_identical_to_cpu": bitexact}
+        pick = sorted(range(len(sizes)), key=lambda v: int(np.prod(sizes[v])))[:3]
+        err = {"fp32": [0.0, 0.0, 1.0], "fp16": [0.0, 0.0, 1.0]}   # max, mean, labels
+        for v in pick:
+            Cs, Ct, a, _ = workloads.binary3d_ellipsoids(sizes[v], seed=1004 * 1000 + v)
+            for flow, key in (("f32", "fp32"), ("f16", "fp16")):
+                r64 = orc.binary(Cs, Ct, a, sizes[v], res[flow][v].iterations, 0.0, cc, st,
+                                 precision="f64")
+                ug = res[flow][v].u.cpu().numpy().astype(np.float64)
+                d = np.abs(ug - r64.u)
+                e = err[key]
+                err[key] = [max(e[0], float(d.max())), max(e[1], float(d.mean())),
+                            min(e[2], float(np.mean((ug > 0.5) == (r64.u > 0.5))))]
+        out["error_vs_fp64_oracle"] = {
+            key: {"max_abs_u_err": e[0], "mean_abs_u_err": e[1], "label_agreement": e[2]}
+            for key, e in err.items()}
+        out["error_vs_fp64_oracle"]["volumes"] = ["x".join(map(str, sizes[v])) for v in pick]
+        out["error_vs_fp64_oracle"]["note"] = (
+            "worst volume of the sample per figure; each GPU result against the FP64 "
+            "oracle run for the same iteration count")
     return out
 
 
