@@ -660,28 +660,27 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         };
 
         // ---- pipeline ----------------------------------------------------
-        // Step s, phase X: S3(s-1) and S1(s+2); barrier; phase Y: S2a(s+2) and
-        // S2b(s+1), then the loads of the S3 operands of plane s and of raw
-        // plane s+3; barrier.  Steps z0-4 .. z0-1 are the prologue (gk, b1 and
-        // g of the planes below the chunk, no output), step z0 sets the carried
-        // z face of plane z0, steps z0+1 .. z1 output planes z0 .. z1-1.  Every
-        // load is issued right after the last use of its registers, so it stays
-        // in flight across the barrier and S3.  The hardware counts loads and
-        // stores together in issue order, and the compiler's wait counts are
-        // exact only when every path issues the same memory operations after a
-        // load: so the three phases are separate loops whose steps all end with
-        // the same loads in the same order, past the chunk's last raw plane the
-        // raw loads read plane -3 (never used), and lanes outside the volume
-        // store past the field instead of branching around their stores.
-        auto tail = [&](int s, Raw &r, Out &o, bool out) {
+        // Step s, phase X: S3(s-1), the loads of the S3 operands of plane s,
+        // and S1(s+2); barrier; phase Y: S2a(s+2) and S2b(s+1), then the loads
+        // of raw plane s+3; barrier.  Steps z0-4 .. z0-1 are the prologue (gk,
+        // b1 and g of the planes below the chunk, no output), step z0 sets the
+        // carried z face of plane z0, steps z0+1 .. z1 output planes
+        // z0 .. z1-1.  Every load is issued right after the last use of its
+        // registers, so it stays in flight across the barriers and the stages
+        // between: the S3 operands behind S1, both barriers, S2a and S2b; the
+        // raw plane behind S3.  The hardware counts loads and stores together
+        // in issue order, and the compiler's wait counts are exact only when
+        // every path issues the same memory operations after a load: so the
+        // phases are separate loops whose steps all issue the same loads in the
+        // same order, past the chunk's last raw plane the raw loads read plane
+        // -3 (never used), and lanes outside the volume store past the field
+        // instead of branching around their stores.
+        auto tail = [&](int s, Raw &r) {
             __syncthreads();
             if (s + 2 <= z1 + 1) s2a(s + 2, r);
             if (s + 1 >= z0 - 1 && s + 1 <= z1) s2b(s + 1, r);
             // the loads below reuse the registers S2a and S2b just read: keep the
             // scheduler from hoisting them (that would need two raw planes live)
-            __builtin_amdgcn_sched_barrier(0);
-            if (out) load_out(s, o);   // consumed by S3(s) at the next step
-            // issued after the S3 operands, so that S3's waits do not cover them
             __builtin_amdgcn_sched_barrier(0);
             load_raw(s + 3 <= z1 + 1 ? s + 3 : -3, r);   // S1 at the next step
             __syncthreads();
@@ -696,17 +695,22 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             load_raw(z0 - 2, r);
             for (int s = z0 - 4; s < z0; ++s) {   // prologue (s + 2 <= z0 + 1 <= z1 + 1)
                 s1(s + 2, r);
-                tail(s, r, o, false);
+                tail(s, r);
             }
             s3_carry(z0 - 1);                     // step z0
             __builtin_amdgcn_sched_barrier(0);
+            load_out(z0, o);                      // consumed by S3(z0) at the next step
+            __builtin_amdgcn_sched_barrier(0);
             s1(z0 + 2, r);
-            tail(z0, r, o, true);
+            tail(z0, r);
             for (int s = z0 + 1; s <= z1; ++s) {
                 s3(s - 1, o);
-                __builtin_amdgcn_sched_barrier(0);   // S3 and S1 not interleaved (registers)
+                // S3 has read o: refill it; S1 and S2 are free of memory operations
+                __builtin_amdgcn_sched_barrier(0);
+                load_out(s, o);
+                __builtin_amdgcn_sched_barrier(0);
                 if (s + 2 <= z1 + 1) s1(s + 2, r);
-                tail(s, r, o, true);
+                tail(s, r);
             }
         } else {
             // one plane: its stages in dependency order, no z pipeline; alpha of
