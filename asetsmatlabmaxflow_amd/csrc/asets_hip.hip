@@ -103,6 +103,15 @@ struct Carver {
     static size_t sz(int64_t n, size_t es) { return ((size_t)n * es + 255) & ~(size_t)255; }
 };
 
+// Kernel-timing events of one call, destroyed on every return path.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
 asets_status check_params(const asets_params *p) {
     if (!p) return fail(ASETS_EBADARG, "params is NULL");
     if (p->max_iter < 0) return fail(ASETS_EBADARG, "max_iter must be >= 0 (got %d)", p->max_iter);
@@ -132,26 +141,27 @@ inline bool fused_fits(int64_t ny, int64_t nx, int64_t nz) {
 
 // alpha <= 0 anywhere is an error when at least one iteration runs
 // (asetsBinaryMF3D_mex.c:317-319 raises it inside the first projection pass).
-__global__ void k_alpha_check(const float *alpha, int64_t n, int32_t *bad) {
+__global__ void k_alpha_check(const float *alpha, int64_t n, int32_t *bad, int32_t *done) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     int found = 0;
     for (; i < n; i += stride)
         if (alpha[i] <= 0.0f) found = 1;
-    if (found) atomicOr(bad, 1);
+    if (found) {
+        atomicOr(bad, 1);
+        if (done) atomicOr(done, 1);
+    }
 }
 
-asets_status alpha_ok(const float *alpha, int64_t n, hipStream_t s, DeviceCache &dc,
-                      int32_t *scratch_i32) {
-    HIPCHK(hipMemsetAsync(scratch_i32, 0, sizeof(int32_t), s));
-    int64_t blocks = std::min<int64_t>(cdiv(n, kBlock), 4096);
-    k_alpha_check<<<(unsigned)blocks, kBlock, 0, s>>>(alpha, n, scratch_i32);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(dc.host_flag, scratch_i32, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (*dc.host_flag) return fail(ASETS_EALPHA, "alpha(x) must be positive. Exiting...");
-    return ASETS_OK;
+// Enqueue the alpha > 0 check into the solve's flags (cleared before): a
+// violation stops every following iteration at once, and the host reports it
+// at its first poll of the flags (run_loop), before any output is written.
+void alpha_check_async(const float *alpha, int64_t n, Flags *f, hipStream_t s) {
+    const int64_t blocks = std::min<int64_t>(cdiv(n, kBlock), 4096);
+    k_alpha_check<<<(unsigned)blocks, kBlock, 0, s>>>(alpha, n, &f->bad_alpha, &f->done);
 }
+
+asets_status alpha_error() { return fail(ASETS_EALPHA, "alpha(x) must be positive. Exiting..."); }
 
 inline void fill(float *p, float v, int64_t n, hipStream_t s) {
     if (n > 0) k_fill_f32<<<(unsigned)cdiv(n, kBlock), kBlock, 0, s>>>(p, v, n);
@@ -162,9 +172,10 @@ inline void fill(float *p, float v, int64_t n, hipStream_t s) {
 // kernels were enqueued.
 template <typename F>
 asets_status run_loop(int32_t max_iter, float err_bound, int32_t poll_opt, hipStream_t s,
-                      DeviceCache &dc, Flags *flags, F &&enqueue_iter) {
+                      DeviceCache &dc, Flags *flags, int32_t *itn, F &&enqueue_iter) {
     int32_t poll = poll_opt > 0 ? poll_opt : (err_bound > 0.0f ? 16 : 256);
     int32_t it = 0;
+    *itn = max_iter;   // no stop before max_iter
     while (it < max_iter) {
         int32_t k = std::min(poll, max_iter - it);
         for (int32_t j = 0; j < k; ++j) {
@@ -173,10 +184,15 @@ asets_status run_loop(int32_t max_iter, float err_bound, int32_t poll_opt, hipSt
         }
         it += k;
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(dc.host_flag, &flags->done, sizeof(int32_t),
+        // done, bad_alpha and it_num in one read
+        HIPCHK(hipMemcpyAsync(dc.host_flag, &flags->done, 3 * sizeof(int32_t),
                               hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        if (*dc.host_flag) break;
+        if (dc.host_flag[1]) return alpha_error();
+        if (dc.host_flag[0]) {
+            *itn = dc.host_flag[2];
+            break;
+        }
         if (poll < 128 && err_bound > 0.0f && poll_opt <= 0) poll *= 2;
     }
     return ASETS_OK;
@@ -284,6 +300,23 @@ __global__ void k_init_binary_t(const float *Cs, const float *Ct, const float *u
     Io<T>::put(pt + kPair * g, mt);
 }
 
+// The whole initial state of a cold start in one pass: u and the (ps, pt) pair
+// as k_init_binary_t, the (bx, by) pair and bz zero.
+template <typename T>
+__global__ void k_init_binary_cold_t(const float *Cs, const float *Ct, float *u, T *pspt, T *bxby,
+                                     T *bz, int64_t N) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    const float cs = Cs[g], ct = Ct[g];
+    const float m = fmin_ref(cs, ct);
+    u[g] = (cs - ct >= 0.0f) ? 1.0f : 0.0f;   // asetsBinaryMF3D_mex.c:215-219
+    Io<T>::put(pspt + kPair * g, m);
+    Io<T>::put(pspt + kPair * g + 1, m);
+    Io<T>::put(bxby + kPair * g, 0.0f);
+    Io<T>::put(bxby + kPair * g + 1, 0.0f);
+    Io<T>::put(bz + g, 0.0f);
+}
+
 // Initial flow faces of a resumed solve: dst = src, except the faces with axis
 // index 0, which are the zero-flux boundary (never updated; asetsBinaryMF3D_mex.c
 // leaves index 0 of bx, by, bz at 0) and are forced to 0.  axis: 0 = y (by),
@@ -348,6 +381,7 @@ struct FusedRun {
     hipStream_t s;
     hipEvent_t ev0, ev1;
     int32_t poll;
+    int32_t *itn;   // host: iteration at which the stop test fired, or max_iter
 };
 
 // Workspace bytes of a fused solve with flow element size es.
@@ -370,11 +404,16 @@ asets_status run_fused(const FusedRun &r, Carver &cv, DeviceCache &dc) {
         buf[b].by = buf[b].bx + 1;
         buf[b].bz = cv.take<T>(N);
     }
-    k_init_binary_t<T><<<nb, kBlock, 0, s>>>(r.Cs, r.Ct, r.u0, r.ps0, r.pt0, buf[0].u,
-                                            buf[0].ps, buf[0].pt, N);
-    T *const f0[3] = {buf[0].bx, buf[0].by, buf[0].bz};
-    init_faces<T>(f0, r.b0, r.ny, r.nx, r.nz, kPair, s);
-    zero_faces<T>(buf[1].bx, buf[1].by, buf[1].bz, N, kPair, s);
+    if (!r.u0) {
+        k_init_binary_cold_t<T><<<nb, kBlock, 0, s>>>(r.Cs, r.Ct, buf[0].u, buf[0].ps,
+                                                     buf[0].bx, buf[0].bz, N);
+    } else {
+        k_init_binary_t<T><<<nb, kBlock, 0, s>>>(r.Cs, r.Ct, r.u0, r.ps0, r.pt0, buf[0].u,
+                                                buf[0].ps, buf[0].pt, N);
+        T *const f0[3] = {buf[0].bx, buf[0].by, buf[0].bz};
+        init_faces<T>(f0, r.b0, r.ny, r.nx, r.nz, kPair, s);
+    }
+    // buf[1] needs no initialisation: iteration 0 writes every voxel of it
     HIPCHK(hipGetLastError());
 
     FusedArgs<T> fa{};
@@ -405,13 +444,12 @@ asets_status run_fused(const FusedRun &r, Carver &cv, DeviceCache &dc) {
                                       r.it_num, r.flags);
         return ASETS_OK;
     };
-    asets_status st = run_loop(max_iter, r.p->err_bound, r.poll, s, dc, r.flags, iter);
+    int32_t itn = 0;
+    asets_status st = run_loop(max_iter, r.p->err_bound, r.poll, s, dc, r.flags, &itn, iter);
     if (st) return st;
     if (r.ev1) HIPCHK(hipEventRecord(r.ev1, s));
+    *r.itn = itn;
     // final state lives in buf[n_exec & 1]
-    int32_t itn = 0;
-    HIPCHK(hipMemcpyAsync(&itn, r.it_num, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
     const int32_t n_exec = (itn < max_iter) ? itn + 1 : max_iter;
     const FusedState<T> &fin = buf[n_exec & 1];
     HIPCHK(hipMemcpyAsync(r.u, fin.u, N * 4, hipMemcpyDeviceToDevice, s));
@@ -569,19 +607,16 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
     Carver cv(dc.dev);
     Flags *flags = cv.take<Flags>(1);
     double *partials = cv.take<double>(nparts);
-    int32_t *scratch_i32 = cv.take<int32_t>(1);
-
-    if (max_iter > 0) {
-        st = alpha_ok(alpha, N, s, dc, scratch_i32);
-        if (st) return st;
-    }
 
     k_fill_i32<<<1, 64, 0, s>>>((int32_t *)flags, 0, sizeof(Flags) / 4);
+    if (max_iter > 0) alpha_check_async(alpha, N, flags, s);
     k_fill_i32<<<1, 1, 0, s>>>(it_num, max_iter, 1);
     fill(cvg, 0.0f, max_iter, s);
     HIPCHK(hipGetLastError());
+    int32_t itn = max_iter;
 
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    EventPair evp;
+    hipEvent_t &ev0 = evp.a, &ev1 = evp.b;
     if (o && o->kernel_ms) {
         HIPCHK(hipEventCreate(&ev0));
         HIPCHK(hipEventCreate(&ev1));
@@ -622,7 +657,7 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
                                           it_num, flags);
             return ASETS_OK;
         };
-        st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc, flags, iter);
+        st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc, flags, &itn, iter);
         if (st) return st;
         if (ev1) HIPCHK(hipEventRecord(ev1, s));
         if (o_ps) HIPCHK(hipMemcpyAsync(o_ps, ps, N * 4, hipMemcpyDeviceToDevice, s));
@@ -635,7 +670,7 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
                     {w.b0[0], w.b0[1], w.b0[2]}, u, cvg, it_num,
                     o_ps, o_pt, o_bx, o_by, o_bz, tc, ntile_y, ntile_x, nzc, nblocks,
                     grid_fused, nparts, partials, flags, s, ev0, ev1,
-                    o ? o->poll_every : 0};
+                    o ? o->poll_every : 0, &itn};
         st = (flow16 == ASETS_FLOW_F16) ? run_fused<__half>(fr, cv, dc) : run_fused<float>(fr, cv, dc);
         if (st) return st;
     }
@@ -643,12 +678,8 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
     if (ev0) {
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
-        int32_t itn = 0;
-        HIPCHK(hipMemcpy(&itn, it_num, sizeof(int32_t), hipMemcpyDeviceToHost));
         const int32_t n_exec = (itn < max_iter) ? itn + 1 : max_iter;
         *o->kernel_ms = n_exec > 0 ? (double)ms / n_exec : 0.0;
-        (void)hipEventDestroy(ev0);
-        (void)hipEventDestroy(ev1);
     }
     if (seconds)
         *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
@@ -702,18 +733,15 @@ asets_status potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny
     Carver cv(dc.dev);
     Flags *flags = cv.take<Flags>(1);
     double *partials = cv.take<double>(nparts);
-    int32_t *scratch_i32 = cv.take<int32_t>(1);
     float *ps = cv.take<float>(N);
 
-    if (max_iter > 0) {
-        st = alpha_ok(alpha, NL, s, dc, scratch_i32);
-        if (st) return st;
-    }
     k_fill_i32<<<1, 64, 0, s>>>((int32_t *)flags, 0, sizeof(Flags) / 4);
+    if (max_iter > 0) alpha_check_async(alpha, NL, flags, s);
     k_fill_i32<<<1, 1, 0, s>>>(it_num, max_iter, 1);
     fill(cvg, 0.0f, max_iter, s);
 
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    EventPair evp;
+    hipEvent_t &ev0 = evp.a, &ev1 = evp.b;
     if (o && o->kernel_ms) {
         HIPCHK(hipEventCreate(&ev0));
         HIPCHK(hipEventCreate(&ev1));
@@ -812,19 +840,16 @@ asets_status potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny
     }
     HIPCHK(hipGetLastError());
     if (ev0) HIPCHK(hipEventRecord(ev0, s));
-    st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc, flags, iter);
+    int32_t itn = max_iter;
+    st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc, flags, &itn, iter);
     if (st) return st;
     if (ev1) HIPCHK(hipEventRecord(ev1, s));
     HIPCHK(hipStreamSynchronize(s));
     if (ev0) {
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
-        int32_t itn = 0;
-        HIPCHK(hipMemcpy(&itn, it_num, sizeof(int32_t), hipMemcpyDeviceToHost));
         const int32_t n_exec = (itn < max_iter) ? itn + 1 : max_iter;
         *o->kernel_ms = n_exec > 0 ? (double)ms / n_exec : 0.0;
-        (void)hipEventDestroy(ev0);
-        (void)hipEventDestroy(ev1);
     }
     if (seconds)
         *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
@@ -915,17 +940,37 @@ void launch_batched(int tx, const BatchVol<T> *vols, const int2 *work, int nwork
     }
 }
 
-__global__ void k_batch_reset(int32_t *vdone, int32_t *viter, int nvol, int32_t max_iter,
-                              BatchCtl *ctl) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < nvol) {
+// One workgroup per volume: its stop flag, iteration counter, cvg (zeros) and
+// it_num (max_iter, the value when the stop test never fires); the control words.
+__global__ void k_batch_reset(const BatchMeta *meta, int32_t *vdone, int32_t *viter, int nvol,
+                              int32_t max_iter, BatchCtl *ctl) {
+    const int v = blockIdx.x;
+    const BatchMeta m = meta[v];
+    for (int32_t i = threadIdx.x; i < max_iter; i += blockDim.x) m.cvg[i] = 0.0f;
+    if (threadIdx.x == 0) {
+        *m.it_num = max_iter;
         vdone[v] = (max_iter <= 0) ? 1 : 0;
         viter[v] = 0;
+        if (v == 0) {
+            ctl->remaining = (max_iter <= 0) ? 0 : nvol;
+            ctl->work_next = 0;
+        }
     }
-    if (v == 0) {
-        ctl->remaining = (max_iter <= 0) ? 0 : nvol;
-        ctl->work_next = 0;
-    }
+}
+
+// u of every volume from the state buffer its last iteration wrote ((iterations
+// executed) & 1) to the caller's array; blockIdx.y = volume.
+template <typename T>
+__global__ void k_batch_take_u(const BatchVol<T> *vols, const BatchMeta *meta, int32_t max_iter) {
+    const int v = blockIdx.y;
+    const BatchMeta m = meta[v];
+    const int32_t itn = *m.it_num;
+    const int32_t n_exec = (itn < max_iter) ? itn + 1 : max_iter;
+    const float *src = vols[v].u[n_exec & 1];
+    const int64_t N = (int64_t)m.n;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * blockDim.x)
+        m.u_out[i] = src[i];
 }
 
 template <typename T>
@@ -933,6 +978,7 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
                          double *seconds, const asets_opts *o) {
     auto t_start = std::chrono::steady_clock::now();
     if (nvol <= 0 || !vols) return fail(ASETS_EBADARG, "need at least one volume");
+    if (nvol > 65535) return fail(ASETS_EBADARG, "at most 65535 volumes per call (got %d)", nvol);
     asets_status st = check_params(p);
     if (st) return st;
     int64_t total = 0;
@@ -974,7 +1020,7 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
     size_t need = Carver::sz(1, sizeof(BatchCtl)) + Carver::sz(nvol, sizeof(BatchVol<T>)) +
                   Carver::sz(nvol, sizeof(BatchMeta)) + 2 * Carver::sz(nvol, 4) +
                   Carver::sz(max_items, sizeof(int2)) + Carver::sz(max_items, sizeof(double)) +
-                  Carver::sz(1, 4);
+                  Carver::sz(nvol, 4);
     for (int v = 0; v < nvol; ++v) need += fused_bytes(vols[v].ny * vols[v].nx * vols[v].nz, sizeof(T));
     st = ensure(&dc.dev, &dc.dev_bytes, need);
     if (st) return st;
@@ -986,8 +1032,9 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
     int32_t *viter = cv.take<int32_t>(nvol);
     int2 *d_work = cv.take<int2>(max_items);
     double *partials = cv.take<double>(max_items);
-    int32_t *scratch_i32 = cv.take<int32_t>(1);
+    int32_t *vbad = cv.take<int32_t>(nvol);   // alpha <= 0 somewhere in volume v
 
+    if (max_iter > 0) k_fill_i32<<<(unsigned)cdiv(nvol, 256), 256, 0, s>>>(vbad, 0, nvol);
     std::vector<BatchVol<T>> hv(nvol);
     std::vector<BatchMeta> hm(nvol);
     for (int v = 0; v < nvol; ++v) {
@@ -1011,26 +1058,35 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
         hm[v].n = (double)N;
         hm[v].cvg = V.cvg;
         hm[v].it_num = V.it_num;
+        hm[v].u_out = V.u;
         if (max_iter > 0) {
-            st = alpha_ok(V.alpha, N, s, dc, scratch_i32);
-            if (st) return fail(st, "volume %d: %s", v, asets_last_error());
+            const int64_t blocks = std::min<int64_t>(cdiv(N, kBlock), 4096);
+            k_alpha_check<<<(unsigned)blocks, kBlock, 0, s>>>(V.alpha, N, vbad + v, nullptr);
         }
+    }
+    if (max_iter > 0) {
+        // one read of every volume's check
+        std::vector<int32_t> bad(nvol);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(bad.data(), vbad, nvol * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int v = 0; v < nvol; ++v)
+            if (bad[v])
+                return fail(ASETS_EALPHA, "volume %d: alpha(x) must be positive. Exiting...", v);
     }
 
     // (re)initialise every volume's state and the control words
+    // (needs d_meta: after set_tiling)
     auto init_all = [&]() -> asets_status {
         for (int v = 0; v < nvol; ++v) {
             const BatchVol<T> &b = hv[v];
             const int64_t N = (int64_t)b.ny * b.nx * b.nz;
             const unsigned nb = (unsigned)cdiv(N, kBlock);
-            k_init_binary_t<T><<<nb, kBlock, 0, s>>>(b.Cs, b.Ct, nullptr, nullptr, nullptr,
-                                                    b.u[0], b.ps[0], b.pt[0], N);
-            for (int k = 0; k < 2; ++k)
-                zero_faces<T>(b.bx[k], b.by[k], b.bz[k], N, kPair, s);
-            fill(vols[v].cvg, 0.0f, max_iter, s);
-            k_fill_i32<<<1, 1, 0, s>>>(vols[v].it_num, max_iter, 1);
+            // buffer 1 needs none: a volume's first iteration writes every voxel of it
+            k_init_binary_cold_t<T><<<nb, kBlock, 0, s>>>(b.Cs, b.Ct, b.u[0], b.ps[0], b.bx[0],
+                                                         b.bz[0], N);
         }
-        k_batch_reset<<<(unsigned)cdiv(nvol, 256), 256, 0, s>>>(vdone, viter, nvol, max_iter, ctl);
+        k_batch_reset<<<(unsigned)nvol, 256, 0, s>>>(d_meta, vdone, viter, nvol, max_iter, ctl);
         HIPCHK(hipGetLastError());
         return ASETS_OK;
     };
@@ -1075,7 +1131,8 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
         const std::string key = tune_key("batched3d", nvol, total, (int)sizeof(T));
         if (!tune_lookup(key, &cfg)) {
             float best = 1e30f;
-            hipEvent_t e0, e1;
+            EventPair tp;
+            hipEvent_t &e0 = tp.a, &e1 = tp.b;
             HIPCHK(hipEventCreate(&e0));
             HIPCHK(hipEventCreate(&e1));
             for (int tx : {8, 16, 32})
@@ -1097,8 +1154,6 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
                         cfg = TileCfg{tx, zc};
                     }
                 }
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
             tune_store(key, cfg);
         }
     }
@@ -1107,7 +1162,8 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
     st = init_all();
     if (st) return st;
 
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    EventPair evp;
+    hipEvent_t &ev0 = evp.a, &ev1 = evp.b;
     if (o && o->kernel_ms) {
         HIPCHK(hipEventCreate(&ev0));
         HIPCHK(hipEventCreate(&ev1));
@@ -1143,23 +1199,15 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
         if (lst) return lst;
     }
     if (ev1) HIPCHK(hipEventRecord(ev1, s));
-    // u of each volume lives in buffer (iterations executed) & 1
-    std::vector<int32_t> itn(nvol);
-    for (int v = 0; v < nvol; ++v)
-        HIPCHK(hipMemcpyAsync(&itn[v], vols[v].it_num, 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    for (int v = 0; v < nvol; ++v) {
-        const int32_t n_exec = (itn[v] < max_iter) ? itn[v] + 1 : max_iter;
-        const int64_t N = (int64_t)hv[v].ny * hv[v].nx * hv[v].nz;
-        HIPCHK(hipMemcpyAsync(vols[v].u, hv[v].u[n_exec & 1], N * 4, hipMemcpyDeviceToDevice, s));
-    }
+    // u of each volume lives in buffer (iterations executed) & 1: one copy kernel
+    // reads each volume's it_num on the device
+    k_batch_take_u<T><<<dim3(64, (unsigned)nvol), 256, 0, s>>>(d_vols, d_meta, max_iter);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
     if (ev0) {
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
         *o->kernel_ms = ms;   // whole batched loop (ms)
-        (void)hipEventDestroy(ev0);
-        (void)hipEventDestroy(ev1);
     }
     if (seconds)
         *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
@@ -1349,19 +1397,15 @@ asets_status ishikawa_dev(int ndim, const float *Ct, const float *alpha, int64_t
     Carver cv(dc.dev);
     Flags *flags = cv.take<Flags>(1);
     double *partials = cv.take<double>(nparts);
-    int32_t *scratch_i32 = cv.take<int32_t>(1);
     float *pt = cv.take<float>(N * L);
     float *dv = cv.take<float>(NM);
     float *bb[2][3];
     for (int b = 0; b < 2; ++b)
         for (int k = 0; k < 3; ++k) bb[b][k] = cv.take<float>(NM);
 
-    if (max_iter > 0) {
-        // only the first L-1 alpha fields are used (asetsIshikawa3D_mex.c:377-379)
-        st = alpha_ok(alpha, NM, s, dc, scratch_i32);
-        if (st) return st;
-    }
     k_fill_i32<<<1, 64, 0, s>>>((int32_t *)flags, 0, sizeof(Flags) / 4);
+    // only the first L-1 alpha fields are used (asetsIshikawa3D_mex.c:377-379)
+    if (max_iter > 0) alpha_check_async(alpha, NM, flags, s);
     k_fill_i32<<<1, 1, 0, s>>>(it_num, max_iter, 1);
     fill(cvg, 0.0f, max_iter, s);
     for (int b = 0; b < 2; ++b)
@@ -1369,7 +1413,8 @@ asets_status ishikawa_dev(int ndim, const float *Ct, const float *alpha, int64_t
     k_init_ishikawa<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Ct, u, pt, N, (int)L);
     HIPCHK(hipGetLastError());
 
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    EventPair evp;
+    hipEvent_t &ev0 = evp.a, &ev1 = evp.b;
     if (o && o->kernel_ms) {
         HIPCHK(hipEventCreate(&ev0));
         HIPCHK(hipEventCreate(&ev1));
@@ -1420,19 +1465,16 @@ asets_status ishikawa_dev(int ndim, const float *Ct, const float *alpha, int64_t
         return ASETS_OK;
     };
     if (ev0) HIPCHK(hipEventRecord(ev0, s));
-    st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc, flags, iter);
+    int32_t itn = max_iter;
+    st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc, flags, &itn, iter);
     if (st) return st;
     if (ev1) HIPCHK(hipEventRecord(ev1, s));
     HIPCHK(hipStreamSynchronize(s));
     if (ev0) {
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
-        int32_t itn = 0;
-        HIPCHK(hipMemcpy(&itn, it_num, sizeof(int32_t), hipMemcpyDeviceToHost));
         const int32_t n_exec = (itn < max_iter) ? itn + 1 : max_iter;
         *o->kernel_ms = n_exec > 0 ? (double)ms / n_exec : 0.0;
-        (void)hipEventDestroy(ev0);
-        (void)hipEventDestroy(ev1);
     }
     if (seconds)
         *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

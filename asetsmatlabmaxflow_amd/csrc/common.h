@@ -16,8 +16,10 @@ constexpr int kWave = 64;
 // Flag words shared by all kernels of one solve (device memory).
 //   flags[0]  : 1 once some iteration met cvg <= err_bound; later kernels no-op
 struct Flags {
-    int32_t done;
-    int32_t pad[63];
+    int32_t done;       // stop: converged, or alpha invalid (every later iteration returns)
+    int32_t bad_alpha;  // alpha <= 0 somewhere (set before the first iteration)
+    int32_t it_num;     // iteration at which the stop test fired (valid when done)
+    int32_t pad[61];
 };
 
 __device__ __forceinline__ float fmin_ref(float a, float b) {

@@ -326,7 +326,8 @@ asets_status hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz, asets_hmf_nod
         pa.fast_cc = fast_cc;
     }
 
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    EventPair evp;
+    hipEvent_t &ev0 = evp.a, &ev1 = evp.b;
     if (o && o->kernel_ms) {
         HIPCHK(hipEventCreate(&ev0));
         HIPCHK(hipEventCreate(&ev1));
@@ -403,8 +404,6 @@ asets_status hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz, asets_hmf_nod
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
         *o->kernel_ms = num_iter > 0 ? (double)ms / num_iter : 0.0;
-        (void)hipEventDestroy(ev0);
-        (void)hipEventDestroy(ev1);
     }
     if (seconds)
         *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

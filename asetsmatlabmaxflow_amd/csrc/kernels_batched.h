@@ -29,6 +29,7 @@ struct BatchMeta {
     double n;          // voxels
     float *cvg;        // [max_iter]
     int32_t *it_num;
+    float *u_out;      // the caller's u
 };
 
 struct BatchCtl {

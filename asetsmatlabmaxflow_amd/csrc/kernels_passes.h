@@ -260,6 +260,7 @@ __global__ void k_finalize(const double *partials, int64_t nparts, double denom,
         cvg[it] = c;
         if (STRICT ? (c < err_bound) : (c <= err_bound)) {
             f->done = 1;
+            f->it_num = it;
             *it_num = it;
         }
     }

@@ -191,7 +191,7 @@ asets_status asets_ishikawa_dev(int ndim, const float *Ct, const float *alpha,
 /* ---- batched binary 3D: many volumes of uneven size, one launch per iteration
  * No reference counterpart (the reference solves one volume per call).  Each
  * volume has its own sizes, arrays, cvg/it_num outputs and stop decision; a
- * converged volume drops out of the work list. */
+ * converged volume drops out of the work list.  1 <= nvol <= 65535. */
 typedef struct asets_volume {
     const float *Cs, *Ct, *alpha;   /* device */
     int64_t ny, nx, nz;

@@ -337,6 +337,38 @@ def test_batched_vs_oracle(cuda):
         assert _bits_equal(r.u.cpu().numpy(), ref.u), sh
 
 
+def test_alpha_error_every_solve(cuda):
+    """alpha <= 0 anywhere (asetsBinaryMF3D_mex.c:317-319): the binary solves (fused fp32
+    and fp16, per-pass), Potts and the batched solve report it, the batched one naming
+    the volume.  The check runs on the device before the first iteration and is read
+    with the first stop-flag poll; an error leaves the library usable, and a following
+    solve equals the oracle."""
+    from asetsmatlabmaxflow_amd import _native as nat
+
+    shape = (8, 6, 4)
+    Cs, Ct, a = workloads.random_smooth_binary(shape, seed=1, alpha=0.1)
+    bad = a.copy()
+    bad[3, 2, 1] = -1.0
+    g = [solver.to_device(x) for x in (Cs, Ct, a)]
+    b = [solver.to_device(x) for x in (Cs, Ct, bad)]
+    for kernel, flow, eb in (("fused", "f32", 1e-4), ("fused", "f16", 0.0),
+                             ("passes", "f32", 0.0)):
+        with pytest.raises(nat.AsetsError, match="alpha"):
+            solver.binary(*b, shape, 40, eb, 0.25, 0.11, kernel=kernel, flow=flow)
+    with pytest.raises(nat.AsetsError, match="volume 1: alpha"):
+        solver.binary_batched([tuple(g) + (shape,), tuple(b) + (shape,)], 10, 0.0, 0.25, 0.11)
+    Ctp, ap = workloads.random_smooth_potts((8, 6, 4), 3, seed=2)
+    ap = ap.copy()
+    ap[1, 1, 1, 2] = 0.0
+    with pytest.raises(nat.AsetsError, match="alpha"):
+        solver.potts(*(solver.to_device(x) for x in (Ctp, ap)), (8, 6, 4), 3, 10, 0.0, 0.2, 0.11)
+    r = solver.binary(*g, shape, 10, 0.0, 0.25, 0.11)
+    ref = orc.binary(Cs, Ct, a, shape, 10, 0.0, 0.25, 0.11)
+    assert _bits_equal(r.u.cpu().numpy(), ref.u)
+    rb, _, _ = solver.binary_batched([tuple(g) + (shape,)], 10, 0.0, 0.25, 0.11)
+    assert rb[0].it_num == 10 and _bits_equal(rb[0].u.cpu().numpy(), ref.u)
+
+
 def test_potts_voronoi_fused_vs_passes_and_stop(cuda):
     """Potts workload generator at reduced size: fused == per-pass kernels bit for bit over
     100 iterations, and the stop rule fires on the same iteration."""
