@@ -226,23 +226,27 @@ TileCfg pick_tiles(int64_t ny, int64_t nx, int64_t nz, const asets_opts *o) {
 // Fused kernel instances: (tile width TX, threads per workgroup).  Wider tiles
 // use more threads so each thread keeps few positions in registers.
 template <bool IS3D, int TX, int NT, typename T>
-void launch_fused_t(const FusedArgs<T> &a, int grid, hipStream_t s) {
+void launch_fused_t(const FusedArgs<T> &a, int grid, hipStream_t s, bool first) {
     // occupancy target: 4 waves per SIMD (<= 128 VGPRs) for the 512- and
     // 1024-thread tiles, 3 for the 256-thread tile (LDS allows 3 per CU)
     constexpr int W = (NT >= 512) ? 4 : 3;
-    kf_binary<IS3D, TX, NT, W, T><<<grid, NT, 0, s>>>(a);
+    if (first) kf_binary<IS3D, TX, NT, W, T, true><<<grid, NT, 0, s>>>(a);
+    else kf_binary<IS3D, TX, NT, W, T><<<grid, NT, 0, s>>>(a);
 }
 
+// first: the cold-start first iteration (the initial cut formed from Cs and Ct,
+// alpha checked into a.alpha_bad / a.alpha_stop)
 template <typename T>
-void launch_fused(bool is3d, int tx, const FusedArgs<T> &a, int grid, hipStream_t s) {
+void launch_fused(bool is3d, int tx, const FusedArgs<T> &a, int grid, hipStream_t s,
+                  bool first = false) {
     if (is3d) {
-        if (tx == 8) launch_fused_t<true, 8, 256, T>(a, grid, s);
-        else if (tx == 32) launch_fused_t<true, 32, 1024, T>(a, grid, s);
-        else launch_fused_t<true, 16, 512, T>(a, grid, s);
+        if (tx == 8) launch_fused_t<true, 8, 256, T>(a, grid, s, first);
+        else if (tx == 32) launch_fused_t<true, 32, 1024, T>(a, grid, s, first);
+        else launch_fused_t<true, 16, 512, T>(a, grid, s, first);
     } else {
-        if (tx == 8) launch_fused_t<false, 8, 256, T>(a, grid, s);
-        else if (tx == 32) launch_fused_t<false, 32, 1024, T>(a, grid, s);
-        else launch_fused_t<false, 16, 512, T>(a, grid, s);
+        if (tx == 8) launch_fused_t<false, 8, 256, T>(a, grid, s, first);
+        else if (tx == 32) launch_fused_t<false, 32, 1024, T>(a, grid, s, first);
+        else launch_fused_t<false, 16, 512, T>(a, grid, s, first);
     }
 }
 
@@ -404,10 +408,14 @@ asets_status run_fused(const FusedRun &r, Carver &cv, DeviceCache &dc) {
         buf[b].by = buf[b].bx + 1;
         buf[b].bz = cv.take<T>(N);
     }
-    if (!r.u0) {
+    const int32_t max_iter = r.p->max_iter;
+    // cold start: iteration 0 forms the initial state itself (and checks alpha);
+    // with no iteration buf[0] must hold it for the outputs
+    const bool first = !r.u0;
+    if (first && max_iter == 0) {
         k_init_binary_cold_t<T><<<nb, kBlock, 0, s>>>(r.Cs, r.Ct, buf[0].u, buf[0].ps,
                                                      buf[0].bx, buf[0].bz, N);
-    } else {
+    } else if (!first) {
         k_init_binary_t<T><<<nb, kBlock, 0, s>>>(r.Cs, r.Ct, r.u0, r.ps0, r.pt0, buf[0].u,
                                                 buf[0].ps, buf[0].pt, N);
         T *const f0[3] = {buf[0].bx, buf[0].by, buf[0].bz};
@@ -434,12 +442,13 @@ asets_status run_fused(const FusedRun &r, Carver &cv, DeviceCache &dc) {
     fa.nblocks = r.nblocks;
     fa.partials = r.partials;
     fa.flags = r.flags;
-    const int32_t max_iter = r.p->max_iter;
+    fa.alpha_bad = &r.flags->bad_alpha;
+    fa.alpha_stop = &r.flags->done;
     if (r.ev0) HIPCHK(hipEventRecord(r.ev0, s));
     auto iter = [&](int32_t it) -> asets_status {
         fa.in = buf[it & 1];
         fa.out = buf[(it + 1) & 1];
-        launch_fused<T>(r.is3d, r.tc.tx, fa, r.grid, s);
+        launch_fused<T>(r.is3d, r.tc.tx, fa, r.grid, s, first && it == 0);
         k_finalize<<<1, 1024, 0, s>>>(r.partials, r.nparts, (double)N, it, r.p->err_bound, r.cvg,
                                       r.it_num, r.flags);
         return ASETS_OK;
@@ -609,7 +618,8 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
     double *partials = cv.take<double>(nparts);
 
     k_fill_i32<<<1, 64, 0, s>>>((int32_t *)flags, 0, sizeof(Flags) / 4);
-    if (max_iter > 0) alpha_check_async(alpha, N, flags, s);
+    // a fused cold start checks alpha in its first iteration
+    if (max_iter > 0 && !(fused && !u0)) alpha_check_async(alpha, N, flags, s);
     k_fill_i32<<<1, 1, 0, s>>>(it_num, max_iter, 1);
     fill(cvg, 0.0f, max_iter, s);
     HIPCHK(hipGetLastError());
@@ -921,29 +931,42 @@ int batched_grid() {
     return std::max(1, cus) * std::max(1, per);
 }
 
+template <int TX, int NT, int W, typename T>
+void launch_batched_t(const BatchVol<T> *vols, const int2 *work, int nwork, const int32_t *vdone,
+                      const int32_t *viter, double *partials, BatchCtl *ctl, float cc,
+                      float steps, int fast_cc, bool first, int32_t *vbad, hipStream_t s) {
+    const int g = std::min(nwork, batched_grid<TX, NT, T>());
+    if (first)
+        kf_binary_batched<TX, NT, W, T, true><<<g, NT, 0, s>>>(vols, work, nwork, vdone, viter,
+                                                               partials, ctl, cc, steps,
+                                                               fast_cc, vbad);
+    else
+        kf_binary_batched<TX, NT, W, T><<<g, NT, 0, s>>>(vols, work, nwork, vdone, viter,
+                                                         partials, ctl, cc, steps, fast_cc,
+                                                         vbad);
+}
+
+// first: the cold-start first iteration of every volume (checks alpha into vbad)
 template <typename T>
 void launch_batched(int tx, const BatchVol<T> *vols, const int2 *work, int nwork,
                     const int32_t *vdone, const int32_t *viter, double *partials, BatchCtl *ctl,
-                    float cc, float steps, int fast_cc, hipStream_t s) {
-    if (tx == 8) {
-        const int g = std::min(nwork, batched_grid<8, 256, T>());
-        kf_binary_batched<8, 256, 3, T><<<g, 256, 0, s>>>(vols, work, nwork, vdone, viter,
-                                                          partials, ctl, cc, steps, fast_cc);
-    } else if (tx == 32) {
-        const int g = std::min(nwork, batched_grid<32, 1024, T>());
-        kf_binary_batched<32, 1024, 4, T><<<g, 1024, 0, s>>>(vols, work, nwork, vdone, viter,
-                                                             partials, ctl, cc, steps, fast_cc);
-    } else {
-        const int g = std::min(nwork, batched_grid<16, 512, T>());
-        kf_binary_batched<16, 512, 4, T><<<g, 512, 0, s>>>(vols, work, nwork, vdone, viter,
-                                                           partials, ctl, cc, steps, fast_cc);
-    }
+                    float cc, float steps, int fast_cc, bool first, int32_t *vbad,
+                    hipStream_t s) {
+    if (tx == 8)
+        launch_batched_t<8, 256, 3, T>(vols, work, nwork, vdone, viter, partials, ctl, cc, steps,
+                                       fast_cc, first, vbad, s);
+    else if (tx == 32)
+        launch_batched_t<32, 1024, 4, T>(vols, work, nwork, vdone, viter, partials, ctl, cc,
+                                         steps, fast_cc, first, vbad, s);
+    else
+        launch_batched_t<16, 512, 4, T>(vols, work, nwork, vdone, viter, partials, ctl, cc, steps,
+                                        fast_cc, first, vbad, s);
 }
 
 // One workgroup per volume: its stop flag, iteration counter, cvg (zeros) and
 // it_num (max_iter, the value when the stop test never fires); the control words.
-__global__ void k_batch_reset(const BatchMeta *meta, int32_t *vdone, int32_t *viter, int nvol,
-                              int32_t max_iter, BatchCtl *ctl) {
+__global__ void k_batch_reset(const BatchMeta *meta, int32_t *vdone, int32_t *viter,
+                              int32_t *vbad, int nvol, int32_t max_iter, BatchCtl *ctl) {
     const int v = blockIdx.x;
     const BatchMeta m = meta[v];
     for (int32_t i = threadIdx.x; i < max_iter; i += blockDim.x) m.cvg[i] = 0.0f;
@@ -951,8 +974,10 @@ __global__ void k_batch_reset(const BatchMeta *meta, int32_t *vdone, int32_t *vi
         *m.it_num = max_iter;
         vdone[v] = (max_iter <= 0) ? 1 : 0;
         viter[v] = 0;
+        vbad[v] = 0;
         if (v == 0) {
             ctl->remaining = (max_iter <= 0) ? 0 : nvol;
+            ctl->bad_alpha = 0;
             ctl->work_next = 0;
         }
     }
@@ -1034,7 +1059,6 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
     double *partials = cv.take<double>(max_items);
     int32_t *vbad = cv.take<int32_t>(nvol);   // alpha <= 0 somewhere in volume v
 
-    if (max_iter > 0) k_fill_i32<<<(unsigned)cdiv(nvol, 256), 256, 0, s>>>(vbad, 0, nvol);
     std::vector<BatchVol<T>> hv(nvol);
     std::vector<BatchMeta> hm(nvol);
     for (int v = 0; v < nvol; ++v) {
@@ -1059,26 +1083,13 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
         hm[v].cvg = V.cvg;
         hm[v].it_num = V.it_num;
         hm[v].u_out = V.u;
-        if (max_iter > 0) {
-            const int64_t blocks = std::min<int64_t>(cdiv(N, kBlock), 4096);
-            k_alpha_check<<<(unsigned)blocks, kBlock, 0, s>>>(V.alpha, N, vbad + v, nullptr);
-        }
-    }
-    if (max_iter > 0) {
-        // one read of every volume's check
-        std::vector<int32_t> bad(nvol);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(bad.data(), vbad, nvol * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        for (int v = 0; v < nvol; ++v)
-            if (bad[v])
-                return fail(ASETS_EALPHA, "volume %d: alpha(x) must be positive. Exiting...", v);
     }
 
-    // (re)initialise every volume's state and the control words
+    // reset the control words and (with_state) every volume's initial state; a solve
+    // that iterates starts with the cold-start iteration instead
     // (needs d_meta: after set_tiling)
-    auto init_all = [&]() -> asets_status {
-        for (int v = 0; v < nvol; ++v) {
+    auto init_all = [&](bool with_state) -> asets_status {
+        for (int v = 0; with_state && v < nvol; ++v) {
             const BatchVol<T> &b = hv[v];
             const int64_t N = (int64_t)b.ny * b.nx * b.nz;
             const unsigned nb = (unsigned)cdiv(N, kBlock);
@@ -1086,7 +1097,8 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
             k_init_binary_cold_t<T><<<nb, kBlock, 0, s>>>(b.Cs, b.Ct, b.u[0], b.ps[0], b.bx[0],
                                                          b.bz[0], N);
         }
-        k_batch_reset<<<(unsigned)nvol, 256, 0, s>>>(d_meta, vdone, viter, nvol, max_iter, ctl);
+        k_batch_reset<<<(unsigned)nvol, 256, 0, s>>>(d_meta, vdone, viter, vbad, nvol, max_iter,
+                                                     ctl);
         HIPCHK(hipGetLastError());
         return ASETS_OK;
     };
@@ -1114,9 +1126,9 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
         HIPCHK(hipStreamSynchronize(s));
         return ASETS_OK;
     };
-    auto one_iter = [&](int tx) {
+    auto one_iter = [&](int tx, bool first = false) {
         launch_batched<T>(tx, d_vols, d_work, nwork, vdone, viter, partials, ctl, p->cc, p->steps,
-                          fast_cc, s);
+                          fast_cc, first, vbad, s);
         k_finalize_batched<<<nvol, 256, 0, s>>>(d_meta, vdone, viter, partials, p->err_bound,
                                                 max_iter, ctl);
     };
@@ -1139,7 +1151,7 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
                 for (int zc : {8, 16, 32, 64}) {
                     st = set_tiling(TileCfg{tx, zc});
                     if (st) return st;
-                    st = init_all();
+                    st = init_all(true);
                     if (st) return st;
                     one_iter(tx);   // warm-up
                     HIPCHK(hipEventRecord(e0, s));
@@ -1159,7 +1171,7 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
     }
     st = set_tiling(cfg);
     if (st) return st;
-    st = init_all();
+    st = init_all(max_iter == 0);
     if (st) return st;
 
     EventPair evp;
@@ -1169,39 +1181,57 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
         HIPCHK(hipEventCreate(&ev1));
         HIPCHK(hipEventRecord(ev0, s));
     }
-    // K iterations captured once as a graph, replayed until every volume stopped
-    int32_t done_iters = 0;
+    // the cold-start iteration, then K iterations captured once as a graph, replayed
+    // until every volume stopped; each poll reads (remaining, bad_alpha)
     if (max_iter > 0) {
-        const int K = std::max(1, std::min(8, (int)max_iter));
+        one_iter(cfg.tx, true);
+        int32_t done_iters = 1;
+        const int K = std::max(1, std::min(8, (int)max_iter - 1));
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
-        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < K; ++k) one_iter(cfg.tx);
-        HIPCHK(hipStreamEndCapture(s, &graph));
-        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        if (max_iter > 1) {
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < K; ++k) one_iter(cfg.tx);
+            HIPCHK(hipStreamEndCapture(s, &graph));
+            HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        }
         asets_status lst = ASETS_OK;
-        while (done_iters < max_iter) {
-            if (hipGraphLaunch(exec, s) != hipSuccess) {
-                lst = fail(ASETS_EDEVICE, "hipGraphLaunch failed");
-                break;
+        for (;;) {
+            if (done_iters < max_iter) {
+                if (hipGraphLaunch(exec, s) != hipSuccess) {
+                    lst = fail(ASETS_EDEVICE, "hipGraphLaunch failed");
+                    break;
+                }
+                done_iters += K;
             }
-            done_iters += K;
-            if (hipMemcpyAsync(dc.host_flag, &ctl->remaining, 4, hipMemcpyDeviceToHost, s) !=
-                    hipSuccess ||
+            if (hipMemcpyAsync(dc.host_flag, &ctl->remaining, 2 * sizeof(int32_t),
+                               hipMemcpyDeviceToHost, s) != hipSuccess ||
                 hipStreamSynchronize(s) != hipSuccess) {
                 lst = fail(ASETS_EDEVICE, "stream error in the batched loop");
                 break;
             }
-            if (*dc.host_flag <= 0) break;
+            if (dc.host_flag[1]) {   // alpha <= 0: name the first such volume
+                std::vector<int32_t> bad(nvol);
+                if (hipMemcpy(bad.data(), vbad, nvol * sizeof(int32_t), hipMemcpyDeviceToHost) !=
+                    hipSuccess) {
+                    lst = fail(ASETS_EDEVICE, "stream error in the batched loop");
+                    break;
+                }
+                int v = 0;
+                while (v < nvol - 1 && !bad[v]) ++v;
+                lst = fail(ASETS_EALPHA, "volume %d: alpha(x) must be positive. Exiting...", v);
+                break;
+            }
+            if (dc.host_flag[0] <= 0 || done_iters >= max_iter) break;
         }
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
         if (lst) return lst;
     }
     if (ev1) HIPCHK(hipEventRecord(ev1, s));
     // u of each volume lives in buffer (iterations executed) & 1: one copy kernel
     // reads each volume's it_num on the device
-    k_batch_take_u<T><<<dim3(64, (unsigned)nvol), 256, 0, s>>>(d_vols, d_meta, max_iter);
+    k_batch_take_u<T><<<dim3(256, (unsigned)nvol), 256, 0, s>>>(d_vols, d_meta, max_iter);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
     if (ev0) {

@@ -34,8 +34,9 @@ struct BatchMeta {
 
 struct BatchCtl {
     int32_t remaining;   // volumes still iterating
+    int32_t bad_alpha;   // some volume has alpha <= 0 (first iteration)
     int32_t work_next;   // work-list cursor of the running iteration
-    int32_t pad[62];
+    int32_t pad[61];
 };
 
 // Values read through a wave-uniform index are uniform, but the compiler cannot
@@ -49,10 +50,14 @@ __device__ __forceinline__ P *uni(P *p) {
     return (P *)(((uint64_t)hi << 32) | lo);
 }
 
-template <int TX, int NT, int WAVES, typename T>
+// FIRST: the first iteration of every volume (cold start, fused_tile), which also
+// checks alpha (vbad[v] = 1 and ctl->bad_alpha = 1 when alpha <= 0 somewhere in
+// volume v).
+template <int TX, int NT, int WAVES, typename T, bool FIRST = false>
 __global__ __launch_bounds__(NT, WAVES) void kf_binary_batched(
     const BatchVol<T> *vols, const int2 *work, int nwork, const int32_t *vdone,
-    const int32_t *viter, double *partials, BatchCtl *ctl, float cc, float steps, int fast_cc) {
+    const int32_t *viter, double *partials, BatchCtl *ctl, float cc, float steps, int fast_cc,
+    int32_t *vbad) {
     __shared__ double red[16];
     __shared__ int item;
     for (;;) {
@@ -88,7 +93,9 @@ __global__ __launch_bounds__(NT, WAVES) void kf_binary_batched(
         a.fast_cc = fast_cc;
         a.partials = nullptr;
         a.flags = nullptr;
-        const double e = fused_tile<true, TX, NT, T>(a, lb);
+        a.alpha_bad = FIRST ? vbad + v : nullptr;
+        a.alpha_stop = FIRST ? &ctl->bad_alpha : nullptr;
+        const double e = fused_tile<true, TX, NT, T, 0, FIRST>(a, lb);
         const double s = block_sum(e, red);
         if (threadIdx.x == 0) partials[uni(V.work0) + lb] = s;
         __syncthreads();

@@ -63,6 +63,9 @@ struct FusedArgs {
     int fast_cc;            // cc is in the range where div_fast is exact
     double *partials;       // [gridDim.x]
     const Flags *flags;
+    // cold-start first iteration: set to 1 when alpha <= 0 at some voxel (stop: also
+    // set, when not null, so that the following iterations return at once)
+    int32_t *alpha_bad, *alpha_stop;
 };
 
 // ---- buffer access --------------------------------------------------------
@@ -107,6 +110,8 @@ template <> struct Io<float> {
     }
     static __device__ __forceinline__ float get(const float *p) { return *p; }
     static __device__ __forceinline__ void put(float *p, float v) { *p = v; }
+    // the value a store and a load give back
+    static __device__ __forceinline__ float stored(float v) { return v; }
 };
 template <> struct Io<__half> {
     static constexpr int kBytes = 2;
@@ -120,6 +125,9 @@ template <> struct Io<__half> {
     }
     static __device__ __forceinline__ float get(const __half *p) { return __half2float(*p); }
     static __device__ __forceinline__ void put(__half *p, float v) { *p = __float2half_rn(v); }
+    static __device__ __forceinline__ float stored(float v) {
+        return __half2float(__float2half_rn(v));
+    }
 };
 
 __device__ __forceinline__ float ldf(__amdgpu_buffer_rsrc_t r, int off) {
@@ -317,10 +325,16 @@ __device__ __forceinline__ float rescale(float g_hi, float g_lo, float b, bool t
 // in.pt holds pt_{k+1} (gk = dv - ((pt_k - pt_{k+1}) + u_k/cc), :287-288); the
 // output stage stores only the faces and dv_k (:470-496) to out.ps; the sink-flow
 // chain and the multipliers run afterwards (k_ishikawa_couple).
-template <bool IS3D, int TX, int NT, typename T, int MODE = 0>
+// FIRST (binary only): the first iteration of a cold start.  The state it reads
+// is the initial cut of asetsBinaryMF3D_mex.c:215-219 -- u = (Cs - Ct >= 0),
+// ps = pt = min(Cs, Ct) (rounded as the flow storage type stores it), all flows
+// 0 -- formed in registers from Cs and Ct instead of loaded, and the alpha
+// values it loads are checked (> 0, asetsBinaryMF3D_mex.c:317-319).
+template <bool IS3D, int TX, int NT, typename T, int MODE = 0, bool FIRST = false>
 __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
     using IO = Io<T>;
     constexpr bool POTTS = (MODE == 1), ISHI = (MODE == 2);
+    static_assert(!FIRST || MODE == 0, "the cold-start iteration is the binary model's");
     constexpr bool PAIR = (MODE == 0);   // binary state: (ps, pt), (bx, by) interleaved
     constexpr int PB = 2 * IO::kBytes;   // bytes of a pair
     static_assert(PAIR || IO::kBytes == 4, "fp16 flow storage is for the binary model");
@@ -342,6 +356,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
 
     const int tid = threadIdx.x;
     double e_acc = 0.0;
+    bool abad = false;   // FIRST: alpha <= 0 seen
     {
         const int tyi = lb % a.ntile_y;
         const int txi = (lb / a.ntile_y) % a.ntile_x;
@@ -401,6 +416,21 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         // load raw plane q (u, ps, pt, bx own/high, by own/high), bz own of
         // plane q+1, and alpha of plane q-1 (for S2b one step later)
         auto load_raw = [&](int q, Raw &r) {
+            if constexpr (FIRST) {
+                const PRef rcs = pref(a.Cs, q, nz, 4, plane);
+                const PRef rct = pref(a.Ct, q, nz, 4, plane);
+#pragma unroll
+                for (int k = 0; k < K2; ++k) {
+                    const float cs = ldf(rcs, off2[k]), ct = ldf(rct, off2[k]);
+                    r.u[k] = (cs - ct >= 0.0f) ? 1.0f : 0.0f;
+                    r.ps[k] = r.pt[k] = IO::stored(fmin_ref(cs, ct));
+                    r.bxl[k] = r.bxr[k] = r.byl[k] = r.byr[k] = r.bzn[k] = 0.0f;
+                }
+                const PRef ra = pref(a.alpha, q - 1, nz, 4, plane);
+#pragma unroll
+                for (int k = 0; k < K1; ++k) r.al[k] = ldf(ra, off1[k]);
+                return;
+            }
             const PRef ru = pref(a.in.u, q, nz, 4, plane);
             // the high x face of the last column is the next plane's (or past the
             // field's) x = 0 face, and the high y face of the last row the next
@@ -520,7 +550,9 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     zr = B1Z[sn][l];
                 }
                 const float v = proj_factor_fast<IS3D>(xr, xl, yr, yl, zr, zl, r.al[k]);
-                if (in1(k, idx)) go[idx] = (((fl1 >> k) & 1) && qin) ? v : 1.0f;
+                const bool live = ((fl1 >> k) & 1) && qin;
+                if (in1(k, idx)) go[idx] = live ? v : 1.0f;
+                if constexpr (FIRST) abad |= live && r.al[k] <= 0.0f;
             }
         };
 
@@ -532,6 +564,20 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         auto load_out = [&](int s, Out &o) {
             if constexpr (ISHI) return;   // the Ishikawa output stage needs none of them
             const int sv = (s >= z0 && s < z1) ? s : -1;   // outside: empty descriptors
+            if constexpr (FIRST) {
+                const PRef rcs = pref(a.Cs, sv, nz, 4, plane);
+                const PRef rct = pref(a.Ct, sv, nz, 4, plane);
+#pragma unroll
+                for (int k = 0; k < KO; ++k) {
+                    const int i = tid % TY, j = tid / TY + k * (NT / TY);
+                    const int off = (x0 + j) * ny + (y0 + i);
+                    o.cs[k] = ldf(rcs, off);
+                    o.ct[k] = ldf(rct, off);
+                    o.u[k] = (o.cs[k] - o.ct[k] >= 0.0f) ? 1.0f : 0.0f;
+                    o.pt[k] = IO::stored(fmin_ref(o.cs[k], o.ct[k]));
+                }
+                return;
+            }
             const PRef ru = pref(a.in.u, sv, nz, 4, plane);
             const PRef rpt = PAIR ? pref(a.in.ps, sv, nz, PB, plane)
                                   : pref(POTTS ? a.in.ps : a.in.pt, sv, nz, IO::kBytes, plane);
@@ -689,9 +735,11 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         Out o;
         if constexpr (IS3D) {
             // bz own of plane z0-2 (raw plane z0-3 would carry it as its "next" bz)
-            const PRef rbz = pref(a.in.bz, z0 - 2, nz, IO::kBytes, plane);
+            if constexpr (!FIRST) {
+                const PRef rbz = pref(a.in.bz, z0 - 2, nz, IO::kBytes, plane);
 #pragma unroll
-            for (int k = 0; k < K2; ++k) bz_cur[k] = ld<IO>(rbz, off2[k]);
+                for (int k = 0; k < K2; ++k) bz_cur[k] = ld<IO>(rbz, off2[k]);
+            }
             load_raw(z0 - 2, r);
             for (int s = z0 - 4; s < z0; ++s) {   // prologue (s + 2 <= z0 + 1 <= z1 + 1)
                 s1(s + 2, r);
@@ -729,6 +777,12 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             s3(0, o);
         }
     }
+    if constexpr (FIRST) {
+        if (abad && a.alpha_bad) {
+            atomicOr(a.alpha_bad, 1);
+            if (a.alpha_stop) atomicOr(a.alpha_stop, 1);
+        }
+    }
     return e_acc;
 }
 
@@ -739,12 +793,12 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 }
 
 // One volume: one workgroup per work item, partial |e| sums per workgroup.
-template <bool IS3D, int TX, int NT, int WAVES, typename T>
+template <bool IS3D, int TX, int NT, int WAVES, typename T, bool FIRST = false>
 __global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
     __shared__ double red[16];
     if (a.flags->done) return;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    const double e = (lb < a.nblocks) ? fused_tile<IS3D, TX, NT, T>(a, lb) : 0.0;
+    const double e = (lb < a.nblocks) ? fused_tile<IS3D, TX, NT, T, 0, FIRST>(a, lb) : 0.0;
     const double s = block_sum(e, red);
     if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
 }

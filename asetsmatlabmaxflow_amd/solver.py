@@ -213,31 +213,44 @@ def binary_batched(volumes, max_iter: int, err_bound: float, cc: float, steps: f
 
     volumes: list of (Cs, Ct, alpha, (ny, nx, nz)) with flat float32 CUDA tensors.
     Returns (results, seconds, loop_ms): one Result per volume (u, cvg, it_num)."""
-    vols = (nat.Volume * len(volumes))()
-    keep = []
+    nv = len(volumes)
+    if nv == 0:
+        raise ValueError("need at least one volume")
+    sizes = []
     for i, (Cs, Ct, alpha, shape) in enumerate(volumes):
         ny, nx, nz = shape
         N = ny * nx * nz
         for t in (Cs, Ct, alpha):
             if t.numel() != N:
                 raise ValueError(f"volume {i}: array has {t.numel()} elements, shape implies {N}")
-        u = torch.empty(N, dtype=torch.float32, device=Cs.device)
-        cvg = torch.empty(max(max_iter, 1), dtype=torch.float32, device=Cs.device)
-        itn = torch.empty(1, dtype=torch.int32, device=Cs.device)
-        keep.append((u, cvg, itn))
+        sizes.append(N)
+    dev = volumes[0][0].device
+    # one allocation per output kind: u of every volume back to back, a cvg row and an
+    # it_num slot per volume (one read-back of all iteration counts at the end)
+    starts = [0]
+    for N in sizes:
+        starts.append(starts[-1] + (N + 63) // 64 * 64)   # 256-byte aligned slices
+    u_all = torch.empty(starts[-1], dtype=torch.float32, device=dev)
+    cvg_all = torch.empty((nv, max(max_iter, 1)), dtype=torch.float32, device=dev)
+    itn_all = torch.empty(nv, dtype=torch.int32, device=dev)
+    vols = (nat.Volume * nv)()
+    us = []
+    for i, (Cs, Ct, alpha, shape) in enumerate(volumes):
+        ny, nx, nz = shape
+        u = u_all[starts[i]:starts[i] + sizes[i]]
+        us.append(u)
         vols[i] = nat.Volume(_ptr(Cs), _ptr(Ct), _ptr(alpha), ny, nx, nz, u.data_ptr(),
-                             cvg.data_ptr(), itn.data_ptr())
+                             cvg_all[i].data_ptr(), itn_all[i:i + 1].data_ptr())
     secs = C.c_double(0.0)
     kms = C.c_double(0.0) if time_kernel else None
     handle = stream.cuda_stream if stream is not None else None
     opts = nat.make_opts(handle, _FLOWS[flow], nat.KERNEL_FUSED, 0, tile_x, zchunk, kms)
     p = nat.make_params(max_iter, err_bound, cc, steps)
-    nat.check(nat.lib().asets_binary3d_batched_dev(len(volumes), vols, C.byref(p),
-                                                   C.byref(secs), C.byref(opts)))
+    nat.check(nat.lib().asets_binary3d_batched_dev(nv, vols, C.byref(p), C.byref(secs),
+                                                   C.byref(opts)))
     out = []
-    for u, cvg, itn in keep:
-        it = int(itn.item())
-        r = Result(u=u, cvg=cvg[:max_iter], it_num=it, seconds=secs.value)
+    for i, it in enumerate(itn_all.cpu().tolist()):
+        r = Result(u=us[i], cvg=cvg_all[i, :max_iter], it_num=it, seconds=secs.value)
         r._executed = it + 1 if it < max_iter else max_iter
         out.append(r)
     return out, secs.value, (kms.value if kms is not None else None)

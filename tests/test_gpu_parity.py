@@ -337,6 +337,46 @@ def test_batched_vs_oracle(cuda):
         assert _bits_equal(r.u.cpu().numpy(), ref.u), sh
 
 
+@pytest.mark.parametrize("flow", ["f32", "f16"])
+@pytest.mark.parametrize("shape", [(37, 29, 21), (70, 45)])
+def test_cold_start_iteration_equals_initial_state(cuda, flow, shape):
+    """A cold start forms the initial cut (u = Cs - Ct >= 0, ps = pt = min(Cs, Ct) as the
+    flow storage rounds it, flows 0; asetsBinaryMF3D_mex.c:215-219) inside its first
+    fused iteration.  The same solve warm-started from that state (written by a separate
+    kernel, first iteration reading it back) gives the same bits: u, cvg and the stop
+    iteration, at fixed counts and under a stop rule."""
+    Cs, Ct, a = workloads.random_smooth_binary(shape, seed=11, alpha=0.2)
+    d = [solver.to_device(x) for x in (Cs, Ct, a)]
+    u0 = solver.to_device((Cs - Ct >= 0).astype(np.float32))
+    pt0 = solver.to_device(np.where(Cs < Ct, Cs, Ct).astype(np.float32))
+    for mi, eb in ((1, 0.0), (7, 0.0), (300, 2e-3)):
+        rc = solver.binary(*d, shape, mi, eb, 0.3, 0.16, kernel="fused", flow=flow)
+        rw = solver.binary(*d, shape, mi, eb, 0.3, 0.16, kernel="fused", flow=flow,
+                           u0=u0, pt0=pt0)
+        assert rc.it_num == rw.it_num, (mi, eb)
+        assert torch.equal(rc.u.view(torch.int32), rw.u.view(torch.int32)), (mi, eb)
+        assert torch.equal(rc.cvg.view(torch.int32), rw.cvg.view(torch.int32)), (mi, eb)
+    if flow == "f32":   # and the oracle, one iteration
+        ref = orc.binary(Cs, Ct, a, shape, 1, 0.0, 0.3, 0.16)
+        r1 = solver.binary(*d, shape, 1, 0.0, 0.3, 0.16, kernel="fused")
+        assert _bits_equal(r1.u.cpu().numpy(), ref.u)
+
+
+def test_batched_zero_and_one_iteration(cuda):
+    """Batched solve with maxIter 0 (u is the initial cut, it_num 0) and 1 (the
+    cold-start iteration alone) against the oracle."""
+    host = _batch_inputs()[:3]
+    dev = [tuple(solver.to_device(x) for x in v[:3]) + (v[3],) for v in host]
+    r0, _, _ = solver.binary_batched(dev, 0, 0.0, 0.35, 0.11)
+    r1, _, _ = solver.binary_batched(dev, 1, 0.0, 0.35, 0.11)
+    for (Cs, Ct, a, sh), a0, a1 in zip(host, r0, r1):
+        assert a0.it_num == 0
+        np.testing.assert_array_equal(a0.u.cpu().numpy(),
+                                      (Cs - Ct >= 0).astype(np.float32).ravel(order="F"))
+        ref = orc.binary(Cs, Ct, a, sh, 1, 0.0, 0.35, 0.11)
+        assert a1.it_num == 1 and _bits_equal(a1.u.cpu().numpy(), ref.u), sh
+
+
 def test_alpha_error_every_solve(cuda):
     """alpha <= 0 anywhere (asetsBinaryMF3D_mex.c:317-319): the binary solves (fused fp32
     and fp16, per-pass), Potts and the batched solve report it, the batched one naming
