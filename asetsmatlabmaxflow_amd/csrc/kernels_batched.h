@@ -55,9 +55,9 @@ __device__ __forceinline__ P *uni(P *p) {
 // volume v).
 template <int TX, int NT, int WAVES, typename T, bool FIRST = false>
 __global__ __launch_bounds__(NT, WAVES) void kf_binary_batched(
-    const BatchVol<T> *vols, const int2 *work, int nwork, const int32_t *vdone,
-    const int32_t *viter, double *partials, BatchCtl *ctl, float cc, float steps, int fast_cc,
-    int32_t *vbad) {
+    const BatchVol<T> *__restrict__ vols, const int2 *__restrict__ work, int nwork,
+    const int32_t *vdone, const int32_t *viter, double *partials, BatchCtl *ctl, float cc,
+    float steps, int fast_cc, int32_t *vbad) {
     __shared__ double red[16];
     __shared__ int item;
     for (;;) {

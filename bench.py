@@ -166,14 +166,20 @@ def bench_binary3d(args):
                              "max_abs_u_diff_vs_fp32": float((r16.u - u_out).abs().max().item())}
 
     if not args.no_tol:
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        rt = solve(20000, 1e-4)
-        torch.cuda.synchronize()
-        ttt = time.perf_counter() - t1
+        # whole solve calls (setup, iterations, stop polls, output copy) from the host:
+        # the first call of this configuration, then the median of the next five
+        runs = []
+        for _ in range(6):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            rt = solve(20000, 1e-4)
+            torch.cuda.synchronize()
+            runs.append(time.perf_counter() - t1)
+        ttt = sorted(runs[1:])[2]
         lab = (rt.u > 0.5)
         e = solver.energy_binary(dCs, dCt, dA, rt.u, shape)
-        out["time_to_tol"] = {"err_bound": 1e-4, "seconds": ttt, "it_num": rt.it_num,
+        out["time_to_tol"] = {"err_bound": 1e-4, "seconds": ttt, "first_call_seconds": runs[0],
+                              "runs_seconds": runs, "it_num": rt.it_num,
                               "iterations": rt.iterations,
                               "final_cvg": float(rt.cvg[rt.iterations - 1].item()),
                               "fg_fraction": float(lab.float().mean().item()),
@@ -192,14 +198,18 @@ def bench_binary3d(args):
             e_star = solver.energy_binary(dCs, dCt, dA, ref.u, shape, stream=stream)[0]
             net.forward(dCs, dCt, dA, shape, 1)   # warm-up
             torch.cuda.synchronize()
-            t3 = time.perf_counter()
-            u0, pt0 = net.forward(dCs, dCt, dA, shape, 1)
-            torch.cuda.synchronize()
-            t_inf = time.perf_counter() - t3
-            rw = solver.binary(dCs, dCt, dA, shape, 20000, 1e-4, cc, steps, u0=u0, pt0=pt0,
-                               stream=stream)
-            torch.cuda.synchronize()
-            t_warm = time.perf_counter() - t3
+            # median of three (inference + solve), as the cold median above
+            t_infs, t_warms = [], []
+            for _ in range(3):
+                t3 = time.perf_counter()
+                u0, pt0 = net.forward(dCs, dCt, dA, shape, 1)
+                torch.cuda.synchronize()
+                t_infs.append(time.perf_counter() - t3)
+                rw = solver.binary(dCs, dCt, dA, shape, 20000, 1e-4, cc, steps, u0=u0,
+                                   pt0=pt0, stream=stream)
+                torch.cuda.synchronize()
+                t_warms.append(time.perf_counter() - t3)
+            t_inf, t_warm = sorted(t_infs)[1], sorted(t_warms)[1]
             e_warm = solver.energy_binary(dCs, dCt, dA, rw.u, shape, stream=stream)[0]
             e_net = solver.energy_binary(dCs, dCt, dA, u0, shape, stream=stream)[0]
             lab_ref = ref.u > 0.5

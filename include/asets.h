@@ -24,6 +24,14 @@
  * float; the reference sums in float32 sequentially, so near err_bound the two
  * can stop on different iterations.  At a fixed iteration count u is
  * bit-identical to the reference operation order.
+ *
+ * alpha <= 0 anywhere (checked when max_iter > 0, as the reference checks it
+ * inside its first projection pass) returns ASETS_EALPHA with the reference's
+ * message.  The check runs on the device ahead of (or, for a cold start of the
+ * fused binary solver, within) the first iteration and is read at the first
+ * stop-flag poll.  On that or any other error the contents of the device
+ * output arrays (u, cvg, it_num) are unspecified; the host-pointer entry
+ * points (and the mex files) write no output on error.
  */
 #ifndef ASETS_H
 #define ASETS_H
