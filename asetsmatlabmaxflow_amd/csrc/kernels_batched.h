@@ -95,7 +95,8 @@ __global__ __launch_bounds__(NT, WAVES) void kf_binary_batched(
         a.flags = nullptr;
         a.alpha_bad = FIRST ? vbad + v : nullptr;
         a.alpha_stop = FIRST ? &ctl->bad_alpha : nullptr;
-        const double e = fused_tile<true, TX, NT, T, 0, FIRST>(a, lb);
+        // normal cache policy: streaming stores measured slower for these volumes
+        const double e = fused_tile<true, TX, NT, T, 0, FIRST, false>(a, lb);
         const double s = block_sum(e, red);
         if (threadIdx.x == 0) partials[uni(V.work0) + lb] = s;
         __syncthreads();

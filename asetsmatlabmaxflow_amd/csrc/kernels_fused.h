@@ -38,6 +38,8 @@
 #pragma once
 #include <hip/hip_fp16.h>
 
+#include <type_traits>
+
 #include "common.h"
 
 namespace asets {
@@ -98,6 +100,17 @@ __device__ __forceinline__ int boff(int e, unsigned bytes) { return (int)((unsig
 // around them).
 __device__ __forceinline__ int drop_elem(unsigned bytes) { return (int)(0xfffffff0u / bytes); }
 
+// Cache-policy operand of the buffer instructions: 0, or non-temporal
+// (streaming).  The single-volume 3D kernels store their output planes, which no
+// workgroup reads back within the launch, and load the output stage's Cs and Ct,
+// which only their own workgroup reads, as streaming: they then displace less of
+// the raw planes' halo that neighbouring tiles re-read from L2.  (Measured: at
+// 256^3 the binary kernel reads 10 % fewer bytes and runs 3.7 % faster.  Where the
+// working set stays in the caches from one launch to the next -- 2D, and the
+// batched kernel's many small volumes -- streaming stores measured slower, so
+// those kernels use 0.)
+constexpr int kStream = 2;
+
 template <typename T> struct Io;
 template <> struct Io<float> {
     static constexpr int kBytes = 4;
@@ -105,8 +118,9 @@ template <> struct Io<float> {
     static __device__ __forceinline__ float load(__amdgpu_buffer_rsrc_t r, int off) {
         return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, boff(off, 4), 0, 0));
     }
+    template <int CP = 0>
     static __device__ __forceinline__ void store(__amdgpu_buffer_rsrc_t r, int off, float v) {
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, boff(off, 4), 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, boff(off, 4), 0, CP);
     }
     static __device__ __forceinline__ float get(const float *p) { return *p; }
     static __device__ __forceinline__ void put(float *p, float v) { *p = v; }
@@ -119,9 +133,10 @@ template <> struct Io<__half> {
         unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, boff(off, 2), 0, 0);
         return __half2float(__ushort_as_half(b));
     }
+    template <int CP = 0>
     static __device__ __forceinline__ void store(__amdgpu_buffer_rsrc_t r, int off, float v) {
         __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(__float2half_rn(v)), r,
-                                              boff(off, 2), 0, 0);
+                                              boff(off, 2), 0, CP);
     }
     static __device__ __forceinline__ float get(const __half *p) { return __half2float(*p); }
     static __device__ __forceinline__ void put(__half *p, float v) { *p = __float2half_rn(v); }
@@ -134,14 +149,16 @@ __device__ __forceinline__ float ldf(__amdgpu_buffer_rsrc_t r, int off) {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, boff(off, 4), 0, 0));
 }
 __device__ __forceinline__ float ldf(const PRef &p, int off) { return ldf(p.r, p.b + off); }
+// fp32 load with cache policy CP
+template <int CP>
+__device__ __forceinline__ float ldf_cp(const PRef &p, int off) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(p.r, boff(p.b + off, 4), 0, CP));
+}
 template <typename IO>
 __device__ __forceinline__ float ld(const PRef &p, int off) { return IO::load(p.r, p.b + off); }
-template <typename IO>
-__device__ __forceinline__ void st(const PRef &p, int off, float v) { IO::store(p.r, p.b + off, v); }
-// store of lane-dependent validity: invalid lanes store past the field (dropped)
-template <typename IO>
-__device__ __forceinline__ void stm(const PRef &p, int off, float v, bool ok) {
-    IO::store(p.r, ok ? p.b + off : drop_elem(IO::kBytes), v);
+template <typename IO, int CP = 0>
+__device__ __forceinline__ void st(const PRef &p, int off, float v) {
+    IO::template store<CP>(p.r, p.b + off, v);
 }
 // One in-plane offset for all stores of an output position to fields of one plane
 // (plane offset b): invalid lanes get the element 2^29-1 relative to the field,
@@ -173,24 +190,22 @@ __device__ __forceinline__ float2 ld_pair<__half>(const PRef &p, int off) {
     return make_float2(__half2float(__ushort_as_half((unsigned short)(w & 0xffffu))),
                        __half2float(__ushort_as_half((unsigned short)(w >> 16))));
 }
-// pair store; ok == false stores past the field (dropped)
-template <typename T>
+// pair store with cache policy CP; ok == false stores past the field (dropped)
+template <typename T, int CP = 0>
 __device__ __forceinline__ void st_pair(const PRef &p, int off, float lo, float hi,
-                                        bool ok = true);
-template <>
-__device__ __forceinline__ void st_pair<float>(const PRef &p, int off, float lo, float hi,
-                                               bool ok) {
-    u32x2 w;
-    w.x = __float_as_uint(lo);
-    w.y = __float_as_uint(hi);
-    __builtin_amdgcn_raw_buffer_store_b64(w, p.r, boff(ok ? p.b + off : drop_elem(8), 8), 0, 0);
-}
-template <>
-__device__ __forceinline__ void st_pair<__half>(const PRef &p, int off, float lo, float hi,
-                                                bool ok) {
-    const uint32_t w = (uint32_t)__half_as_ushort(__float2half_rn(lo)) |
-                       ((uint32_t)__half_as_ushort(__float2half_rn(hi)) << 16);
-    __builtin_amdgcn_raw_buffer_store_b32(w, p.r, boff(ok ? p.b + off : drop_elem(4), 4), 0, 0);
+                                        bool ok = true) {
+    if constexpr (std::is_same<T, float>::value) {
+        u32x2 w;
+        w.x = __float_as_uint(lo);
+        w.y = __float_as_uint(hi);
+        __builtin_amdgcn_raw_buffer_store_b64(w, p.r, boff(ok ? p.b + off : drop_elem(8), 8), 0,
+                                              CP);
+    } else {
+        const uint32_t w = (uint32_t)__half_as_ushort(__float2half_rn(lo)) |
+                           ((uint32_t)__half_as_ushort(__float2half_rn(hi)) << 16);
+        __builtin_amdgcn_raw_buffer_store_b32(w, p.r, boff(ok ? p.b + off : drop_elem(4), 4), 0,
+                                              CP);
+    }
 }
 
 // ---- correctly rounded division, short form ------------------------------
@@ -330,9 +345,12 @@ __device__ __forceinline__ float rescale(float g_hi, float g_lo, float b, bool t
 // ps = pt = min(Cs, Ct) (rounded as the flow storage type stores it), all flows
 // 0 -- formed in registers from Cs and Ct instead of loaded, and the alpha
 // values it loads are checked (> 0, asetsBinaryMF3D_mex.c:317-319).
-template <bool IS3D, int TX, int NT, typename T, int MODE = 0, bool FIRST = false>
+// STREAM: output stores and the output stage's Cs/Ct loads are non-temporal (kStream).
+template <bool IS3D, int TX, int NT, typename T, int MODE = 0, bool FIRST = false,
+          bool STREAM = IS3D>
 __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
     using IO = Io<T>;
+    constexpr int CP = STREAM ? kStream : 0;
     constexpr bool POTTS = (MODE == 1), ISHI = (MODE == 2);
     static_assert(!FIRST || MODE == 0, "the cold-start iteration is the binary model's");
     constexpr bool PAIR = (MODE == 0);   // binary state: (ps, pt), (bx, by) interleaved
@@ -590,8 +608,8 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                 o.u[k] = ldf(ru, off);
                 if constexpr (PAIR) o.pt[k] = ld_pair<T>(rpt, off).y;
                 else o.pt[k] = ld<IO>(rpt, off);
-                if (!POTTS) o.cs[k] = ldf(rcs, off);
-                o.ct[k] = ldf(rct, off);
+                if (!POTTS) o.cs[k] = ldf_cp<CP>(rcs, off);
+                o.ct[k] = ldf_cp<CP>(rct, off);
             }
         };
 
@@ -665,10 +683,10 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                 const bool ok = y < ny && x < nx;
                 const int off = masked_off(ok, x * ny + y, wu.b);
                 if constexpr (ISHI) {
-                    st<IO>(wps, off, d);
-                    st<IO>(wbx, off, bxl);
-                    st<IO>(wby, off, byl);
-                    if (IS3D) st<IO>(wbz, off, bzl);
+                    st<IO, CP>(wps, off, d);
+                    st<IO, CP>(wbx, off, bxl);
+                    st<IO, CP>(wby, off, byl);
+                    if (IS3D) st<IO, CP>(wbz, off, bzl);
                     continue;
                 }
                 const float uo = o.u[k];
@@ -677,11 +695,11 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     // sink flow of this label (asetsPotts3D_mex.c:457-466), ft = dv + pt
                     const float t = (o.pt[k] + uc) - d;   // o.pt holds ps here
                     const float npt = fmin_ref(t, o.ct[k]);
-                    st<IO>(wps, off, d + npt);
-                    st<IO>(wpt, off, npt);
-                    st<IO>(wbx, off, bxl);
-                    st<IO>(wby, off, byl);
-                    if (IS3D) st<IO>(wbz, off, bzl);
+                    st<IO, CP>(wps, off, d + npt);
+                    st<IO, CP>(wpt, off, npt);
+                    st<IO, CP>(wbx, off, bxl);
+                    st<IO, CP>(wby, off, byl);
+                    if (IS3D) st<IO, CP>(wbz, off, bzl);
                 } else {
                     // source/sink flows and multiplier (asetsBinaryMF3D_mex.c:419-432)
                     float f = ((o.pt[k] - uc) + d) + one_cc;
@@ -690,17 +708,17 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     const float npt = fmin_ref(f, o.ct[k]);
                     const float e = cc * ((npt + d) - nps);
                     e_acc += ok ? fabs((double)e) : 0.0;
-                    st<Io<float>>(wu, off, uo - e);
+                    st<Io<float>, CP>(wu, off, uo - e);
                     if constexpr (PAIR) {
-                        st_pair<T>(wps, off, nps, npt);
-                        st_pair<T>(wbx, off, bxl, byl);
+                        st_pair<T, CP>(wps, off, nps, npt);
+                        st_pair<T, CP>(wbx, off, bxl, byl);
                     } else {
-                        st<IO>(wps, off, nps);
-                        st<IO>(wpt, off, npt);
-                        st<IO>(wbx, off, bxl);
-                        st<IO>(wby, off, byl);
+                        st<IO, CP>(wps, off, nps);
+                        st<IO, CP>(wpt, off, npt);
+                        st<IO, CP>(wbx, off, bxl);
+                        st<IO, CP>(wby, off, byl);
                     }
-                    if (IS3D) st<IO>(wbz, off, bzl);
+                    if (IS3D) st<IO, CP>(wbz, off, bzl);
                 }
             }
         };

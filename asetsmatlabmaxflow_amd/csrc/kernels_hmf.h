@@ -150,6 +150,9 @@ __device__ __forceinline__ float hmf_cap(float t, float al) {
 // stored when their g is formed.
 template <bool IS3D, int TX, int NT>
 __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
+    // streaming stores in 3D (kStream); the 2D tree's fields stay in the caches
+    // between the flow and sink launches, where streaming measured 15 % slower
+    constexpr int CP = IS3D ? kStream : 0;
     constexpr int TY = 64;
     constexpr int W2 = TY + 4, H2 = TX + 4;
     constexpr int W1 = TY + 2, H1 = TX + 2;
@@ -281,7 +284,7 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
             const float un = r.u[k] + cc * ((r.pp[k] - d) - r.pt[k]);
             const float uc = div_fast(un, cc, rcc, fast_cc);
             const float v = live ? steps * (((d + r.pt[k]) - r.pp[k]) - uc) : 0.0f;
-            if ((f & 16) && qown) st<Io<float>>(wu, off2[k], un);
+            if ((f & 16) && qown) st<Io<float>, CP>(wu, off2[k], un);
             if (in2(k, idx)) {
                 gk[idx] = v;
                 if (IS3D) B1Z[sl][idx] = (live && q > 0) ? (bz_cur[k] + v) - gkp[k] : 0.0f;
@@ -395,10 +398,10 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
             }
             if (y < ny && x < nx) {
                 const int off = x * ny + y;
-                st<Io<float>>(wd, off, d);
-                st<Io<float>>(wx, off, bxl);
-                st<Io<float>>(wy, off, byl);
-                if (IS3D) st<Io<float>>(wz, off, bzl);
+                st<Io<float>, CP>(wd, off, d);
+                st<Io<float>, CP>(wx, off, bxl);
+                st<Io<float>, CP>(wy, off, byl);
+                if (IS3D) st<Io<float>, CP>(wz, off, bzl);
             }
         }
     };
