@@ -18,7 +18,8 @@
 // TY (=64, the contiguous y axis) x TX voxels and a range of z planes, and
 // streams through the planes.  Every raw plane is loaded from HBM once per
 // workgroup; its loads are issued at the end of the previous step and consumed
-// after the step's output stage.  Per plane step:
+// after the step's output stage.  The output stage's own operands are issued
+// right after the previous output stage, a whole step ahead.  Per plane step:
 //   S1 (plane s+2)  gk over the tile + 2-voxel halo -> LDS GK ring (2), and
 //                   the stepped z face below plane s+2 -> LDS B1Z ring (3)
 //   S2a(plane s+2)  stepped in-plane faces b1 from gk -> LDS B1X/B1Y ring (3)
@@ -34,7 +35,9 @@
 // plane outside the volume, or past the end of the field, return 0 -- exactly
 // the zero-flux value of a high face past the last voxel.  The state is
 // double-buffered (in -> out) because neighbouring workgroups read the halo of
-// the previous iterate.
+// the previous iterate.  The first iteration of a cold start reads no state: it
+// forms the initial cut from Cs and Ct (fused_tile's FIRST).  The single-volume
+// 3D kernels store with the streaming cache policy (kStream).
 #pragma once
 #include <hip/hip_fp16.h>
 
