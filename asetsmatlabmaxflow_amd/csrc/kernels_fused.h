@@ -843,7 +843,9 @@ struct PottsArgs {
 
 // All labels' flow updates of one iteration in one launch: workgroup b handles
 // tile b % nblocks of label b / nblocks (labels are independent inside an
-// iteration, asetsPotts3D_mex.c:172-191).
+// iteration, asetsPotts3D_mex.c:172-191).  Label-major, so that each XCD holds
+// whole slabs of one label and neighbouring tiles share their halo in L2 (label
+// fastest, sharing the label-independent ps instead, measured 3 % slower).
 template <bool IS3D, int TX, int NT, int WAVES>
 __global__ __launch_bounds__(NT, WAVES) void kf_potts_flow(PottsArgs p) {
     if (p.flags->done) return;
@@ -874,6 +876,7 @@ __global__ __launch_bounds__(NT, WAVES) void kf_potts_flow(PottsArgs p) {
     a.fast_cc = p.fast_cc;
     a.partials = nullptr;
     a.flags = nullptr;
+    a.alpha_bad = a.alpha_stop = nullptr;
     (void)fused_tile<IS3D, TX, NT, float, 1>(a, lb);
 }
 
@@ -984,6 +987,7 @@ __global__ __launch_bounds__(NT, WAVES) void kf_ishikawa_flow(IshiArgs p) {
     a.fast_cc = p.fast_cc;
     a.partials = nullptr;
     a.flags = nullptr;
+    a.alpha_bad = a.alpha_stop = nullptr;
     (void)fused_tile<IS3D, TX, NT, float, 2>(a, lb);
 }
 
