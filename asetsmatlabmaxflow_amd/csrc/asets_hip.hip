@@ -447,13 +447,27 @@ asets_status run_fused(const FusedRun &r, Carver &cv, DeviceCache &dc) {
     fa.flags = r.flags;
     fa.alpha_bad = &r.flags->bad_alpha;
     fa.alpha_stop = &r.flags->done;
+    // err_bound 0 (a fixed iteration count; the stop test fires only on an exact
+    // fixed point): no launch has to wait for an iteration's stop test, so it runs
+    // inside the next iteration's kernel (DeferredStop, same bits as k_finalize),
+    // on partials double-buffered by iteration parity; k_finalize only after the
+    // last.  A stop found one launch late is still exact: that launch wrote the
+    // other state buffer, and every later launch returns at once.
+    const bool deferred = (r.p->err_bound == 0.0f);
+    double *const P[2] = {r.partials, r.partials + r.nparts};
     if (r.ev0) HIPCHK(hipEventRecord(r.ev0, s));
     auto iter = [&](int32_t it) -> asets_status {
         fa.in = buf[it & 1];
         fa.out = buf[(it + 1) & 1];
+        fa.partials = deferred ? P[it & 1] : r.partials;
+        fa.stop = DeferredStop{};
+        if (deferred && it > 0)
+            fa.stop = DeferredStop{P[(it - 1) & 1], r.nparts, (double)N, it - 1,
+                                   r.p->err_bound, r.cvg, r.it_num, r.flags};
         launch_fused<T>(r.is3d, r.tc.tx, fa, r.grid, s, first && it == 0);
-        k_finalize<<<1, 1024, 0, s>>>(r.partials, r.nparts, (double)N, it, r.p->err_bound, r.cvg,
-                                      r.it_num, r.flags);
+        if (!deferred || it == max_iter - 1)
+            k_finalize<<<1, 1024, 0, s>>>(fa.partials, r.nparts, (double)N, it, r.p->err_bound,
+                                          r.cvg, r.it_num, r.flags);
         return ASETS_OK;
     };
     int32_t itn = 0;
@@ -608,8 +622,8 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
 
     // workspace
     const size_t es = (flow16 == ASETS_FLOW_F16) ? sizeof(__half) : sizeof(float);
-    size_t need = Carver::sz(1, sizeof(Flags)) + Carver::sz(nparts, sizeof(double)) +
-                  Carver::sz(1, sizeof(int32_t));
+    // fused: two partial buffers (DeferredStop)
+    size_t need = Carver::sz(1, sizeof(Flags)) + Carver::sz(2 * nparts, sizeof(double));
     if (fused)
         need += fused_bytes(N, es);
     else
@@ -618,7 +632,7 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
     if (st) return st;
     Carver cv(dc.dev);
     Flags *flags = cv.take<Flags>(1);
-    double *partials = cv.take<double>(nparts);
+    double *partials = cv.take<double>(2 * nparts);
 
     k_fill_i32<<<1, 64, 0, s>>>((int32_t *)flags, 0, sizeof(Flags) / 4);
     // a fused cold start checks alpha in its first iteration

@@ -55,6 +55,20 @@ struct FusedState {
     T *ps, *pt, *bx, *by, *bz;
 };
 
+// Stop test of iteration `it`, folded into the next iteration's launch (solves
+// with err_bound 0, where no launch waits on it): the partial |e| sums of `it`,
+// its cvg slot and the stop outputs.  partials == nullptr: none.
+struct DeferredStop {
+    const double *partials;
+    int64_t nparts;
+    double denom;
+    int32_t it;
+    float err_bound;
+    float *cvg;
+    int32_t *it_num;
+    Flags *flags;
+};
+
 template <typename T>
 struct FusedArgs {
     const float *Cs, *Ct, *alpha;
@@ -71,6 +85,7 @@ struct FusedArgs {
     // cold-start first iteration: set to 1 when alpha <= 0 at some voxel (stop: also
     // set, when not null, so that the following iterations return at once)
     int32_t *alpha_bad, *alpha_stop;
+    DeferredStop stop;      // the previous iteration's stop test (kf_binary)
 };
 
 // ---- buffer access --------------------------------------------------------
@@ -807,6 +822,33 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
     return e_acc;
 }
 
+// The stop test of a previous iteration, as k_finalize (1024 threads) computes it,
+// by a workgroup of blockDim.x threads (a divisor of 1024): each thread plays
+// 1024 / blockDim.x of k_finalize's threads -- the same strided sums, the same
+// wave sums of the same 64 values, the 16 wave sums added in the same order --
+// so cvg has the same bits.  scratch: >= 16 doubles of LDS, free on entry.
+__device__ __forceinline__ void deferred_stop(const DeferredStop &d, double *scratch) {
+    constexpr int kFinalizeThreads = 1024;
+    for (int vt = threadIdx.x; vt < kFinalizeThreads; vt += blockDim.x) {
+        double v = 0.0;
+        for (int64_t i = vt; i < d.nparts; i += kFinalizeThreads) v += d.partials[i];
+        v = wave_sum(v);
+        if ((vt & (kWave - 1)) == 0) scratch[vt / kWave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < kFinalizeThreads / kWave; ++w) r += scratch[w];
+        const float c = (float)(r / d.denom);
+        d.cvg[d.it] = c;
+        if (c <= d.err_bound) {   // asetsBinaryMF3D_mex.c:183-188
+            d.flags->done = 1;
+            d.flags->it_num = d.it;
+            *d.it_num = d.it;
+        }
+    }
+}
+
 // XCD-aware block order: hardware dispatches block b to XCD b % 8; remap so
 // that consecutive logical tiles share an XCD and its L2.
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
@@ -814,6 +856,8 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 }
 
 // One volume: one workgroup per work item, partial |e| sums per workgroup.
+// The first dispatched workgroup also runs the previous iteration's deferred
+// stop test, when there is one (it starts first, so it has the most slack).
 template <bool IS3D, int TX, int NT, int WAVES, typename T, bool FIRST = false>
 __global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
     __shared__ double red[16];
@@ -822,6 +866,10 @@ __global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
     const double e = (lb < a.nblocks) ? fused_tile<IS3D, TX, NT, T, 0, FIRST>(a, lb) : 0.0;
     const double s = block_sum(e, red);
     if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
+    if (blockIdx.x == 0 && a.stop.partials) {
+        __syncthreads();   // thread 0 has read red
+        deferred_stop(a.stop, red);
+    }
 }
 
 // ---- Potts model -----------------------------------------------------------

@@ -377,6 +377,24 @@ def test_batched_zero_and_one_iteration(cuda):
         assert a1.it_num == 1 and _bits_equal(a1.u.cpu().numpy(), ref.u), sh
 
 
+@pytest.mark.parametrize("flow", ["f32", "f16"])
+@pytest.mark.parametrize("shape,tile_x", [((70, 45, 33), 8), ((70, 45, 33), 16),
+                                          ((70, 45, 33), 32), ((130, 77), 16)])
+def test_deferred_stop_test_same_bits(cuda, flow, shape, tile_x):
+    """errBound 0 folds each iteration's stop test into the next iteration's kernel
+    (workgroups of 256, 512 or 1024 threads playing the finalize kernel's 1024); a
+    negative errBound (never met) keeps the separate finalize kernel.  The cvg traces
+    and u are the same bits."""
+    Cs, Ct, a = workloads.random_smooth_binary(shape, seed=23, alpha=0.15)
+    d = [solver.to_device(x) for x in (Cs, Ct, a)]
+    r0 = solver.binary(*d, shape, 25, 0.0, 0.3, 0.14, kernel="fused", flow=flow, tile_x=tile_x)
+    rn = solver.binary(*d, shape, 25, -1.0, 0.3, 0.14, kernel="fused", flow=flow, tile_x=tile_x)
+    assert r0.it_num == rn.it_num == 25
+    assert torch.equal(r0.cvg.view(torch.int32), rn.cvg.view(torch.int32))
+    assert torch.equal(r0.u.view(torch.int32), rn.u.view(torch.int32))
+    assert bool((r0.cvg > 0).all())
+
+
 def test_alpha_error_every_solve(cuda):
     """alpha <= 0 anywhere (asetsBinaryMF3D_mex.c:317-319): the binary solves (fused fp32
     and fp16, per-pass), Potts and the batched solve report it, the batched one naming
