@@ -276,31 +276,14 @@ __device__ __forceinline__ float div_fast(float x, float d, float r, bool d_ok =
 // Lanes that do not use a division feed it benign operands (1) so they never
 // trigger the exact fallback.
 //
-// Before any FP64 work an fp32 screen proves that bound for most waves: with
-// s = the fp32 sum of the six squares (fused multiply-adds, 6 roundings), the
-// real sum is at most s*(1 + 8*2^-24) when no square underflows, and the
-// double-rounded sum at most that times (1 + 2^-50); an underflowing square
-// (|b| < 2^-63) adds at most 2^-126 each.  The test
-//     s * 0.5*(1 + 2^-18)  <=  (a*a) * (1 - 2^-20)          (fp32, both sides rounded)
-// therefore implies t*0.5 <= alpha^2 whenever alpha is in [2^-50, 2^50]
-// (the slack alpha^2 * 2^-19 covers 3*2^-126).  An inf or NaN square fails
-// the test and takes the exact path.
+// There is no fp32 screen ahead of the FP64 sum: non-packed FP64 FMAs issue at
+// the fp32 rate on gfx950, so a screen could save at most the six FP64 FMAs of a
+// wave it clears, while costing as many fp32 operations on every wave; on the
+// 256^3 workload almost no wave is cleared (most hold a voxel on the |p| = alpha
+// boundary), and dropping the screen measured 2.4 % faster.
 template <bool IS3D>
 __device__ __forceinline__ float proj_factor_fast(float xr, float xl, float yr, float yl,
                                                   float zr, float zl, float a) {
-    {
-        float s = xr * xr;
-        s = __builtin_fmaf(xl, xl, s);
-        s = __builtin_fmaf(yr, yr, s);
-        s = __builtin_fmaf(yl, yl, s);
-        if (IS3D) {
-            s = __builtin_fmaf(zr, zr, s);
-            s = __builtin_fmaf(zl, zl, s);
-        }
-        const bool a_ok = a >= 0x1p-50f && a <= 0x1p50f;
-        const bool one = a_ok && (s * 0x1.00004p-1f <= (a * a) * 0x1.ffffep-1f);
-        if (wave_all(one)) return 1.0f;
-    }
     double t = (double)xr * (double)xr;
     t = __builtin_fma((double)xl, (double)xl, t);
     t = __builtin_fma((double)yr, (double)yr, t);
