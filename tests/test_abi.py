@@ -96,6 +96,12 @@ def test_bad_arguments_rejected_before_device_work():
     # oversized plane
     assert L.asets_binary3d(ptr, ptr, ptr, 1 << 16, 1 << 15, 1, C.byref(p), ptr, ptr,
                             C.byref(itn), None, None) == nat.ASETS_ETOOBIG
+    # batched: 1 to 65,535 volumes
+    vols = (nat.Volume * 1)()
+    assert L.asets_binary3d_batched_dev(0, vols, C.byref(p), None, None) == nat.ASETS_EBADARG
+    assert L.asets_binary3d_batched_dev(65536, vols, C.byref(p), None,
+                                        None) == nat.ASETS_EBADARG
+    assert b"65535" in L.asets_last_error()
 
 
 def test_mex_argument_validation():
