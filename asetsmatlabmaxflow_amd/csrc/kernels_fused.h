@@ -103,8 +103,9 @@ struct PRef {
 
 __device__ __forceinline__ PRef pref(const void *base, int q, int nz, int bytes_per_elem,
                                      int plane) {
-    return PRef{__builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0,
-                                                  nz * plane * bytes_per_elem, 0x00020000),
+    // num_records in unsigned arithmetic: an 8-byte pair field may pass 2^31 bytes
+    const unsigned bytes = (unsigned)nz * (unsigned)plane * (unsigned)bytes_per_elem;
+    return PRef{__builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)bytes, 0x00020000),
                 q * plane};
 }
 
