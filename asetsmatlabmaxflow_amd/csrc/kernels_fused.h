@@ -365,16 +365,17 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
     constexpr int K2 = (N2 + NT - 1) / NT;    // radius-2 positions per thread
     constexpr int K1 = (N1 + NT - 1) / NT;    // radius-1 positions per thread
     constexpr int KO = (TY * TX) / NT;        // output voxels per thread
+    constexpr int NH = N2 - TY * TX;          // halo positions of the radius-2 region
     static_assert((TY * TX) % NT == 0, "tile must be a multiple of the block");
+    static_assert(KO + (NH + NT - 1) / NT == K2, "radius-2 slots: own positions, then halo");
 
     __shared__ float GK[2][N2];
     __shared__ float B1X[3][N2], B1Y[3][N2], B1Z[IS3D ? 3 : 1][IS3D ? N2 : 1];
     __shared__ float G[2][N1];
-    // only the last position slot of a thread can fall past the region
-    auto in2 = [&](int k, int idx) { return k < K2 - 1 || (N2 % NT) == 0 || idx < N2; };
-    auto in1 = [&](int k, int idx) { return k < K1 - 1 || (N1 % NT) == 0 || idx < N1; };
-
     const int tid = threadIdx.x;
+    // radius-2 slot k holds a position (only the last halo slot can run past NH)
+    auto in2 = [&](int k) { return k < KO || (NH % NT) == 0 || tid + (k - KO) * NT < NH; };
+    auto in1 = [&](int k, int idx) { return k < K1 - 1 || (N1 % NT) == 0 || idx < N1; };
     double e_acc = 0.0;
     bool abad = false;   // FIRST: alpha <= 0 seen
     {
@@ -391,22 +392,48 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         const bool fast_cc = a.fast_cc != 0;
 
         // ---- per-thread position tables (fixed for all planes) ------------
-        // radius-2 positions: in-plane offset and flags
+        // radius-2 region (W2 x H2, region row i, column j): slots 0 .. KO-1
+        // are the tile's own positions in the output stage's order (row
+        // tid % TY, column tid / TY + k * (NT / TY)), so the u and pt a thread
+        // loads for S1 are the ones its output stage uses three steps later;
+        // slots KO .. K2-1 hold the NH halo positions, halo index h = tid +
+        // (k - KO) * NT: the two columns either side of the tile (h < 4 * TY,
+        // TY rows each), then the two rows either side (4 per column).
+        // Per slot: in-plane offset, LDS index, and flags
         //   bit0 inside the volume, bit1 x>0, bit2 y>0, bit3 i>=1 && j>=1
         static_assert(K2 <= 16, "flag word holds 16 positions");
         int off2[K2];
+        int l2h[K2 - KO];
+        const int l2o = (2 + tid / TY) * W2 + 2 + tid % TY;   // LDS index of slot 0
         uint64_t fl2 = 0;
 #pragma unroll
         for (int k = 0; k < K2; ++k) {
-            const int idx = tid + k * NT;
-            const int i = idx % W2, j = idx / W2;
+            int i, j;
+            if (k < KO) {
+                i = 2 + tid % TY;
+                j = 2 + tid / TY + k * (NT / TY);
+            } else {
+                const int h = tid + (k - KO) * NT;
+                if (h < 4 * TY) {
+                    const int c = h / TY;
+                    i = 2 + h % TY;
+                    j = (c < 2) ? c : H2 - 4 + c;
+                } else {
+                    const int h2 = h - 4 * TY, r4 = h2 & 3;
+                    i = (r4 < 2) ? r4 : W2 - 4 + r4;
+                    j = in2(k) ? h2 >> 2 : 0;
+                }
+                l2h[k - KO] = j * W2 + i;
+            }
             const int y = y0 - 2 + i, x = x0 - 2 + j;
             off2[k] = x * ny + y;
-            const bool in = idx < N2 && y >= 0 && y < ny && x >= 0 && x < nx;
+            const bool ok = in2(k);
+            const bool in = ok && y >= 0 && y < ny && x >= 0 && x < nx;
             const uint64_t f = (in ? 1u : 0u) | (x > 0 ? 2u : 0u) | (y > 0 ? 4u : 0u) |
-                               ((idx < N2 && i >= 1 && j >= 1) ? 8u : 0u);
+                               ((ok && i >= 1 && j >= 1) ? 8u : 0u);
             fl2 |= f << (4 * k);
         }
+        auto l2 = [&](int k) { return k < KO ? l2o + k * (NT / TY) * W2 : l2h[k - KO]; };
         // radius-1 positions: LDS index (in the N2 layout), offset, inside flag
         static_assert(K1 <= 32, "flag word holds 32 positions");
         int off1[K1], l2i1[K1];
@@ -509,14 +536,14 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             const bool qin = q >= 0 && q < nz;
 #pragma unroll
             for (int k = 0; k < K2; ++k) {
-                const int idx = tid + k * NT;
+                const int idx = l2(k);
                 const uint32_t f = (uint32_t)(fl2 >> (4 * k));
                 const bool live = (f & 1) && qin;
                 float d = ((r.byr[k] - r.byl[k]) + r.bxr[k]) - r.bxl[k];
                 if (IS3D) d = (d + r.bzn[k]) - bz_cur[k];
                 const float uc = div_fast(r.u[k], cc, rcc, fast_cc);
                 const float v = live ? d - ((r.ps[k] - r.pt[k]) + uc) : 0.0f;
-                if (in2(k, idx)) {
+                if (in2(k)) {
                     gk[idx] = v;
                     if (IS3D) {
                         const float bz1 =
@@ -536,11 +563,11 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             const bool qin = q >= 0 && q < nz;
 #pragma unroll
             for (int k = 0; k < K2; ++k) {
-                const int idx = tid + k * NT;
+                const int idx = l2(k);
                 const uint32_t f = (uint32_t)(fl2 >> (4 * k));
                 const bool live = (f & 1) && qin;
                 // positions on the first row/column of the region are never read
-                if (in2(k, idx)) {
+                if (in2(k)) {
                     const float c = gq[idx];
                     const float cx = gq[(f & 8) ? idx - W2 : idx];
                     const float cy = gq[(f & 8) ? idx - 1 : idx];
@@ -576,42 +603,33 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             }
         };
 
-        // S3 operands of plane s at the output positions
-        // (binary: u, pt, Cs, Ct; Potts: u, ps, -, Ct)
+        // S3 operands of plane s at the output positions, loaded one step
+        // ahead (binary: Cs, Ct; Potts: -, Ct)
         struct Out {
-            float u[KO], pt[KO], cs[KO], ct[KO];
+            float cs[KO], ct[KO];
         };
         auto load_out = [&](int s, Out &o) {
             if constexpr (ISHI) return;   // the Ishikawa output stage needs none of them
             const int sv = (s >= z0 && s < z1) ? s : -1;   // outside: empty descriptors
-            if constexpr (FIRST) {
-                const PRef rcs = pref(a.Cs, sv, nz, 4, plane);
-                const PRef rct = pref(a.Ct, sv, nz, 4, plane);
-#pragma unroll
-                for (int k = 0; k < KO; ++k) {
-                    const int i = tid % TY, j = tid / TY + k * (NT / TY);
-                    const int off = (x0 + j) * ny + (y0 + i);
-                    o.cs[k] = ldf(rcs, off);
-                    o.ct[k] = ldf(rct, off);
-                    o.u[k] = (o.cs[k] - o.ct[k] >= 0.0f) ? 1.0f : 0.0f;
-                    o.pt[k] = IO::stored(fmin_ref(o.cs[k], o.ct[k]));
-                }
-                return;
-            }
-            const PRef ru = pref(a.in.u, sv, nz, 4, plane);
-            const PRef rpt = PAIR ? pref(a.in.ps, sv, nz, PB, plane)
-                                  : pref(POTTS ? a.in.ps : a.in.pt, sv, nz, IO::kBytes, plane);
             const PRef rcs = pref(a.Cs, POTTS ? -1 : sv, nz, 4, plane);
             const PRef rct = pref(a.Ct, sv, nz, 4, plane);
 #pragma unroll
             for (int k = 0; k < KO; ++k) {
-                const int i = tid % TY, j = tid / TY + k * (NT / TY);
-                const int off = (x0 + j) * ny + (y0 + i);
-                o.u[k] = ldf(ru, off);
-                if constexpr (PAIR) o.pt[k] = ld_pair<T>(rpt, off).y;
-                else o.pt[k] = ld<IO>(rpt, off);
+                const int off = off2[k];   // slot k < KO: output position k
                 if (!POTTS) o.cs[k] = ldf_cp<CP>(rcs, off);
                 o.ct[k] = ldf_cp<CP>(rct, off);
+            }
+        };
+        // S3 operands kept from the raw plane S1 read (own-position slots):
+        // binary u and pt, Potts u and ps (the label-shared source flow)
+        struct Keep {
+            float u[KO], p[KO];
+        };
+        auto keep = [&](Keep &kp, const Raw &r) {
+#pragma unroll
+            for (int k = 0; k < KO; ++k) {
+                kp.u[k] = r.u[k];
+                kp.p[k] = POTTS ? r.ps[k] : r.pt[k];
             }
         };
 
@@ -641,7 +659,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         };
 
         // S3: output plane s (z0 <= s < z1)
-        auto s3 = [&](int s, const Out &o) {
+        auto s3 = [&](int s, const Out &o, const Keep &kp) {
             const int sl = (s + 3) % 3;
             const float *g0 = G[s & 1];
             const float *g1 = G[(s + 1) & 1];
@@ -691,11 +709,11 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     if (IS3D) st<IO, CP>(wbz, off, bzl);
                     continue;
                 }
-                const float uo = o.u[k];
+                const float uo = kp.u[k];
                 const float uc = div_fast(uo, cc, rcc, fast_cc);
                 if (POTTS) {
                     // sink flow of this label (asetsPotts3D_mex.c:457-466), ft = dv + pt
-                    const float t = (o.pt[k] + uc) - d;   // o.pt holds ps here
+                    const float t = (kp.p[k] + uc) - d;   // kp.p holds ps here
                     const float npt = fmin_ref(t, o.ct[k]);
                     st<IO, CP>(wps, off, d + npt);
                     st<IO, CP>(wpt, off, npt);
@@ -704,7 +722,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     if (IS3D) st<IO, CP>(wbz, off, bzl);
                 } else {
                     // source/sink flows and multiplier (asetsBinaryMF3D_mex.c:419-432)
-                    float f = ((o.pt[k] - uc) + d) + one_cc;
+                    float f = ((kp.p[k] - uc) + d) + one_cc;
                     const float nps = fmin_ref(f, o.cs[k]);
                     f = (nps + uc) - d;
                     const float npt = fmin_ref(f, o.ct[k]);
@@ -731,7 +749,9 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         // of raw plane s+3; barrier.  Steps z0-4 .. z0-1 are the prologue (gk,
         // b1 and g of the planes below the chunk, no output), step z0 sets the
         // carried z face of plane z0, steps z0+1 .. z1 output planes
-        // z0 .. z1-1.  Every load is issued right after the last use of its
+        // z0 .. z1-1.  S3's u and pt (Potts: ps) are the raw plane's own-position
+        // slots, kept in registers from S1 three steps earlier (k0, k1, k2), so
+        // the output stage loads only Cs and Ct.  Every load is issued right after the last use of its
         // registers, so it stays in flight across the barriers and the stages
         // between: the S3 operands behind S1, both barriers, S2a and S2b; the
         // raw plane behind S3.  The hardware counts loads and stores together
@@ -753,6 +773,8 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         };
         Raw r;
         Out o;
+        // kept S3 operands of planes s-1, s, s+1 at the start of step s
+        Keep k0, k1, k2;
         if constexpr (IS3D) {
             // bz own of plane z0-2 (raw plane z0-3 would carry it as its "next" bz)
             if constexpr (!FIRST) {
@@ -763,6 +785,9 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             load_raw(z0 - 2, r);
             for (int s = z0 - 4; s < z0; ++s) {   // prologue (s + 2 <= z0 + 1 <= z1 + 1)
                 s1(s + 2, r);
+                k0 = k1;
+                k1 = k2;
+                keep(k2, r);
                 tail(s, r);
             }
             s3_carry(z0 - 1);                     // step z0
@@ -770,14 +795,22 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             load_out(z0, o);                      // consumed by S3(z0) at the next step
             __builtin_amdgcn_sched_barrier(0);
             s1(z0 + 2, r);
+            k0 = k1;
+            k1 = k2;
+            keep(k2, r);
             tail(z0, r);
             for (int s = z0 + 1; s <= z1; ++s) {
-                s3(s - 1, o);
+                s3(s - 1, o, k0);
                 // S3 has read o: refill it; S1 and S2 are free of memory operations
                 __builtin_amdgcn_sched_barrier(0);
                 load_out(s, o);
                 __builtin_amdgcn_sched_barrier(0);
-                if (s + 2 <= z1 + 1) s1(s + 2, r);
+                k0 = k1;
+                k1 = k2;
+                if (s + 2 <= z1 + 1) {
+                    s1(s + 2, r);
+                    keep(k2, r);
+                }
                 tail(s, r);
             }
         } else {
@@ -788,13 +821,14 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
 #pragma unroll
             for (int k = 0; k < K1; ++k) r.al[k] = ldf(ra, off1[k]);
             load_out(0, o);
+            keep(k0, r);
             s1(0, r);
             __syncthreads();
             s2a(0, r);
             __syncthreads();
             s2b(0, r);
             __syncthreads();
-            s3(0, o);
+            s3(0, o, k0);
         }
     }
     if constexpr (FIRST) {
