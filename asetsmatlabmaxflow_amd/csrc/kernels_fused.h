@@ -366,8 +366,10 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
     constexpr int K1 = (N1 + NT - 1) / NT;    // radius-1 positions per thread
     constexpr int KO = (TY * TX) / NT;        // output voxels per thread
     constexpr int NH = N2 - TY * TX;          // halo positions of the radius-2 region
+    constexpr int NH1 = N1 - TY * TX;         // halo positions of the radius-1 region
     static_assert((TY * TX) % NT == 0, "tile must be a multiple of the block");
     static_assert(KO + (NH + NT - 1) / NT == K2, "radius-2 slots: own positions, then halo");
+    static_assert(KO + (NH1 + NT - 1) / NT == K1, "radius-1 slots: own positions, then halo");
 
     __shared__ float GK[2][N2];
     __shared__ float B1X[3][N2], B1Y[3][N2], B1Z[IS3D ? 3 : 1][IS3D ? N2 : 1];
@@ -375,7 +377,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
     const int tid = threadIdx.x;
     // radius-2 slot k holds a position (only the last halo slot can run past NH)
     auto in2 = [&](int k) { return k < KO || (NH % NT) == 0 || tid + (k - KO) * NT < NH; };
-    auto in1 = [&](int k, int idx) { return k < K1 - 1 || (N1 % NT) == 0 || idx < N1; };
+    auto in1 = [&](int k) { return k < KO || (NH1 % NT) == 0 || tid + (k - KO) * NT < NH1; };
     double e_acc = 0.0;
     bool abad = false;   // FIRST: alpha <= 0 seen
     {
@@ -434,20 +436,41 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             fl2 |= f << (4 * k);
         }
         auto l2 = [&](int k) { return k < KO ? l2o + k * (NT / TY) * W2 : l2h[k - KO]; };
-        // radius-1 positions: LDS index (in the N2 layout), offset, inside flag
+        // radius-1 region (W1 x H1, row i1, column j1), slots ordered as the
+        // radius-2 ones: own positions first (output order), then the NH1 halo
+        // positions (the column either side, then the row either side, 2 per
+        // column).  Per slot: offset, index in the G (N1) layout and in the N2
+        // layout, inside flag.
         static_assert(K1 <= 32, "flag word holds 32 positions");
-        int off1[K1], l2i1[K1];
+        int off1[K1], l1h[K1 - KO], l2i1h[K1 - KO];
+        const int l1o = (1 + tid / TY) * W1 + 1 + tid % TY;   // G index of slot 0
         uint32_t fl1 = 0;
 #pragma unroll
         for (int k = 0; k < K1; ++k) {
-            const int idx = tid + k * NT;
-            const int i1 = idx % W1, j1 = idx / W1;
+            int i1, j1;
+            if (k < KO) {
+                i1 = 1 + tid % TY;
+                j1 = 1 + tid / TY + k * (NT / TY);
+            } else {
+                const int h = tid + (k - KO) * NT;
+                if (h < 2 * TY) {
+                    i1 = 1 + h % TY;
+                    j1 = (h < TY) ? 0 : H1 - 1;
+                } else {
+                    const int h2 = h - 2 * TY;
+                    i1 = (h2 & 1) ? W1 - 1 : 0;
+                    j1 = in1(k) ? h2 >> 1 : 0;
+                }
+                l1h[k - KO] = j1 * W1 + i1;
+                l2i1h[k - KO] = (j1 + 1) * W2 + (i1 + 1);
+            }
             const int y = y0 - 1 + i1, x = x0 - 1 + j1;
             off1[k] = x * ny + y;
-            l2i1[k] = idx < N1 ? (j1 + 1) * W2 + (i1 + 1) : W2 + 1;
-            const bool in = idx < N1 && y >= 0 && y < ny && x >= 0 && x < nx;
+            const bool in = in1(k) && y >= 0 && y < ny && x >= 0 && x < nx;
             fl1 |= (in ? 1u : 0u) << k;
         }
+        auto l1 = [&](int k) { return k < KO ? l1o + k * (NT / TY) * W1 : l1h[k - KO]; };
+        auto l2i1 = [&](int k) { return k < KO ? l2(k) : l2i1h[k - KO]; };
 
         // ---- raw plane registers (prefetched one step ahead) --------------
         struct Raw {
@@ -587,8 +610,8 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             const bool qin = q >= 0 && q < nz;
 #pragma unroll
             for (int k = 0; k < K1; ++k) {
-                const int idx = tid + k * NT;
-                const int l = l2i1[k];
+                const int idx = l1(k);
+                const int l = l2i1(k);
                 const float xl = B1X[sl][l], xr = B1X[sl][l + W2];
                 const float yl = B1Y[sl][l], yr = B1Y[sl][l + 1];
                 float zl = 0.0f, zr = 0.0f;
@@ -598,7 +621,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                 }
                 const float v = proj_factor_fast<IS3D>(xr, xl, yr, yl, zr, zl, r.al[k]);
                 const bool live = ((fl1 >> k) & 1) && qin;
-                if (in1(k, idx)) go[idx] = live ? v : 1.0f;
+                if (in1(k)) go[idx] = live ? v : 1.0f;
                 if constexpr (FIRST) abad |= live && r.al[k] <= 0.0f;
             }
         };
