@@ -160,17 +160,20 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
     constexpr int K2 = (N2 + NT - 1) / NT;
     constexpr int K1 = (N1 + NT - 1) / NT;
     constexpr int KO = (TY * TX) / NT;
+    constexpr int NH = N2 - TY * TX, NH1 = N1 - TY * TX;   // halo positions
     static_assert((TY * TX) % NT == 0, "tile must be a multiple of the block");
+    static_assert(KO + (NH + NT - 1) / NT == K2, "radius-2 slots: own positions, then halo");
+    static_assert(KO + (NH1 + NT - 1) / NT == K1, "radius-1 slots: own positions, then halo");
     static_assert(K2 <= 8, "flag word holds 8 positions");
     static_assert(K1 <= 32, "flag word holds 32 positions");
 
     __shared__ float GK[2][N2];
     __shared__ float B1X[3][N2], B1Y[3][N2], B1Z[IS3D ? 3 : 1][IS3D ? N2 : 1];
     __shared__ float G[2][N1];
-    auto in2 = [&](int k, int idx) { return k < K2 - 1 || (N2 % NT) == 0 || idx < N2; };
-    auto in1 = [&](int k, int idx) { return k < K1 - 1 || (N1 % NT) == 0 || idx < N1; };
-
     const int tid = threadIdx.x;
+    // slot k holds a position (only the last halo slot can run past the halo)
+    auto in2 = [&](int k) { return k < KO || (NH % NT) == 0 || tid + (k - KO) * NT < NH; };
+    auto in1 = [&](int k) { return k < KO || (NH1 % NT) == 0 || tid + (k - KO) * NT < NH1; };
     const int slot = lb % a.nflow;
     const int rest = lb / a.nflow;
     const int n = a.flow[slot];
@@ -196,34 +199,72 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
     const float rcc = rcp_refined(cc);
     const bool fast_cc = a.fast_cc != 0;
 
-    // radius-2 positions: bit0 inside, bit1 x>0, bit2 y>0, bit3 i>=1 && j>=1,
-    // bit4 inside the tile (its label is stored here)
-    int off2[K2];
+    // Position slots as in the binary tile (kernels_fused.h): slots 0 .. KO-1
+    // are the tile's own positions (row tid % TY, column tid / TY + k * (NT /
+    // TY)), the halo follows (the columns either side, then the rows either
+    // side).  Radius-2 flags: bit0 inside, bit1 x>0, bit2 y>0, bit3 i>=1 &&
+    // j>=1, bit4 inside the tile (its label is stored here)
+    int off2[K2], l2h[K2 - KO];
+    const int l2o = (2 + tid / TY) * W2 + 2 + tid % TY;
     uint64_t fl2 = 0;
 #pragma unroll
     for (int k = 0; k < K2; ++k) {
-        const int idx = tid + k * NT;
-        const int i = idx % W2, j = idx / W2;
+        int i, j;
+        if (k < KO) {
+            i = 2 + tid % TY;
+            j = 2 + tid / TY + k * (NT / TY);
+        } else {
+            const int h = tid + (k - KO) * NT;
+            if (h < 4 * TY) {
+                const int c = h / TY;
+                i = 2 + h % TY;
+                j = (c < 2) ? c : H2 - 4 + c;
+            } else {
+                const int h2 = h - 4 * TY, r4 = h2 & 3;
+                i = (r4 < 2) ? r4 : W2 - 4 + r4;
+                j = in2(k) ? h2 >> 2 : 0;
+            }
+            l2h[k - KO] = j * W2 + i;
+        }
         const int y = y0 - 2 + i, x = x0 - 2 + j;
         off2[k] = x * ny + y;
-        const bool in = idx < N2 && y >= 0 && y < ny && x >= 0 && x < nx;
-        const bool own = in && i >= 2 && i < TY + 2 && j >= 2 && j < TX + 2;
+        const bool ok = in2(k);
+        const bool in = ok && y >= 0 && y < ny && x >= 0 && x < nx;
+        const bool own = in && k < KO;
         const uint64_t f = (in ? 1u : 0u) | (x > 0 ? 2u : 0u) | (y > 0 ? 4u : 0u) |
-                           ((idx < N2 && i >= 1 && j >= 1) ? 8u : 0u) | (own ? 16u : 0u);
+                           ((ok && i >= 1 && j >= 1) ? 8u : 0u) | (own ? 16u : 0u);
         fl2 |= f << (8 * k);
     }
-    int off1[K1], l2i1[K1];
+    auto l2 = [&](int k) { return k < KO ? l2o + k * (NT / TY) * W2 : l2h[k - KO]; };
+    int off1[K1], l1h[K1 - KO], l2i1h[K1 - KO];
+    const int l1o = (1 + tid / TY) * W1 + 1 + tid % TY;
     uint32_t fl1 = 0;
 #pragma unroll
     for (int k = 0; k < K1; ++k) {
-        const int idx = tid + k * NT;
-        const int i1 = idx % W1, j1 = idx / W1;
+        int i1, j1;
+        if (k < KO) {
+            i1 = 1 + tid % TY;
+            j1 = 1 + tid / TY + k * (NT / TY);
+        } else {
+            const int h = tid + (k - KO) * NT;
+            if (h < 2 * TY) {
+                i1 = 1 + h % TY;
+                j1 = (h < TY) ? 0 : H1 - 1;
+            } else {
+                const int h2 = h - 2 * TY;
+                i1 = (h2 & 1) ? W1 - 1 : 0;
+                j1 = in1(k) ? h2 >> 1 : 0;
+            }
+            l1h[k - KO] = j1 * W1 + i1;
+            l2i1h[k - KO] = (j1 + 1) * W2 + (i1 + 1);
+        }
         const int y = y0 - 1 + i1, x = x0 - 1 + j1;
         off1[k] = x * ny + y;
-        l2i1[k] = idx < N1 ? (j1 + 1) * W2 + (i1 + 1) : W2 + 1;
-        const bool in = idx < N1 && y >= 0 && y < ny && x >= 0 && x < nx;
+        const bool in = in1(k) && y >= 0 && y < ny && x >= 0 && x < nx;
         fl1 |= (in ? 1u : 0u) << k;
     }
+    auto l1 = [&](int k) { return k < KO ? l1o + k * (NT / TY) * W1 : l1h[k - KO]; };
+    auto l2i1 = [&](int k) { return k < KO ? l2(k) : l2i1h[k - KO]; };
 
     struct Raw {
         float u[K2], pt[K2], pp[K2], bxl[K2], bxr[K2], byl[K2], byr[K2], bzn[K2];
@@ -276,7 +317,7 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
         const PRef wu = pref(u_out, q, nz, 4, plane);
 #pragma unroll
         for (int k = 0; k < K2; ++k) {
-            const int idx = tid + k * NT;
+            const int idx = l2(k);
             const uint32_t f = (uint32_t)(fl2 >> (8 * k));
             const bool live = (f & 1) && qin;
             float d = ((r.byr[k] - r.byl[k]) + r.bxr[k]) - r.bxl[k];
@@ -284,8 +325,8 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
             const float un = r.u[k] + cc * ((r.pp[k] - d) - r.pt[k]);
             const float uc = div_fast(un, cc, rcc, fast_cc);
             const float v = live ? steps * (((d + r.pt[k]) - r.pp[k]) - uc) : 0.0f;
-            if ((f & 16) && qown) st<Io<float>, CP>(wu, off2[k], un);
-            if (in2(k, idx)) {
+            if (k < KO && (f & 16) && qown) st<Io<float>, CP>(wu, off2[k], un);
+            if (in2(k)) {
                 gk[idx] = v;
                 if (IS3D) B1Z[sl][idx] = (live && q > 0) ? (bz_cur[k] + v) - gkp[k] : 0.0f;
             }
@@ -300,10 +341,10 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
         const bool qin = q >= 0 && q < nz;
 #pragma unroll
         for (int k = 0; k < K2; ++k) {
-            const int idx = tid + k * NT;
+            const int idx = l2(k);
             const uint32_t f = (uint32_t)(fl2 >> (8 * k));
             const bool live = (f & 1) && qin;
-            if (in2(k, idx)) {
+            if (in2(k)) {
                 const float c = gq[idx];
                 const float cx = gq[(f & 8) ? idx - W2 : idx];
                 const float cy = gq[(f & 8) ? idx - 1 : idx];
@@ -328,8 +369,8 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
         const bool qin = q >= 0 && q < nz;
 #pragma unroll
         for (int k = 0; k < K1; ++k) {
-            const int idx = tid + k * NT;
-            const int l = l2i1[k];
+            const int idx = l1(k);
+            const int l = l2i1(k);
             const float xl = B1X[sl][l], xr = B1X[sl][l + W2];
             const float yl = B1Y[sl][l], yr = B1Y[sl][l + 1];
             float t = yl * yl + yr * yr;
@@ -341,7 +382,7 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
             const float al = amap ? r.al[k] : as;
             const bool live = ((fl1 >> k) & 1) && qin;
             const float v = live ? hmf_cap(t, al) : 1.0f;
-            if (in1(k, idx)) go[idx] = v;
+            if (in1(k)) go[idx] = v;
         }
     };
 
