@@ -61,6 +61,24 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// ((0 + p[i]) + p[i + st]) + ... over i < n, added in index order; the loads
+// are issued eight at a time ahead of their adds, so a long strided sum waits
+// on memory once per eight elements instead of once per element.
+__device__ __forceinline__ double strided_sum(const double *p, int64_t n, int64_t i,
+                                              int64_t st) {
+    constexpr int B = 8;
+    double v = 0.0;
+    for (; i + (B - 1) * st < n; i += B * st) {
+        double t[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) t[b] = p[i + b * st];
+#pragma unroll
+        for (int b = 0; b < B; ++b) v += t[b];
+    }
+    for (; i < n; i += st) v += p[i];
+    return v;
+}
+
 // Block-wide sum (blockDim.x a multiple of 64, <= 1024); result valid in
 // thread 0.  scratch: >= 16 doubles of LDS.
 __device__ __forceinline__ double block_sum(double v, double *scratch) {

@@ -871,8 +871,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
 __device__ __forceinline__ void deferred_stop(const DeferredStop &d, double *scratch) {
     constexpr int kFinalizeThreads = 1024;
     for (int vt = threadIdx.x; vt < kFinalizeThreads; vt += blockDim.x) {
-        double v = 0.0;
-        for (int64_t i = vt; i < d.nparts; i += kFinalizeThreads) v += d.partials[i];
+        double v = strided_sum(d.partials, d.nparts, vt, kFinalizeThreads);
         v = wave_sum(v);
         if ((vt & (kWave - 1)) == 0) scratch[vt / kWave] = v;
     }

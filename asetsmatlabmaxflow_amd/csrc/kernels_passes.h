@@ -252,8 +252,7 @@ __global__ void k_finalize(const double *partials, int64_t nparts, double denom,
                            float err_bound, float *cvg, int32_t *it_num, Flags *f) {
     __shared__ double red[16];
     if (f->done) return;
-    double v = 0.0;
-    for (int64_t i = threadIdx.x; i < nparts; i += blockDim.x) v += partials[i];
+    const double v = strided_sum(partials, nparts, threadIdx.x, blockDim.x);
     double s = block_sum(v, red);
     if (threadIdx.x == 0) {
         const float c = (float)(s / denom);
