@@ -851,17 +851,26 @@ asets_status potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny
         pa.fast_cc = (std::fabs(p->cc) >= 0x1p-16f && std::fabs(p->cc) <= 0x1p16f) ? 1 : 0;
         pa.flags = flags;
         grid = (int)cdiv((int64_t)pa.nblocks * L, 8) * 8;
-        iter = [&](int32_t it) -> asets_status {
+        // errBound 0 (a fixed iteration count): iteration it's stop test runs in
+        // the first workgroup of the next flow launch, and k_finalize only after
+        // the last iteration
+        const bool deferred = (p->err_bound == 0.0f);
+        iter = [&, deferred](int32_t it) -> asets_status {
             pa.pt_in = ptb[it & 1];
             pa.pt_out = ptb[(it + 1) & 1];
             for (int k = 0; k < 3; ++k) {
                 pa.b_in[k] = bb[it & 1][k];
                 pa.b_out[k] = bb[(it + 1) & 1][k];
             }
+            pa.stop = DeferredStop{};
+            if (deferred && it > 0)
+                pa.stop = DeferredStop{partials, nparts, (double)NL, it - 1, p->err_bound, cvg,
+                                       it_num, flags};
             launch_potts_flow(is3d, tc.tx, pa, grid, s);
             launch_potts_couple((int)L, pa.ft, u, ps, N, p->cc, pa.fast_cc, partials, flags, s);
-            k_finalize<<<1, 1024, 0, s>>>(partials, nparts, (double)NL, it, p->err_bound, cvg,
-                                          it_num, flags);
+            if (!deferred || it == max_iter - 1)
+                k_finalize<<<1, 1024, 0, s>>>(partials, nparts, (double)NL, it, p->err_bound,
+                                              cvg, it_num, flags);
             return ASETS_OK;
         };
     }
@@ -1495,13 +1504,24 @@ asets_status ishikawa_dev(int ndim, const float *Ct, const float *alpha, int64_t
     ia.fast_cc = (std::fabs(p->cc) >= 0x1p-16f && std::fabs(p->cc) <= 0x1p16f) ? 1 : 0;
     ia.flags = flags;
     const int grid = (int)cdiv((int64_t)ia.nblocks * M, 8) * 8;
+    // errBound 0 (a fixed iteration count): iteration it's stop test runs in the
+    // first workgroup of the next flow launch, and k_finalize only after the last
+    // iteration.  The deferred test stops on cvg <= bound; 2D's strict cvg < 0
+    // is cvg <= the largest float below 0 (never, as cvg >= 0).
+    const bool deferred = (p->err_bound == 0.0f);
+    const float deferred_bound = is3d ? p->err_bound : std::nextafter(0.0f, -1.0f);
     auto iter = [&](int32_t it) -> asets_status {
         for (int k = 0; k < 3; ++k) {
             ia.b_in[k] = bb[it & 1][k];
             ia.b_out[k] = bb[(it + 1) & 1][k];
         }
+        ia.stop = DeferredStop{};
+        if (deferred && it > 0)
+            ia.stop = DeferredStop{partials, nparts, (double)NM, it - 1, deferred_bound, cvg,
+                                   it_num, flags};
         launch_ishikawa_flow(is3d, tc.tx, ia, grid, s);
         launch_ishikawa_couple((int)L, dv, pt, u, Ct, N, p->cc, ia.fast_cc, partials, flags, s);
+        if (deferred && it != max_iter - 1) return ASETS_OK;
         // 3D stops on cvg <= errBound (asetsIshikawa3D_mex.c:213), 2D on < (2D :205)
         if (is3d)
             k_finalize<false><<<1, 1024, 0, s>>>(partials, nparts, (double)NM, it,

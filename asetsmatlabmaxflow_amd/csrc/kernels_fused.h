@@ -927,6 +927,7 @@ struct PottsArgs {
     int nblocks;               // per label
     int fast_cc;
     const Flags *flags;
+    DeferredStop stop;         // the previous iteration's stop test (partials null: none)
 };
 
 // All labels' flow updates of one iteration in one launch: workgroup b handles
@@ -934,8 +935,13 @@ struct PottsArgs {
 // iteration, asetsPotts3D_mex.c:172-191).  Label-major, so that each XCD holds
 // whole slabs of one label and neighbouring tiles share their halo in L2 (label
 // fastest, sharing the label-independent ps instead, measured 3 % slower).
+// The first dispatched workgroup also runs the previous iteration's deferred stop
+// test, when there is one: this launch writes only the other pt and face buffers
+// and ft, so a stop found here leaves that iteration's state whole, and the
+// coupling kernel after it (the only writer of u and ps) returns at once.
 template <bool IS3D, int TX, int NT, int WAVES>
 __global__ __launch_bounds__(NT, WAVES) void kf_potts_flow(PottsArgs p) {
+    __shared__ double red[16];
     if (p.flags->done) return;
     const int total = p.nblocks * p.nlab;
     const int gb = xcd_remap(blockIdx.x, gridDim.x);
@@ -966,6 +972,7 @@ __global__ __launch_bounds__(NT, WAVES) void kf_potts_flow(PottsArgs p) {
     a.flags = nullptr;
     a.alpha_bad = a.alpha_stop = nullptr;
     (void)fused_tile<IS3D, TX, NT, float, 1>(a, lb);
+    if (blockIdx.x == 0 && p.stop.partials) deferred_stop(p.stop, red);
 }
 
 // Source flow and multipliers, labels coupled per voxel in label order
@@ -1040,12 +1047,17 @@ struct IshiArgs {
     int nblocks;               // per flow field
     int fast_cc;
     const Flags *flags;
+    DeferredStop stop;         // the previous iteration's stop test (partials null: none)
 };
 
 // All M flow updates of one iteration in one launch (the label loop of
-// asetsIshikawa3D_mex.c:162-187 has no dependence between labels).
+// asetsIshikawa3D_mex.c:162-187 has no dependence between labels).  The first
+// dispatched workgroup runs the previous iteration's deferred stop test as in
+// kf_potts_flow: this launch writes only dv and the other face buffers, and the
+// coupling kernel (the only writer of pt and u) returns at once after a stop.
 template <bool IS3D, int TX, int NT, int WAVES>
 __global__ __launch_bounds__(NT, WAVES) void kf_ishikawa_flow(IshiArgs p) {
+    __shared__ double red[16];
     if (p.flags->done) return;
     const int total = p.nblocks * p.nflow;
     const int gb = xcd_remap(blockIdx.x, gridDim.x);
@@ -1077,6 +1089,7 @@ __global__ __launch_bounds__(NT, WAVES) void kf_ishikawa_flow(IshiArgs p) {
     a.flags = nullptr;
     a.alpha_bad = a.alpha_stop = nullptr;
     (void)fused_tile<IS3D, TX, NT, float, 2>(a, lb);
+    if (blockIdx.x == 0 && p.stop.partials) deferred_stop(p.stop, red);
 }
 
 // Sink flows (asetsIshikawa3D_mex.c:498-576: bottom, middle and top layer, in

@@ -79,6 +79,20 @@ def test_exact_zero_error_stop_rules(cuda):
         assert _bits_equal(r.u.cpu().numpy(), ref.u)
 
 
+@pytest.mark.parametrize("shape,L", [((45, 38, 29), 3), ((33, 70, 17), 5), ((70, 53), 4)])
+def test_deferred_stop_test_same_bits(cuda, shape, L):
+    """errBound 0 runs each iteration's stop test in the first workgroup of the next flow
+    launch (3D: cvg <= 0, 2D: cvg < 0, never); a negative errBound keeps the finalize
+    kernel after every iteration.  u, cvg and the iteration count are the same bits."""
+    Ct, a = _costs(shape, L, 5 * L + 1)
+    r0 = _run(Ct, a, shape, L, 25, 0.0, 0.75, 0.1)
+    rn = _run(Ct, a, shape, L, 25, -1.0, 0.75, 0.1)
+    assert r0.it_num == rn.it_num == 25
+    assert _bits_equal(r0.cvg.cpu().numpy(), rn.cvg.cpu().numpy())
+    assert _bits_equal(r0.u.cpu().numpy(), rn.u.cpu().numpy())
+    assert bool((r0.cvg > 0).all())
+
+
 def test_mex_boundary(cuda):
     sh3, L = (31, 22, 9), 4
     Ct, a = _costs(sh3, L, 5)

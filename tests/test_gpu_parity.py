@@ -395,6 +395,22 @@ def test_deferred_stop_test_same_bits(cuda, flow, shape, tile_x):
     assert bool((r0.cvg > 0).all())
 
 
+@pytest.mark.parametrize("shape,nlab", [((33, 20, 11), 6), ((70, 9, 5), 4), ((65, 18, 12), 3),
+                                        ((64, 48), 4)])
+def test_potts_deferred_stop_test_same_bits(cuda, shape, nlab):
+    """Fused Potts: errBound 0 runs each iteration's stop test in the first workgroup of
+    the next flow launch; a negative errBound (never met) keeps the finalize kernel
+    after every iteration.  u, cvg and the iteration count are the same bits."""
+    Ct, a = workloads.random_smooth_potts(shape, nlab, seed=sum(shape) + 3 * nlab)
+    d = [solver.to_device(x) for x in (Ct, a)]
+    r0 = solver.potts(*d, shape, nlab, 25, 0.0, 0.25, 0.11, kernel="fused")
+    rn = solver.potts(*d, shape, nlab, 25, -1.0, 0.25, 0.11, kernel="fused")
+    assert r0.it_num == rn.it_num == 25
+    assert torch.equal(r0.cvg.view(torch.int32), rn.cvg.view(torch.int32))
+    assert torch.equal(r0.u.view(torch.int32), rn.u.view(torch.int32))
+    assert bool((r0.cvg > 0).all())
+
+
 def test_alpha_error_every_solve(cuda):
     """alpha <= 0 anywhere (asetsBinaryMF3D_mex.c:317-319): the binary solves (fused fp32
     and fp16, per-pass), Potts and the batched solve report it, the batched one naming
