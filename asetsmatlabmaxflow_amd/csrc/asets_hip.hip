@@ -203,25 +203,51 @@ struct TileCfg {
     int zchunk;
 };
 
+// z-chunk length of the single-volume fused kernels for `tiles` tiles per z
+// chunk layer (all labels or flow fields of a launch).  A workgroup streams its
+// chunk plus the 4-plane prologue; the 64 x 16 tile's LDS allows two resident
+// workgroups per CU, and two give about 1.25x the throughput of one.  A CU given
+// k = ceil(W / CUs) of the W workgroups then takes about (k == 1 ? 1 : k / 1.25)
+// workgroup-times, and the chunk (at least 8 planes) with the least modelled
+// time wins.  A grid just past one resident round is what it avoids: Ishikawa at
+// 192^3 with 5 flow fields (180 tiles) took 0.731 ms per iteration with 64-plane
+// chunks (540 workgroups) and 0.623 ms with the 48 planes the model picks; for
+// the binary kernel it picks what a resident-round rule picked (256^3: 32).
+int pick_zchunk(int64_t tiles, int64_t nz) {
+    if (nz <= 8) return (int)std::max<int64_t>(nz, 1);
+    int cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0) {
+        (void)hipGetLastError();
+        cus = 256;
+    }
+    double best = 0.0;
+    int64_t best_zc = nz;
+    for (int64_t nzc = 1; nzc <= nz / 8; ++nzc) {
+        const int64_t zc = cdiv(nz, nzc);
+        const int64_t k = cdiv(tiles * cdiv(nz, zc), cus);
+        const double cost = (k == 1 ? 1.0 : (double)k / 1.25) * (double)(zc + 4);
+        if (nzc == 1 || cost < best) {
+            best = cost;
+            best_zc = zc;
+        }
+    }
+    return (int)best_zc;
+}
+
 // Tile shape for the fused kernel.  Default: 64 x 16 tiles (512 threads, two
-// workgroups per CU by LDS) and z chunks sized so the grid is about one
-// resident round of 2 workgroups x 256 CUs; a chunk is at least 8 planes so
-// the 4-plane z prologue stays a small overhead.  A single plane with fewer
-// 64 x 16 tiles than CUs takes 64 x 8 tiles: twice the workgroups on its short,
-// latency-bound iteration (256^2: 9.15 vs 9.43 us).  Overridden by asets_opts.
+// workgroups per CU by LDS) and z chunks from pick_zchunk.  A single plane with
+// fewer 64 x 16 tiles than CUs takes 64 x 8 tiles: twice the workgroups on its
+// short, latency-bound iteration (256^2: 9.15 vs 9.43 us).  Overridden by
+// asets_opts.
 TileCfg pick_tiles(int64_t ny, int64_t nx, int64_t nz, const asets_opts *o) {
     TileCfg t;
     if (o && o->tile_x > 0) t.tx = o->tile_x;
     else t.tx = (nz == 1 && cdiv(ny, 64) * cdiv(nx, 16) < 256) ? 8 : 16;
     if (t.tx != 8 && t.tx != 16 && t.tx != 32) t.tx = 16;
-    int64_t tiles = cdiv(ny, 64) * cdiv(nx, t.tx);
-    int zc;
-    if (o && o->zchunk > 0) {
-        zc = o->zchunk;
-    } else {
-        int64_t nzc = std::max<int64_t>(1, (512 + tiles / 2) / tiles);
-        zc = (int)std::max<int64_t>(8, cdiv(nz, nzc));
-    }
+    const int64_t tiles = cdiv(ny, 64) * cdiv(nx, t.tx);
+    const int64_t zc = (o && o->zchunk > 0) ? o->zchunk : pick_zchunk(tiles, nz);
     t.zchunk = (int)std::min<int64_t>(zc, nz);
     return t;
 }
@@ -826,12 +852,8 @@ asets_status potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny
         tc.tx = (o && o->tile_x > 0) ? o->tile_x : 16;
         if (tc.tx != 8 && tc.tx != 32) tc.tx = 16;
         const int64_t tiles = cdiv(ny, 64) * cdiv(nx, tc.tx) * L;
-        if (o && o->zchunk > 0) {
-            tc.zchunk = (int)std::min<int64_t>(o->zchunk, nz);
-        } else {
-            const int64_t nzc = std::max<int64_t>(1, (512 + tiles / 2) / tiles);
-            tc.zchunk = (int)std::min<int64_t>(nz, std::max<int64_t>(8, cdiv(nz, nzc)));
-        }
+        tc.zchunk = (int)std::min<int64_t>(
+            nz, (o && o->zchunk > 0) ? (int64_t)o->zchunk : pick_zchunk(tiles, nz));
         pa.Ct = Ct;
         pa.alpha = alpha;
         pa.u = u;
@@ -1479,12 +1501,8 @@ asets_status ishikawa_dev(int ndim, const float *Ct, const float *alpha, int64_t
     tc.tx = (o && o->tile_x > 0) ? o->tile_x : 16;
     if (tc.tx != 8 && tc.tx != 32) tc.tx = 16;
     const int64_t tiles = cdiv(ny, 64) * cdiv(nx, tc.tx) * M;
-    if (o && o->zchunk > 0) {
-        tc.zchunk = (int)std::min<int64_t>(o->zchunk, nz);
-    } else {
-        const int64_t nzc = std::max<int64_t>(1, (512 + tiles / 2) / tiles);
-        tc.zchunk = (int)std::min<int64_t>(nz, std::max<int64_t>(8, cdiv(nz, nzc)));
-    }
+    tc.zchunk = (int)std::min<int64_t>(
+        nz, (o && o->zchunk > 0) ? (int64_t)o->zchunk : pick_zchunk(tiles, nz));
     IshiArgs ia{};
     ia.alpha = alpha;
     ia.u = u;
