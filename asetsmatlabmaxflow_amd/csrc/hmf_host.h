@@ -100,7 +100,8 @@ asets_status hmf_plan(int ndim, const asets_hmf_node *in, int32_t M, HmfPlan &pl
     return ASETS_OK;
 }
 
-// z-chunk length of the fused flow kernel: a workgroup streams chunk + 4 planes
+// z-chunk length of the fused flow kernel with 64 x 8 or 64 x 32 tiles (the
+// 64 x 16 tile uses pick_zchunk): a workgroup streams chunk + 4 planes
 // (the 4-plane prologue), and the grid runs in rounds of `slots` resident
 // workgroups, so the time is about rounds(tiles * chunks) * (chunk + 4).  The
 // smallest such cost wins; ties go to the longer chunk (fewer halo re-reads).
@@ -283,8 +284,11 @@ asets_status hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz, asets_hmf_nod
         tx = (o && o->tile_x > 0) ? o->tile_x : 16;
         if (tx != 8 && tx != 32) tx = 16;
         const int64_t tiles = cdiv(ny, 64) * cdiv(nx, tx) * nflow;
+        // the default 64 x 16 tile: the binary kernel's per-CU load model (192^3
+        // two-level tree: 64 -> 32-plane chunks, 0.967 -> 0.910 ms per iteration)
         const int zc = (o && o->zchunk > 0) ? (int)std::min<int64_t>(o->zchunk, nz)
-                                            : hmf_zchunk(tiles, nz, hmf_resident(is3d, tx));
+                       : (tx == 16) ? pick_zchunk(tiles, nz)
+                                    : hmf_zchunk(tiles, nz, hmf_resident(is3d, tx));
         fa.nodes = d_nodes;
         fa.flow = d_flow;
         fa.nflow = nflow;
