@@ -76,8 +76,9 @@ typedef struct asets_opts {
     int32_t flow_dtype;    /* ASETS_FLOW_*                       (binary only)                     */
     int32_t kernel;        /* ASETS_KERNEL_*                                                        */
     int32_t poll_every;    /* iterations enqueued between host polls of the stop flag; 0 = auto     */
-    int32_t tile_x;        /* fused kernel tile width along x; 0 = autotuned                        */
-    int32_t zchunk;        /* fused kernel planes per workgroup; 0 = autotuned                      */
+    int32_t tile_x;        /* fused kernel tile width along x (8, 16, 32); 0 = library's choice     */
+    int32_t zchunk;        /* fused kernel planes per workgroup; 0 = library's choice (one volume:  */
+                           /* a per-CU load model; batched: timed on the caller's data, cached)     */
     int32_t reserved[3];
     double *kernel_ms;     /* out (optional): mean device time per iteration of the solve loop      */
 } asets_opts;
