@@ -24,7 +24,7 @@ EXPORTED = (
     "asets_binary3d_batched_dev", "asets_binary_energy_dev", "asets_potts_energy_dev",
     "asets_cost_loglik_dev", "asets_cost_neglog_dev", "asets_alpha_contrast_dev",
     "asets_labels_dev", "asets_cost_l1_dev", "asets_cost_smooth_labels_dev", "asets_hmf_dev",
-    "asets_selftest_division", "asets_last_error",
+    "asets_selftest_division", "asets_hbm_stream_dev", "asets_last_error",
     "asets_version", "asets_release",
 )
 
@@ -106,8 +106,10 @@ def lib() -> C.CDLL:
         L.asets_cost_smooth_labels_dev.argtypes = [P, I64, I64, P, C.c_int32, P, C.c_int32, P,
                                                    PO]
         L.asets_hmf_dev.argtypes = [C.c_int, I64, I64, I64, C.POINTER(HmfNode), C.c_int32,
-                                    C.c_int32, C.c_float, C.c_float, PD, PO]
+                                    C.c_int32, C.c_float, C.c_double, PD, PO]
         L.asets_selftest_division.argtypes = [C.c_int, C.c_float, C.POINTER(C.c_uint64)]
+        L.asets_hbm_stream_dev.argtypes = [C.c_int, P, I64, C.c_int32, C.c_int32, C.c_int32,
+                                           PD, C.POINTER(C.c_int64), PO]
         for name in EXPORTED:
             if name in ("asets_last_error", "asets_version"):
                 getattr(L, name).restype = C.c_char_p

@@ -30,6 +30,7 @@
 #include "kernels_prepost.h"
 #include "kernels_fused.h"
 #include "kernels_hmf.h"
+#include "kernels_membench.h"
 #include "kernels_passes.h"
 
 using namespace asets;
@@ -170,9 +171,10 @@ inline void fill(float *p, float v, int64_t n, hipStream_t s) {
 // Enqueue iterations [0, max_iter) via enqueue_iter(it), polling the stop flag
 // on the host every poll iterations.  Returns the number of iterations whose
 // kernels were enqueued.
+// hflag: host memory for the three flag words read at each poll.
 template <typename F>
 asets_status run_loop(int32_t max_iter, float err_bound, int32_t poll_opt, hipStream_t s,
-                      DeviceCache &dc, Flags *flags, int32_t *itn, F &&enqueue_iter) {
+                      int32_t *hflag, Flags *flags, int32_t *itn, F &&enqueue_iter) {
     int32_t poll = poll_opt > 0 ? poll_opt : (err_bound > 0.0f ? 16 : 256);
     int32_t it = 0;
     *itn = max_iter;   // no stop before max_iter
@@ -185,12 +187,12 @@ asets_status run_loop(int32_t max_iter, float err_bound, int32_t poll_opt, hipSt
         it += k;
         HIPCHK(hipGetLastError());
         // done, bad_alpha and it_num in one read
-        HIPCHK(hipMemcpyAsync(dc.host_flag, &flags->done, 3 * sizeof(int32_t),
-                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(hflag, &flags->done, 3 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              s));
         HIPCHK(hipStreamSynchronize(s));
-        if (dc.host_flag[1]) return alpha_error();
-        if (dc.host_flag[0]) {
-            *itn = dc.host_flag[2];
+        if (hflag[1]) return alpha_error();
+        if (hflag[0]) {
+            *itn = hflag[2];
             break;
         }
         if (poll < 128 && err_bound > 0.0f && poll_opt <= 0) poll *= 2;
@@ -423,7 +425,7 @@ size_t fused_bytes(int64_t N, size_t es) {
 }
 
 template <typename T>
-asets_status run_fused(const FusedRun &r, Carver &cv, DeviceCache &dc) {
+asets_status run_fused(const FusedRun &r, Carver &cv, int32_t *hflag) {
     const int64_t N = r.ny * r.nx * r.nz;
     const hipStream_t s = r.s;
     const unsigned nb = (unsigned)cdiv(N, kBlock);
@@ -497,7 +499,7 @@ asets_status run_fused(const FusedRun &r, Carver &cv, DeviceCache &dc) {
         return ASETS_OK;
     };
     int32_t itn = 0;
-    asets_status st = run_loop(max_iter, r.p->err_bound, r.poll, s, dc, r.flags, &itn, iter);
+    asets_status st = run_loop(max_iter, r.p->err_bound, r.poll, s, hflag, r.flags, &itn, iter);
     if (st) return st;
     if (r.ev1) HIPCHK(hipEventRecord(r.ev1, s));
     *r.itn = itn;
@@ -710,7 +712,7 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
                                           it_num, flags);
             return ASETS_OK;
         };
-        st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc, flags, &itn, iter);
+        st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc.host_flag, flags, &itn, iter);
         if (st) return st;
         if (ev1) HIPCHK(hipEventRecord(ev1, s));
         if (o_ps) HIPCHK(hipMemcpyAsync(o_ps, ps, N * 4, hipMemcpyDeviceToDevice, s));
@@ -724,7 +726,8 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
                     o_ps, o_pt, o_bx, o_by, o_bz, tc, ntile_y, ntile_x, nzc, nblocks,
                     grid_fused, nparts, partials, flags, s, ev0, ev1,
                     o ? o->poll_every : 0, &itn};
-        st = (flow16 == ASETS_FLOW_F16) ? run_fused<__half>(fr, cv, dc) : run_fused<float>(fr, cv, dc);
+        st = (flow16 == ASETS_FLOW_F16) ? run_fused<__half>(fr, cv, dc.host_flag)
+                                          : run_fused<float>(fr, cv, dc.host_flag);
         if (st) return st;
     }
     HIPCHK(hipStreamSynchronize(s));
@@ -899,7 +902,7 @@ asets_status potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny
     HIPCHK(hipGetLastError());
     if (ev0) HIPCHK(hipEventRecord(ev0, s));
     int32_t itn = max_iter;
-    st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc, flags, &itn, iter);
+    st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc.host_flag, flags, &itn, iter);
     if (st) return st;
     if (ev1) HIPCHK(hipEventRecord(ev1, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -936,6 +939,14 @@ std::string tune_key(const char *kind, int32_t nvol, int64_t vox, int es) {
     return buf;
 }
 
+// The tilings the batched solver runs: the three kernel tile widths, and z chunks of
+// at least kBatchMinZchunk planes (its work list and partials are sized for that).
+constexpr int kBatchMinZchunk = 8;
+bool batched_cfg_ok(TileCfg c) {
+    return (c.tx == 8 || c.tx == 16 || c.tx == 32) && c.zchunk >= kBatchMinZchunk &&
+           c.zchunk <= (1 << 20);
+}
+
 bool tune_lookup(const std::string &key, TileCfg *out) {
     std::lock_guard<std::mutex> lk(g_tune.mu);
     const char *path = std::getenv("ASETS_TUNE_CACHE");
@@ -944,12 +955,15 @@ bool tune_lookup(const std::string &key, TileCfg *out) {
         if (FILE *f = std::fopen(path, "r")) {
             char k[256];
             int tx, zc;
-            while (std::fscanf(f, "%255s %d %d", k, &tx, &zc) == 3) g_tune.m[k] = TileCfg{tx, zc};
+            // a line a later build or a hand edit left with a tiling the batched kernel
+            // does not have (tile widths 8, 16, 32; chunks of 8 planes or more) is dropped
+            while (std::fscanf(f, "%255s %d %d", k, &tx, &zc) == 3)
+                if (batched_cfg_ok(TileCfg{tx, zc})) g_tune.m[k] = TileCfg{tx, zc};
             std::fclose(f);
         }
     }
     auto it = g_tune.m.find(key);
-    if (it == g_tune.m.end()) return false;
+    if (it == g_tune.m.end() || !batched_cfg_ok(it->second)) return false;
     *out = it->second;
     return true;
 }
@@ -1086,10 +1100,18 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
     const int32_t max_iter = p->max_iter;
     const int fast_cc = (std::fabs(p->cc) >= 0x1p-16f && std::fabs(p->cc) <= 0x1p16f) ? 1 : 0;
 
-    // work-list capacity: the finest tiling the tuner may pick (tx 8, zchunk 8)
+    // work-list capacity: the finest tiling any source may pick (tx 8, zchunk 8); an
+    // explicit tiling outside what batched_cfg_ok allows is rejected here, and
+    // set_tiling checks the count it builds against the capacity
+    if (o && o->tile_x > 0 && o->tile_x != 8 && o->tile_x != 16 && o->tile_x != 32)
+        return fail(ASETS_EBADARG, "tile_x must be 8, 16 or 32 (got %d)", o->tile_x);
+    if (o && o->zchunk > 0 && o->zchunk < kBatchMinZchunk)
+        return fail(ASETS_EBADARG, "the batched solver needs zchunk >= %d (got %d)",
+                    kBatchMinZchunk, o->zchunk);
     int64_t max_items = 0;
     for (int v = 0; v < nvol; ++v)
-        max_items += cdiv(vols[v].ny, 64) * cdiv(vols[v].nx, 8) * cdiv(vols[v].nz, 8);
+        max_items += cdiv(vols[v].ny, 64) * cdiv(vols[v].nx, 8) *
+                     cdiv(vols[v].nz, kBatchMinZchunk);
     size_t need = Carver::sz(1, sizeof(BatchCtl)) + Carver::sz(nvol, sizeof(BatchVol<T>)) +
                   Carver::sz(nvol, sizeof(BatchMeta)) + 2 * Carver::sz(nvol, 4) +
                   Carver::sz(max_items, sizeof(int2)) + Carver::sz(max_items, sizeof(double)) +
@@ -1167,6 +1189,9 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
             hm[v].nblocks = b.nblocks;
             for (int lb = 0; lb < b.nblocks; ++lb) w.push_back(make_int2(v, lb));
         }
+        if ((int64_t)w.size() > max_items)
+            return fail(ASETS_EBADARG, "tiling (%d, %d) exceeds the work-list capacity", c.tx,
+                        c.zchunk);
         nwork = (int)w.size();
         HIPCHK(hipMemcpyAsync(d_work, w.data(), w.size() * sizeof(int2), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(d_vols, hv.data(), nvol * sizeof(BatchVol<T>), hipMemcpyHostToDevice, s));
@@ -1185,7 +1210,7 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
     TileCfg cfg{16, 32};
     const bool fixed = o && o->tile_x > 0;
     if (fixed) {
-        cfg.tx = (o->tile_x == 8 || o->tile_x == 32) ? o->tile_x : 16;
+        cfg.tx = o->tile_x;
         cfg.zchunk = o->zchunk > 0 ? o->zchunk : 32;
     } else if (max_iter > 0) {
         const std::string key = tune_key("batched3d", nvol, total, (int)sizeof(T));
@@ -1551,7 +1576,7 @@ asets_status ishikawa_dev(int ndim, const float *Ct, const float *alpha, int64_t
     };
     if (ev0) HIPCHK(hipEventRecord(ev0, s));
     int32_t itn = max_iter;
-    st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc, flags, &itn, iter);
+    st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc.host_flag, flags, &itn, iter);
     if (st) return st;
     if (ev1) HIPCHK(hipEventRecord(ev1, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -1959,6 +1984,77 @@ asets_status asets_selftest_division(int mode, float d, uint64_t *mismatches) {
     return ASETS_OK;
 }
 
+asets_status asets_hbm_stream_dev(int mode, void *buf, int64_t bytes, int32_t flags,
+                                  int32_t grid, int32_t iters, double *ms, int64_t *traffic,
+                                  const asets_opts *o) {
+    if (!buf || !ms || !traffic) return fail(ASETS_EBADARG, "null argument");
+    if (mode < kStreamRead || mode > kStreamMix) return fail(ASETS_EBADARG, "mode 0, 1 or 2");
+    if (((uintptr_t)buf & 255) != 0) return fail(ASETS_EBADARG, "buf must be 256-byte aligned");
+    if (iters < 1 || grid < 0) return fail(ASETS_EBADARG, "iters >= 1 and grid >= 0");
+    const bool nt = (flags & 1) != 0, unroll1 = (flags & 2) != 0;
+    int dev = 0, cus = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int g = grid > 0 ? grid : 8 * cus;
+    hipStream_t s = o ? (hipStream_t)o->stream : nullptr;
+    char *b = (char *)buf;
+    std::function<void()> launch;
+    if (mode == kStreamRead) {
+        const int64_t sink = (int64_t)g * 256 * 4;
+        const int64_t n16 = (bytes - sink) / 16;
+        if (n16 < 1) return fail(ASETS_EBADARG, "buffer too small");
+        *traffic = n16 * 16;
+        const f32x4 *src = (const f32x4 *)b;
+        float *snk = (float *)(b + n16 * 16);
+        launch = [=]() {
+            if (nt && unroll1) k_stream_read<1, true><<<g, 256, 0, s>>>(src, n16, snk);
+            else if (nt) k_stream_read<4, true><<<g, 256, 0, s>>>(src, n16, snk);
+            else if (unroll1) k_stream_read<1, false><<<g, 256, 0, s>>>(src, n16, snk);
+            else k_stream_read<4, false><<<g, 256, 0, s>>>(src, n16, snk);
+        };
+    } else if (mode == kStreamCopy) {
+        const int64_t n16 = (bytes / 2) / 256 * 16;   // halves 256-byte aligned
+        if (n16 < 1) return fail(ASETS_EBADARG, "buffer too small");
+        *traffic = 2 * n16 * 16;
+        const f32x4 *src = (const f32x4 *)b;
+        f32x4 *dst = (f32x4 *)(b + n16 * 16);
+        launch = [=]() {
+            if (nt && unroll1) k_stream_copy<1, true><<<g, 256, 0, s>>>(src, dst, n16);
+            else if (nt) k_stream_copy<4, true><<<g, 256, 0, s>>>(src, dst, n16);
+            else if (unroll1) k_stream_copy<1, false><<<g, 256, 0, s>>>(src, dst, n16);
+            else k_stream_copy<4, false><<<g, 256, 0, s>>>(src, dst, n16);
+        };
+    } else {
+        const int64_t N = bytes / 60 / 1024 * 1024;
+        if (N < 1024) return fail(ASETS_EBADARG, "buffer too small");
+        *traffic = 60 * N;
+        MixFields f;
+        char *q = b;
+        for (int k = 0; k < 5; ++k, q += 4 * N) f.in4[k] = (const f32x4 *)q;
+        for (int k = 0; k < 2; ++k, q += 8 * N) f.in8[k] = (const f32x4 *)q;
+        for (int k = 0; k < 2; ++k, q += 4 * N) f.out4[k] = (f32x4 *)q;
+        for (int k = 0; k < 2; ++k, q += 8 * N) f.out8[k] = (f32x4 *)q;
+        const int64_t n16 = N / 4;
+        launch = [=]() {
+            if (nt) k_stream_mix<true><<<g, 256, 0, s>>>(f, n16);
+            else k_stream_mix<false><<<g, 256, 0, s>>>(f, n16);
+        };
+    }
+    EventPair ev;
+    HIPCHK(hipEventCreate(&ev.a));
+    HIPCHK(hipEventCreate(&ev.b));
+    launch();   // untimed
+    HIPCHK(hipEventRecord(ev.a, s));
+    for (int32_t i = 0; i < iters; ++i) launch();
+    HIPCHK(hipEventRecord(ev.b, s));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventSynchronize(ev.b));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, ev.a, ev.b));
+    *ms = (double)t / iters;
+    return ASETS_OK;
+}
+
 asets_status asets_cost_l1_dev(const float *img, int64_t n, const float *models, int32_t nlab,
                                float *Ct, const asets_opts *o) {
     return cost_l1(img, n, models, nlab, Ct, o);
@@ -1971,7 +2067,7 @@ asets_status asets_cost_smooth_labels_dev(const int32_t *labels, int64_t ny, int
 }
 
 asets_status asets_hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz, asets_hmf_node *nodes,
-                           int32_t nnodes, int32_t num_iter, float steps, float cc,
+                           int32_t nnodes, int32_t num_iter, float steps, double cc,
                            double *seconds, const asets_opts *o) {
     return hmf_dev(ndim, ny, nx, nz, nodes, nnodes, num_iter, steps, cc, seconds, o);
 }

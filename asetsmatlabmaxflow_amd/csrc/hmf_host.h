@@ -180,9 +180,12 @@ void launch_hmf_passes(const HmfPass &p, int nflow, bool shape, hipStream_t s) {
 }
 
 asets_status hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz, asets_hmf_node *in,
-                     int32_t M, int32_t num_iter, float steps, float cc, double *seconds,
+                     int32_t M, int32_t num_iter, float steps, double cc_d, double *seconds,
                      const asets_opts *o) {
     auto t_start = std::chrono::steady_clock::now();
+    // MATLAB's cc is a double: fp32 operations use (float)cc; the root's 1/cc .*
+    // ones(h.D) (asetsHMF3D.m:320) is a double array rounded where it meets fp32 data
+    const float cc = (float)cc_d;
     if (ndim != 2 && ndim != 3) return fail(ASETS_EBADARG, "ndim must be 2 or 3");
     if (ndim == 2 && nz != 1) return fail(ASETS_EBADARG, "2D solve needs nz == 1");
     if (!in) return fail(ASETS_EBADARG, "nodes is NULL");
@@ -273,7 +276,7 @@ asets_status hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz, asets_hmf_nod
     HIPCHK(hipGetLastError());
 
     const int fast_cc = (std::fabs(cc) >= 0x1p-16f && std::fabs(cc) <= 0x1p16f) ? 1 : 0;
-    const float one_cc = 1.0f / cc;
+    const float one_cc = (float)(1.0 / cc_d);
     const size_t sink_lds = (size_t)pl.levels * 4 * kBlock * sizeof(float);
     const unsigned nbv = (unsigned)cdiv(N, kBlock);
 

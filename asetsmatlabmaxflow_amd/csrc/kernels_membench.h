@@ -1,0 +1,114 @@
+// kernels_membench.h -- HBM ceiling kernels: what this chip gives a plain stream,
+// the yardstick for the solver kernels' roofline fractions.  No reference
+// counterpart.
+//
+// Every kernel is a grid-stride loop over 16-byte (dwordx4) elements with UNROLL
+// independent elements in flight per lane, optionally with the non-temporal
+// (streaming) cache policy.  Modes:
+//   kStreamRead  read n16 elements, fold them into one value per lane (written
+//                once, so the loads are not dead)
+//   kStreamCopy  dst[i] = src[i] for n16 elements (1:1 read:write)
+//   kStreamMix   the binary solver's per-voxel traffic over N voxels (N a multiple
+//                of 4): read Cs, Ct, alpha, u, bz (4 B per voxel) and the (ps, pt),
+//                (bx, by) pairs (8 B), write u, bz and the two pairs: 36 B read and
+//                24 B written per voxel, as the fused iteration moves them, without
+//                its stencil.  One element = 4 voxels.
+#pragma once
+#include "common.h"
+
+namespace asets {
+
+enum StreamMode { kStreamRead = 0, kStreamCopy = 1, kStreamMix = 2 };
+
+// 16-byte element (a clang vector: the non-temporal builtins take those)
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct MixFields {
+    const f32x4 *in4[5];    // Cs, Ct, alpha, u, bz       [N/4]
+    const f32x4 *in8[2];    // (ps, pt), (bx, by) pairs    [N/2]
+    f32x4 *out4[2];         // u, bz                       [N/4]
+    f32x4 *out8[2];         // (ps, pt), (bx, by)          [N/2]
+};
+
+template <bool NT>
+__device__ __forceinline__ f32x4 ld16(const f32x4 *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT>
+__device__ __forceinline__ void st16(f32x4 *p, f32x4 v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <int UNROLL, bool NT>
+__global__ __launch_bounds__(256) void k_stream_read(const f32x4 *src, int64_t n16,
+                                                     float *sink) {
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float acc = 0.0f;
+    for (; i + (UNROLL - 1) * st < n16; i += UNROLL * st) {
+        f32x4 v[UNROLL];
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) v[k] = ld16<NT>(src + i + k * st);
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) acc += (v[k].x + v[k].y) + (v[k].z + v[k].w);
+    }
+    for (; i < n16; i += st) {
+        const f32x4 v = ld16<NT>(src + i);
+        acc += (v.x + v.y) + (v.z + v.w);
+    }
+    sink[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
+template <int UNROLL, bool NT>
+__global__ __launch_bounds__(256) void k_stream_copy(const f32x4 *src, f32x4 *dst,
+                                                     int64_t n16) {
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + (UNROLL - 1) * st < n16; i += UNROLL * st) {
+        f32x4 v[UNROLL];
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) v[k] = ld16<NT>(src + i + k * st);
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) st16<NT>(dst + i + k * st, v[k]);
+    }
+    for (; i < n16; i += st) st16<NT>(dst + i, ld16<NT>(src + i));
+}
+
+// n16 = N/4 elements of 4 voxels; the outputs are cheap functions of all nine
+// inputs, so every load is live.
+template <bool NT>
+__global__ __launch_bounds__(256) void k_stream_mix(MixFields f, int64_t n16) {
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += st) {
+        f32x4 a[5], p[2][2];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) a[k] = ld16<NT>(f.in4[k] + i);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            p[k][0] = ld16<NT>(f.in8[k] + 2 * i);
+            p[k][1] = ld16<NT>(f.in8[k] + 2 * i + 1);
+        }
+        f32x4 s;
+        s.x = (a[0].x + a[1].x) + a[2].x;
+        s.y = (a[0].y + a[1].y) + a[2].y;
+        s.z = (a[0].z + a[1].z) + a[2].z;
+        s.w = (a[0].w + a[1].w) + a[2].w;
+        f32x4 o0 = a[3], o1 = a[4];
+        o0.x += s.x; o0.y += s.y; o0.z += s.z; o0.w += s.w;
+        o1.x -= s.x; o1.y -= s.y; o1.z -= s.z; o1.w -= s.w;
+        st16<NT>(f.out4[0] + i, o0);
+        st16<NT>(f.out4[1] + i, o1);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            f32x4 q0 = p[k][0], q1 = p[k][1];
+            q0.x += s.x; q0.y += s.y; q0.z += s.z; q0.w += s.w;
+            q1.x += s.w; q1.y += s.z; q1.z += s.y; q1.w += s.x;
+            st16<NT>(f.out8[k] + 2 * i, q0);
+            st16<NT>(f.out8[k] + 2 * i + 1, q1);
+        }
+    }
+}
+
+}  // namespace asets

@@ -16,6 +16,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     size_t n;
     asets_params p;
     float *u, *cvg;
+    mxArray *cvg_tmp = NULL;
     int32_t it_num = 0;
     double secs = 0.0;
     mwSize dims[2];
@@ -43,10 +44,10 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     }
     dims[0] = (mwSize)ny;
     dims[1] = (mwSize)nx;
-    u = amx_outputs(plhs, 2, dims, p.max_iter, &cvg);
+    u = amx_outputs(nlhs, plhs, 2, dims, p.max_iter, &cvg, &cvg_tmp);
     amx_check(asets_binary_warm(2, (const float *)mxGetData(prhs[0]),
                                 (const float *)mxGetData(prhs[1]),
                                 (const float *)mxGetData(prhs[2]), ny, nx, 1, &p, u0, pt0, u,
                                 cvg, &it_num, &secs, NULL));
-    amx_finish(nlhs, plhs, it_num, secs, "binary max flow 2D");
+    amx_finish(nlhs, plhs, it_num, secs, "binary max flow 2D", cvg_tmp);
 }

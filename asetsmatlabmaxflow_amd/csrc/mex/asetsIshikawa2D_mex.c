@@ -16,6 +16,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     size_t n, na;
     asets_params p;
     float *u, *cvg;
+    mxArray *cvg_tmp = NULL;
     int32_t it_num = 0;
     double secs = 0.0;
     mwSize dims[4];
@@ -41,9 +42,9 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     dims[0] = (mwSize)ny;
     dims[1] = (mwSize)nx;
     dims[2] = (mwSize)(nl - 1);
-    u = amx_outputs(plhs, 3, dims, p.max_iter, &cvg);
+    u = amx_outputs(nlhs, plhs, 3, dims, p.max_iter, &cvg, &cvg_tmp);
     amx_check(asets_ishikawa2d((const float *)mxGetData(prhs[0]),
                                (const float *)mxGetData(prhs[1]), ny, nx, nl, &p, u, cvg,
                                &it_num, &secs, NULL));
-    amx_finish(nlhs, plhs, it_num, secs, "ishikawa model max flow 2D");
+    amx_finish(nlhs, plhs, it_num, secs, "ishikawa model max flow 2D", cvg_tmp);
 }

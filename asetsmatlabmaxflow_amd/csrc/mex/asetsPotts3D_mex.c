@@ -14,6 +14,7 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     size_t n;
     asets_params p;
     float *u, *cvg;
+    mxArray *cvg_tmp = NULL;
     int32_t it_num = 0;
     double secs = 0.0;
     mwSize dims[4];
@@ -36,8 +37,8 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     dims[1] = (mwSize)nx;
     dims[2] = (mwSize)nz;
     dims[3] = (mwSize)nl;
-    u = amx_outputs(plhs, 4, dims, p.max_iter, &cvg);
+    u = amx_outputs(nlhs, plhs, 4, dims, p.max_iter, &cvg, &cvg_tmp);
     amx_check(asets_potts3d((const float *)mxGetData(prhs[0]), (const float *)mxGetData(prhs[1]),
                             ny, nx, nz, nl, &p, u, cvg, &it_num, &secs, NULL));
-    amx_finish(nlhs, plhs, it_num, secs, "potts model max flow 3D");
+    amx_finish(nlhs, plhs, it_num, secs, "potts model max flow 3D", cvg_tmp);
 }

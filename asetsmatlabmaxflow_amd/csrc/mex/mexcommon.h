@@ -57,24 +57,36 @@ static int64_t amx_dim(float v, const char *what) {
 }
 
 /* Outputs u and erriter (asetsBinaryMF3D_mex.c:91-104); numIter and time are
- * created by amx_finish once the solve has returned. */
-static float *amx_outputs(mxArray *plhs[], mwSize ndim, const mwSize *udims, int32_t max_iter,
-                          float **cvg) {
+ * created by amx_finish once the solve has returned.  plhs has max(nlhs, 1)
+ * slots: with nlhs < 2 erriter is not returned, so the solver writes it into a
+ * temporary array (*tmp, destroyed by amx_finish; MATLAB also frees it if the
+ * call ends in an error). */
+static float *amx_outputs(int nlhs, mxArray *plhs[], mwSize ndim, const mwSize *udims,
+                          int32_t max_iter, float **cvg, mxArray **tmp) {
     mwSize d2[2];
+    mxArray *c;
     plhs[0] = mxCreateNumericArray(ndim, udims, mxSINGLE_CLASS, mxREAL);
     d2[0] = (mwSize)max_iter;
     d2[1] = 1;
-    plhs[1] = mxCreateNumericArray(2, d2, mxSINGLE_CLASS, mxREAL);
-    if (!plhs[0] || !plhs[1]) mexErrMsgTxt("out of memory");
-    *cvg = (float *)mxGetData(plhs[1]);
+    c = mxCreateNumericArray(2, d2, mxSINGLE_CLASS, mxREAL);
+    if (!plhs[0] || !c) mexErrMsgTxt("out of memory");
+    if (nlhs >= 2) {
+        plhs[1] = c;
+        *tmp = NULL;
+    } else {
+        *tmp = c;
+    }
+    *cvg = (float *)mxGetData(c);
     return (float *)mxGetData(plhs[0]);
 }
 
 /* numIter (uint16) and time (single) outputs (asetsBinaryMF3D_mex.c:106-118)
- * and the reference's one-line report (:132). */
+ * and the reference's one-line report (:132); tmp: the erriter array
+ * amx_outputs made when it is not returned. */
 static void amx_finish(int nlhs, mxArray *plhs[], int32_t it_num, double secs,
-                       const char *label) {
+                       const char *label, mxArray *tmp) {
     mwSize d2[2] = {1, 1};
+    mxDestroyArray(tmp);
     if (nlhs > 2) {
         plhs[2] = mxCreateNumericArray(2, d2, mxUINT16_CLASS, mxREAL);
         if (!plhs[2]) mexErrMsgTxt("out of memory");

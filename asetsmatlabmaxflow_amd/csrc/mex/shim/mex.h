@@ -56,11 +56,15 @@ int mexPrintf(const char *fmt, ...);
 /* ---- harness (not part of MATLAB's API) ---- */
 typedef void (*mex_function_t)(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]);
 /* Calls fn; returns 0 on normal return, 1 if it raised mexErrMsgTxt (message
- * copied into err).  Outputs created before an error are freed. */
+ * copied into err).  plhs has max(nlhs, 1) slots.  As in MATLAB, arrays fn
+ * created and did not return in plhs are freed when it returns, and every array
+ * it created is freed when it raises an error. */
 int mexshim_call(mex_function_t fn, int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[],
                  char *err, size_t errlen);
 /* 0: mexPrintf writes to stdout (default); 1: mexPrintf output is dropped. */
 void mexshim_set_quiet(int quiet);
+/* Arrays created and not yet freed, all threads (a leak check for tests). */
+long mexshim_live_arrays(void);
 
 #ifdef __cplusplus
 }

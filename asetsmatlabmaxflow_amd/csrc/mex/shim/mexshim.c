@@ -19,13 +19,32 @@ struct mxArray_tag {
     void *data;
 };
 
-#define MAX_LIVE 16
+/* Arrays created inside mexshim_call are recorded, so that -- as MATLAB does --
+ * the ones the function does not return are freed when it ends, normally or
+ * through mexErrMsgTxt. */
+#define MAX_LIVE 64
 
 static __thread jmp_buf *t_jmp = NULL;
 static __thread char t_msg[1024];
 static __thread mxArray **t_outs = NULL;
 static __thread int t_nouts = 0;
+static __thread mxArray *t_live[MAX_LIVE];
+static __thread int t_nlive = 0;
 static int g_quiet = 0;
+static long g_arrays = 0;   /* arrays allocated and not yet freed (all threads) */
+
+static void live_add(mxArray *a) {
+    if (t_jmp && t_nlive < MAX_LIVE) t_live[t_nlive++] = a;
+}
+
+static void live_remove(mxArray *a) {
+    int i;
+    for (i = 0; i < t_nlive; ++i)
+        if (t_live[i] == a) {
+            t_live[i] = t_live[--t_nlive];
+            return;
+        }
+}
 
 static size_t elem_size(mxClassID c) {
     switch (c) {
@@ -62,6 +81,8 @@ mxArray *mxCreateNumericArray(mwSize ndim, const mwSize *dims, mxClassID cls, mx
         free(a);
         return NULL;
     }
+    live_add(a);
+    __sync_fetch_and_add(&g_arrays, 1);
     return a;
 }
 
@@ -74,9 +95,13 @@ mxArray *mxCreateNumericMatrix(mwSize m, mwSize n, mxClassID cls, mxComplexity c
 
 void mxDestroyArray(mxArray *a) {
     if (!a) return;
+    live_remove(a);
     free(a->data);
     free(a);
+    __sync_fetch_and_sub(&g_arrays, 1);
 }
+
+long mexshim_live_arrays(void) { return __sync_fetch_and_add(&g_arrays, 0); }
 
 void *mxGetData(const mxArray *a) { return a ? a->data : NULL; }
 mwSize mxGetNumberOfElements(const mxArray *a) { return a ? a->numel : 0; }
@@ -122,8 +147,11 @@ int mexshim_call(mex_function_t fn, int nlhs, mxArray *plhs[], int nrhs, const m
     jmp_buf *prev = t_jmp;
     mxArray **prev_outs = t_outs;
     int prev_n = t_nouts;
-    int i;
-    for (i = 0; i < nlhs; ++i) plhs[i] = NULL;
+    /* plhs has max(nlhs, 1) slots (MATLAB's ans) */
+    const int nslots = nlhs < 1 ? 1 : nlhs;
+    int i, j;
+    for (i = 0; i < nslots; ++i) plhs[i] = NULL;
+    t_nlive = 0;
     t_jmp = &jb;
     t_outs = plhs;
     t_nouts = nlhs;
@@ -132,15 +160,29 @@ int mexshim_call(mex_function_t fn, int nlhs, mxArray *plhs[], int nrhs, const m
         t_jmp = prev;
         t_outs = prev_outs;
         t_nouts = prev_n;
+        /* free the temporaries: every recorded array not returned in plhs */
+        for (i = 0; i < t_nlive; ++i) {
+            int returned = 0;
+            for (j = 0; j < nslots; ++j) returned |= (plhs[j] == t_live[i]);
+            if (!returned) {
+                free(t_live[i]->data);
+                free(t_live[i]);
+                __sync_fetch_and_sub(&g_arrays, 1);
+            }
+        }
+        t_nlive = 0;
         if (err && errlen) err[0] = '\0';
         return 0;
     }
-    /* error path: free what the function had created */
-    for (i = 0; i < nlhs; ++i) {
-        mxDestroyArray(plhs[i]);
-        plhs[i] = NULL;
-    }
+    /* error path: free everything the function had created */
     t_jmp = prev;
+    for (i = 0; i < t_nlive; ++i) {
+        free(t_live[i]->data);
+        free(t_live[i]);
+        __sync_fetch_and_sub(&g_arrays, 1);
+    }
+    t_nlive = 0;
+    for (i = 0; i < nslots; ++i) plhs[i] = NULL;
     t_outs = prev_outs;
     t_nouts = prev_n;
     if (err && errlen) snprintf(err, errlen, "%s", t_msg);

@@ -53,8 +53,15 @@ def _load_shim() -> C.CDLL:
         s.mexshim_call.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.c_int,
                                    C.POINTER(C.c_void_p), C.c_char_p, C.c_size_t]
         s.mexshim_set_quiet.argtypes = [C.c_int]
+        s.mexshim_live_arrays.restype = C.c_long
+        s.mexshim_live_arrays.argtypes = []
         _shim = s
     return _shim
+
+
+def live_arrays() -> int:
+    """mxArrays allocated through the shim and not yet freed (a leak check)."""
+    return int(_load_shim().mexshim_live_arrays())
 
 
 def set_quiet(quiet: bool) -> None:
@@ -125,6 +132,8 @@ def call(name: str, *args, nlhs: int = 4):
         for i in range(nlhs):
             outs.append(_from_mx(plhs[i]))
             s.mxDestroyArray(plhs[i])
+        if nlhs == 0 and plhs[0]:
+            s.mxDestroyArray(plhs[0])   # MATLAB's ans
         return outs
     finally:
         for m in made:

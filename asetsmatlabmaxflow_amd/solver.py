@@ -284,3 +284,20 @@ def energy_potts(Ct, alpha, u, shape, nlab: int, *, stream=None) -> np.ndarray:
         ndim, _ptr(Ct), _ptr(alpha), ny, nx, nz, nlab, _ptr(u),
         out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(opts)))
     return out
+
+
+def hbm_stream(mode: str, buf, *, nontemporal: bool = True, one_in_flight: bool = False,
+               grid: int = 0, iters: int = 10, stream=None) -> tuple[float, int]:
+    """Time a plain stream over the device tensor ``buf`` (asets_hbm_stream_dev):
+    mode "read", "copy" or "mix" (the binary solver's 36 B read + 24 B written per voxel,
+    without the stencil).  Returns (ms per launch, bytes read plus written per launch)."""
+    m = {"read": 0, "copy": 1, "mix": 2}[mode]
+    if not (buf.is_cuda and buf.is_contiguous()):
+        raise ValueError("expected a contiguous CUDA tensor")
+    ms, traffic = C.c_double(), C.c_int64()
+    opts = nat.make_opts(_stream_handle(stream), nat.FLOW_F32, nat.KERNEL_AUTO, 0, 0, 0, None)
+    flags = (1 if nontemporal else 0) | (2 if one_in_flight else 0)
+    nat.check(nat.lib().asets_hbm_stream_dev(
+        m, buf.data_ptr(), buf.numel() * buf.element_size(), flags, grid, iters,
+        C.byref(ms), C.byref(traffic), C.byref(opts)))
+    return ms.value, traffic.value

@@ -21,6 +21,8 @@ Other workloads (one JSON line each, same keys):
             tests/runTestHMFLinearModel.m, seed 1005), 200 fixed iterations
   hmf3d     192^3 hierarchical max-flow, root -> 2 branches -> 3 leaves (seed 1006), 100
             fixed iterations
+  hbm       the HBM ceiling: plain read, copy and solver-traffic-mix streams with
+            16-byte accesses over about 1 GB (no solver; the yardstick of the rooflines)
 
 Algorithmic bytes per voxel-iteration: binary 3D 60 (fp32 flows) / 40 (fp16 flows),
 binary 2D 52, Potts 3D 48L+8, Ishikawa 3D 12L+36(L-1) (read Ct, alpha; read and write u,
@@ -537,6 +539,48 @@ def bench_batched(args):
     return out
 
 
+def bench_hbm(args):
+    """HBM ceiling: plain 16-byte-per-lane streams (read, copy, and the binary solver's
+    traffic mix without its stencil) over about 1 GB, each swept over grid size, cache
+    policy and loads in flight; the best of each is the ceiling that kernel shape gets."""
+    import torch
+
+    from asetsmatlabmaxflow_amd import solver
+
+    dev = torch.device("cuda:0")
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    N = 256 ** 3
+    buf = torch.ones(60 * N // 4, dtype=torch.float32, device=dev)   # 1006.6 MB
+    torch.cuda.synchronize()
+    rows = []
+    for mode in ("read", "copy", "mix"):
+        for nt in (True, False):
+            for one in ((False, True) if mode != "mix" else (False,)):
+                for per_cu in (2, 4, 8, 16):
+                    ms, traffic = solver.hbm_stream(mode, buf, nontemporal=nt,
+                                                    one_in_flight=one, grid=per_cu * cus,
+                                                    iters=args.steps)
+                    rows.append({"mode": mode, "nontemporal": nt,
+                                 "in_flight_per_lane": 1 if one else 4,
+                                 "workgroups_per_cu": per_cu, "ms": ms, "bytes": traffic,
+                                 "GBps": traffic / (ms * 1e-3) / 1e9})
+    best = {m: max((r for r in rows if r["mode"] == m), key=lambda r: r["GBps"])
+            for m in ("read", "copy", "mix")}
+    mix = best["mix"]
+    out = _base(args, "HBM GB/s ceiling: plain streams with 16-byte accesses (read, copy, "
+                      "binary-solver traffic mix)", mix["GBps"], "GB/s", mix["ms"], "f32",
+                "synthetic (ones)",
+                {"workload": "hbm streams over 1006.6 MB (mix: 256^3 voxels x 60 B)",
+                 "cus": cus})
+    out["roofline"] = {"bound": "hbm", "achieved": mix["GBps"], "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": mix["GBps"] / HBM_PEAK_GBS, "traffic": mix["bytes"],
+                       "traffic_source": "bytes the stream moves by construction"}
+    out["best"] = best
+    out["sweep"] = rows
+    return out
+
+
+def main():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -544,7 +588,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="binary3d",
                     choices=["binary3d", "binary2d", "potts3d", "batched", "ishikawa3d", "hmf2d",
-                             "hmf3d"])
+                             "hmf3d", "hbm"])
     ap.add_argument("--iters", type=int, default=200)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--nvol", type=int, default=64)
@@ -566,7 +610,7 @@ def main():
         sys.exit(2)
     fn = {"binary3d": bench_binary3d, "binary2d": bench_binary2d, "potts3d": bench_potts3d,
           "batched": bench_batched, "ishikawa3d": bench_ishikawa3d, "hmf2d": bench_hmf2d,
-          "hmf3d": bench_hmf3d}[args.workload]
+          "hmf3d": bench_hmf3d, "hbm": bench_hbm}[args.workload]
     out = fn(args)
     if args.copy_calibration > 0:
         import torch

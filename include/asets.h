@@ -300,7 +300,10 @@ asets_status asets_cost_smooth_labels_dev(const int32_t *labels, int64_t ny, int
  * along dim 1 (px), faces[1] along dim 2 (py), faces[2] along dim 3 (pz), each
  * voxel holding the face to its lower neighbour, so px(i,j,k) = faces[0][g] at
  * row i (0-based) and the index-0 faces are 0.  Output pointers may be NULL.
- * ndim == 2 needs nz == 1.  At most 4096 nodes and 16 levels of internal nodes. */
+ * ndim == 2 needs nz == 1.  At most 4096 nodes and 16 levels of internal nodes.
+ * cc is a double, as MATLAB's: the fp32 updates use (float)cc, while the root's
+ * sink term 1/cc .* ones(h.D) (asetsHMF3D.m:320) is evaluated in double and
+ * rounded to fp32 where it meets the children's sink flows. */
 typedef struct asets_hmf_node {
     int32_t parent;          /* parent entry, -1 for the root                              */
     float alpha;             /* scalar alpha, used when alpha_map is NULL                  */
@@ -316,7 +319,7 @@ typedef struct asets_hmf_node {
 
 asets_status asets_hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz,
                            asets_hmf_node *nodes, int32_t nnodes, int32_t num_iter,
-                           float steps, float cc, double *seconds, const asets_opts *o);
+                           float steps, double cc, double *seconds, const asets_opts *o);
 
 /* ---- misc ---------------------------------------------------------------- */
 /* Self-check of the short correctly-rounded division form the kernels use for
@@ -325,6 +328,22 @@ asets_status asets_hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz,
  * (|d| in [2^-16, 2^16]); mode 1: 1/x for every x in that range.  Returns the
  * number of bitwise mismatches in *mismatches (0 expected). */
 asets_status asets_selftest_division(int mode, float d, uint64_t *mismatches);
+
+/* HBM ceiling (measurement only; no reference counterpart): times a plain stream
+ * over the caller's device buffer buf of `bytes` bytes (256-byte aligned) with
+ * 16-byte accesses per lane.  mode 0 reads the buffer (less a small sink the lanes
+ * write their folded value to); mode 1 copies its first half to its second half;
+ * mode 2 moves the binary solver's per-voxel traffic over the N voxels the buffer
+ * holds (N = bytes/60 rounded down to a multiple of 1024): read Cs, Ct, alpha, u, bz
+ * (4 B) and the (ps, pt), (bx, by) pairs (8 B), write u, bz and the two pairs --
+ * 36 B read and 24 B written per voxel, without the stencil.  flags bit 0:
+ * non-temporal cache policy; bit 1: one element per lane in flight instead of four
+ * (modes 0, 1).  grid: workgroups of 256 threads, 0 = 8 per CU.  One untimed launch,
+ * then `iters` launches timed with HIP events on the stream of o: *ms = mean time
+ * per launch, *traffic = bytes read plus written per launch. */
+asets_status asets_hbm_stream_dev(int mode, void *buf, int64_t bytes, int32_t flags,
+                                  int32_t grid, int32_t iters, double *ms, int64_t *traffic,
+                                  const asets_opts *o);
 
 const char *asets_last_error(void);
 const char *asets_version(void);

@@ -188,7 +188,7 @@ EXPORT int orc_energy_potts(const double *Ct, const double *alpha, int64_t ny, i
 EXPORT int orc_hmf_f32(int64_t ny, int64_t nx, int64_t nz, int is3d, int nnodes,
                        const int32_t *parent, const float *Ct, const float *alpha,
                        const float *l_in, const int32_t *has_l, int num_it, float steps,
-                       float cc, float *u, float *pt, float *div, float *f, float *l_out,
+                       double cc, float *u, float *pt, float *div, float *f, float *l_out,
                        double *data_cost, double *regu_cost)
 {
     return hmf_f32(ny, nx, nz, is3d, nnodes, parent, Ct, alpha, l_in, has_l, num_it, steps,

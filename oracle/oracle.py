@@ -52,8 +52,8 @@ def lib():
             f.restype = C.c_int
         for name, real in (("orc_hmf_f32", C.c_float), ("orc_hmf_f64", C.c_double)):
             f = getattr(L, name)
-            f.argtypes = [I64, I64, I64, C.c_int, C.c_int, P, P, P, P, P, C.c_int, real, real,
-                          P, P, P, P, P, P, P]
+            f.argtypes = [I64, I64, I64, C.c_int, C.c_int, P, P, P, P, P, C.c_int, real,
+                          C.c_double, P, P, P, P, P, P, P]
             f.restype = C.c_int
         L.orc_energy_binary.argtypes = [P, P, P, I64, I64, I64, C.c_int, P, P]
         L.orc_energy_binary.restype = C.c_int

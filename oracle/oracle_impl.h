@@ -528,14 +528,17 @@ static int SFX(ishikawa)(const REAL *Ct, const REAL *alpha,
  *   .^2 is x*x, .^0.5 is the correctly rounded square root;
  *   min(Ct, x) is (Ct < x) ? Ct : x;  max(x, 0) is (x > 0) ? x : +0;
  *   the masks (px>0 & l>0).*l.*a select one product or +0;
- *   1/cc of the root is (REAL)1 / cc; the 0.001 of the normalisation is (REAL)0.001;
+ *   cc is MATLAB's double: every operation with REAL arrays uses (REAL)cc, but the
+ *   root's 1/cc .* ones(h.D) (asetsHMF3D.m:320, asetsHMF2D.m likewise) is a double
+ *   array, rounded to REAL only where it meets the children's pt: (REAL)(1.0/cc);
+ *   the 0.001 of the normalisation is (REAL)0.001;
  *   the costs are sums, in double, of the REAL products u*Ct and u*div.
  */
 #define HMF_MAXN 4096
 
 static void SFX(hmf_sink)(int n, const int32_t *parent, const int32_t *first_child,
-                          const int32_t *next_sib, int64_t N, REAL cc, const REAL *Ct,
-                          REAL *u, REAL *pt, REAL *div)
+                          const int32_t *next_sib, int64_t N, REAL cc, REAL root_cc,
+                          const REAL *Ct, REAL *u, REAL *pt, REAL *div)
 {
     /* children first (bottom-up), then this node (asetsHMF3D.m:307-336) */
     int c, nc = 0;
@@ -543,7 +546,7 @@ static void SFX(hmf_sink)(int n, const int32_t *parent, const int32_t *first_chi
     REAL *ptn = pt + (int64_t)n * N;
     const REAL *dvn = div + (int64_t)n * N, *un = u + (int64_t)n * N;
     for (c = first_child[n]; c >= 0; c = next_sib[c]) {
-        SFX(hmf_sink)(c, parent, first_child, next_sib, N, cc, Ct, u, pt, div);
+        SFX(hmf_sink)(c, parent, first_child, next_sib, N, cc, root_cc, Ct, u, pt, div);
         ++nc;
     }
     if (nc == 0) {
@@ -556,7 +559,7 @@ static void SFX(hmf_sink)(int n, const int32_t *parent, const int32_t *first_chi
         return;
     }
     for (g = 0; g < N; ++g) {
-        REAL acc = (parent[n] < 0) ? (REAL)1 / cc
+        REAL acc = (parent[n] < 0) ? root_cc
                                    : (pt[(int64_t)parent[n] * N + g] - dvn[g]) + un[g] / cc;
         for (c = first_child[n]; c >= 0; c = next_sib[c]) {
             const int64_t i = (int64_t)c * N + g;
@@ -568,13 +571,14 @@ static void SFX(hmf_sink)(int n, const int32_t *parent, const int32_t *first_chi
 
 static int SFX(hmf)(int64_t ny, int64_t nx, int64_t nz, int is3d, int nnodes,
                     const int32_t *parent, const REAL *Ct, const REAL *alpha,
-                    const REAL *l_in, const int32_t *has_l, int num_it, REAL steps, REAL cc,
+                    const REAL *l_in, const int32_t *has_l, int num_it, REAL steps, double cc_d,
                     REAL *u, REAL *pt, REAL *div, REAL *f, REAL *l_out,
                     double *data_cost, double *regu_cost)
 {
     const int64_t N = nx * ny * nz;
     const int64_t st[3] = {1, ny, nx * ny};
     const int nd = is3d ? 3 : 2;
+    const REAL cc = (REAL)cc_d, root_cc = (REAL)(1.0 / cc_d);
     int32_t first_child[HMF_MAXN], next_sib[HMF_MAXN], last[HMF_MAXN], order[HMF_MAXN];
     int n, k, it, root = -1, norder = 0;
     int64_t g;
@@ -634,7 +638,7 @@ static int SFX(hmf)(int64_t ny, int64_t nx, int64_t nz, int is3d, int nnodes,
     }
 
     for (it = 0; it < num_it; ++it) {
-        SFX(hmf_sink)(root, parent, first_child, next_sib, N, cc, Ct, u, pt, div);
+        SFX(hmf_sink)(root, parent, first_child, next_sib, N, cc, root_cc, Ct, u, pt, div);
 
         /* labels (top-down; order-free): u = u + cc*(P.pt - div - pt) */
         for (n = 0; n < nnodes; ++n) {

@@ -130,6 +130,25 @@ def test_mex_argument_validation():
         mex.asetsBinaryMF3D_mex(f, f, f, pars, u0=f[:2], pt0=f)
 
 
+def test_mex_temporaries_freed():
+    """With nlhs below 2 erriter is a temporary of the mex function, and an error after
+    the outputs exist (here: no device, so the solve fails) frees them: no mxArray is
+    left behind either way, as MATLAB frees a MEX function's temporaries."""
+    mex.set_quiet(True)
+    f = np.ones((4, 3, 2), np.float32)
+    pars = np.array([4, 3, 2, 3, 0.0, 0.25, 0.11], np.float32)
+    base = mex.live_arrays()
+    for nlhs in (0, 1, 2, 4):
+        try:
+            mex.call("asetsBinaryMF3D_mex", f, f, f, pars, nlhs=nlhs)
+        except mex.MexError:
+            pass
+        assert mex.live_arrays() == base, f"nlhs={nlhs}"
+    with pytest.raises(mex.MexError):
+        mex.call("asetsBinaryMF3D_mex", f, f, f, pars[:3], nlhs=1)
+    assert mex.live_arrays() == base
+
+
 def test_mx_roundtrip():
     a = np.asfortranarray(np.arange(24, dtype=np.float32).reshape(4, 3, 2))
     m = mex._to_mx(a)

@@ -93,6 +93,20 @@ def test_hmf_matches_oracle_bitwise(cuda, shape, parent, amap, kernel, its):
     assert np.abs(ref.faces).max() > 1e-3   # flows moved
 
 
+@pytest.mark.parametrize("shape,kernel", [((30, 22, 7), "fused"), ((33, 25), "passes")])
+def test_hmf_root_term_of_a_double_cc(cuda, shape, kernel):
+    """cc = 0.12: the root's 1/cc .* ones(h.D) (asetsHMF3D.m:320) is MATLAB's double
+    quotient rounded to single, one ulp below 1/single(cc); both kernel families match the
+    oracle bit for bit with it."""
+    from asetsmatlabmaxflow_amd.hmf import solve_tree
+    print(f"hmf cc=0.12 case shape={shape} kernel={kernel}", flush=True)
+    assert np.float32(1.0 / 0.12) != np.float32(1) / np.float32(0.12)
+    Ct, alpha, _ = _problem(shape, NESTED, seed=5, alpha_map=False)
+    res = solve_tree(NESTED, Ct, alpha, shape, 12, 0.12, 0.12, kernel=kernel)
+    ref = O.hmf(NESTED, Ct, alpha, shape, 12, 0.12, 0.12)
+    _check(res, ref, shape, len(NESTED))
+
+
 @pytest.mark.parametrize("shape", [(40, 30), (24, 20, 9)])
 def test_hmf_shape_constraint_matches_oracle_bitwise(cuda, shape):
     from asetsmatlabmaxflow_amd.hmf import _matlab_view, solve_tree

@@ -175,6 +175,21 @@ def test_mex_binary3d(cuda):
     assert _bits_equal(u, ref.u)
 
 
+def test_mex_fewer_outputs(cuda):
+    """[u] = asetsBinaryMF3D_mex(...) and a call with no output: erriter is a temporary,
+    u is the same, and no mxArray is left behind."""
+    mex.set_quiet(True)
+    shape = (21, 17, 9)
+    Cs, Ct, a = workloads.random_smooth_binary(shape, seed=21, alpha=0.1)
+    pars = np.array([21, 17, 9, 35, 0.0, 0.25, 0.11], dtype=np.float32)
+    base = mex.live_arrays()
+    (u,) = mex.call("asetsBinaryMF3D_mex", Cs, Ct, a, pars, nlhs=1)
+    assert mex.call("asetsBinaryMF3D_mex", Cs, Ct, a, pars, nlhs=0) == []
+    ref = orc.binary(Cs, Ct, a, shape, 35, 0.0, 0.25, 0.11)
+    assert _bits_equal(u, ref.u)
+    assert mex.live_arrays() == base
+
+
 def test_mex_binary2d_config1(cuda):
     """256x256 ellipse image with the pars of the 2D workload, 300 fixed iterations."""
     mex.set_quiet(True)
@@ -441,6 +456,44 @@ def test_alpha_error_every_solve(cuda):
     assert _bits_equal(r.u.cpu().numpy(), ref.u)
     rb, _, _ = solver.binary_batched([tuple(g) + (shape,)], 10, 0.0, 0.25, 0.11)
     assert rb[0].it_num == 10 and _bits_equal(rb[0].u.cpu().numpy(), ref.u)
+
+
+def test_potts_config3_workload_vs_oracle(cuda):
+    """The 192^3 Potts workload's generator (L = 6 smoothed Voronoi labels, alpha 0.8,
+    cc 0.25, steps 0.11, 300 fixed iterations) at 64x56x48: u of the fused and the
+    per-pass kernels equals the oracle's bit for bit (asetsPotts3D_mex.c:144-217,
+    474-521), and cvg agrees with the oracle's FP64 sum."""
+    shape, L, its = (64, 56, 48), 6, 300
+    print(f"potts workload vs oracle {shape} L={L} {its} iterations", flush=True)
+    Ct, a, _ = workloads.potts3d_voronoi(shape, nlab=L, seed=1003)
+    cc, st = workloads.POTTS3D_CC, workloads.POTTS3D_STEPS
+    ref = orc.potts(Ct, a, shape, L, its, 0.0, cc, st)
+    d = [solver.to_device(x) for x in (Ct, a)]
+    for kern in ("fused", "passes"):
+        r = solver.potts(*d, shape, L, its, 0.0, cc, st, kernel=kern)
+        assert r.it_num == ref.it_num == its
+        u = r.u.cpu().numpy()
+        assert _bits_equal(u, ref.u), (kern, _report(u, ref.u))
+        np.testing.assert_allclose(r.cvg.cpu().numpy(), ref.cvg64, rtol=1e-6, atol=0)
+
+
+def test_potts_config3_full_size_properties(cuda):
+    """The 192^3, L = 6 Potts workload at full size, 300 fixed iterations: the labels sum
+    to 1 at every voxel to round-off (the coupling of asetsPotts3D_mex.c:495-514 keeps
+    sum_l u_l = 1 in exact arithmetic; the oracle's fp32 trajectory stays within 1.2e-6
+    of it at 48x40x32), and the fused and per-pass kernels agree bit for bit."""
+    shape, L, its = (192, 192, 192), 6, 300
+    print(f"potts workload full size {shape} L={L}", flush=True)
+    Ct, a, _ = workloads.potts3d_voronoi(shape, nlab=L, seed=1003)
+    cc, st = workloads.POTTS3D_CC, workloads.POTTS3D_STEPS
+    d = [solver.to_device(x) for x in (Ct, a)]
+    del Ct, a
+    rf = solver.potts(*d, shape, L, its, 0.0, cc, st, kernel="fused")
+    s = rf.u.view(L, -1).double().sum(0)
+    assert float((s - 1).abs().max()) < 1e-5
+    rp = solver.potts(*d, shape, L, its, 0.0, cc, st, kernel="passes")
+    assert torch.equal(rf.u.view(torch.int32), rp.u.view(torch.int32))
+    assert torch.equal(rf.cvg, rp.cvg)
 
 
 def test_potts_voronoi_fused_vs_passes_and_stop(cuda):

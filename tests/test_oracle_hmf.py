@@ -37,6 +37,9 @@ def hmf_arrays(parent, Ct, alpha, shape, num_it, steps, cc, l=None):
     D = tuple(shape)
     nd = len(D)
     M = len(parent)
+    # MATLAB's steps and cc are doubles; with single arrays they act as single, except
+    # the root's 1/cc .* ones(h.D), a double array rounded where it meets single data
+    root_cc = F32(1.0 / float(cc))
     steps, cc = F32(steps), F32(cc)
     kids = [[c for c in range(M) if parent[c] == n] for n in range(M)]
     root = list(parent).index(-1)
@@ -65,7 +68,7 @@ def hmf_arrays(parent, Ct, alpha, shape, num_it, steps, cc, l=None):
         if not kids[n]:
             pt[n] = np.minimum(ct[n], (pt[parent[n]] - div[n]) + u[n] / cc)
             return
-        acc = np.full(D, F32(1) / cc) if n == root else (pt[parent[n]] - div[n]) + u[n] / cc
+        acc = np.full(D, root_cc) if n == root else (pt[parent[n]] - div[n]) + u[n] / cc
         for c in kids[n]:
             acc = ((acc + pt[c]) + div[c]) - u[c] / cc
         pt[n] = acc / F32(len(kids[n]) if n == root else 1 + len(kids[n]))
@@ -310,6 +313,26 @@ def test_axis_direction_exempts_the_flows_along_it(sign):
 # ---------------------------------------------------------------------------
 # independent whole-array restatement, fp32, bit for bit
 # ---------------------------------------------------------------------------
+def test_root_sink_term_is_the_double_quotient_rounded():
+    """asetsHMF3D.m:320 sets the root's sink flow to 1/cc .* ones(h.D) in double; it is
+    rounded to single where it meets the children's single pt.  For cc = 0.12 that value,
+    single(1/0.12) = 8.333333, is one ulp below the single quotient 1/single(0.12) =
+    8.333334.  One iteration from zero with constant costs keeps every spatial flow and
+    div at 0, so the root's pt is (single(1/cc) + sum of the leaves' pt) / 2, rounded in
+    that order, and the leaves' pt = min(Ct, 0)."""
+    cc = 0.12
+    assert F32(1.0 / cc) != F32(1) / F32(cc)
+    shape = (5, 4)
+    c1, c2 = F32(-0.25), F32(-0.5)
+    r = O.hmf([-1, 0, 0], [None, np.full(shape, c1), np.full(shape, c2)], [0, 1.0, 1.0],
+              shape, 1, 0.1, cc)
+    want = ((F32(1.0 / cc) + c1) + c2) / F32(2)
+    np.testing.assert_array_equal(r.pt[1], np.full_like(r.pt[1], c1))
+    np.testing.assert_array_equal(r.pt[0], np.full_like(r.pt[0], want))
+    wrong = ((F32(1) / F32(cc) + c1) + c2) / F32(2)
+    assert want != wrong
+
+
 @pytest.mark.parametrize("shape,shape_constraint", [((13, 11), False), ((13, 11), True),
                                                     ((9, 8, 7), False), ((9, 8, 7), True)])
 def test_array_restatement_bitwise(shape, shape_constraint):
