@@ -50,6 +50,12 @@ def _oracle():
     return orc
 
 
+def _progress(msg):
+    """One line on stderr as each phase of a run ends (the JSON line stays alone on
+    stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def _host_cpu():
     """(model name, logical CPUs visible to this process)."""
     model = "unknown"
@@ -114,6 +120,7 @@ def bench_binary3d(args):
     t0 = time.time()
     Cs, Ct, alpha, _ = workloads.binary3d_ellipsoids(shape, seed=1002)
     gen_s = time.time() - t0
+    _progress(f"volume generated in {gen_s:.1f} s")
     dev = torch.device("cuda:0")
     dCs, dCt, dA = (solver.to_device(x, dev) for x in (Cs, Ct, alpha))
     cc, steps = workloads.BINARY3D_CC, workloads.BINARY3D_STEPS
@@ -137,6 +144,7 @@ def bench_binary3d(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     k_ms = float(np.mean(kms))
+    _progress(f"{args.steps} timed steps: {wall:.3f} s, {k_ms:.4f} ms per iteration")
     # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs of
     # this kernel and workload (scaled by a known-size copy), committed under profiles/
     traffic = None
@@ -178,6 +186,7 @@ def bench_binary3d(args):
             torch.cuda.synchronize()
             runs.append(time.perf_counter() - t1)
         ttt = sorted(runs[1:])[2]
+        _progress(f"time to tolerance: {ttt * 1e3:.2f} ms")
         lab = (rt.u > 0.5)
         e = solver.energy_binary(dCs, dCt, dA, rt.u, shape)
         out["time_to_tol"] = {"err_bound": 1e-4, "seconds": ttt, "first_call_seconds": runs[0],
@@ -227,6 +236,7 @@ def bench_binary3d(args):
             "label_agreement_cold": float(((rt.u > 0.5) == lab_ref).float().mean().item()),
             "weights": "asetsmatlabmaxflow_amd/weights/warmstart_unet.npz"}
 
+    _progress("GPU legs done")
     if not args.no_cpu:
         orc = _oracle()
         k = max(1, args.cpu_iters)
@@ -580,7 +590,6 @@ def bench_hbm(args):
     return out
 
 
-def main():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
