@@ -20,6 +20,7 @@ EXPORTED = (
     "asets_binary2d", "asets_binary3d", "asets_binary_warm", "asets_potts2d", "asets_potts3d",
     "asets_ishikawa2d", "asets_ishikawa3d", "asets_ishikawa_dev",
     "asets_binary_dev", "asets_binary_dev_state", "asets_binary_dev_resume",
+    "asets_binary_dev_noalloc",
     "asets_potts_dev",
     "asets_binary3d_batched_dev", "asets_binary_energy_dev", "asets_potts_energy_dev",
     "asets_cost_loglik_dev", "asets_cost_neglog_dev", "asets_alpha_contrast_dev",
@@ -86,6 +87,8 @@ def lib() -> C.CDLL:
         L.asets_potts3d.argtypes = [P, P, I64, I64, I64, I64, PP, P, P, P, PD, PO]
         L.asets_binary_dev.argtypes = [C.c_int, P, P, P, I64, I64, I64, PP, P, P, P, P, P, PD,
                                        PO]
+        L.asets_binary_dev_noalloc.argtypes = [C.c_int, P, P, P, I64, I64, I64, PP, P, P, P,
+                                               P, P, PD, PO]
         L.asets_binary_dev_state.argtypes = [C.c_int, P, P, P, I64, I64, I64, PP, P, P, P, P, P,
                                              PD, P, P, P, P, P, PO]
         L.asets_binary_dev_resume.argtypes = [C.c_int, P, P, P, I64, I64, I64, PP, P, P, P, P,

@@ -595,11 +595,23 @@ struct WarmIn {
     const float *b0[3] = {nullptr, nullptr, nullptr};
 };
 
+// A workspace the caller fixes instead of the library's cached one (the
+// allocation-free entry): a device region, its size, the mutex that serialises
+// its users, and host memory for the stop-flag polls.
+struct FixedWs {
+    void *dev;
+    size_t bytes;
+    std::mutex *mu;
+    int32_t *hflag;
+};
+
+// fw == nullptr: the cached per-device workspace (grown with hipMalloc as needed).
 asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float *alpha,
                         int64_t ny, int64_t nx, int64_t nz, const asets_params *p,
                         const WarmIn &w, float *u, float *cvg,
                         int32_t *it_num, double *seconds, float *o_ps, float *o_pt,
-                        float *o_bx, float *o_by, float *o_bz, const asets_opts *o) {
+                        float *o_bx, float *o_by, float *o_bz, const asets_opts *o,
+                        const FixedWs *fw = nullptr) {
     auto t_start = std::chrono::steady_clock::now();
     if (ndim != 2 && ndim != 3) return fail(ASETS_EBADARG, "ndim must be 2 or 3");
     if (ndim == 2 && nz != 1) return fail(ASETS_EBADARG, "2D solve needs nz == 1");
@@ -632,10 +644,14 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     if (dev < 0 || dev >= kMaxDevices) return fail(ASETS_EDEVICE, "device id %d", dev);
+    if (fw && o && o->kernel_ms)
+        return fail(ASETS_EBADARG, "kernel timing needs HIP events: not available without "
+                                   "allocation");
     DeviceCache &dc = g_cache[dev];
-    std::lock_guard<std::mutex> lock(dc.mu);
+    std::lock_guard<std::mutex> lock(fw ? *fw->mu : dc.mu);
     hipStream_t s = o ? (hipStream_t)o->stream : nullptr;
-    if (!dc.host_flag) HIPCHK(hipHostMalloc((void **)&dc.host_flag, 64));
+    if (!fw && !dc.host_flag) HIPCHK(hipHostMalloc((void **)&dc.host_flag, 64));
+    int32_t *const hflag = fw ? fw->hflag : dc.host_flag;
 
     const bool is3d = (ndim == 3);
     const int64_t N = ny * nx * nz;
@@ -656,9 +672,15 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
         need += fused_bytes(N, es);
     else
         need += 7 * Carver::sz(N, sizeof(float));
-    st = ensure(&dc.dev, &dc.dev_bytes, need);
-    if (st) return st;
-    Carver cv(dc.dev);
+    if (fw) {
+        if (need > fw->bytes)
+            return fail(ASETS_ETOOBIG, "the solve needs %zu bytes of workspace; the "
+                                       "allocation-free entry has %zu", need, fw->bytes);
+    } else {
+        st = ensure(&dc.dev, &dc.dev_bytes, need);
+        if (st) return st;
+    }
+    Carver cv(fw ? fw->dev : dc.dev);
     Flags *flags = cv.take<Flags>(1);
     double *partials = cv.take<double>(2 * nparts);
 
@@ -712,7 +734,7 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
                                           it_num, flags);
             return ASETS_OK;
         };
-        st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, dc.host_flag, flags, &itn, iter);
+        st = run_loop(max_iter, p->err_bound, o ? o->poll_every : 0, s, hflag, flags, &itn, iter);
         if (st) return st;
         if (ev1) HIPCHK(hipEventRecord(ev1, s));
         if (o_ps) HIPCHK(hipMemcpyAsync(o_ps, ps, N * 4, hipMemcpyDeviceToDevice, s));
@@ -726,8 +748,8 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
                     o_ps, o_pt, o_bx, o_by, o_bz, tc, ntile_y, ntile_x, nzc, nblocks,
                     grid_fused, nparts, partials, flags, s, ev0, ev1,
                     o ? o->poll_every : 0, &itn};
-        st = (flow16 == ASETS_FLOW_F16) ? run_fused<__half>(fr, cv, dc.host_flag)
-                                          : run_fused<float>(fr, cv, dc.host_flag);
+        st = (flow16 == ASETS_FLOW_F16) ? run_fused<__half>(fr, cv, hflag)
+                                          : run_fused<float>(fr, cv, hflag);
         if (st) return st;
     }
     HIPCHK(hipStreamSynchronize(s));
@@ -1796,6 +1818,43 @@ asets_status cost_smooth_labels(const int32_t *lab, int64_t ny, int64_t nx, cons
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// the allocation-free binary solve (asets_binary_dev_noalloc)
+// ---------------------------------------------------------------------------
+// Its whole device workspace (two state buffers, partial sums, flags) is carved from
+// this array: each device the library runs on holds its own copy until the process
+// ends.  In host code `scratch` names the host-side stand-in, not device memory: the
+// device address is taken once per device with hipGetSymbolAddress (that device
+// current) and only that pointer is used.
+static __device__ __align__(256) unsigned char scratch[ASETS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+struct ScratchDev {
+    std::mutex init_mu;   // guards ptr
+    std::mutex mu;        // serialises the solves that use the array
+    void *ptr = nullptr;
+};
+ScratchDev g_scratch[kMaxDevices];
+
+// Device address of this device's scratch (current device = dev).
+asets_status scratch_address(int dev, void **out) {
+    ScratchDev &sd = g_scratch[dev];
+    std::lock_guard<std::mutex> lk(sd.init_mu);
+    if (!sd.ptr) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return fail(ASETS_EDEVICE, "hipGetSymbolAddress(scratch) failed: %s",
+                        hipGetErrorString(e));
+        }
+        sd.ptr = p;
+    }
+    *out = sd.ptr;
+    return ASETS_OK;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // extern "C" entry points
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -1863,6 +1922,26 @@ asets_status asets_binary_dev(int ndim, const float *Cs, const float *Ct, const 
     w.pt0 = pt0;
     return binary_dev(ndim, Cs, Ct, alpha, ny, nx, nz, p, w, u, cvg, it_num, seconds, nullptr,
                       nullptr, nullptr, nullptr, nullptr, o);
+}
+
+asets_status asets_binary_dev_noalloc(int ndim, const float *Cs, const float *Ct,
+                                      const float *alpha, int64_t ny, int64_t nx, int64_t nz,
+                                      const asets_params *p, const float *u0, const float *pt0,
+                                      float *u, float *cvg, int32_t *it_num, double *seconds,
+                                      const asets_opts *o) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(ASETS_EDEVICE, "device id %d", dev);
+    void *ws = nullptr;
+    asets_status st = scratch_address(dev, &ws);
+    if (st) return st;
+    int32_t hflag[4] = {0, 0, 0, 0};   // stop-flag polls read into pageable host memory
+    const FixedWs fw{ws, (size_t)ASETS_NOALLOC_SCRATCH_BYTES, &g_scratch[dev].mu, hflag};
+    WarmIn w;
+    w.u0 = u0;
+    w.pt0 = pt0;
+    return binary_dev(ndim, Cs, Ct, alpha, ny, nx, nz, p, w, u, cvg, it_num, seconds, nullptr,
+                      nullptr, nullptr, nullptr, nullptr, o, &fw);
 }
 
 asets_status asets_binary_dev_state(int ndim, const float *Cs, const float *Ct,

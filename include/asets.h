@@ -157,6 +157,31 @@ asets_status asets_binary_dev(int ndim, const float *Cs, const float *Ct, const 
                               float *u, float *cvg, int32_t *it_num, double *seconds,
                               const asets_opts *o);
 
+/* Workspace of asets_binary_dev_noalloc: a file-scope device array of this many
+ * bytes per device, held from the library's first use of a device until the
+ * process ends. */
+#define ASETS_NOALLOC_SCRATCH_BYTES 104857600
+
+/* asets_binary_dev without allocation.  No counterpart in the reference; it is
+ * for callers that cannot allocate: code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc.  Same arguments and results as
+ * asets_binary_dev (u bit for bit), but the solve's device workspace (two
+ * state buffers, per-workgroup partial sums, flags) is carved from the
+ * library's ASETS_NOALLOC_SCRATCH_BYTES array, and the stop flag is polled
+ * into host stack memory: no hipMalloc, hipHostMalloc or hipEventCreate.  A
+ * solve needing more workspace returns ASETS_ETOOBIG before any device work:
+ * the fused fp32 solver needs about 48 bytes per voxel (128^3 fits), fp16
+ * flows 28, the per-pass kernels 28.  o->kernel_ms must be NULL.  The array
+ * keeps the last call's contents; every call initialises what it reads.  All
+ * calls on one device share it: the library serialises them (each returns
+ * after its work on the stream is complete), so calls from several host
+ * threads wait for each other instead of racing. */
+asets_status asets_binary_dev_noalloc(int ndim, const float *Cs, const float *Ct,
+                                      const float *alpha, int64_t ny, int64_t nx, int64_t nz,
+                                      const asets_params *p, const float *u0, const float *pt0,
+                                      float *u, float *cvg, int32_t *it_num, double *seconds,
+                                      const asets_opts *o);
+
 /* Same as asets_binary_dev, additionally returning the final flows
  * (device pointers, N floats each, any may be NULL): ps, pt and the three
  * face fields in own-face convention (bx[g] is the face between x-1 and x). */
