@@ -2067,7 +2067,8 @@ asets_status asets_hbm_stream_dev(int mode, void *buf, int64_t bytes, int32_t fl
                                   int32_t grid, int32_t iters, double *ms, int64_t *traffic,
                                   const asets_opts *o) {
     if (!buf || !ms || !traffic) return fail(ASETS_EBADARG, "null argument");
-    if (mode < kStreamRead || mode > kStreamMix) return fail(ASETS_EBADARG, "mode 0, 1 or 2");
+    if (mode < kStreamRead || mode > kStreamMixPacked)
+        return fail(ASETS_EBADARG, "mode 0, 1, 2 or 3");
     if (((uintptr_t)buf & 255) != 0) return fail(ASETS_EBADARG, "buf must be 256-byte aligned");
     if (iters < 1 || grid < 0) return fail(ASETS_EBADARG, "iters >= 1 and grid >= 0");
     const bool nt = (flags & 1) != 0, unroll1 = (flags & 2) != 0;
@@ -2102,6 +2103,18 @@ asets_status asets_hbm_stream_dev(int mode, void *buf, int64_t bytes, int32_t fl
             else if (nt) k_stream_copy<4, true><<<g, 256, 0, s>>>(src, dst, n16);
             else if (unroll1) k_stream_copy<1, false><<<g, 256, 0, s>>>(src, dst, n16);
             else k_stream_copy<4, false><<<g, 256, 0, s>>>(src, dst, n16);
+        };
+    } else if (mode == kStreamMixPacked) {
+        const int64_t N = bytes / 60 / 1024 * 1024;
+        if (N < 1024) return fail(ASETS_EBADARG, "buffer too small");
+        *traffic = 60 * N;
+        const f32x4 *in = (const f32x4 *)b;
+        const f32x4 *sin = (const f32x4 *)(b + 12 * N);
+        f32x4 *sout = (f32x4 *)(b + 36 * N);
+        const int64_t n16 = N / 4;
+        launch = [=]() {
+            if (nt) k_stream_mix_packed<true><<<g, 256, 0, s>>>(in, sin, sout, n16);
+            else k_stream_mix_packed<false><<<g, 256, 0, s>>>(in, sin, sout, n16);
         };
     } else {
         const int64_t N = bytes / 60 / 1024 * 1024;

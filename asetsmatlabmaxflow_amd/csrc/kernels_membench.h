@@ -13,12 +13,17 @@
 //                (bx, by) pairs (8 B), write u, bz and the two pairs: 36 B read and
 //                24 B written per voxel, as the fused iteration moves them, without
 //                its stencil.  One element = 4 voxels.
+//   kStreamMixPacked  the same 36 B read and 24 B written per voxel through three
+//                arrays instead of eleven: the inputs (12 B per voxel), the state read
+//                (24 B) and the state written (24 B), each interleaved by blocks --
+//                for every 64 elements (one wave), field k is a contiguous 1 KiB
+//                block -- so each load and store is one coalesced 1 KiB wave access.
 #pragma once
 #include "common.h"
 
 namespace asets {
 
-enum StreamMode { kStreamRead = 0, kStreamCopy = 1, kStreamMix = 2 };
+enum StreamMode { kStreamRead = 0, kStreamCopy = 1, kStreamMix = 2, kStreamMixPacked = 3 };
 
 // 16-byte element (a clang vector: the non-temporal builtins take those)
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -108,6 +113,25 @@ __global__ __launch_bounds__(256) void k_stream_mix(MixFields f, int64_t n16) {
             st16<NT>(f.out8[k] + 2 * i, q0);
             st16<NT>(f.out8[k] + 2 * i + 1, q1);
         }
+    }
+}
+
+// n16 = N/4 elements of 4 voxels (a multiple of 64): 3 input, 6 state-in and 6
+// state-out f32x4 each.
+template <bool NT>
+__global__ __launch_bounds__(256) void k_stream_mix_packed(const f32x4 *in, const f32x4 *sin,
+                                                           f32x4 *sout, int64_t n16) {
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += st) {
+        const int64_t blk = i >> 6, lane = i & 63;
+        f32x4 a[3], q[6];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) a[k] = ld16<NT>(in + (blk * 3 + k) * 64 + lane);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) q[k] = ld16<NT>(sin + (blk * 6 + k) * 64 + lane);
+        const f32x4 s = (a[0] + a[1]) + a[2];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) st16<NT>(sout + (blk * 6 + k) * 64 + lane, q[k] + s);
     }
 }
 

@@ -563,9 +563,9 @@ def bench_hbm(args):
     buf = torch.ones(60 * N // 4, dtype=torch.float32, device=dev)   # 1006.6 MB
     torch.cuda.synchronize()
     rows = []
-    for mode in ("read", "copy", "mix"):
+    for mode in ("read", "copy", "mix", "mix_packed"):
         for nt in (True, False):
-            for one in ((False, True) if mode != "mix" else (False,)):
+            for one in ((False, True) if mode in ("read", "copy") else (False,)):
                 for per_cu in (2, 4, 8, 16):
                     ms, traffic = solver.hbm_stream(mode, buf, nontemporal=nt,
                                                     one_in_flight=one, grid=per_cu * cus,
@@ -575,7 +575,7 @@ def bench_hbm(args):
                                  "workgroups_per_cu": per_cu, "ms": ms, "bytes": traffic,
                                  "GBps": traffic / (ms * 1e-3) / 1e9})
     best = {m: max((r for r in rows if r["mode"] == m), key=lambda r: r["GBps"])
-            for m in ("read", "copy", "mix")}
+            for m in ("read", "copy", "mix", "mix_packed")}
     mix = best["mix"]
     out = _base(args, "HBM GB/s ceiling: plain streams with 16-byte accesses (read, copy, "
                       "binary-solver traffic mix)", mix["GBps"], "GB/s", mix["ms"], "f32",

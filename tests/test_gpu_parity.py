@@ -342,6 +342,38 @@ def test_batched_matches_single(cuda, flow, tmp_path, monkeypatch):
     assert len((tmp_path / "tune.txt").read_text().splitlines()) == 1
 
 
+@pytest.mark.parametrize("flow", ["f32", "f16"])
+def test_batched_workload_shapes_to_1e4(cuda, flow, tmp_path, monkeypatch):
+    """The batched workload's first 8 volumes (its generator and seeds: axes uniform in
+    {64..192}, volume v from seed 1004*1000+v) in one batched call to errBound 1e-4: each
+    equals its own single-volume solve bit for bit, stop iteration included; against the
+    FP64 oracle run for the same iteration count on the two smallest volumes, max |u -
+    u_f64| stays below 1e-5 with fp32 flows (measured 7.8e-7) and 5e-3 with fp16 flows
+    (measured 1.4e-3), with at least 99.99 % of the labels u > 0.5 equal."""
+    monkeypatch.setenv("ASETS_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    rng = np.random.default_rng(1004)
+    sizes = [tuple(int(v) for v in rng.integers(64, 193, 3)) for _ in range(8)]
+    print(f"batched workload shapes {sizes} flow={flow}", flush=True)
+    host = []
+    for v, sh in enumerate(sizes):
+        Cs, Ct, a, _ = workloads.binary3d_ellipsoids(sh, seed=1004 * 1000 + v)
+        host.append((Cs, Ct, a, sh))
+    dev = [tuple(solver.to_device(x) for x in h[:3]) + (h[3],) for h in host]
+    cc, st = workloads.BINARY3D_CC, workloads.BINARY3D_STEPS
+    res, _, _ = solver.binary_batched(dev, 2000, 1e-4, cc, st, flow=flow)
+    for (Cs, Ct, a, sh), r in zip(dev, res):
+        single = solver.binary(Cs, Ct, a, sh, 2000, 1e-4, cc, st, kernel="fused", flow=flow)
+        assert r.it_num == single.it_num < 2000, sh
+        assert torch.equal(r.u.view(torch.int32), single.u.view(torch.int32)), sh
+    bound = 1e-5 if flow == "f32" else 5e-3
+    for v in sorted(range(len(sizes)), key=lambda i: int(np.prod(sizes[i])))[:2]:
+        Cs, Ct, a, sh = host[v]
+        r64 = orc.binary(Cs, Ct, a, sh, res[v].iterations, 0.0, cc, st, precision="f64")
+        u = res[v].u.cpu().numpy().astype(np.float64)
+        assert np.abs(u - r64.u).max() < bound, sh
+        assert np.mean((u > 0.5) == (r64.u > 0.5)) >= 0.9999, sh
+
+
 def test_batched_vs_oracle(cuda):
     """Fixed iteration count: batched u equals the CPU oracle bit for bit."""
     host = _batch_inputs()[:3]
