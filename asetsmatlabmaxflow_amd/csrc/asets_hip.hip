@@ -140,6 +140,19 @@ inline bool fused_fits(int64_t ny, int64_t nx, int64_t nz) {
     return ny * nx * (nz + 2) * 4 < ((int64_t)1 << 31);
 }
 
+// Bytes per plane of the block-interleaved binary state (BlkLayout) for flow
+// element size es.
+inline int64_t blk_plane_bytes(int64_t ny, int64_t nx, size_t es) {
+    return nx * cdiv(ny, 64) * (256 + 320 * (int64_t)es);
+}
+// The block-interleaved binary kernel addresses one state buffer with 32-bit byte
+// offsets, planes -3 to nz+2 (out-of-range planes must fall past the buffer): the
+// buffer plus four planes must stay below 2^31 bytes (about 430^3 voxels with fp32
+// flows).  Larger volumes run the planar fused kernel.
+inline bool fused_fits_blk(int64_t ny, int64_t nx, int64_t nz, size_t es) {
+    return (nz + 4) * blk_plane_bytes(ny, nx, es) < ((int64_t)1 << 31);
+}
+
 // alpha <= 0 anywhere is an error when at least one iteration runs
 // (asetsBinaryMF3D_mex.c:317-319 raises it inside the first projection pass).
 __global__ void k_alpha_check(const float *alpha, int64_t n, int32_t *bad, int32_t *done) {
@@ -257,27 +270,32 @@ TileCfg pick_tiles(int64_t ny, int64_t nx, int64_t nz, const asets_opts *o) {
 // Fused kernel instances: (tile width TX, threads per workgroup).  Wider tiles
 // use more threads so each thread keeps few positions in registers.
 template <bool IS3D, int TX, int NT, typename T>
-void launch_fused_t(const FusedArgs<T> &a, int grid, hipStream_t s, bool first) {
+void launch_fused_t(const FusedArgs<T> &a, int grid, hipStream_t s, bool first, bool blk) {
     // occupancy target: 4 waves per SIMD (<= 128 VGPRs) for the 512- and
     // 1024-thread tiles, 3 for the 256-thread tile (LDS allows 3 per CU)
     constexpr int W = (NT >= 512) ? 4 : 3;
-    if (first) kf_binary<IS3D, TX, NT, W, T, true><<<grid, NT, 0, s>>>(a);
-    else kf_binary<IS3D, TX, NT, W, T><<<grid, NT, 0, s>>>(a);
+    if (blk) {
+        if (first) kf_binary<IS3D, TX, NT, W, T, true, true><<<grid, NT, 0, s>>>(a);
+        else kf_binary<IS3D, TX, NT, W, T, false, true><<<grid, NT, 0, s>>>(a);
+    } else {
+        if (first) kf_binary<IS3D, TX, NT, W, T, true, false><<<grid, NT, 0, s>>>(a);
+        else kf_binary<IS3D, TX, NT, W, T, false, false><<<grid, NT, 0, s>>>(a);
+    }
 }
 
 // first: the cold-start first iteration (the initial cut formed from Cs and Ct,
-// alpha checked into a.alpha_bad / a.alpha_stop)
+// alpha checked into a.alpha_bad / a.alpha_stop); blk: block-interleaved state
 template <typename T>
 void launch_fused(bool is3d, int tx, const FusedArgs<T> &a, int grid, hipStream_t s,
-                  bool first = false) {
+                  bool first, bool blk) {
     if (is3d) {
-        if (tx == 8) launch_fused_t<true, 8, 256, T>(a, grid, s, first);
-        else if (tx == 32) launch_fused_t<true, 32, 1024, T>(a, grid, s, first);
-        else launch_fused_t<true, 16, 512, T>(a, grid, s, first);
+        if (tx == 8) launch_fused_t<true, 8, 256, T>(a, grid, s, first, blk);
+        else if (tx == 32) launch_fused_t<true, 32, 1024, T>(a, grid, s, first, blk);
+        else launch_fused_t<true, 16, 512, T>(a, grid, s, first, blk);
     } else {
-        if (tx == 8) launch_fused_t<false, 8, 256, T>(a, grid, s, first);
-        else if (tx == 32) launch_fused_t<false, 32, 1024, T>(a, grid, s, first);
-        else launch_fused_t<false, 16, 512, T>(a, grid, s, first);
+        if (tx == 8) launch_fused_t<false, 8, 256, T>(a, grid, s, first, blk);
+        else if (tx == 32) launch_fused_t<false, 32, 1024, T>(a, grid, s, first, blk);
+        else launch_fused_t<false, 16, 512, T>(a, grid, s, first, blk);
     }
 }
 
@@ -397,6 +415,79 @@ __global__ void k_to_f32(float *dst, const T *src, int64_t n, int stride) {
     if (i < n) dst[i] = Io<T>::get(src + stride * i);
 }
 
+// Initial block-interleaved binary state (BlkLayout), one thread per lane of a
+// block.  Cold start (u0 null): u = (Cs - Ct >= 0), ps = pt = min(Cs, Ct), flows 0
+// (asetsBinaryMF3D_mex.c:215-219).  Warm start or resume: u0, ps0 (null: pt0), pt0
+// and the flows b0 (null: 0), whose faces with axis index 0 -- the zero-flux
+// boundary, never updated -- are forced to 0.  Padding lanes y >= ny: all 0.  Flows
+// are rounded as the storage type stores them.
+template <typename T>
+__global__ void k_blk_init(const float *Cs, const float *Ct, const float *u0, const float *ps0,
+                           const float *pt0, const float *bx0, const float *by0,
+                           const float *bz0, char *st, int ny, int nx, int nz, int nyb) {
+    using BL = BlkLayout<T>;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nz * nx * nyb * 64) return;
+    const int64_t b = t >> 6;
+    const int ln = (int)(t & 63);
+    const int yb = (int)(b % nyb);
+    const int64_t xz = b / nyb;
+    const int x = (int)(xz % nx), z = (int)(xz / nx);
+    const int y = yb * 64 + ln;
+    float u = 0.0f, ps = 0.0f, pt = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
+    if (y < ny) {
+        const int64_t g = ((int64_t)z * nx + x) * ny + y;
+        if (u0) {
+            u = u0[g];
+            pt = pt0[g];
+            ps = ps0 ? ps0[g] : pt;
+        } else {
+            const float cs = Cs[g], ct = Ct[g];
+            u = (cs - ct >= 0.0f) ? 1.0f : 0.0f;
+            ps = pt = fmin_ref(cs, ct);
+        }
+        if (bx0 && x > 0) bx = bx0[g];
+        if (by0 && y > 0) by = by0[g];
+        if (bz0 && z > 0) bz = bz0[g];
+    }
+    char *blk = st + b * BL::kBytes;
+    ((float *)(blk + BL::kU))[ln] = u;
+    T *pp = (T *)(blk + BL::kPS) + 2 * ln;
+    T *bb = (T *)(blk + BL::kBB) + 2 * ln;
+    Io<T>::put(pp, ps);
+    Io<T>::put(pp + 1, pt);
+    Io<T>::put(bb, bx);
+    Io<T>::put(bb + 1, by);
+    Io<T>::put((T *)(blk + BL::kBZ) + ln, bz);
+}
+
+// u and, where requested, the flows of a block-interleaved state to planar fp32
+// arrays (own-face convention), one thread per lane of a block.
+template <typename T>
+__global__ void k_blk_take(const char *st, float *u, float *ps, float *pt, float *bx, float *by,
+                           float *bz, int ny, int nx, int nz, int nyb) {
+    using BL = BlkLayout<T>;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nz * nx * nyb * 64) return;
+    const int64_t b = t >> 6;
+    const int ln = (int)(t & 63);
+    const int yb = (int)(b % nyb);
+    const int64_t xz = b / nyb;
+    const int x = (int)(xz % nx), z = (int)(xz / nx);
+    const int y = yb * 64 + ln;
+    if (y >= ny) return;
+    const int64_t g = ((int64_t)z * nx + x) * ny + y;
+    const char *blk = st + b * BL::kBytes;
+    const T *pp = (const T *)(blk + BL::kPS) + 2 * ln;
+    const T *bb = (const T *)(blk + BL::kBB) + 2 * ln;
+    if (u) u[g] = ((const float *)(blk + BL::kU))[ln];
+    if (ps) ps[g] = Io<T>::get(pp);
+    if (pt) pt[g] = Io<T>::get(pp + 1);
+    if (bx) bx[g] = Io<T>::get(bb);
+    if (by) by[g] = Io<T>::get(bb + 1);
+    if (bz) bz[g] = Io<T>::get((const T *)(blk + BL::kBZ) + ln);
+}
+
 // Everything the fused solve loop needs (one volume).
 struct FusedRun {
     bool is3d;
@@ -417,6 +508,9 @@ struct FusedRun {
     hipEvent_t ev0, ev1;
     int32_t poll;
     int32_t *itn;   // host: iteration at which the stop test fired, or max_iter
+    bool blk;       // block-interleaved state (BlkLayout) instead of planar fields
+    int nyb;        // blocks per column (blk)
+    int pb;         // state bytes per plane (blk)
 };
 
 // Workspace bytes of a fused solve with flow element size es.
@@ -432,6 +526,12 @@ asets_status run_fused(const FusedRun &r, Carver &cv, int32_t *hflag) {
     // two state buffers; iteration i reads buf[i&1] and writes buf[(i+1)&1]
     FusedState<T> buf[2];
     for (int b = 0; b < 2; ++b) {
+        if (r.blk) {   // one block-interleaved array; every pointer is its base
+            char *base = cv.take<char>(r.nz * (int64_t)r.pb);
+            buf[b] = FusedState<T>{(float *)base, (T *)base, (T *)base, (T *)base, (T *)base,
+                                   (T *)base};
+            continue;
+        }
         buf[b].u = cv.take<float>(N);
         buf[b].ps = cv.take<T>(kPair * N);
         buf[b].pt = buf[b].ps + 1;
@@ -443,7 +543,13 @@ asets_status run_fused(const FusedRun &r, Carver &cv, int32_t *hflag) {
     // cold start: iteration 0 forms the initial state itself (and checks alpha);
     // with no iteration buf[0] must hold it for the outputs
     const bool first = !r.u0;
-    if (first && max_iter == 0) {
+    const unsigned nbb = (unsigned)cdiv(r.nz * r.nx * (int64_t)r.nyb * 64, kBlock);
+    if (r.blk) {
+        if (!first || max_iter == 0)
+            k_blk_init<T><<<nbb, kBlock, 0, s>>>(r.Cs, r.Ct, r.u0, r.ps0, r.pt0, r.b0[0],
+                                                 r.b0[1], r.b0[2], (char *)buf[0].u, (int)r.ny,
+                                                 (int)r.nx, (int)r.nz, r.nyb);
+    } else if (first && max_iter == 0) {
         k_init_binary_cold_t<T><<<nb, kBlock, 0, s>>>(r.Cs, r.Ct, buf[0].u, buf[0].ps,
                                                      buf[0].bx, buf[0].bz, N);
     } else if (!first) {
@@ -475,6 +581,8 @@ asets_status run_fused(const FusedRun &r, Carver &cv, int32_t *hflag) {
     fa.flags = r.flags;
     fa.alpha_bad = &r.flags->bad_alpha;
     fa.alpha_stop = &r.flags->done;
+    fa.nyb = r.nyb;
+    fa.pb = r.pb;
     // err_bound 0 (a fixed iteration count; the stop test fires only on an exact
     // fixed point): no launch has to wait for an iteration's stop test, so it runs
     // inside the next iteration's kernel (DeferredStop, same bits as k_finalize),
@@ -492,7 +600,7 @@ asets_status run_fused(const FusedRun &r, Carver &cv, int32_t *hflag) {
         if (deferred && it > 0)
             fa.stop = DeferredStop{P[(it - 1) & 1], r.nparts, (double)N, it - 1,
                                    r.p->err_bound, r.cvg, r.it_num, r.flags};
-        launch_fused<T>(r.is3d, r.tc.tx, fa, r.grid, s, first && it == 0);
+        launch_fused<T>(r.is3d, r.tc.tx, fa, r.grid, s, first && it == 0, r.blk);
         if (!deferred || it == max_iter - 1)
             k_finalize<<<1, 1024, 0, s>>>(fa.partials, r.nparts, (double)N, it, r.p->err_bound,
                                           r.cvg, r.it_num, r.flags);
@@ -506,6 +614,13 @@ asets_status run_fused(const FusedRun &r, Carver &cv, int32_t *hflag) {
     // final state lives in buf[n_exec & 1]
     const int32_t n_exec = (itn < max_iter) ? itn + 1 : max_iter;
     const FusedState<T> &fin = buf[n_exec & 1];
+    if (r.blk) {
+        k_blk_take<T><<<nbb, kBlock, 0, s>>>((const char *)fin.u, r.u, r.o_ps, r.o_pt, r.o_bx,
+                                             r.o_by, r.o_bz, (int)r.ny, (int)r.nx, (int)r.nz,
+                                             r.nyb);
+        HIPCHK(hipGetLastError());
+        return ASETS_OK;
+    }
     HIPCHK(hipMemcpyAsync(r.u, fin.u, N * 4, hipMemcpyDeviceToDevice, s));
     const T *srcs[5] = {fin.ps, fin.pt, fin.bx, fin.by, fin.bz};
     float *dsts[5] = {r.o_ps, r.o_pt, r.o_bx, r.o_by, r.o_bz};
@@ -668,7 +783,12 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
     const size_t es = (flow16 == ASETS_FLOW_F16) ? sizeof(__half) : sizeof(float);
     // fused: two partial buffers (DeferredStop)
     size_t need = Carver::sz(1, sizeof(Flags)) + Carver::sz(2 * nparts, sizeof(double));
-    if (fused)
+    // the block-interleaved state unless a buffer would pass its 2^31-byte limit
+    const bool blk = fused && fused_fits_blk(ny, nx, nz, es);
+    const int64_t pb = blk_plane_bytes(ny, nx, es);
+    if (blk)
+        need += 2 * Carver::sz(nz * pb, 1);
+    else if (fused)
         need += fused_bytes(N, es);
     else
         need += 7 * Carver::sz(N, sizeof(float));
@@ -747,7 +867,7 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
                     {w.b0[0], w.b0[1], w.b0[2]}, u, cvg, it_num,
                     o_ps, o_pt, o_bx, o_by, o_bz, tc, ntile_y, ntile_x, nzc, nblocks,
                     grid_fused, nparts, partials, flags, s, ev0, ev1,
-                    o ? o->poll_every : 0, &itn};
+                    o ? o->poll_every : 0, &itn, blk, (int)cdiv(ny, 64), (int)pb};
         st = (flow16 == ASETS_FLOW_F16) ? run_fused<__half>(fr, cv, hflag)
                                           : run_fused<float>(fr, cv, hflag);
         if (st) return st;
@@ -2067,8 +2187,8 @@ asets_status asets_hbm_stream_dev(int mode, void *buf, int64_t bytes, int32_t fl
                                   int32_t grid, int32_t iters, double *ms, int64_t *traffic,
                                   const asets_opts *o) {
     if (!buf || !ms || !traffic) return fail(ASETS_EBADARG, "null argument");
-    if (mode < kStreamRead || mode > kStreamMixPacked)
-        return fail(ASETS_EBADARG, "mode 0, 1, 2 or 3");
+    if (mode < kStreamRead || mode > kStreamMixState)
+        return fail(ASETS_EBADARG, "mode 0 to 4");
     if (((uintptr_t)buf & 255) != 0) return fail(ASETS_EBADARG, "buf must be 256-byte aligned");
     if (iters < 1 || grid < 0) return fail(ASETS_EBADARG, "iters >= 1 and grid >= 0");
     const bool nt = (flags & 1) != 0, unroll1 = (flags & 2) != 0;
@@ -2103,6 +2223,19 @@ asets_status asets_hbm_stream_dev(int mode, void *buf, int64_t bytes, int32_t fl
             else if (nt) k_stream_copy<4, true><<<g, 256, 0, s>>>(src, dst, n16);
             else if (unroll1) k_stream_copy<1, false><<<g, 256, 0, s>>>(src, dst, n16);
             else k_stream_copy<4, false><<<g, 256, 0, s>>>(src, dst, n16);
+        };
+    } else if (mode == kStreamMixState) {
+        const int64_t N = bytes / 60 / 1024 * 1024;
+        if (N < 1024) return fail(ASETS_EBADARG, "buffer too small");
+        *traffic = 60 * N;
+        const f32x4 *cs = (const f32x4 *)b, *ct = (const f32x4 *)(b + 4 * N);
+        const f32x4 *al = (const f32x4 *)(b + 8 * N);
+        const f32x4 *sin = (const f32x4 *)(b + 12 * N);
+        f32x4 *sout = (f32x4 *)(b + 36 * N);
+        const int64_t n16 = N / 4;
+        launch = [=]() {
+            if (nt) k_stream_mix_state<true><<<g, 256, 0, s>>>(cs, ct, al, sin, sout, n16);
+            else k_stream_mix_state<false><<<g, 256, 0, s>>>(cs, ct, al, sin, sout, n16);
         };
     } else if (mode == kStreamMixPacked) {
         const int64_t N = bytes / 60 / 1024 * 1024;

@@ -86,6 +86,9 @@ struct FusedArgs {
     // set, when not null, so that the following iterations return at once)
     int32_t *alpha_bad, *alpha_stop;
     DeferredStop stop;      // the previous iteration's stop test (kf_binary)
+    // block-interleaved binary state (BlkLayout; in.u / out.u are the buffers' bases)
+    int nyb;                // 64-row blocks per column: ceil(ny / 64)
+    int pb;                 // bytes per plane of the state: nx * nyb * block bytes
 };
 
 // ---- buffer access --------------------------------------------------------
@@ -227,6 +230,89 @@ __device__ __forceinline__ void st_pair(const PRef &p, int off, float lo, float 
     }
 }
 
+// ---- block-interleaved binary state ---------------------------------------
+// The fused binary solver (kf_binary) stores a state buffer as one array of
+// blocks: block (x, z, yb) holds the 64 voxels y = 64*yb .. 64*yb + 63 of column
+// x of plane z, field after field -- u (fp32), the (ps, pt) pairs, the (bx, by)
+// pairs, bz -- so a plane of the whole state is one contiguous run and an
+// iteration reads one state array and writes one (a plain stream of the
+// iteration's bytes through five arrays instead of eleven runs about 11 % faster
+// on MI355X).  Block index (z * nx + x) * nyb + yb, nyb = ceil(ny / 64); the lanes
+// y >= ny of a column's last block are padding whose faces are kept at 0.  A high
+// face past the volume reads 0 as in the planar layout: the y + 1 face of the last
+// row is the next column's y = 0 face (or a padding lane), the x + 1 face of the
+// last column the next plane's x = 0 face, a plane past the volume lies past the
+// buffer (bounds check).
+template <typename T>
+struct BlkLayout {
+    static constexpr int E = (int)sizeof(T);
+    static constexpr int kU = 0;                 // u: 64 x 4 B
+    static constexpr int kPS = 256;              // (ps, pt): 64 x 2E
+    static constexpr int kBB = 256 + 128 * E;    // (bx, by): 64 x 2E
+    static constexpr int kBZ = 256 + 256 * E;    // bz: 64 x E
+    static constexpr int kBytes = 256 + 320 * E; // block bytes (fp32 1536, fp16 896)
+};
+
+// raw buffer access at a byte offset (unsigned: negative planes wrap above the buffer)
+__device__ __forceinline__ float ldb_f32(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+template <typename T>
+__device__ __forceinline__ float ldb_t(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    if constexpr (std::is_same<T, float>::value) {
+        return ldb_f32(r, off);
+    } else {
+        const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0);
+        return __half2float(__ushort_as_half(b));
+    }
+}
+template <typename T>
+__device__ __forceinline__ float2 ldb_pair(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    if constexpr (std::is_same<T, float>::value) {
+        const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+        return make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
+    } else {
+        const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
+        return make_float2(__half2float(__ushort_as_half((unsigned short)(w & 0xffffu))),
+                           __half2float(__ushort_as_half((unsigned short)(w >> 16))));
+    }
+}
+template <int CP>
+__device__ __forceinline__ void stb_f32(__amdgpu_buffer_rsrc_t r, unsigned off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)off, 0, CP);
+}
+template <typename T, int CP>
+__device__ __forceinline__ void stb_t(__amdgpu_buffer_rsrc_t r, unsigned off, float v) {
+    if constexpr (std::is_same<T, float>::value)
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)off, 0, CP);
+    else
+        __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(__float2half_rn(v)), r, (int)off,
+                                              0, CP);
+}
+template <typename T, int CP>
+__device__ __forceinline__ void stb_pair(__amdgpu_buffer_rsrc_t r, unsigned off, float lo,
+                                         float hi) {
+    if constexpr (std::is_same<T, float>::value) {
+        u32x2 w;
+        w.x = __float_as_uint(lo);
+        w.y = __float_as_uint(hi);
+        __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)off, 0, CP);
+    } else {
+        const uint32_t w = (uint32_t)__half_as_ushort(__float2half_rn(lo)) |
+                           ((uint32_t)__half_as_ushort(__float2half_rn(hi)) << 16);
+        __builtin_amdgcn_raw_buffer_store_b32(w, r, (int)off, 0, CP);
+    }
+}
+// descriptor over a whole block-interleaved state buffer of nz planes of pb bytes
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t blk_rsrc(const void *base, int nz, int pb) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0,
+                                             (int)((unsigned)nz * (unsigned)pb), 0x00020000);
+}
+// Base offset of a store to drop: past every block-interleaved buffer fused_fits_blk
+// allows (< 2^31 bytes), and adding a field and lane offset (< 4096) stays below 2^32,
+// so the bounds check drops it.
+constexpr unsigned kBlkDrop = 0xfffff000u;
+
 // ---- correctly rounded division, short form ------------------------------
 // The compiler's IEEE fp32 division is: refine a hardware reciprocal of the
 // (scaled) divisor with two FMAs, multiply, then two remainder/correction FMA
@@ -348,10 +434,14 @@ __device__ __forceinline__ float rescale(float g_hi, float g_lo, float b, bool t
 // 0 -- formed in registers from Cs and Ct instead of loaded, and the alpha
 // values it loads are checked (> 0, asetsBinaryMF3D_mex.c:317-319).
 // STREAM: output stores and the output stage's Cs/Ct loads are non-temporal (kStream).
+// BLK (binary only): the state is block-interleaved (BlkLayout; a.in.u and a.out.u
+// are the two buffers), Cs, Ct and alpha stay planar.
 template <bool IS3D, int TX, int NT, typename T, int MODE = 0, bool FIRST = false,
-          bool STREAM = IS3D>
+          bool STREAM = IS3D, bool BLK = false>
 __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
     using IO = Io<T>;
+    using BL = BlkLayout<T>;
+    static_assert(!BLK || MODE == 0, "the block-interleaved state is the binary model's");
     constexpr int CP = STREAM ? kStream : 0;
     constexpr bool POTTS = (MODE == 1), ISHI = (MODE == 2);
     static_assert(!FIRST || MODE == 0, "the cold-start iteration is the binary model's");
@@ -407,6 +497,10 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         static_assert(K2 <= 16, "flag word holds 16 positions");
         int off2[K2];
         int l2h[K2 - KO];
+        // BLK: byte offsets within a state plane of the slot's u, its pairs, its bz
+        // and the pairs of the voxel at y + 1 (fp32: bz at the u offset; fp16: the
+        // pairs at the u offset)
+        unsigned bu2[BLK ? K2 : 1], bp2[BLK ? K2 : 1], bz2[BLK ? K2 : 1], byp2[BLK ? K2 : 1];
         const int l2o = (2 + tid / TY) * W2 + 2 + tid % TY;   // LDS index of slot 0
         uint64_t fl2 = 0;
 #pragma unroll
@@ -430,6 +524,14 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             }
             const int y = y0 - 2 + i, x = x0 - 2 + j;
             off2[k] = x * ny + y;
+            if constexpr (BLK) {
+                const int yy = y + 64, yb = yy / 64 - 1, ln = yy % 64;   // y >= -2
+                const unsigned ob = (unsigned)((x * a.nyb + yb) * BL::kBytes);
+                bu2[k] = ob + 4u * (unsigned)ln;
+                bp2[k] = ob + (unsigned)(2 * BL::E * ln);
+                bz2[k] = ob + (unsigned)(BL::E * ln);
+                byp2[k] = (ln < 63) ? bp2[k] + 2u * BL::E : ob + (unsigned)BL::kBytes;
+            }
             const bool ok = in2(k);
             const bool in = ok && y >= 0 && y < ny && x >= 0 && x < nx;
             const uint64_t f = (in ? 1u : 0u) | (x > 0 ? 2u : 0u) | (y > 0 ? 4u : 0u) |
@@ -496,6 +598,32 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     r.u[k] = (cs - ct >= 0.0f) ? 1.0f : 0.0f;
                     r.ps[k] = r.pt[k] = IO::stored(fmin_ref(cs, ct));
                     r.bxl[k] = r.bxr[k] = r.byl[k] = r.byr[k] = r.bzn[k] = 0.0f;
+                }
+                const PRef ra = pref(a.alpha, q - 1, nz, 4, plane);
+#pragma unroll
+                for (int k = 0; k < K1; ++k) r.al[k] = ldf(ra, off1[k]);
+                return;
+            }
+            if constexpr (BLK) {
+                const __amdgpu_buffer_rsrc_t rs = blk_rsrc(a.in.u, nz, a.pb);
+                const unsigned P = (unsigned)q * (unsigned)a.pb;
+                const unsigned XR = (unsigned)(a.nyb * BL::kBytes);   // the next column
+#pragma unroll
+                for (int k = 0; k < K2; ++k) {
+                    r.u[k] = ldb_f32(rs, P + bu2[k] + BL::kU);
+                    const float2 pp = ldb_pair<T>(rs, P + bp2[k] + BL::kPS);
+                    r.ps[k] = pp.x;
+                    r.pt[k] = pp.y;
+                    const float2 bb = ldb_pair<T>(rs, P + bp2[k] + BL::kBB);
+                    r.bxl[k] = bb.x;
+                    r.byl[k] = bb.y;
+                    r.bxr[k] = ldb_pair<T>(rs, P + XR + bp2[k] + BL::kBB).x;
+                    r.byr[k] = ldb_pair<T>(rs, P + byp2[k] + BL::kBB).y;
+                }
+                if (IS3D) {
+#pragma unroll
+                    for (int k = 0; k < K2; ++k)
+                        r.bzn[k] = ldb_t<T>(rs, P + (unsigned)a.pb + bz2[k] + BL::kBZ);
                 }
                 const PRef ra = pref(a.alpha, q - 1, nz, 4, plane);
 #pragma unroll
@@ -752,6 +880,22 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     const float npt = fmin_ref(f, o.ct[k]);
                     const float e = cc * ((npt + d) - nps);
                     e_acc += ok ? fabs((double)e) : 0.0;
+                    if constexpr (BLK) {
+                        // every lane of a column inside the volume stores (the padding
+                        // lanes y >= ny keep their faces at 0: they are 0 here); a
+                        // column past the volume is dropped
+                        const __amdgpu_buffer_rsrc_t ws = blk_rsrc(a.out.u, nz, a.pb);
+                        const unsigned P = (unsigned)s * (unsigned)a.pb;
+                        const bool okx = x < nx;
+                        const unsigned ou = okx ? P + bu2[k] : kBlkDrop;
+                        const unsigned op = okx ? P + bp2[k] : kBlkDrop;
+                        const unsigned oz = okx ? P + bz2[k] : kBlkDrop;
+                        stb_f32<CP>(ws, ou + BL::kU, uo - e);
+                        stb_pair<T, CP>(ws, op + BL::kPS, nps, npt);
+                        stb_pair<T, CP>(ws, op + BL::kBB, bxl, byl);
+                        if (IS3D) stb_t<T, CP>(ws, oz + BL::kBZ, bzl);
+                        continue;
+                    }
                     st<Io<float>, CP>(wu, off, uo - e);
                     if constexpr (PAIR) {
                         st_pair<T, CP>(wps, off, nps, npt);
@@ -808,7 +952,12 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         Keep k0, k1, k2;
         if constexpr (IS3D) {
             // bz own of plane z0-2 (raw plane z0-3 would carry it as its "next" bz)
-            if constexpr (!FIRST) {
+            if constexpr (!FIRST && BLK) {
+                const __amdgpu_buffer_rsrc_t rs = blk_rsrc(a.in.u, nz, a.pb);
+                const unsigned P = (unsigned)(z0 - 2) * (unsigned)a.pb;
+#pragma unroll
+                for (int k = 0; k < K2; ++k) bz_cur[k] = ldb_t<T>(rs, P + bz2[k] + BL::kBZ);
+            } else if constexpr (!FIRST) {
                 const PRef rbz = pref(a.in.bz, z0 - 2, nz, IO::kBytes, plane);
 #pragma unroll
                 for (int k = 0; k < K2; ++k) bz_cur[k] = ld<IO>(rbz, off2[k]);
@@ -908,12 +1057,14 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 // One volume: one workgroup per work item, partial |e| sums per workgroup.
 // The first dispatched workgroup also runs the previous iteration's deferred
 // stop test, when there is one (it starts first, so it has the most slack).
-template <bool IS3D, int TX, int NT, int WAVES, typename T, bool FIRST = false>
+// BLK: block-interleaved state (BlkLayout), else planar fields.
+template <bool IS3D, int TX, int NT, int WAVES, typename T, bool FIRST = false, bool BLK = true>
 __global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
     __shared__ double red[16];
     if (a.flags->done) return;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    const double e = (lb < a.nblocks) ? fused_tile<IS3D, TX, NT, T, 0, FIRST>(a, lb) : 0.0;
+    const double e =
+        (lb < a.nblocks) ? fused_tile<IS3D, TX, NT, T, 0, FIRST, IS3D, BLK>(a, lb) : 0.0;
     const double s = block_sum(e, red);
     if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
     if (blockIdx.x == 0 && a.stop.partials) {

@@ -18,12 +18,21 @@
 //                (24 B) and the state written (24 B), each interleaved by blocks --
 //                for every 64 elements (one wave), field k is a contiguous 1 KiB
 //                block -- so each load and store is one coalesced 1 KiB wave access.
+//   kStreamMixState  the same bytes with the inputs in their own three arrays (Cs,
+//                Ct, alpha as the caller holds them) and only the state block-
+//                interleaved (state read, state written): five arrays.
 #pragma once
 #include "common.h"
 
 namespace asets {
 
-enum StreamMode { kStreamRead = 0, kStreamCopy = 1, kStreamMix = 2, kStreamMixPacked = 3 };
+enum StreamMode {
+    kStreamRead = 0,
+    kStreamCopy = 1,
+    kStreamMix = 2,
+    kStreamMixPacked = 3,
+    kStreamMixState = 4
+};
 
 // 16-byte element (a clang vector: the non-temporal builtins take those)
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -130,6 +139,25 @@ __global__ __launch_bounds__(256) void k_stream_mix_packed(const f32x4 *in, cons
 #pragma unroll
         for (int k = 0; k < 6; ++k) q[k] = ld16<NT>(sin + (blk * 6 + k) * 64 + lane);
         const f32x4 s = (a[0] + a[1]) + a[2];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) st16<NT>(sout + (blk * 6 + k) * 64 + lane, q[k] + s);
+    }
+}
+
+// kStreamMixState: three planar input arrays (n16 elements each) and the state
+// block-interleaved as in k_stream_mix_packed.
+template <bool NT>
+__global__ __launch_bounds__(256) void k_stream_mix_state(const f32x4 *cs, const f32x4 *ct,
+                                                          const f32x4 *al, const f32x4 *sin,
+                                                          f32x4 *sout, int64_t n16) {
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += st) {
+        const int64_t blk = i >> 6, lane = i & 63;
+        f32x4 q[6];
+        const f32x4 a0 = ld16<NT>(cs + i), a1 = ld16<NT>(ct + i), a2 = ld16<NT>(al + i);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) q[k] = ld16<NT>(sin + (blk * 6 + k) * 64 + lane);
+        const f32x4 s = (a0 + a1) + a2;
 #pragma unroll
         for (int k = 0; k < 6; ++k) st16<NT>(sout + (blk * 6 + k) * 64 + lane, q[k] + s);
     }

@@ -290,9 +290,9 @@ def hbm_stream(mode: str, buf, *, nontemporal: bool = True, one_in_flight: bool 
                grid: int = 0, iters: int = 10, stream=None) -> tuple[float, int]:
     """Time a plain stream over the device tensor ``buf`` (asets_hbm_stream_dev):
     mode "read", "copy", "mix" (the binary solver's 36 B read + 24 B written per voxel over
-    its eleven arrays, without the stencil) or "mix_packed" (the same bytes through three
-    interleaved arrays).  Returns (ms per launch, bytes read plus written per launch)."""
-    m = {"read": 0, "copy": 1, "mix": 2, "mix_packed": 3}[mode]
+    its eleven arrays, without the stencil), "mix_packed" (the same bytes through three
+    interleaved arrays) or "mix_state" (inputs in three arrays, the state interleaved).  Returns (ms per launch, bytes read plus written per launch)."""
+    m = {"read": 0, "copy": 1, "mix": 2, "mix_packed": 3, "mix_state": 4}[mode]
     if not (buf.is_cuda and buf.is_contiguous()):
         raise ValueError("expected a contiguous CUDA tensor")
     ms, traffic = C.c_double(), C.c_int64()

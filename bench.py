@@ -41,6 +41,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+REF_ITERS = 2000        # fixed iterations of the converged reference solve
+LABEL_TARGET = 0.9999   # label agreement the errBound 1e-4 solve is held to
 
 
 def _oracle():
@@ -160,6 +162,10 @@ def bench_binary3d(args):
                 {"workload": f"binary3d {n}^3 ellipsoids, contrast alpha, cc 0.35 steps 0.11",
                  "shape": list(shape), "iters_per_step": args.iters, "kernel": args.kernel})
     out["roofline"] = _roofline(60, N, k_ms, traffic)
+    out["roofline"]["traffic_source"] = (
+        "not measured in this run: the constant of profiles/kf_binary_256cubed.json, from "
+        "rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs of this kernel and workload "
+        "scaled by a 256 MiB device copy" if traffic is not None else None)
     out["gen_s"] = gen_s
 
     # fp16 flow storage (ps, pt, bx, by, bz in fp16; u and arithmetic fp32): 40 B/voxel-iter
@@ -174,6 +180,29 @@ def bench_binary3d(args):
                              "bytes_per_voxel_iter": 40,
                              "achieved_GBps": 40 * N / (r16.kernel_ms * 1e-3) / 1e9,
                              "max_abs_u_diff_vs_fp32": float((r16.u - u_out).abs().max().item())}
+        if not args.no_cpu:
+            # both flow storages against the FP64 oracle run for the same iterations on
+            # the central (n/2)^3 crop of the volume
+            orc = _oracle()
+            crop = tuple(slice(n // 4, n // 4 + n // 2) for _ in range(3))
+            cs = [np.asfortranarray(x[crop]) for x in (Cs, Ct, alpha)]
+            csh = cs[0].shape
+            dcs = [solver.to_device(x, dev) for x in cs]
+            r64 = orc.binary(*cs, csh, args.iters, 0.0, cc, steps, precision="f64")
+            err = {}
+            for flow in ("f32", "f16"):
+                with torch.cuda.stream(stream):
+                    rc_ = solver.binary(*dcs, csh, args.iters, 0.0, cc, steps, flow=flow,
+                                        stream=stream)
+                torch.cuda.synchronize()
+                ug = rc_.u.cpu().numpy().astype(np.float64)
+                d = np.abs(ug - r64.u)
+                err[flow] = {"max_abs_u_err": float(d.max()), "mean_abs_u_err": float(d.mean()),
+                             "label_agreement": float(np.mean((ug > 0.5) == (r64.u > 0.5)))}
+            out["fp16_flows"]["error_vs_fp64_oracle"] = {
+                "crop": "x".join(map(str, csh)), "iterations": args.iters,
+                "fp16_flows": err["f16"], "fp32_flows": err["f32"]}
+            _progress("fp16 error against the FP64 oracle done")
 
     if not args.no_tol:
         # whole solve calls (setup, iterations, stop polls, output copy) from the host:
@@ -195,6 +224,37 @@ def bench_binary3d(args):
                               "final_cvg": float(rt.cvg[rt.iterations - 1].item()),
                               "fg_fraction": float(lab.float().mean().item()),
                               "energy": float(e[0])}
+        # labels (u > 0.5) against a converged solve (REF_ITERS fixed iterations), the
+        # >= 99.99 % agreement target, and the largest errBound that meets it
+        ref_u = solve(REF_ITERS, 0.0).u.clone()
+        lab_ref = ref_u > 0.5
+
+        def agree(u):
+            return float(((u > 0.5) == lab_ref).float().mean().item())
+
+        sweep = []
+        for eb in (1e-4, 5e-5, 2e-5, 1e-5, 5e-6, 2e-6, 1e-6, 5e-7):
+            ts = []
+            for _ in range(3):
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                r_eb = solve(20000, eb)
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t1)
+            sweep.append({"err_bound": eb, "seconds": sorted(ts)[1],
+                          "iterations": r_eb.iterations, "label_agreement": agree(r_eb.u)})
+            if sweep[-1]["label_agreement"] >= LABEL_TARGET:
+                break
+        first = next((x for x in sweep if x["label_agreement"] >= LABEL_TARGET), None)
+        out["time_to_tol"]["labels_vs_converged"] = {
+            "reference": f"{REF_ITERS} fixed iterations",
+            "label_agreement": agree(rt.u), "target": LABEL_TARGET,
+            "meets_target": agree(rt.u) >= LABEL_TARGET,
+            "first_err_bound_meeting_target": first["err_bound"] if first else None,
+            "time_to_tol_at_that_err_bound_s": first["seconds"] if first else None,
+            "iterations_at_that_err_bound": first["iterations"] if first else None,
+            "sweep": sweep}
+        _progress("label agreement sweep done")
 
     # learned warm start (network inference + solve from (u0, pt0)) vs the cold start,
     # both to errBound 1e-4; energies against a 2000-iteration reference solve
@@ -205,7 +265,7 @@ def bench_binary3d(args):
         net = WarmStartNet(dev)
         net.load(wpath)
         with torch.cuda.stream(stream):
-            ref = solver.binary(dCs, dCt, dA, shape, 2000, 0.0, cc, steps, stream=stream)
+            ref = solver.binary(dCs, dCt, dA, shape, REF_ITERS, 0.0, cc, steps, stream=stream)
             e_star = solver.energy_binary(dCs, dCt, dA, ref.u, shape, stream=stream)[0]
             net.forward(dCs, dCt, dA, shape, 1)   # warm-up
             torch.cuda.synchronize()
@@ -563,7 +623,7 @@ def bench_hbm(args):
     buf = torch.ones(60 * N // 4, dtype=torch.float32, device=dev)   # 1006.6 MB
     torch.cuda.synchronize()
     rows = []
-    for mode in ("read", "copy", "mix", "mix_packed"):
+    for mode in ("read", "copy", "mix", "mix_packed", "mix_state"):
         for nt in (True, False):
             for one in ((False, True) if mode in ("read", "copy") else (False,)):
                 for per_cu in (2, 4, 8, 16):
@@ -575,7 +635,7 @@ def bench_hbm(args):
                                  "workgroups_per_cu": per_cu, "ms": ms, "bytes": traffic,
                                  "GBps": traffic / (ms * 1e-3) / 1e9})
     best = {m: max((r for r in rows if r["mode"] == m), key=lambda r: r["GBps"])
-            for m in ("read", "copy", "mix", "mix_packed")}
+            for m in ("read", "copy", "mix", "mix_packed", "mix_state")}
     mix = best["mix"]
     out = _base(args, "HBM GB/s ceiling: plain streams with 16-byte accesses (read, copy, "
                       "binary-solver traffic mix)", mix["GBps"], "GB/s", mix["ms"], "f32",
