@@ -1,0 +1,198 @@
+"""Primal-dual bounds of the continuous max-flow model, and why they do not close on the
+reference iteration (FP64 oracle, CPU).
+
+Model (asetsBinaryMF3D_mex.c): max sum ps s.t. ps <= Cs, pt <= Ct, div p - ps + pt = 0 and
+the per-voxel flow capacity 0.5 * (sum of its six faces' p^2) <= alpha^2 (the norm of
+projStep, :321-331).  For feasible flows the dual value is D(p) = sum min(Cs, Ct + div p);
+for u in [0, 1] the primal value is P(u) = sum (1-u) Cs + u Ct + sigma_K(grad u), where
+sigma_K is the support function of the capacity set.  By Lagrange duality over the
+capacity constraints, sigma_K(w) = min over lambda >= 0 of
+    F(lambda) = sum_v lambda_v alpha_v^2 + sum_f w_f^2 / (2 (lambda_v1(f) + lambda_v2(f))),
+so F at any lambda bounds it from above, and red-black coordinate descent on lambda
+(per-voxel Newton) drives F to sigma_K.  D(p~) <= min P <= P_lambda(u) then brackets the
+optimum, with p~ the flows scaled face by face to the smaller feasibility factor of the
+face's two voxels.
+
+The reference's rescale (asetsBinaryMF3D_mex.c:337-397) scales each face by the mean of its
+two voxels' factors instead of projecting, so its fixed point is not the saddle point of
+this problem: its converged flows exceed the capacity, and the bracket around its iterates
+stalls.  These tests pin the bounds (weak duality, tightness of F) and that finding.
+"""
+import numpy as np
+
+import oracle as O
+from asetsmatlabmaxflow_amd import workloads
+
+SHAPE = (24, 20, 16)   # (Ny, Nx, Nz)
+
+
+def _grid(a):
+    ny, nx, nz = SHAPE
+    a = np.asarray(a, np.float64)
+    return (a.ravel(order="F") if a.ndim > 1 else a).reshape(nz, nx, ny)
+
+
+def _hi(f, ax):   # own face of the +ax neighbour (0 past the volume); axes (z, x, y)
+    g = np.zeros_like(f)
+    sl = [slice(None)] * 3
+    src = list(sl)
+    sl[ax] = slice(None, -1)
+    src[ax] = slice(1, None)
+    g[tuple(sl)] = f[tuple(src)]
+    return g
+
+
+def _lo(f, ax):   # value at the -ax neighbour (0 outside)
+    g = np.zeros_like(f)
+    sl = [slice(None)] * 3
+    src = list(sl)
+    sl[ax] = slice(1, None)
+    src[ax] = slice(None, -1)
+    g[tuple(sl)] = f[tuple(src)]
+    return g
+
+
+AX = (1, 2, 0)   # bx along x (axis 1), by along y (axis 2), bz along z (axis 0)
+
+
+def _norm(f):
+    return np.sqrt(0.5 * sum(f[k] ** 2 + _hi(f[k], AX[k]) ** 2 for k in range(3)))
+
+
+def _div(f):
+    return sum(_hi(f[k], AX[k]) - f[k] for k in range(3))
+
+
+def dual(Cs, Ct, al, faces):
+    """D(p~): the faces scaled to the smaller feasibility factor of their two voxels."""
+    n = _norm(faces)
+    s = np.where(n > al, al / np.where(n > 0, n, 1.0), 1.0)
+    sc = [faces[k] * np.minimum(s, _lo(s, AX[k])) for k in range(3)]
+    assert (_norm(sc) <= al * (1 + 1e-12)).all()
+    return np.minimum(Cs, Ct + _div(sc)).sum()
+
+
+def _wfaces(u):
+    w = [u - _lo(u, AX[k]) for k in range(3)]
+    for k in range(3):
+        idx = [slice(None)] * 3
+        idx[AX[k]] = 0
+        w[k][tuple(idx)] = 0.0
+    return w
+
+
+def F(lam, al, w):
+    tot = (al ** 2 * lam).sum()
+    for k in range(3):
+        c = 0.5 * w[k] ** 2
+        L = lam + _lo(lam, AX[k])
+        tot += np.where(c > 0, c / np.where(L > 0, L, 1.0), 0.0).sum()
+        assert not ((c > 0) & (L <= 0)).any()
+    return tot
+
+
+def sweep(lam, al, w, color, newton=40):
+    """Minimise F over the voxels of one checkerboard colour, neighbours fixed."""
+    cs, ms = [], []
+    for k in range(3):
+        c = 0.5 * w[k] ** 2
+        cs += [c, _hi(c, AX[k])]
+        ms += [_lo(lam, AX[k]), _hi(lam, AX[k])]
+    Cm, Mm = np.stack(cs), np.stack(ms)
+    a2 = al ** 2
+    csum = Cm.sum(0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        d0 = a2 - np.where(Cm > 0, Cm / np.where(Mm > 0, Mm, 0.0) ** 2, 0.0).sum(0)
+    d0 = np.where(np.isnan(d0), -np.inf, d0)
+    active = (csum > 0) & (d0 < 0)
+    mmin = np.where(Cm > 0, Mm, np.inf).min(0)
+    x = np.where(active, np.sqrt(csum) / al - np.where(np.isfinite(mmin), mmin, 0.0), 1.0)
+    x = np.where(x > 0, x, 1e-3)
+    for _ in range(newton):
+        den = x + Mm
+        den = np.where(den > 0, den, 1.0)
+        t = np.where(Cm > 0, Cm / den ** 2, 0.0)
+        g = a2 - t.sum(0)
+        h = 2.0 * np.where(Cm > 0, t / den, 0.0).sum(0)
+        xn = x - g / np.where(h > 0, h, 1.0)
+        x = np.where(xn > 0, xn, x / 4)
+    x = np.where(active, x, 0.0)
+    zz, xx, yy = np.indices(lam.shape)
+    return np.where(((zz + xx + yy) % 2) == color, x, lam)
+
+
+def primal_bound(Cs, Ct, al, u, sweeps=30):
+    ub = np.clip(u, 0.0, 1.0)
+    w = _wfaces(ub)
+    nrm = np.sqrt(sum(w[k] ** 2 + _hi(w[k], AX[k]) ** 2 for k in range(3)))
+    lam = nrm / (2.0 * np.sqrt(2.0) * al)   # equal split of every face: the voxel-centred TV
+    data = ((1 - ub) * Cs + ub * Ct).sum()
+    vox = data + F(lam, al, w)
+    for i in range(sweeps):
+        lam = sweep(lam, al, w, i % 2)
+    return data + F(lam, al, w), vox, lam, w
+
+
+def _problem():
+    Cs, Ct, al, _ = workloads.binary3d_ellipsoids(SHAPE, seed=5, count=3)
+    return Cs, Ct, al, _grid(Cs), _grid(Ct), _grid(al)
+
+
+def _state(Cs, Ct, al, its):
+    r = O.binary(Cs, Ct, al, SHAPE, its, 0.0, 0.35, 0.11, precision="f64", want_state=True)
+    return _grid(r.u), [_grid(r.state[k]) for k in ("bx", "by", "bz")]
+
+
+def test_weak_duality_any_state():
+    """For any u in [0, 1], any lambda and any flows (scaled to feasibility),
+    P_lambda(u) >= D(p~): random states and a few-iteration solve."""
+    Cs, Ct, al, cs, ct, a = _problem()
+    rng = np.random.default_rng(3)
+    for trial in range(3):
+        u = rng.uniform(-0.2, 1.2, cs.shape)
+        faces = [rng.normal(0, 0.5, cs.shape) for _ in range(3)]
+        P, vox, _, _ = primal_bound(cs, ct, a, u, sweeps=4)
+        D = dual(cs, ct, a, faces)
+        assert P >= D and vox >= P - 1e-9 * abs(P)
+    u, faces = _state(Cs, Ct, al, 30)
+    P, vox, _, _ = primal_bound(cs, ct, a, u, sweeps=4)
+    assert vox >= P >= dual(cs, ct, a, faces)
+
+
+def test_lambda_descent_reaches_the_support_function():
+    """F(lambda) after the sweeps equals sigma_K(w): the inner maximiser p_f = w_f /
+    Lambda_f, scaled to feasibility, gives <w, p> within 1e-6 of F (both bound sigma_K,
+    from below and above)."""
+    Cs, Ct, al, cs, ct, a = _problem()
+    u, _ = _state(Cs, Ct, al, 400)
+    ub = np.clip(u, 0, 1)
+    _, _, lam, w = primal_bound(cs, ct, a, ub, sweeps=60)
+    up = F(lam, a, w)
+    p = []
+    for k in range(3):
+        L = lam + _lo(lam, AX[k])
+        p.append(np.where(w[k] != 0, w[k] / np.where(L > 0, L, 1.0), 0.0))
+    n = _norm(p)
+    s = np.where(n > a, a / np.where(n > 0, n, 1.0), 1.0)
+    p = [p[k] * np.minimum(s, _lo(s, AX[k])) for k in range(3)]
+    low = sum((w[k] * p[k]).sum() for k in range(3))
+    assert low <= up * (1 + 1e-12)
+    assert (up - low) <= 1e-6 * up
+
+
+def test_reference_fixed_point_is_not_the_saddle_point():
+    """Converged FP64 solves: the stored flows exceed the capacity (the rescale averages the
+    two voxels' factors), the primal bound is tight (the voxel-centred TV bound it
+    replaces is looser by about 5x), and the bracket D(p~) <= min <= P(u) stays open at a
+    constant width between 1000 and 3000 iterations instead of closing."""
+    Cs, Ct, al, cs, ct, a = _problem()
+    gaps = []
+    for its in (1000, 3000):
+        u, faces = _state(Cs, Ct, al, its)
+        assert (_norm(faces) / a).max() > 1.5
+        P, vox, _, _ = primal_bound(cs, ct, a, u, sweeps=30)
+        D = dual(cs, ct, a, faces)
+        assert P >= D
+        gaps.append((P - D) / abs(D))
+        assert (vox - D) > 3 * (P - D)
+    assert gaps[1] > 1e-3 and abs(gaps[1] - gaps[0]) < 0.05 * gaps[0]
