@@ -426,13 +426,14 @@ __global__ void k_blk_init(const float *Cs, const float *Ct, const float *u0, co
                            const float *pt0, const float *bx0, const float *by0,
                            const float *bz0, char *st, int ny, int nx, int nz, int nyb) {
     using BL = BlkLayout<T>;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nz * nx * nyb * 64) return;
-    const int64_t b = t >> 6;
-    const int ln = (int)(t & 63);
-    const int yb = (int)(b % nyb);
-    const int64_t xz = b / nyb;
-    const int x = (int)(xz % nx), z = (int)(xz / nx);
+    // 32-bit indices: the layout's buffers stay below 2^31 bytes (fused_fits_blk)
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= nz * nx * nyb * 64) return;
+    const int b = t >> 6;
+    const int ln = t & 63;
+    const int yb = b % nyb;
+    const int xz = b / nyb;
+    const int x = xz % nx, z = xz / nx;
     const int y = yb * 64 + ln;
     float u = 0.0f, ps = 0.0f, pt = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
     if (y < ny) {
@@ -450,7 +451,7 @@ __global__ void k_blk_init(const float *Cs, const float *Ct, const float *u0, co
         if (by0 && y > 0) by = by0[g];
         if (bz0 && z > 0) bz = bz0[g];
     }
-    char *blk = st + b * BL::kBytes;
+    char *blk = st + (size_t)b * BL::kBytes;
     ((float *)(blk + BL::kU))[ln] = u;
     T *pp = (T *)(blk + BL::kPS) + 2 * ln;
     T *bb = (T *)(blk + BL::kBB) + 2 * ln;
@@ -467,17 +468,18 @@ template <typename T>
 __global__ void k_blk_take(const char *st, float *u, float *ps, float *pt, float *bx, float *by,
                            float *bz, int ny, int nx, int nz, int nyb) {
     using BL = BlkLayout<T>;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nz * nx * nyb * 64) return;
-    const int64_t b = t >> 6;
-    const int ln = (int)(t & 63);
-    const int yb = (int)(b % nyb);
-    const int64_t xz = b / nyb;
-    const int x = (int)(xz % nx), z = (int)(xz / nx);
+    // 32-bit indices: the layout's buffers stay below 2^31 bytes (fused_fits_blk)
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= nz * nx * nyb * 64) return;
+    const int b = t >> 6;
+    const int ln = t & 63;
+    const int yb = b % nyb;
+    const int xz = b / nyb;
+    const int x = xz % nx, z = xz / nx;
     const int y = yb * 64 + ln;
     if (y >= ny) return;
     const int64_t g = ((int64_t)z * nx + x) * ny + y;
-    const char *blk = st + b * BL::kBytes;
+    const char *blk = st + (size_t)b * BL::kBytes;
     const T *pp = (const T *)(blk + BL::kPS) + 2 * ln;
     const T *bb = (const T *)(blk + BL::kBB) + 2 * ln;
     if (u) u[g] = ((const float *)(blk + BL::kU))[ln];
@@ -1124,47 +1126,58 @@ void tune_store(const std::string &key, TileCfg c) {
 // ---------------------------------------------------------------------------
 // batched binary 3D
 // ---------------------------------------------------------------------------
-template <int TX, int NT, typename T>
+template <int TX, int NT, typename T, bool BLK>
 int batched_grid() {
     constexpr int W = (NT >= 512) ? 4 : 3;
     int dev = 0, cus = 0, per = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kf_binary_batched<TX, NT, W, T>, NT,
-                                                       0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per, kf_binary_batched<TX, NT, W, T, false, BLK>, NT, 0);
     return std::max(1, cus) * std::max(1, per);
 }
 
-template <int TX, int NT, int W, typename T>
+template <int TX, int NT, int W, typename T, bool BLK>
 void launch_batched_t(const BatchVol<T> *vols, const int2 *work, int nwork, const int32_t *vdone,
                       const int32_t *viter, double *partials, BatchCtl *ctl, float cc,
                       float steps, int fast_cc, bool first, int32_t *vbad, hipStream_t s) {
-    const int g = std::min(nwork, batched_grid<TX, NT, T>());
+    const int g = std::min(nwork, batched_grid<TX, NT, T, BLK>());
     if (first)
-        kf_binary_batched<TX, NT, W, T, true><<<g, NT, 0, s>>>(vols, work, nwork, vdone, viter,
-                                                               partials, ctl, cc, steps,
-                                                               fast_cc, vbad);
+        kf_binary_batched<TX, NT, W, T, true, BLK><<<g, NT, 0, s>>>(
+            vols, work, nwork, vdone, viter, partials, ctl, cc, steps, fast_cc, vbad);
     else
-        kf_binary_batched<TX, NT, W, T><<<g, NT, 0, s>>>(vols, work, nwork, vdone, viter,
-                                                         partials, ctl, cc, steps, fast_cc,
-                                                         vbad);
+        kf_binary_batched<TX, NT, W, T, false, BLK><<<g, NT, 0, s>>>(
+            vols, work, nwork, vdone, viter, partials, ctl, cc, steps, fast_cc, vbad);
 }
 
-// first: the cold-start first iteration of every volume (checks alpha into vbad)
+// first: the cold-start first iteration of every volume (checks alpha into vbad);
+// blk: block-interleaved state buffers
 template <typename T>
 void launch_batched(int tx, const BatchVol<T> *vols, const int2 *work, int nwork,
                     const int32_t *vdone, const int32_t *viter, double *partials, BatchCtl *ctl,
                     float cc, float steps, int fast_cc, bool first, int32_t *vbad,
-                    hipStream_t s) {
+                    hipStream_t s, bool blk) {
+    if (blk) {
+        if (tx == 8)
+            launch_batched_t<8, 256, 3, T, true>(vols, work, nwork, vdone, viter, partials, ctl,
+                                                 cc, steps, fast_cc, first, vbad, s);
+        else if (tx == 32)
+            launch_batched_t<32, 1024, 4, T, true>(vols, work, nwork, vdone, viter, partials,
+                                                   ctl, cc, steps, fast_cc, first, vbad, s);
+        else
+            launch_batched_t<16, 512, 4, T, true>(vols, work, nwork, vdone, viter, partials,
+                                                  ctl, cc, steps, fast_cc, first, vbad, s);
+        return;
+    }
     if (tx == 8)
-        launch_batched_t<8, 256, 3, T>(vols, work, nwork, vdone, viter, partials, ctl, cc, steps,
-                                       fast_cc, first, vbad, s);
+        launch_batched_t<8, 256, 3, T, false>(vols, work, nwork, vdone, viter, partials, ctl, cc,
+                                              steps, fast_cc, first, vbad, s);
     else if (tx == 32)
-        launch_batched_t<32, 1024, 4, T>(vols, work, nwork, vdone, viter, partials, ctl, cc,
-                                         steps, fast_cc, first, vbad, s);
+        launch_batched_t<32, 1024, 4, T, false>(vols, work, nwork, vdone, viter, partials, ctl,
+                                                cc, steps, fast_cc, first, vbad, s);
     else
-        launch_batched_t<16, 512, 4, T>(vols, work, nwork, vdone, viter, partials, ctl, cc, steps,
-                                        fast_cc, first, vbad, s);
+        launch_batched_t<16, 512, 4, T, false>(vols, work, nwork, vdone, viter, partials, ctl,
+                                               cc, steps, fast_cc, first, vbad, s);
 }
 
 // One workgroup per volume: its stop flag, iteration counter, cvg (zeros) and
@@ -1188,14 +1201,32 @@ __global__ void k_batch_reset(const BatchMeta *meta, int32_t *vdone, int32_t *vi
 }
 
 // u of every volume from the state buffer its last iteration wrote ((iterations
-// executed) & 1) to the caller's array; blockIdx.y = volume.
-template <typename T>
+// executed) & 1) to the caller's array; blockIdx.y = volume.  BLK: the buffers are
+// block-interleaved (one thread per lane of a block; 32-bit indices, as the
+// layout's buffers stay below 2^31 bytes).
+template <typename T, bool BLK>
 __global__ void k_batch_take_u(const BatchVol<T> *vols, const BatchMeta *meta, int32_t max_iter) {
     const int v = blockIdx.y;
     const BatchMeta m = meta[v];
     const int32_t itn = *m.it_num;
     const int32_t n_exec = (itn < max_iter) ? itn + 1 : max_iter;
     const float *src = vols[v].u[n_exec & 1];
+    if constexpr (BLK) {
+        const BatchVol<T> &V = vols[v];
+        const int nyb = V.nyb, ny = V.ny;
+        const int lanes = V.nz * V.nx * nyb * 64;
+        for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < lanes;
+             t += gridDim.x * blockDim.x) {
+            const int b = t >> 6, ln = t & 63;
+            const int yb = b % nyb, xz = b / nyb;
+            const int y = yb * 64 + ln;
+            if (y >= ny) continue;
+            const int64_t g = (int64_t)xz * ny + y;
+            m.u_out[g] = ((const float *)((const char *)src + (size_t)b *
+                                          BlkLayout<T>::kBytes + BlkLayout<T>::kU))[ln];
+        }
+        return;
+    }
     const int64_t N = (int64_t)m.n;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -1258,7 +1289,15 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
                   Carver::sz(nvol, sizeof(BatchMeta)) + 2 * Carver::sz(nvol, 4) +
                   Carver::sz(max_items, sizeof(int2)) + Carver::sz(max_items, sizeof(double)) +
                   Carver::sz(nvol, 4);
-    for (int v = 0; v < nvol; ++v) need += fused_bytes(vols[v].ny * vols[v].nx * vols[v].nz, sizeof(T));
+    // block-interleaved state buffers when every volume's fit (fused_fits_blk), else planar
+    bool blk = true;
+    for (int v = 0; v < nvol; ++v)
+        blk = blk && fused_fits_blk(vols[v].ny, vols[v].nx, vols[v].nz, sizeof(T));
+    for (int v = 0; v < nvol; ++v) {
+        const asets_volume &V = vols[v];
+        need += blk ? 2 * Carver::sz(V.nz * blk_plane_bytes(V.ny, V.nx, sizeof(T)), 1)
+                    : fused_bytes(V.ny * V.nx * V.nz, sizeof(T));
+    }
     st = ensure(&dc.dev, &dc.dev_bytes, need);
     if (st) return st;
     Carver cv(dc.dev);
@@ -1280,7 +1319,15 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
         b.Cs = V.Cs;
         b.Ct = V.Ct;
         b.alpha = V.alpha;
+        b.nyb = (int)cdiv(V.ny, 64);
+        b.pb = (int)blk_plane_bytes(V.ny, V.nx, sizeof(T));
         for (int k = 0; k < 2; ++k) {
+            if (blk) {   // one block-interleaved array; every pointer is its base
+                char *base = cv.take<char>(V.nz * (int64_t)b.pb);
+                b.u[k] = (float *)base;
+                b.ps[k] = b.pt[k] = b.bx[k] = b.by[k] = b.bz[k] = (T *)base;
+                continue;
+            }
             b.u[k] = cv.take<float>(N);
             b.ps[k] = cv.take<T>(kPair * N);
             b.pt[k] = b.ps[k] + 1;
@@ -1306,8 +1353,13 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
             const int64_t N = (int64_t)b.ny * b.nx * b.nz;
             const unsigned nb = (unsigned)cdiv(N, kBlock);
             // buffer 1 needs none: a volume's first iteration writes every voxel of it
-            k_init_binary_cold_t<T><<<nb, kBlock, 0, s>>>(b.Cs, b.Ct, b.u[0], b.ps[0], b.bx[0],
-                                                         b.bz[0], N);
+            if (blk)
+                k_blk_init<T><<<(unsigned)cdiv((int64_t)b.nz * b.nx * b.nyb * 64, kBlock), kBlock,
+                                0, s>>>(b.Cs, b.Ct, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                        nullptr, (char *)b.u[0], b.ny, b.nx, b.nz, b.nyb);
+            else
+                k_init_binary_cold_t<T><<<nb, kBlock, 0, s>>>(b.Cs, b.Ct, b.u[0], b.ps[0],
+                                                             b.bx[0], b.bz[0], N);
         }
         k_batch_reset<<<(unsigned)nvol, 256, 0, s>>>(d_meta, vdone, viter, vbad, nvol, max_iter,
                                                      ctl);
@@ -1343,7 +1395,7 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
     };
     auto one_iter = [&](int tx, bool first = false) {
         launch_batched<T>(tx, d_vols, d_work, nwork, vdone, viter, partials, ctl, p->cc, p->steps,
-                          fast_cc, first, vbad, s);
+                          fast_cc, first, vbad, s, blk);
         k_finalize_batched<<<nvol, 256, 0, s>>>(d_meta, vdone, viter, partials, p->err_bound,
                                                 max_iter, ctl);
     };
@@ -1446,7 +1498,10 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
     if (ev1) HIPCHK(hipEventRecord(ev1, s));
     // u of each volume lives in buffer (iterations executed) & 1: one copy kernel
     // reads each volume's it_num on the device
-    k_batch_take_u<T><<<dim3(256, (unsigned)nvol), 256, 0, s>>>(d_vols, d_meta, max_iter);
+    if (blk) k_batch_take_u<T, true><<<dim3(256, (unsigned)nvol), 256, 0, s>>>(d_vols, d_meta,
+                                                                               max_iter);
+    else k_batch_take_u<T, false><<<dim3(256, (unsigned)nvol), 256, 0, s>>>(d_vols, d_meta,
+                                                                             max_iter);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
     if (ev0) {

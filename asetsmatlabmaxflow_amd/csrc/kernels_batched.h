@@ -14,6 +14,8 @@
 
 namespace asets {
 
+// One volume's two state buffers: planar fields, or (BLK kernels) one
+// block-interleaved array each (BlkLayout), u[k] its base.
 template <typename T>
 struct BatchVol {
     const float *Cs, *Ct, *alpha;
@@ -22,6 +24,7 @@ struct BatchVol {
     int ny, nx, nz;
     int ntile_y, ntile_x, nzc, zchunk, nblocks;
     int work0;   // first partial slot of this volume
+    int nyb, pb; // blocks per column, state bytes per plane (block-interleaved state)
 };
 
 struct BatchMeta {
@@ -53,7 +56,8 @@ __device__ __forceinline__ P *uni(P *p) {
 // FIRST: the first iteration of every volume (cold start, fused_tile), which also
 // checks alpha (vbad[v] = 1 and ctl->bad_alpha = 1 when alpha <= 0 somewhere in
 // volume v).
-template <int TX, int NT, int WAVES, typename T, bool FIRST = false>
+// BLK: block-interleaved state buffers (BlkLayout).
+template <int TX, int NT, int WAVES, typename T, bool FIRST = false, bool BLK = true>
 __global__ __launch_bounds__(NT, WAVES) void kf_binary_batched(
     const BatchVol<T> *__restrict__ vols, const int2 *__restrict__ work, int nwork,
     const int32_t *vdone, const int32_t *viter, double *partials, BatchCtl *ctl, float cc,
@@ -95,8 +99,10 @@ __global__ __launch_bounds__(NT, WAVES) void kf_binary_batched(
         a.flags = nullptr;
         a.alpha_bad = FIRST ? vbad + v : nullptr;
         a.alpha_stop = FIRST ? &ctl->bad_alpha : nullptr;
+        a.nyb = uni(V.nyb);
+        a.pb = uni(V.pb);
         // normal cache policy: streaming stores measured slower for these volumes
-        const double e = fused_tile<true, TX, NT, T, 0, FIRST, false>(a, lb);
+        const double e = fused_tile<true, TX, NT, T, 0, FIRST, false, BLK>(a, lb);
         const double s = block_sum(e, red);
         if (threadIdx.x == 0) partials[uni(V.work0) + lb] = s;
         __syncthreads();
