@@ -579,6 +579,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         struct Raw {
             float u[K2], ps[K2], pt[K2], bxl[K2], bxr[K2], byl[K2], byr[K2], bzn[K2];
             float al[K1];
+            float uc[KO];   // u/cc of the own positions, formed by S1 (kept for S3)
         };
         // bz own face of the plane S1 is working on; the next plane's own bz
         // (loaded with the raw plane) becomes it one step later.
@@ -682,7 +683,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         float gkp[K2];
 #pragma unroll
         for (int k = 0; k < K2; ++k) gkp[k] = 0.0f;
-        auto s1 = [&](int q, const Raw &r) {
+        auto s1 = [&](int q, Raw &r) {
             float *gk = GK[q & 1];
             const int sl = (q + 3) % 3;
             const bool qin = q >= 0 && q < nz;
@@ -694,6 +695,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                 float d = ((r.byr[k] - r.byl[k]) + r.bxr[k]) - r.bxl[k];
                 if (IS3D) d = (d + r.bzn[k]) - bz_cur[k];
                 const float uc = div_fast(r.u[k], cc, rcc, fast_cc);
+                if (k < KO) r.uc[k] = uc;
                 const float v = live ? d - ((r.ps[k] - r.pt[k]) + uc) : 0.0f;
                 if (in2(k)) {
                     gk[idx] = v;
@@ -773,15 +775,17 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             }
         };
         // S3 operands kept from the raw plane S1 read (own-position slots):
-        // binary u and pt, Potts u and ps (the label-shared source flow)
+        // binary u and pt, Potts u and ps (the label-shared source flow), and the
+        // u/cc that S1 formed from the same u
         struct Keep {
-            float u[KO], p[KO];
+            float u[KO], p[KO], uc[KO];
         };
         auto keep = [&](Keep &kp, const Raw &r) {
 #pragma unroll
             for (int k = 0; k < KO; ++k) {
                 kp.u[k] = r.u[k];
                 kp.p[k] = POTTS ? r.ps[k] : r.pt[k];
+                kp.uc[k] = r.uc[k];
             }
         };
 
@@ -862,7 +866,7 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     continue;
                 }
                 const float uo = kp.u[k];
-                const float uc = div_fast(uo, cc, rcc, fast_cc);
+                const float uc = kp.uc[k];
                 if (POTTS) {
                     // sink flow of this label (asetsPotts3D_mex.c:457-466), ft = dv + pt
                     const float t = (kp.p[k] + uc) - d;   // kp.p holds ps here
@@ -1003,8 +1007,8 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
 #pragma unroll
             for (int k = 0; k < K1; ++k) r.al[k] = ldf(ra, off1[k]);
             load_out(0, o);
-            keep(k0, r);
             s1(0, r);
+            keep(k0, r);
             __syncthreads();
             s2a(0, r);
             __syncthreads();
