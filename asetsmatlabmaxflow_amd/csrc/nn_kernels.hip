@@ -11,8 +11,8 @@
 // as implicit GEMMs on the gfx950 matrix cores (v_mfma_f32_16x16x32_bf16):
 // M = voxels, N = output channels, K = 27 taps x input channels.  The input
 // gradient is the same convolution with the taps mirrored and the weight
-// matrix transposed; the weight gradient reduces over fixed voxel chunks into
-// partials that one kernel adds in order (deterministic).
+// matrix transposed; the weight gradient is an MFMA GEMM per tap over fixed voxel
+// chunks, whose partials one kernel adds in order (deterministic).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -170,84 +170,159 @@ __global__ void k_relu_mask(u16 *g, const u16 *y, int n) {
 }
 
 // ---------------------------------------------------------------------------
-// weight gradient: dw[k, co, ci] = sum_p g[p, co] * in[p + o(k), ci].  Workgroup
-// = (tap k, 16 x CIT (co, ci) tile, chunk of kWgChunk voxels), one (co, ci) per
-// thread (CIT = 16, or 8 for an 8-channel input); 64 voxels at a time are staged
-// in LDS (their shifted source voxel computed once per voxel).  Each chunk
-// writes its own partial.
+// weight gradient on the matrix cores: for each tap k,
+//   dw[k, co, ci] = sum_p g[p, co] * in[p + o(k), ci],
+// a GEMM with M = cout, N = cin and K = voxels.  Workgroup = (NQ taps, chunk of
+// wg_chunk(P) voxels), 256 threads: NQ = 3 (the taps dx = -1, 0, 1 of one (dz, dy))
+// when a tap has at most 8 output tiles of 16 x 16, else NQ = 1.  For each slab of
+// 32 voxels, G (32 x cout) is staged once and the tap-shifted input (32 x cin; zero
+// where p + o(k) leaves the volume) once per tap, in LDS and transposed, voxels
+// contiguous, so the 8 consecutive K values of an MFMA lane are one 16-byte LDS
+// read.  The NQ x (cout/16) x (cin/16) (tap, output tile) units (cin 8 padded to
+// 16 with zeros) are dealt round-robin to the 4 waves, at most UPW per wave.  Each
+// chunk writes its own partial; k_sum_chunks adds them in order (deterministic).
 // ---------------------------------------------------------------------------
-constexpr int kWgChunk = 4096;
+constexpr int kWgCoMax = 64, kWgCiMax = 128;
+constexpr int kWgLd = 40;   // LDS row stride in bf16 (80 bytes: 16-byte aligned, spread banks)
 
-template <int CIT>
-__global__ __launch_bounds__(16 * CIT) void k_conv3_wgrad(const u16 *__restrict__ in, Geo5 g,
-                                                          int cin, const u16 *__restrict__ go,
-                                                          int cout, float *__restrict__ part) {
-    constexpr int NTH = 16 * CIT;
-    __shared__ float sg[64][17], sx[64][CIT + 1];
-    __shared__ int src[64];
-    const int tap = blockIdx.x;
-    const int nci = cin / CIT;
-    const int cot = blockIdx.y / nci, cit = blockIdx.y % nci;
-    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+// voxels per weight-gradient chunk: about 128 chunks, 1024 to 16384 voxels each,
+// a multiple of the 32-voxel slab
+__host__ __device__ inline int wg_chunk(int64_t P) {
+    int64_t c = (P + 127) / 128;
+    c = c < 1024 ? 1024 : (c > 16384 ? 16384 : c);
+    return (int)((c + 31) / 32 * 32);
+}
+
+// NQ = 3: ntile <= 8, UPW = 6; NQ = 1: ntile <= 32, UPW = 8
+template <int NQ, int UPW>
+__global__ __launch_bounds__(256) void k_conv3_wgrad(const u16 *__restrict__ in, Geo5 g, int cin,
+                                                     const u16 *__restrict__ go, int cout,
+                                                     int chunk, float *__restrict__ part) {
+    __shared__ __attribute__((aligned(16))) u16 Gt[kWgCoMax][kWgLd];
+    __shared__ __attribute__((aligned(16))) u16 Xt[NQ][kWgCiMax][kWgLd];
+    // taps tap0 .. tap0 + NQ - 1: (dz, dy) fixed, dx = dx0 .. dx0 + NQ - 1
+    const int tap0 = NQ * (int)blockIdx.x;
+    const int dz = tap0 / 9 - 1, dy = (tap0 / 3) % 3 - 1, dx0 = tap0 % 3 - 1;
     const int P = g.n * g.d * g.h * g.w;
-    const int pb = blockIdx.z * kWgChunk;
-    const int pe = min(P, pb + kWgChunk);
-    const int tco = threadIdx.x / CIT, tci = threadIdx.x % CIT;
-    float acc = 0.0f;
-    for (int s = pb; s < pe; s += 64) {
-        if (threadIdx.x < 64) {
-            const int p = s + threadIdx.x;
-            int q = -1;
+    const int pb = blockIdx.y * chunk;
+    const int pe = min(P, pb + chunk);
+    const int cinp = cin < 16 ? 16 : cin;   // padded N
+    const int nct = cout / 16, nit = cinp / 16, ntile = nct * nit, nunit = NQ * ntile;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r = lane & 15, kg = lane >> 4;
+    f32x4 acc[UPW];
+#pragma unroll
+    for (int t = 0; t < UPW; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // padding rows of an 8-channel input stay zero
+    if (cin < 16)
+        for (int e = threadIdx.x; e < NQ * (16 - cin) * kWgLd; e += 256) {
+            const int q = e / ((16 - cin) * kWgLd), rem = e % ((16 - cin) * kWgLd);
+            Xt[q][cin + rem / kWgLd][rem % kWgLd] = 0;
+        }
+    const int zyshift = (dz * g.h + dy) * g.w;
+    for (int s0 = pb; s0 < pe; s0 += 32) {
+        __syncthreads();   // the previous slab's MFMA reads are done
+        // G slab: 32 voxels x cout, 8 channels (16 bytes) per load
+        const int gper = cout / 8;
+        for (int e = threadIdx.x; e < 32 * gper; e += 256) {
+            const int v = e / gper, c8 = (e % gper) * 8;
+            const int p = s0 + v;
+            uint4 val = make_uint4(0, 0, 0, 0);
+            if (p < pe) val = *(const uint4 *)(go + (size_t)p * cout + c8);
+            const u16 *h = (const u16 *)&val;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) Gt[c8 + j][v] = h[j];
+        }
+        // shifted input slabs of the NQ taps: NQ x 32 voxels x cin
+        const int xper = cin / 8;
+        for (int e = threadIdx.x; e < NQ * 32 * xper; e += 256) {
+            const int q = e / (32 * xper), rem = e % (32 * xper);
+            const int v = rem / xper, c8 = (rem % xper) * 8;
+            const int dx = dx0 + q;
+            const int p = s0 + v;
+            uint4 val = make_uint4(0, 0, 0, 0);
             if (p < pe) {
                 const int x = p % g.w;
                 int t = p / g.w;
                 const int y = t % g.h;
                 t /= g.h;
                 const int z = t % g.d;
-                const int zz = z + dz, yy = y + dy, xx = x + dx;
-                if ((unsigned)zz < (unsigned)g.d && (unsigned)yy < (unsigned)g.h &&
-                    (unsigned)xx < (unsigned)g.w)
-                    q = p + (dz * g.h + dy) * g.w + dx;
+                if ((unsigned)(z + dz) < (unsigned)g.d && (unsigned)(y + dy) < (unsigned)g.h &&
+                    (unsigned)(x + dx) < (unsigned)g.w)
+                    val = *(const uint4 *)(in + (size_t)(p + zyshift + dx) * cin + c8);
             }
-            src[threadIdx.x] = q;
+            const u16 *h = (const u16 *)&val;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) Xt[q][c8 + j][v] = h[j];
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < 64 * 16; e += NTH) {
-            const int pi = e >> 4, c = e & 15;
-            const int p = s + pi;
-            sg[pi][c] = p < pe ? bf2f(go[(size_t)p * cout + cot * 16 + c]) : 0.0f;
+#pragma unroll
+        for (int t = 0; t < UPW; ++t) {
+            const int u = wv + 4 * t;
+            if (u < nunit) {
+                const int q = u / ntile, tile = u % ntile;
+                const int ct = tile / nit, it = tile % nit;
+                const bf16x8 a = *(const bf16x8 *)&Gt[ct * 16 + r][8 * kg];
+                const bf16x8 b = *(const bf16x8 *)&Xt[q][it * 16 + r][8 * kg];
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+            }
         }
-        for (int e = threadIdx.x; e < 64 * CIT; e += NTH) {
-            const int pi = e / CIT, c = e % CIT;
-            const int q = src[pi];
-            sx[pi][c] = q >= 0 ? bf2f(in[(size_t)q * cin + cit * CIT + c]) : 0.0f;
-        }
-        __syncthreads();
-#pragma unroll 16
-        for (int pi = 0; pi < 64; ++pi) acc = __builtin_fmaf(sg[pi][tco], sx[pi][tci], acc);
-        __syncthreads();
     }
-    const int co = cot * 16 + tco, ci = cit * CIT + tci;
-    part[(((size_t)blockIdx.z * 27 + tap) * cout + co) * cin + ci] = acc;
+    // lane holds rows (co) 4*kg + i of column (ci) r of each of its units
+#pragma unroll
+    for (int t = 0; t < UPW; ++t) {
+        const int u = wv + 4 * t;
+        if (u >= nunit) continue;
+        const int q = u / ntile, tile = u % ntile;
+        const int tap = tap0 + q;
+        const int ct = tile / nit, it = tile % nit;
+        const int ci = it * 16 + r;
+        if (ci >= cin) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int co = ct * 16 + 4 * kg + i;
+            part[(((size_t)blockIdx.y * 27 + tap) * cout + co) * cin + ci] = acc[t][i];
+        }
+    }
 }
 
-// out[i] (+)= sum_c part[c*n + i], chunks added in order
+// out[i] (+)= sum_c part[c*n + i], chunks added in order; the loads are issued eight
+// at a time ahead of their in-order adds
 __global__ void k_sum_chunks(const float *part, int nchunk, int n, float *out, int accumulate) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float s = 0.0f;
-    for (int c = 0; c < nchunk; ++c) s += part[(size_t)c * n + i];
+    int c = 0;
+    for (; c + 8 <= nchunk; c += 8) {
+        float t[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) t[b] = part[(size_t)(c + b) * n + i];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) s += t[b];
+    }
+    for (; c < nchunk; ++c) s += part[(size_t)c * n + i];
     out[i] = accumulate ? out[i] + s : s;
 }
 
-// per-chunk partial sums of g over the voxels: the bias gradient
-__global__ void k_bias_part(const u16 *go, int P, int cout, float *part) {
-    const int co = threadIdx.x;
-    const int pb = blockIdx.x * kWgChunk, pe = min(P, pb + kWgChunk);
-    if (co >= cout) return;
+// per-chunk partial sums of g over the voxels: the bias gradient.  Thread (j, co)
+// of 256 sums the chunk's voxels p = pb + j (mod 256 / cout) in order; the partial
+// sums are then added in j order (deterministic).
+__global__ __launch_bounds__(256) void k_bias_part(const u16 *go, int P, int cout, int chunk,
+                                                   float *part) {
+    __shared__ float red[256];
+    const int nj = 256 / cout;
+    const int co = threadIdx.x % cout, j = threadIdx.x / cout;
+    const int pb = blockIdx.x * chunk, pe = min(P, pb + chunk);
     float s = 0.0f;
-    for (int p = pb; p < pe; ++p) s += bf2f(go[(size_t)p * cout + co]);
-    part[(size_t)blockIdx.x * cout + co] = s;
+    if (j < nj)
+        for (int p = pb + j; p < pe; p += nj) s += bf2f(go[(size_t)p * cout + co]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < cout) {
+        float t = 0.0f;
+        for (int q = 0; q < nj; ++q) t += red[q * cout + threadIdx.x];
+        part[(size_t)blockIdx.x * cout + threadIdx.x] = t;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -503,7 +578,7 @@ asets_status asets_nn_relu_mask(void *g, const void *y, int64_t n, void *stream)
 
 size_t asets_nn_wgrad_scratch_bytes(int n, int d, int h, int w, int cin, int cout) {
     const int64_t P = (int64_t)n * d * h * w;
-    const int64_t nch = cdiv(P, kWgChunk);
+    const int64_t nch = cdiv(P, wg_chunk(P));
     return (size_t)(nch * 27LL * cout * cin + nch * cout) * sizeof(float);
 }
 
@@ -511,31 +586,30 @@ asets_status asets_nn_conv3_wgrad(const void *in, int n, int d, int h, int w, in
                                   const void *gout, int cout, float *dw, float *dbias,
                                   int accumulate, void *scratch, size_t scratch_bytes,
                                   void *stream) {
-    if (cin % 8 || cout % 16 || (cin > 8 && cin % 16))
-        return nn_fail(ASETS_EBADARG, "cout must be a multiple of 16, cin 8 or a multiple of 16");
+    if (cin % 8 || cout % 16 || (cin > 8 && cin % 16) || cout > kWgCoMax || cin > kWgCiMax)
+        return nn_fail(ASETS_EBADARG, "cout must be a multiple of 16 up to 64, cin 8 or a "
+                                      "multiple of 16 up to 128");
     asets_status st = check_geo(n, d, h, w, cin > cout ? cin : cout);
     if (st) return st;
     if (!scratch || scratch_bytes < asets_nn_wgrad_scratch_bytes(n, d, h, w, cin, cout))
         return nn_fail(ASETS_EBADARG, "scratch too small");
     const Geo5 g{n, d, h, w};
     const int P = n * d * h * w;
-    const int nch = (int)cdiv(P, kWgChunk);
+    const int chunk = wg_chunk(P);
+    const int nch = (int)cdiv(P, chunk);
     hipStream_t s = (hipStream_t)stream;
     float *part = (float *)scratch;
-    if (cin == 8) {
-        dim3 grid(27, cout / 16, nch);
-        k_conv3_wgrad<8><<<grid, 128, 0, s>>>((const u16 *)in, g, cin, (const u16 *)gout, cout,
-                                              part);
-    } else {
-        dim3 grid(27, (cout / 16) * (cin / 16), nch);
-        k_conv3_wgrad<16><<<grid, 256, 0, s>>>((const u16 *)in, g, cin, (const u16 *)gout, cout,
-                                               part);
-    }
+    if ((cout / 16) * ((cin < 16 ? 16 : cin) / 16) <= 8)
+        k_conv3_wgrad<3, 6><<<dim3(9, nch), 256, 0, s>>>((const u16 *)in, g, cin,
+                                                         (const u16 *)gout, cout, chunk, part);
+    else
+        k_conv3_wgrad<1, 8><<<dim3(27, nch), 256, 0, s>>>((const u16 *)in, g, cin,
+                                                          (const u16 *)gout, cout, chunk, part);
     const int nw = 27 * cout * cin;
-    k_sum_chunks<<<nblk(nw), kB, 0, s>>>(part, nch, nw, dw, accumulate);
+    k_sum_chunks<<<(unsigned)cdiv(nw, 64), 64, 0, s>>>(part, nch, nw, dw, accumulate);
     if (dbias) {
         float *bp = part + (size_t)nch * nw;
-        k_bias_part<<<nch, 128, 0, s>>>((const u16 *)gout, P, cout, bp);
+        k_bias_part<<<nch, 256, 0, s>>>((const u16 *)gout, P, cout, chunk, bp);
         k_sum_chunks<<<1, 128, 0, s>>>(bp, nch, cout, dbias, accumulate);
     }
     NNCHK();

@@ -49,8 +49,9 @@ asets_status asets_nn_relu_mask(void *g, const void *y, int64_t n, void *stream)
 
 /* Weight and bias gradient of asets_nn_conv3: dw [27][cout][cin] fp32,
  * dbias [cout] fp32 (may be NULL); accumulate != 0 adds to them.  cout a
- * multiple of 16; cin 8 or a multiple of 16.  scratch: device memory of asets_nn_wgrad_scratch_bytes().
- * Fixed reduction order (deterministic). */
+ * multiple of 16 up to 64; cin 8 or a multiple of 16 up to 128.  scratch: device memory of
+ * asets_nn_wgrad_scratch_bytes().  Computed on the matrix cores (bf16 products, fp32
+ * sums) per tap and voxel chunk; the chunks are added in a fixed order (deterministic). */
 size_t asets_nn_wgrad_scratch_bytes(int n, int d, int h, int w, int cin, int cout);
 asets_status asets_nn_conv3_wgrad(const void *in, int n, int d, int h, int w, int cin,
                                   const void *gout, int cout, float *dw, float *dbias,

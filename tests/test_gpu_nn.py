@@ -105,6 +105,13 @@ def test_conv3_backward(cuda, geo, cin, cout):
     _assert_bf16_close(_f(gx), NO.from_ncdhw(X.grad).numpy())
 
 
+@pytest.mark.parametrize("cin,cout", [(8, 16), (48, 16), (128, 64)])
+def test_conv3_backward_many_chunks(cuda, cin, cout):
+    """17,760 voxels: the weight gradient runs over 18 voxel chunks (the last one and its
+    last 32-voxel slab partly filled) whose partials are added in order."""
+    test_conv3_backward(cuda, (1, 20, 24, 37), cin, cout)
+
+
 def test_relu_mask(cuda):
     rng = np.random.default_rng(3)
     g = NO.bf16_round(rng.normal(0, 1, 1000))
