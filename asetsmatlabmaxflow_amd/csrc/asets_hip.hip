@@ -462,6 +462,26 @@ __global__ void k_blk_init(const float *Cs, const float *Ct, const float *u0, co
     Io<T>::put((T *)(blk + BL::kBZ) + ln, bz);
 }
 
+// Initial block-interleaved Potts flows (PottsBlkLayout) of every label: pt = ps, the
+// initial source flow (asetsPotts3D_mex.c:219-258), and faces 0; padding lanes 0.  One
+// thread per lane of a block of a label; st holds L labels of nz planes of pb bytes.
+__global__ void k_init_potts_blk(const float *ps, char *st, int ny, int nx, int nz, int nyb,
+                                 int pb, int L) {
+    const int64_t per = (int64_t)nz * nx * nyb * 64;   // lanes per label
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per * L) return;
+    const int l = (int)(t / per);
+    const int rem = (int)(t % per);
+    const int b = rem >> 6, ln = rem & 63;
+    const int yb = b % nyb, xz = b / nyb;
+    const int y = yb * 64 + ln;
+    float *blk = (float *)(st + (size_t)l * nz * pb + (size_t)b * PottsBlkLayout::kBytes);
+    blk[ln] = (y < ny) ? ps[(int64_t)xz * ny + y] : 0.0f;   // pt
+    blk[64 + ln] = 0.0f;                                   // bx
+    blk[128 + ln] = 0.0f;                                  // by
+    blk[192 + ln] = 0.0f;                                  // bz
+}
+
 // u and, where requested, the flows of a block-interleaved state to planar fp32
 // arrays (own-face convention), one thread per lane of a block.
 template <typename T>
@@ -633,20 +653,22 @@ asets_status run_fused(const FusedRun &r, Carver &cv, int32_t *hflag) {
 }
 
 template <bool IS3D, int TX, int NT>
-void launch_potts_flow_t(const PottsArgs &p, int grid, hipStream_t s) {
+void launch_potts_flow_t(const PottsArgs &p, int grid, hipStream_t s, bool blk) {
     constexpr int W = (NT >= 512) ? 4 : 3;
-    kf_potts_flow<IS3D, TX, NT, W><<<grid, NT, 0, s>>>(p);
+    if (blk) kf_potts_flow<IS3D, TX, NT, W, true><<<grid, NT, 0, s>>>(p);
+    else kf_potts_flow<IS3D, TX, NT, W, false><<<grid, NT, 0, s>>>(p);
 }
 
-void launch_potts_flow(bool is3d, int tx, const PottsArgs &p, int grid, hipStream_t s) {
+void launch_potts_flow(bool is3d, int tx, const PottsArgs &p, int grid, hipStream_t s,
+                       bool blk) {
     if (is3d) {
-        if (tx == 8) launch_potts_flow_t<true, 8, 256>(p, grid, s);
-        else if (tx == 32) launch_potts_flow_t<true, 32, 1024>(p, grid, s);
-        else launch_potts_flow_t<true, 16, 512>(p, grid, s);
+        if (tx == 8) launch_potts_flow_t<true, 8, 256>(p, grid, s, blk);
+        else if (tx == 32) launch_potts_flow_t<true, 32, 1024>(p, grid, s, blk);
+        else launch_potts_flow_t<true, 16, 512>(p, grid, s, blk);
     } else {
-        if (tx == 8) launch_potts_flow_t<false, 8, 256>(p, grid, s);
-        else if (tx == 32) launch_potts_flow_t<false, 32, 1024>(p, grid, s);
-        else launch_potts_flow_t<false, 16, 512>(p, grid, s);
+        if (tx == 8) launch_potts_flow_t<false, 8, 256>(p, grid, s, blk);
+        else if (tx == 32) launch_potts_flow_t<false, 32, 1024>(p, grid, s, blk);
+        else launch_potts_flow_t<false, 16, 512>(p, grid, s, blk);
     }
 }
 
@@ -925,9 +947,15 @@ asets_status potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny
     const Geo G{ny, nx, nz, N, L};
     const int32_t max_iter = p->max_iter;
     const int64_t nparts = cdiv(N, kBlock);
+    // fused: each label's flows block-interleaved (PottsBlkLayout) unless a label's
+    // buffer plus four planes would reach 2^31 bytes (then planar fields)
+    const int nyb = (int)cdiv(ny, 64);
+    const int64_t ppb = nx * nyb * (int64_t)PottsBlkLayout::kBytes;
+    const bool blk = fused && (nz + 4) * ppb < ((int64_t)1 << 31);
     size_t need = Carver::sz(1, sizeof(Flags)) + Carver::sz(nparts, sizeof(double)) +
-                  Carver::sz(1, sizeof(int32_t)) + Carver::sz(N, 4) +
-                  (fused ? 9 : 6) * Carver::sz(NL, 4);
+                  Carver::sz(1, sizeof(int32_t)) + Carver::sz(N, 4);
+    if (blk) need += 2 * Carver::sz(L * nz * ppb, 1) + Carver::sz(NL, 4);
+    else need += (fused ? 9 : 6) * Carver::sz(NL, 4);
     st = ensure(&dc.dev, &dc.dev_bytes, need);
     if (st) return st;
     Carver cv(dc.dev);
@@ -988,14 +1016,28 @@ asets_status potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny
     } else {
         // label-parallel fused flow kernel + per-voxel coupling kernel; u is
         // updated in place in the caller's buffer, pt and faces ping-pong
-        for (int b = 0; b < 2; ++b) {
-            ptb[b] = cv.take<float>(NL);
-            for (int k = 0; k < 3; ++k) bb[b][k] = cv.take<float>(NL);
+        if (blk) {   // one block-interleaved array per buffer, every label's flows
+            for (int b = 0; b < 2; ++b) {
+                ptb[b] = (float *)cv.take<char>(L * nz * ppb);
+                for (int k = 0; k < 3; ++k) bb[b][k] = ptb[b];
+            }
+        } else {
+            for (int b = 0; b < 2; ++b) {
+                ptb[b] = cv.take<float>(NL);
+                for (int k = 0; k < 3; ++k) bb[b][k] = cv.take<float>(NL);
+            }
         }
         float *ft = cv.take<float>(NL);
-        for (int b = 0; b < 2; ++b)
-            for (int k = 0; k < 3; ++k) fill(bb[b][k], 0.0f, NL, s);
-        k_init_potts<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Ct, u, ps, ptb[0], N, L);
+        if (blk) {
+            k_init_potts<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Ct, u, ps, nullptr, N, L);
+            // buffer 1 needs none: the first flow launch writes every lane of it
+            k_init_potts_blk<<<(unsigned)cdiv(L * nz * nx * nyb * 64, kBlock), kBlock, 0, s>>>(
+                ps, (char *)ptb[0], (int)ny, (int)nx, (int)nz, nyb, (int)ppb, (int)L);
+        } else {
+            for (int b = 0; b < 2; ++b)
+                for (int k = 0; k < 3; ++k) fill(bb[b][k], 0.0f, NL, s);
+            k_init_potts<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Ct, u, ps, ptb[0], N, L);
+        }
         tc.tx = (o && o->tile_x > 0) ? o->tile_x : 16;
         if (tc.tx != 8 && tc.tx != 32) tc.tx = 16;
         const int64_t tiles = cdiv(ny, 64) * cdiv(nx, tc.tx) * L;
@@ -1019,6 +1061,8 @@ asets_status potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny
         pa.nblocks = pa.ntile_y * pa.ntile_x * pa.nzc;
         pa.fast_cc = (std::fabs(p->cc) >= 0x1p-16f && std::fabs(p->cc) <= 0x1p16f) ? 1 : 0;
         pa.flags = flags;
+        pa.nyb = nyb;
+        pa.pb = (int)ppb;
         grid = (int)cdiv((int64_t)pa.nblocks * L, 8) * 8;
         // errBound 0 (a fixed iteration count): iteration it's stop test runs in
         // the first workgroup of the next flow launch, and k_finalize only after
@@ -1035,7 +1079,7 @@ asets_status potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny
             if (deferred && it > 0)
                 pa.stop = DeferredStop{partials, nparts, (double)NL, it - 1, p->err_bound, cvg,
                                        it_num, flags};
-            launch_potts_flow(is3d, tc.tx, pa, grid, s);
+            launch_potts_flow(is3d, tc.tx, pa, grid, s, blk);
             launch_potts_couple((int)L, pa.ft, u, ps, N, p->cc, pa.fast_cc, partials, flags, s);
             if (!deferred || it == max_iter - 1)
                 k_finalize<<<1, 1024, 0, s>>>(partials, nparts, (double)NL, it, p->err_bound,

@@ -253,6 +253,14 @@ struct BlkLayout {
     static constexpr int kBytes = 256 + 320 * E; // block bytes (fp32 1536, fp16 896)
 };
 
+// The fused Potts solver (kf_potts_flow) keeps each label's flows the same way:
+// block (x, z, yb) of label l holds pt, bx, by and bz of its 64 voxels, 1 KiB,
+// with the label's blocks one contiguous run; u, ps, Ct and alpha stay planar.
+struct PottsBlkLayout {
+    static constexpr int kPT = 0, kBX = 256, kBY = 512, kBZ = 768;
+    static constexpr int kBytes = 1024;
+};
+
 // raw buffer access at a byte offset (unsigned: negative planes wrap above the buffer)
 __device__ __forceinline__ float ldb_f32(__amdgpu_buffer_rsrc_t r, unsigned off) {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
@@ -434,14 +442,17 @@ __device__ __forceinline__ float rescale(float g_hi, float g_lo, float b, bool t
 // 0 -- formed in registers from Cs and Ct instead of loaded, and the alpha
 // values it loads are checked (> 0, asetsBinaryMF3D_mex.c:317-319).
 // STREAM: output stores and the output stage's Cs/Ct loads are non-temporal (kStream).
-// BLK (binary only): the state is block-interleaved (BlkLayout; a.in.u and a.out.u
-// are the two buffers), Cs, Ct and alpha stay planar.
+// BLK: the state is block-interleaved -- binary: BlkLayout, a.in.u and a.out.u the
+// two buffers; Potts: the label's flows in PottsBlkLayout, a.in.pt and a.out.pt the
+// two buffers -- while Cs, Ct, alpha (and for Potts u, ps and ft) stay planar.
 template <bool IS3D, int TX, int NT, typename T, int MODE = 0, bool FIRST = false,
           bool STREAM = IS3D, bool BLK = false>
 __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
     using IO = Io<T>;
     using BL = BlkLayout<T>;
-    static_assert(!BLK || MODE == 0, "the block-interleaved state is the binary model's");
+    static_assert(!BLK || MODE <= 1, "the block-interleaved state is the binary and Potts "
+                                     "models'");
+    using PL = PottsBlkLayout;
     constexpr int CP = STREAM ? kStream : 0;
     constexpr bool POTTS = (MODE == 1), ISHI = (MODE == 2);
     static_assert(!FIRST || MODE == 0, "the cold-start iteration is the binary model's");
@@ -499,7 +510,8 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         int l2h[K2 - KO];
         // BLK: byte offsets within a state plane of the slot's u, its pairs, its bz
         // and the pairs of the voxel at y + 1 (fp32: bz at the u offset; fp16: the
-        // pairs at the u offset)
+        // pairs at the u offset).  Potts: bu2 is the slot's offset for all four of
+        // its fields and byp2 that of the voxel at y + 1.
         unsigned bu2[BLK ? K2 : 1], bp2[BLK ? K2 : 1], bz2[BLK ? K2 : 1], byp2[BLK ? K2 : 1];
         const int l2o = (2 + tid / TY) * W2 + 2 + tid % TY;   // LDS index of slot 0
         uint64_t fl2 = 0;
@@ -524,7 +536,12 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
             }
             const int y = y0 - 2 + i, x = x0 - 2 + j;
             off2[k] = x * ny + y;
-            if constexpr (BLK) {
+            if constexpr (BLK && POTTS) {
+                const int yy = y + 64, yb = yy / 64 - 1, ln = yy % 64;   // y >= -2
+                const unsigned ob = (unsigned)((x * a.nyb + yb) * PL::kBytes);
+                bu2[k] = ob + 4u * (unsigned)ln;
+                byp2[k] = (ln < 63) ? bu2[k] + 4u : ob + (unsigned)PL::kBytes;
+            } else if constexpr (BLK) {
                 const int yy = y + 64, yb = yy / 64 - 1, ln = yy % 64;   // y >= -2
                 const unsigned ob = (unsigned)((x * a.nyb + yb) * BL::kBytes);
                 bu2[k] = ob + 4u * (unsigned)ln;
@@ -599,6 +616,32 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     r.u[k] = (cs - ct >= 0.0f) ? 1.0f : 0.0f;
                     r.ps[k] = r.pt[k] = IO::stored(fmin_ref(cs, ct));
                     r.bxl[k] = r.bxr[k] = r.byl[k] = r.byr[k] = r.bzn[k] = 0.0f;
+                }
+                const PRef ra = pref(a.alpha, q - 1, nz, 4, plane);
+#pragma unroll
+                for (int k = 0; k < K1; ++k) r.al[k] = ldf(ra, off1[k]);
+                return;
+            }
+            if constexpr (BLK && POTTS) {
+                const __amdgpu_buffer_rsrc_t rs = blk_rsrc(a.in.pt, nz, a.pb);
+                const unsigned P = (unsigned)q * (unsigned)a.pb;
+                const unsigned XR = (unsigned)(a.nyb * PL::kBytes);   // the next column
+                const PRef ru = pref(a.in.u, q, nz, 4, plane);
+                const PRef rps = pref(a.in.ps, q, nz, 4, plane);
+#pragma unroll
+                for (int k = 0; k < K2; ++k) {
+                    r.u[k] = ldf(ru, off2[k]);
+                    r.ps[k] = ldf(rps, off2[k]);
+                    r.pt[k] = ldb_f32(rs, P + bu2[k] + PL::kPT);
+                    r.bxl[k] = ldb_f32(rs, P + bu2[k] + PL::kBX);
+                    r.bxr[k] = ldb_f32(rs, P + XR + bu2[k] + PL::kBX);
+                    r.byl[k] = ldb_f32(rs, P + bu2[k] + PL::kBY);
+                    r.byr[k] = ldb_f32(rs, P + byp2[k] + PL::kBY);
+                }
+                if (IS3D) {
+#pragma unroll
+                    for (int k = 0; k < K2; ++k)
+                        r.bzn[k] = ldb_f32(rs, P + (unsigned)a.pb + bu2[k] + PL::kBZ);
                 }
                 const PRef ra = pref(a.alpha, q - 1, nz, 4, plane);
 #pragma unroll
@@ -872,6 +915,18 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
                     const float t = (kp.p[k] + uc) - d;   // kp.p holds ps here
                     const float npt = fmin_ref(t, o.ct[k]);
                     st<IO, CP>(wps, off, d + npt);
+                    if constexpr (BLK) {
+                        // every lane of a column inside the volume stores its flows (the
+                        // padding lanes' faces are 0 here); a column past it is dropped
+                        const __amdgpu_buffer_rsrc_t ws = blk_rsrc(a.out.pt, nz, a.pb);
+                        const unsigned ob = (x < nx) ? (unsigned)s * (unsigned)a.pb + bu2[k]
+                                                     : kBlkDrop;
+                        stb_f32<CP>(ws, ob + PL::kPT, npt);
+                        stb_f32<CP>(ws, ob + PL::kBX, bxl);
+                        stb_f32<CP>(ws, ob + PL::kBY, byl);
+                        if (IS3D) stb_f32<CP>(ws, ob + PL::kBZ, bzl);
+                        continue;
+                    }
                     st<IO, CP>(wpt, off, npt);
                     st<IO, CP>(wbx, off, bxl);
                     st<IO, CP>(wby, off, byl);
@@ -956,7 +1011,12 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
         Keep k0, k1, k2;
         if constexpr (IS3D) {
             // bz own of plane z0-2 (raw plane z0-3 would carry it as its "next" bz)
-            if constexpr (!FIRST && BLK) {
+            if constexpr (BLK && POTTS) {
+                const __amdgpu_buffer_rsrc_t rs = blk_rsrc(a.in.pt, nz, a.pb);
+                const unsigned P = (unsigned)(z0 - 2) * (unsigned)a.pb;
+#pragma unroll
+                for (int k = 0; k < K2; ++k) bz_cur[k] = ldb_f32(rs, P + bu2[k] + PL::kBZ);
+            } else if constexpr (!FIRST && BLK) {
                 const __amdgpu_buffer_rsrc_t rs = blk_rsrc(a.in.u, nz, a.pb);
                 const unsigned P = (unsigned)(z0 - 2) * (unsigned)a.pb;
 #pragma unroll
@@ -1093,6 +1153,9 @@ struct PottsArgs {
     int fast_cc;
     const Flags *flags;
     DeferredStop stop;         // the previous iteration's stop test (partials null: none)
+    // block-interleaved flows (PottsBlkLayout; pt_in / pt_out the buffers' bases)
+    int nyb;                   // 64-row blocks per column
+    int pb;                    // bytes per plane of one label's flows
 };
 
 // All labels' flow updates of one iteration in one launch: workgroup b handles
@@ -1104,7 +1167,8 @@ struct PottsArgs {
 // test, when there is one: this launch writes only the other pt and face buffers
 // and ft, so a stop found here leaves that iteration's state whole, and the
 // coupling kernel after it (the only writer of u and ps) returns at once.
-template <bool IS3D, int TX, int NT, int WAVES>
+// BLK: block-interleaved label flows (PottsBlkLayout), else planar fields.
+template <bool IS3D, int TX, int NT, int WAVES, bool BLK = true>
 __global__ __launch_bounds__(NT, WAVES) void kf_potts_flow(PottsArgs p) {
     __shared__ double red[16];
     if (p.flags->done) return;
@@ -1121,6 +1185,15 @@ __global__ __launch_bounds__(NT, WAVES) void kf_potts_flow(PottsArgs p) {
                              p.b_in[1] + off, p.b_in[2] + off};
     a.out = FusedState<float>{nullptr, p.ft + off, p.pt_out + off, p.b_out[0] + off,
                               p.b_out[1] + off, p.b_out[2] + off};
+    if constexpr (BLK) {   // label l's flows start l * nz * pb bytes into each buffer
+        const size_t lb_off = (size_t)l * p.nz * p.pb;
+        a.in.pt = (float *)((char *)p.pt_in + lb_off);
+        a.out.pt = (float *)((char *)p.pt_out + lb_off);
+        a.in.bx = a.in.by = a.in.bz = a.in.pt;
+        a.out.bx = a.out.by = a.out.bz = a.out.pt;
+    }
+    a.nyb = p.nyb;
+    a.pb = p.pb;
     a.ny = p.ny;
     a.nx = p.nx;
     a.nz = p.nz;
@@ -1136,7 +1209,7 @@ __global__ __launch_bounds__(NT, WAVES) void kf_potts_flow(PottsArgs p) {
     a.partials = nullptr;
     a.flags = nullptr;
     a.alpha_bad = a.alpha_stop = nullptr;
-    (void)fused_tile<IS3D, TX, NT, float, 1>(a, lb);
+    (void)fused_tile<IS3D, TX, NT, float, 1, false, IS3D, BLK>(a, lb);
     if (blockIdx.x == 0 && p.stop.partials) deferred_stop(p.stop, red);
 }
 

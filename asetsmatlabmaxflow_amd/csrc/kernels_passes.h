@@ -84,7 +84,7 @@ __global__ void k_init_potts(const float *Ct, float *u, float *ps, float *pt, in
     }
     ps[g] = mv;
     for (int64_t l = 0; l < L; ++l) {
-        pt[g + l * N] = mv;
+        if (pt) pt[g + l * N] = mv;   // null: the caller sets pt itself
         u[g + l * N] = (l == mi) ? 1.0f : 0.0f;
     }
 }

@@ -3,6 +3,9 @@
     python -m asetsmatlabmaxflow_amd.train_warmstart --volumes 256 --size 128 --steps 2000 \
         --batch 4 --out weights.npz
 
+A long run can be split: --checkpoint saves weights, AdamW moments and the step count,
+--resume continues from them, and --total-steps keeps one cosine schedule across parts.
+
 Data: volumes of the 256^3 ellipsoid generator (workloads.binary3d_ellipsoids) at
 --size^3, seeds 2000, 2001, ...; targets (u, pt) are the plain solver's state after
 --target-iters fixed iterations (errBound 0).  Loss: mean (u0-u)^2 + mean ((pt0-pt)/10)^2.
@@ -61,28 +64,34 @@ def make_dataset(nvol: int, size: int, seed0: int, target_iters: int, device, wo
 
 
 def train(net, data, size: int, nvol: int, steps: int, batch: int, lr: float, seed: int,
-          log_every: int = 50, log=print):
+          log_every: int = 50, log=print, total_steps: int | None = None):
+    """Run `steps` AdamW steps from the network's own step count (0, or where a
+    checkpoint left it) on a cosine schedule over `total_steps` (default: steps).  Each
+    step takes `batch` volumes drawn at random (batch groups of consecutive volumes)."""
     import torch
 
     N = size ** 3
     shape = (size, size, size)
     ngroups = nvol // batch
-    rng = np.random.default_rng(seed)
+    step0 = net.step_count
+    total = total_steps or (step0 + steps)
+    rng = np.random.default_rng([seed, step0])
     hist = []
     t0 = time.time()
-    for s in range(steps):
+    for i in range(steps):
+        s = step0 + i
         g = int(rng.integers(ngroups))
         sl = slice(g * batch * N, (g + 1) * batch * N)
         Cs, Ct, a = data["Cs"][sl], data["Ct"][sl], data["alpha"][sl]
         u0, pt0 = net.forward(Cs, Ct, a, shape, batch, train=True)
         loss = net.backward(Cs, Ct, u0, pt0, data["tu"][sl], data["tpt"][sl], shape, batch)
-        lr_s = lr * (0.1 + 0.9 * 0.5 * (1 + math.cos(math.pi * s / max(1, steps))))
+        lr_s = lr * (0.1 + 0.9 * 0.5 * (1 + math.cos(math.pi * s / max(1, total))))
         net.adamw_step(lr=lr_s)
-        if (s + 1) % log_every == 0 or s == 0:
+        if (i + 1) % log_every == 0 or i == 0:
             lv = float(loss)
             hist.append((s + 1, lv))
-            log(f"step {s + 1}/{steps} loss {lv:.5f} lr {lr_s:.2e} "
-                f"{(time.time() - t0) / (s + 1) * 1e3:.1f} ms/step")
+            log(f"step {s + 1}/{total} loss {lv:.5f} lr {lr_s:.2e} "
+                f"{(time.time() - t0) / (i + 1) * 1e3:.1f} ms/step")
     torch.cuda.synchronize()
     return hist, time.time() - t0
 
@@ -100,6 +109,11 @@ def main():
     ap.add_argument("--workers", type=int, default=min(8, os.cpu_count() or 1))
     ap.add_argument("--log-every", type=int, default=50)
     ap.add_argument("--init", default=None, help="start from these weights (.npz)")
+    ap.add_argument("--resume", default=None,
+                    help="continue from this checkpoint (weights, AdamW moments, step count)")
+    ap.add_argument("--total-steps", type=int, default=0,
+                    help="length of the cosine schedule (default: this run's last step)")
+    ap.add_argument("--checkpoint", default=None, help="write a checkpoint here at the end")
     ap.add_argument("--out", required=True)
     args = ap.parse_args()
 
@@ -115,9 +129,14 @@ def main():
     net = WarmStartNet(dev, seed=args.seed)
     if args.init:
         net.load(args.init)
+    if args.resume:
+        net.load_checkpoint(args.resume)
     hist, train_s = train(net, data, args.size, args.volumes, args.steps, args.batch, args.lr,
-                          args.seed, args.log_every, log=lambda m: print(m, flush=True))
+                          args.seed, args.log_every, log=lambda m: print(m, flush=True),
+                          total_steps=args.total_steps or None)
     net.save(args.out)
+    if args.checkpoint:
+        net.save_checkpoint(args.checkpoint)
     print(json.dumps({"volumes": args.volumes, "size": args.size, "steps": args.steps,
                       "batch": args.batch, "lr": args.lr, "target_iters": args.target_iters,
                       "dataset_s": data_s, "train_s": train_s,

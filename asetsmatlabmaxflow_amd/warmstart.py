@@ -175,6 +175,22 @@ class WarmStartNet:
     def save(self, path) -> None:
         np.savez(path, **self.state_dict())
 
+    def save_checkpoint(self, path) -> None:
+        """Weights, AdamW moments and step count, to continue a training run exactly."""
+        np.savez(path, w32=self.w32.cpu().numpy(), m=self.m.cpu().numpy(),
+                 v=self.v.cpu().numpy(), step=np.array([self.step_count], np.int64))
+
+    def load_checkpoint(self, path) -> None:
+        with np.load(path, allow_pickle=False) as z:
+            w32, m, v = z["w32"], z["m"], z["v"]
+            if w32.size != self.nparam or m.size != self.nparam or v.size != self.nparam:
+                raise ValueError(f"checkpoint holds {w32.size} parameters, expected "
+                                 f"{self.nparam}")
+            self.load_flat(w32)
+            self.m.copy_(torch.from_numpy(np.ascontiguousarray(m, np.float32)))
+            self.v.copy_(torch.from_numpy(np.ascontiguousarray(v, np.float32)))
+            self.step_count = int(z["step"][0])
+
     def load(self, path) -> None:
         with np.load(path, allow_pickle=False) as z:
             host = np.zeros(self.nparam, dtype=np.float32)
