@@ -74,18 +74,28 @@ struct Geo5 {
 // (and column l%16 of B); it receives rows 4*(l/16)+i, i < 4, of column l%16.
 // cin is a multiple of 8, so the 8 K indices of a lane share one tap and are
 // one 16-byte load.
+//
+// UPCAT: the input is the channel concatenation [up2(in), skip] of a decoder
+// stage, gathered without materialising it: channels ci < ca read the coarse
+// tensor `in` (ca channels, geometry g halved) at the nearest coarse voxel,
+// channels ci >= ca read `skip` (cin - ca channels) at the voxel itself.  ca is a
+// multiple of 8, so a lane's 8 channels come from one source, and the values are
+// those asets_nn_up2 + asets_nn_chan_copy would have stored: same bits.
 // ---------------------------------------------------------------------------
-template <int COUT>
+template <int COUT, bool UPCAT = false>
 __global__ __launch_bounds__(256) void k_conv3_fwd(const u16 *__restrict__ in, Geo5 g, int cin,
                                                    const u16 *__restrict__ w,
                                                    const float *__restrict__ bias, int relu,
-                                                   u16 *__restrict__ out) {
+                                                   u16 *__restrict__ out,
+                                                   const u16 *__restrict__ skip = nullptr,
+                                                   int ca = 0) {
     constexpr int MT = 4, NT = COUT / 16;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int P = g.n * g.d * g.h * g.w;
     const int p0 = ((int)blockIdx.x * 4 + wv) * 64;
     const int r = lane & 15, kg = lane >> 4;
-    int pz[MT], py[MT], px[MT], pb[MT];
+    const int ch = g.h >> 1, cw = g.w >> 1, cs = cin - ca;   // coarse extents (UPCAT)
+    int pz[MT], py[MT], px[MT], pb[MT], pc[MT];
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
         int p = p0 + m * 16 + r;
@@ -97,6 +107,7 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(const u16 *__restrict__ in, G
         t /= g.h;
         const int z = t % g.d;
         pb[m] = p - (z * g.h + py[m]) * g.w - px[m];   // voxel index of (n, 0, 0, 0)
+        pc[m] = (t / g.d) * (g.d >> 1) * ch * cw;       // coarse voxel index of (n, 0, 0, 0)
         pz[m] = v ? z : -1000;   // an invalid row never passes the bounds test
     }
     f32x4 acc[MT][NT];
@@ -118,8 +129,18 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(const u16 *__restrict__ in, G
                             (unsigned)x < (unsigned)g.w;
             uint4 v = make_uint4(0, 0, 0, 0);
             if (ok) {
-                const int q = pb[m] + (z * g.h + y) * g.w + x;
-                v = *(const uint4 *)(in + (size_t)q * cin + ci);
+                if constexpr (UPCAT) {
+                    if (ci < ca) {
+                        const int q = pc[m] + ((z >> 1) * ch + (y >> 1)) * cw + (x >> 1);
+                        v = *(const uint4 *)(in + (size_t)q * ca + ci);
+                    } else {
+                        const int q = pb[m] + (z * g.h + y) * g.w + x;
+                        v = *(const uint4 *)(skip + (size_t)q * cs + (ci - ca));
+                    }
+                } else {
+                    const int q = pb[m] + (z * g.h + y) * g.w + x;
+                    v = *(const uint4 *)(in + (size_t)q * cin + ci);
+                }
             }
             a[m] = __builtin_bit_cast(bf16x8, v);
         }
@@ -554,6 +575,31 @@ asets_status asets_nn_conv3(const void *in, int n, int d, int h, int w, int cin,
     case 96: k_conv3_fwd<96><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y); break;
     case 128: k_conv3_fwd<128><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y); break;
     default: return nn_fail(ASETS_EBADARG, "cout must be 16, 32, 48, 64, 96 or 128");
+    }
+    NNCHK();
+    return ASETS_OK;
+}
+
+asets_status asets_nn_conv3_upcat(const void *coarse, int ca, const void *skip, int n, int d,
+                                  int h, int w, int cin, const void *wt, const float *bias,
+                                  int cout, int relu, void *out, void *stream) {
+    if (cin % 8 || cin > 512) return nn_fail(ASETS_EBADARG, "cin must be a multiple of 8, <= 512");
+    if (ca <= 0 || ca % 8 || ca >= cin)
+        return nn_fail(ASETS_EBADARG, "ca must be a multiple of 8 in (0, cin)");
+    asets_status st = check_geo(n, d, h, w, cin > cout ? cin : cout);
+    if (st) return st;
+    if (d % 2 || h % 2 || w % 2) return nn_fail(ASETS_EBADARG, "upsample needs even extents");
+    if (!coarse || !skip || !wt || !out) return nn_fail(ASETS_EBADARG, "null pointer");
+    const Geo5 g{n, d, h, w};
+    const unsigned grid = (unsigned)cdiv((int64_t)n * d * h * w, 256);
+    hipStream_t s = (hipStream_t)stream;
+    const u16 *x = (const u16 *)coarse, *k = (const u16 *)skip, *wb = (const u16 *)wt;
+    u16 *y = (u16 *)out;
+    switch (cout) {
+    case 16: k_conv3_fwd<16, true><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y, k, ca); break;
+    case 32: k_conv3_fwd<32, true><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y, k, ca); break;
+    case 64: k_conv3_fwd<64, true><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y, k, ca); break;
+    default: return nn_fail(ASETS_EBADARG, "cout must be 16, 32 or 64");
     }
     NNCHK();
     return ASETS_OK;

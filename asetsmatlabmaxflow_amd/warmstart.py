@@ -47,7 +47,7 @@ HEAD_W, HEAD_B = 32, 2
 
 # every extern "C" symbol declared in include/asets_nn.h
 NN_EXPORTED = (
-    "asets_nn_conv3", "asets_nn_flip_transpose", "asets_nn_relu_mask",
+    "asets_nn_conv3", "asets_nn_conv3_upcat", "asets_nn_flip_transpose", "asets_nn_relu_mask",
     "asets_nn_wgrad_scratch_bytes", "asets_nn_conv3_wgrad", "asets_nn_pool2",
     "asets_nn_pool2_bwd", "asets_nn_up2", "asets_nn_up2_bwd", "asets_nn_chan_copy",
     "asets_nn_features", "asets_nn_head", "asets_nn_head_scratch_bytes", "asets_nn_head_bwd",
@@ -70,6 +70,7 @@ def nnlib() -> C.CDLL:
         L = C.CDLL(str(NN_LIB))
         P, I, I64, F = C.c_void_p, C.c_int, C.c_int64, C.c_float
         L.asets_nn_conv3.argtypes = [P, I, I, I, I, I, P, P, I, I, P, P]
+        L.asets_nn_conv3_upcat.argtypes = [P, I, P, I, I, I, I, I, P, P, I, I, P, P]
         L.asets_nn_flip_transpose.argtypes = [P, P, I, I, P]
         L.asets_nn_relu_mask.argtypes = [P, P, I64, P]
         L.asets_nn_wgrad_scratch_bytes.argtypes = [I, I, I, I, I, I]
@@ -224,9 +225,11 @@ class WarmStartNet:
         E = lambda l, c: torch.empty(vox[l] * c, dtype=bf, device=self.device)
         b = {"x0": E(0, 8), "e0a": E(0, 16), "e0": E(0, 16), "p1": E(1, 16), "e1a": E(1, 32),
              "e1": E(1, 32), "p2": E(2, 32), "e2a": E(2, 64), "e2": E(2, 64), "p3": E(3, 64),
-             "e3a": E(3, 64), "e3": E(3, 64), "c2": E(2, 128), "d2": E(2, 64),
-             "c1": E(1, 96), "d1": E(1, 32), "c0": E(0, 48), "d0": E(0, 16)}
+             "e3a": E(3, 64), "e3": E(3, 64), "d2": E(2, 64), "d1": E(1, 32), "d0": E(0, 16)}
         if train:
+            # the backward pass needs the decoder inputs materialised (the weight
+            # gradient reads them); inference gathers them inside the convolution
+            b.update({"c2": E(2, 128), "c1": E(1, 96), "c0": E(0, 48)})
             b.update({"g0_16a": E(0, 16), "g0_16b": E(0, 16), "g0_48": E(0, 48),
                       "g1_32a": E(1, 32), "g1_32b": E(1, 32), "g1_96": E(1, 96),
                       "g1_16": E(1, 16), "g2_64a": E(2, 64), "g2_64b": E(2, 64),
@@ -254,6 +257,21 @@ class WarmStartNet:
                                    _p(self.view(name + ".b")), cout, int(relu), _p(dst),
                                    self._stream()))
 
+    def _dec(self, name, coarse, ca, skip, cat, g, dst, train):
+        """Decoder convolution over [up2(coarse), skip]: materialised into `cat` when
+        training (the backward pass reads it), gathered in the convolution otherwise."""
+        cin, cout = next((ci, co) for nm, ci, co in CONVS if nm == name)
+        L, s = nnlib(), self._stream()
+        if train:
+            _ck(L.asets_nn_up2(_p(coarse), *g, ca, _p(cat), cin, 0, s))
+            _ck(L.asets_nn_chan_copy(_p(cat), cin, ca, _p(skip), cin - ca, 0, cin - ca,
+                                     g[0] * g[1] * g[2] * g[3], 0, s))
+            self._conv(name, cat, g, dst)
+        else:
+            _ck(L.asets_nn_conv3_upcat(_p(coarse), ca, _p(skip), *g, cin,
+                                       _p(self.view(name + ".w", "w16")),
+                                       _p(self.view(name + ".b")), cout, 1, _p(dst), s))
+
     def forward(self, Cs, Ct, alpha, shape, n=1, *, u0=None, pt0=None, train=False):
         """Predict (u0, pt0) for n volumes of `shape` stored back to back in the flat
         float32 device tensors Cs, Ct, alpha (column-major each).  Returns (u0, pt0)."""
@@ -275,15 +293,9 @@ class WarmStartNet:
         _ck(L.asets_nn_pool2(_p(b["e2"]), *G[2], 64, _p(b["p3"]), s))
         self._conv("enc3a", b["p3"], G[3], b["e3a"])
         self._conv("enc3b", b["e3a"], G[3], b["e3"])
-        _ck(L.asets_nn_up2(_p(b["e3"]), *G[2], 64, _p(b["c2"]), 128, 0, s))
-        _ck(L.asets_nn_chan_copy(_p(b["c2"]), 128, 64, _p(b["e2"]), 64, 0, 64, vox[2], 0, s))
-        self._conv("dec2", b["c2"], G[2], b["d2"])
-        _ck(L.asets_nn_up2(_p(b["d2"]), *G[1], 64, _p(b["c1"]), 96, 0, s))
-        _ck(L.asets_nn_chan_copy(_p(b["c1"]), 96, 64, _p(b["e1"]), 32, 0, 32, vox[1], 0, s))
-        self._conv("dec1", b["c1"], G[1], b["d1"])
-        _ck(L.asets_nn_up2(_p(b["d1"]), *G[0], 32, _p(b["c0"]), 48, 0, s))
-        _ck(L.asets_nn_chan_copy(_p(b["c0"]), 48, 32, _p(b["e0"]), 16, 0, 16, vox[0], 0, s))
-        self._conv("dec0", b["c0"], G[0], b["d0"])
+        self._dec("dec2", b["e3"], 64, b["e2"], b.get("c2"), G[2], b["d2"], train)
+        self._dec("dec1", b["d2"], 64, b["e1"], b.get("c1"), G[1], b["d1"], train)
+        self._dec("dec0", b["d1"], 32, b["e0"], b.get("c0"), G[0], b["d0"], train)
         if u0 is None:
             u0 = torch.empty(P, dtype=torch.float32, device=self.device)
         if pt0 is None:

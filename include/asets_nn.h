@@ -38,6 +38,16 @@ asets_status asets_nn_conv3(const void *in, int n, int d, int h, int w, int cin,
                             const void *wt, const float *bias, int cout, int relu, void *out,
                             void *stream);
 
+/* A decoder convolution over the concatenation [up2(coarse), skip] without
+ * materialising it: the same result, bit for bit, as asets_nn_up2(coarse, ...,
+ * ca, cat, cin, 0) + asets_nn_chan_copy(cat, cin, ca, skip, cin - ca, 0, cin - ca,
+ * ...) + asets_nn_conv3(cat, ...).  coarse [n,d/2,h/2,w/2,ca], skip
+ * [n,d,h,w,cin-ca]; d, h, w even; ca a multiple of 8 in (0, cin); cout in
+ * {16,32,64}. */
+asets_status asets_nn_conv3_upcat(const void *coarse, int ca, const void *skip, int n, int d,
+                                  int h, int w, int cin, const void *wt, const float *bias,
+                                  int cout, int relu, void *out, void *stream);
+
 /* Weights of the input-gradient convolution: wt [27][cout][cin] ->
  * wt_bwd [27][cin][cout] with taps mirrored (k -> 26-k).  Then
  * asets_nn_conv3(gout, ..., cout, wt_bwd, NULL, cin, 0, gin) is d loss / d in. */

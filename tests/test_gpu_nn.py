@@ -169,6 +169,44 @@ def test_pool_and_upsample(cuda, geo, c):
     _assert_bf16_close(_f(gxc), NO.from_ncdhw(XC.grad).numpy())
 
 
+@pytest.mark.parametrize("geo,ca,cs,cout", [((2, 4, 6, 8), 64, 64, 64), ((1, 6, 4, 10), 64, 32, 32),
+                                           ((2, 2, 8, 6), 32, 16, 16), ((1, 8, 8, 8), 16, 8, 16)])
+def test_conv3_upcat_equals_materialised(cuda, geo, ca, cs, cout):
+    """The decoder convolution gathering [up2(coarse), skip] in place gives the same bits
+    as upsample + channel copy + convolution."""
+    n, d, h, w = geo
+    P, Pc, cin = n * d * h * w, n * d * h * w // 8, ca + cs
+    rng = np.random.default_rng(P + ca + cs)
+    coarse = _bf(NO.bf16_round(rng.normal(0, 1, (Pc, ca))), cuda)
+    skip = _bf(NO.bf16_round(rng.normal(0, 1, (P, cs))), cuda)
+    wt = _bf(NO.bf16_round(rng.normal(0, 0.2, 27 * cout * cin)), cuda)
+    b = torch.from_numpy(rng.normal(0, 0.5, cout).astype(np.float32)).to(cuda)
+    L = W.nnlib()
+    cat = torch.empty(P * cin, dtype=torch.bfloat16, device=cuda)
+    ref = torch.empty(P * cout, dtype=torch.bfloat16, device=cuda)
+    out = torch.full((P * cout,), 7.0, dtype=torch.bfloat16, device=cuda)
+    W._ck(L.asets_nn_up2(_p(coarse), n, d, h, w, ca, _p(cat), cin, 0, None))
+    W._ck(L.asets_nn_chan_copy(_p(cat), cin, ca, _p(skip), cs, 0, cs, P, 0, None))
+    W._ck(L.asets_nn_conv3(_p(cat), n, d, h, w, cin, _p(wt), _p(b), cout, 1, _p(ref), None))
+    W._ck(L.asets_nn_conv3_upcat(_p(coarse), ca, _p(skip), n, d, h, w, cin, _p(wt), _p(b),
+                                 cout, 1, _p(out), None))
+    torch.cuda.synchronize()
+    assert torch.equal(out.view(torch.int16), ref.view(torch.int16))
+
+
+def test_inference_forward_equals_training_forward(cuda):
+    """The inference forward (decoder inputs gathered in the convolution) and the training
+    forward (decoder inputs materialised) predict the same bits."""
+    shape, n = (16, 24, 8), 2
+    Cs, Ct, A, _, _ = _net_inputs(shape, n, 5)
+    net = W.WarmStartNet(cuda, seed=4)
+    d = [torch.from_numpy(x).to(cuda) for x in (Cs, Ct, A)]
+    u1, p1 = (t.clone() for t in net.forward(d[0], d[1], d[2], shape, n, train=True))
+    u2, p2 = net.forward(d[0], d[1], d[2], shape, n, train=False)
+    torch.cuda.synchronize()
+    assert torch.equal(u1, u2) and torch.equal(p1, p2)
+
+
 def test_chan_copy_and_features(cuda):
     rng = np.random.default_rng(11)
     L = W.nnlib()

@@ -238,6 +238,21 @@ def test_mex_errors_and_edges(cuda):
     assert int(it[0, 0]) == 10
 
 
+def test_fused_planar_fallback(cuda):
+    """A volume whose block-interleaved state would pass 2^31 bytes per buffer runs the
+    same fused kernel on planar fields: a 4352 x 4120 plane ((1 + 4) planes x 4120 columns
+    x 68 blocks x 1536 B > 2^31), 5 iterations, bit for bit with the oracle."""
+    shape = (4352, 4120)
+    nyb = -(-shape[0] // 64)
+    assert 5 * shape[1] * nyb * 1536 >= 2 ** 31
+    print(f"planar fallback {shape}", flush=True)
+    Cs, Ct, a = workloads.random_smooth_binary(shape, seed=41, alpha=0.25)
+    ref = orc.binary(Cs, Ct, a, shape, 5, 0.0, 0.2, 0.16)
+    r = _run_binary(Cs, Ct, a, shape, 5, 0.0, 0.2, 0.16, kernel="fused")
+    u = r.u.cpu().numpy()
+    assert _bits_equal(u, ref.u), _report(u, ref.u)
+
+
 def test_fused_equals_passes_full_size(cuda):
     """256^3 workload: the fused kernel and the per-pass kernels agree bit for bit over
     50 iterations (two independent GPU implementations of the same operation order)."""
