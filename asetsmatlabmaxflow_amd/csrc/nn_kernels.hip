@@ -18,6 +18,8 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "asets_nn.h"
@@ -170,6 +172,367 @@ __global__ __launch_bounds__(256) void k_conv3_fwd(const u16 *__restrict__ in, G
                     out[(size_t)p * COUT + co] = f2bf(v);
                 }
         }
+}
+
+// ---------------------------------------------------------------------------
+// The same convolution with its operands staged in LDS, for rows of at least 64
+// voxels.  A box is 64 (w) x 2 (h) x 2 (d) voxels, one 64-voxel row per wave (4
+// MFMA row tiles, all COUT as in k_conv3_fwd).  The box with its one-voxel halo,
+// 66 x 4 x 4 voxels (zero outside the volume), is copied to LDS once per chunk of
+// CC input channels; every tap then reads its A operand from there instead of
+// gathering it from global memory.  The chunk sits in LDS channel-group major
+// ([group][voxel], 16 bytes = 8 channels per entry), so the 16 rows of an A
+// fragment are 16 consecutive entries.  K runs over taps and channels within a
+// chunk, chunk after chunk: for cin <= 16 that is k_conv3_fwd's order, the same
+// bits.  Outputs go through LDS and leave as 16 contiguous bytes per lane.
+//
+// Such a kernel is bound by instruction issue unless its index arithmetic stays
+// out of the per-box path, so what does not depend on the box is computed once
+// per thread: the staging slots' offsets from the box origin and their boundary
+// classes (a box whose faces all lie inside the volume needs no bounds test; at
+// a face, one mask test per slot) and each K step's LDS offset (the K loop is
+// unrolled at compile time).  Volumes whose extents are not multiples of the box
+// take a per-slot bounds test instead.
+//
+// k_conv3_box (cin = CC = 8 or 16, one chunk): the weights of the whole layer are
+// staged in LDS, in MFMA fragment order, once per workgroup; the grid is
+// persistent, and a workgroup loads its next box into registers while it
+// computes the current one (its compute loop issues no global load, which the
+// in-order load counter would make wait for that prefetch).
+// k_conv3_boxm (cin a multiple of 16, several chunks, and the decoder's
+// concatenated input): one box per workgroup, chunk after chunk, with the
+// weights read from global memory (cache hits) one K step ahead.  UPCAT stages
+// [up2(in), skip] as k_conv3_fwd<COUT, true> gathers it.
+// ---------------------------------------------------------------------------
+constexpr int kBoxW = 64, kBoxH = 2, kBoxD = 2;
+constexpr int kHaloW = kBoxW + 2, kHaloH = kBoxH + 2, kHaloD = kBoxD + 2;
+constexpr int kHaloVox = kHaloW * kHaloH * kHaloD;   // 1056
+
+// fp32 -> bf16 with the gfx950 conversion instruction (round to nearest even)
+__device__ __forceinline__ u16 f2bf_hw(float f) { return __builtin_bit_cast(u16, (__bf16)f); }
+
+template <int COUT, int CC>
+struct BoxCfg {
+    static constexpr int NT = COUT / 16, NG = CC / 8;
+    static constexpr int KS = (27 * CC + 31) / 32;          // K steps per chunk
+    static constexpr int NA = NG * kHaloVox;                // input entries
+    static constexpr int SI = (NA + 255) / 256;             // input slots per thread
+    static constexpr int NB = KS * NT * 64;                 // weight fragments
+    static constexpr int BI = (NB + 255) / 256;             // weight slots per thread
+    static constexpr int NOUT = 4 * 64 * COUT * 2 / 16;     // output staging (uint4)
+    static constexpr int NAO = NA > NOUT ? NA : NOUT;
+};
+
+template <int COUT, int CC, bool UPCAT>
+__global__ __launch_bounds__(256) void k_conv3_box(const u16 *__restrict__ in, Geo5 g, int cin,
+                                                   const u16 *__restrict__ w,
+                                                   const float *__restrict__ bias, int relu,
+                                                   u16 *__restrict__ out,
+                                                   const u16 *__restrict__ skip, int ca) {
+    using B = BoxCfg<COUT, CC>;
+    constexpr int MT = 4, NT = B::NT, NG = B::NG, KS = B::KS;
+    static_assert(!UPCAT, "one chunk: cin = CC, no concatenation");
+    (void)skip;
+    (void)ca;
+    __shared__ uint4 tile[B::NAO];   // input box; then the outputs of the box
+    __shared__ uint4 btile[B::NB];   // weights (one chunk: the whole layer)
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int r = lane & 15, kg = lane >> 4;
+    const int nbw = (g.w + kBoxW - 1) / kBoxW, nbh = (g.h + kBoxH - 1) / kBoxH;
+    const int nbd = (g.d + kBoxD - 1) / kBoxD;
+    const int nbox = g.n * nbd * nbh * nbw;
+    const int wz = wv >> 1, wy = wv & 1;   // this wave's row within the box
+    const int plane = g.h * g.w, vol = g.d * plane;
+    const bool full = g.w % kBoxW == 0 && g.h % kBoxH == 0 && g.d % kBoxD == 0;
+    if ((int)blockIdx.x >= nbox) return;
+
+    // ---- weights: staged once per workgroup ----------------------------------
+#pragma unroll
+    for (int k = 0; k < B::BI; ++k) {
+        const int j = tid + k * 256, ln = j & 63, t = j >> 6;
+        const int q = t % NT, ks = t / NT;
+        const int kk = ks * 32 + 8 * (ln >> 4);
+        const int tap = kk / CC, grp = (kk % CC) / 8;
+        if (j < B::NB) {
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (tap < 27) v = *(const uint4 *)(w + (tap * COUT + q * 16 + (ln & 15)) * cin + 8 * grp);
+            btile[j] = v;
+        }
+    }
+    // ---- per-thread tables (independent of the box) --------------------------
+    // input slot k: LDS entry (12 bits), halo position, boundary class (bit 0/1:
+    // first/last halo column, 2/3: row, 4/5: plane), valid; voxel offset from the
+    // box origin
+    unsigned spk[B::SI];   // lds | hx << 12 | hy << 19 | hz << 21 | cls << 23 | valid << 29
+    int soff[B::SI];
+#pragma unroll
+    for (int k = 0; k < B::SI; ++k) {
+        const int i = tid + k * 256;
+        const int v = i / NG, grp = i % NG;   // group fastest: coalesced loads
+        const int hx = v % kHaloW, hy = (v / kHaloW) % kHaloH, hz = v / (kHaloW * kHaloH);
+        const unsigned cls = (hx == 0 ? 1u : 0u) | (hx == kHaloW - 1 ? 2u : 0u) |
+                             (hy == 0 ? 4u : 0u) | (hy == kHaloH - 1 ? 8u : 0u) |
+                             (hz == 0 ? 16u : 0u) | (hz == kHaloD - 1 ? 32u : 0u);
+        spk[k] = (unsigned)(grp * kHaloVox + v) | ((unsigned)hx << 12) | ((unsigned)hy << 19) |
+                 ((unsigned)hz << 21) | (cls << 23) | ((i < B::NA ? 1u : 0u) << 29);
+        soff[k] = (((hz - 1) * g.h + (hy - 1)) * g.w + (hx - 1)) * cin + 8 * grp;
+    }
+    // LDS entry of K step ks's A operand for this lane (row tile 0); the last step
+    // has taps past 26 in some lane groups
+    int aofs[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+        const int kk = ks * 32 + 8 * kg;
+        const int tap = min(kk / CC, 26), grp = (kk % CC) / 8;
+        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+        aofs[ks] = grp * kHaloVox + ((wz + 1 + dz) * kHaloH + (wy + 1 + dy)) * kHaloW + 1 + dx + r;
+    }
+    const bool last_ok = (KS - 1) * 32 + 8 * kg < 27 * CC;
+
+    struct Box {
+        int n, x0, y0, z0;
+        unsigned cls;   // boundary classes whose halo positions lie outside the volume
+    };
+    auto box_of = [&](int b) {
+        Box o;
+        const int bw = b % nbw;
+        b /= nbw;
+        const int bh = b % nbh;
+        b /= nbh;
+        const int bd = b % nbd;
+        o.n = b / nbd;
+        o.x0 = bw * kBoxW;
+        o.y0 = bh * kBoxH;
+        o.z0 = bd * kBoxD;
+        o.cls = (bw == 0 ? 1u : 0u) | (bw == nbw - 1 ? 2u : 0u) | (bh == 0 ? 4u : 0u) |
+                (bh == nbh - 1 ? 8u : 0u) | (bd == 0 ? 16u : 0u) | (bd == nbd - 1 ? 32u : 0u);
+        return o;
+    };
+    uint4 sv[B::SI];   // the next box's input
+    auto load_box = [&](const Box &X) {
+        const u16 *src = in + ((size_t)X.n * vol + X.z0 * plane + X.y0 * g.w + X.x0) * cin;
+#pragma unroll
+        for (int k = 0; k < B::SI; ++k) {
+            const unsigned pk = spk[k];
+            bool ok = (pk >> 29) & 1u;
+            if (full) {
+                ok = ok && ((pk >> 23) & X.cls) == 0u;
+            } else {
+                const int x = X.x0 - 1 + (int)((pk >> 12) & 127u);
+                const int y = X.y0 - 1 + (int)((pk >> 19) & 3u);
+                const int z = X.z0 - 1 + (int)((pk >> 21) & 3u);
+                ok = ok && (unsigned)x < (unsigned)g.w && (unsigned)y < (unsigned)g.h &&
+                     (unsigned)z < (unsigned)g.d;
+            }
+            sv[k] = make_uint4(0, 0, 0, 0);
+            if (ok) sv[k] = *(const uint4 *)(src + soff[k]);
+        }
+    };
+
+    int b = blockIdx.x;
+    Box X = box_of(b);
+    load_box(X);
+    float bco[NT];
+#pragma unroll
+    for (int q = 0; q < NT; ++q) bco[q] = bias ? bias[q * 16 + r] : 0.0f;
+    for (;;) {
+        const int nb = b + gridDim.x;
+        const bool more = nb < nbox;
+        __syncthreads();   // the previous box's LDS reads are done (and the weights stored)
+#pragma unroll
+        for (int k = 0; k < B::SI; ++k)
+            if ((spk[k] >> 29) & 1u) tile[spk[k] & 4095u] = sv[k];
+        __syncthreads();
+        const Box Xc = X;
+        if (more) {
+            X = box_of(nb);
+            load_box(X);   // in flight during this box's MFMAs
+        }
+        f32x4 acc[MT][NT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int q = 0; q < NT; ++q) acc[m][q] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            bf16x8 a[MT], bb[NT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                uint4 v = tile[aofs[ks] + 16 * m];
+                if (ks == KS - 1 && !last_ok) v = make_uint4(0, 0, 0, 0);
+                a[m] = __builtin_bit_cast(bf16x8, v);
+            }
+#pragma unroll
+            for (int q = 0; q < NT; ++q)
+                bb[q] = __builtin_bit_cast(bf16x8, btile[(ks * NT + q) * 64 + lane]);
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int q = 0; q < NT; ++q)
+                    acc[m][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], bb[q], acc[m][q], 0, 0, 0);
+        }
+        // outputs: into LDS in the MFMA layout, then 16 contiguous bytes per lane
+        const int z = Xc.z0 + wz, y = Xc.y0 + wy;
+        const bool rowok = z < g.d && y < g.h;
+        const size_t prow = (size_t)Xc.n * vol + (size_t)z * plane + (size_t)y * g.w;
+        __syncthreads();   // every wave has read its last A operand
+        u16 *ot = (u16 *)tile + wv * 64 * COUT;
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int q = 0; q < NT; ++q) {
+                    float v = acc[m][q][i] + bco[q];
+                    if (relu) v = v > 0.0f ? v : 0.0f;
+                    ot[(m * 16 + 4 * kg + i) * COUT + q * 16 + r] = f2bf_hw(v);
+                }
+        __syncthreads();
+        constexpr int G = COUT / 8;   // 16-byte groups per voxel
+#pragma unroll
+        for (int e = lane; e < 64 * G; e += 64) {
+            const int x = Xc.x0 + e / G;
+            if (rowok && x < g.w)
+                *(uint4 *)(out + (prow + x) * COUT + (e % G) * 8) = *(const uint4 *)(ot + e * 8);
+        }
+        if (!more) break;
+        b = nb;
+    }
+}
+
+// Layers of more than one chunk (cin > 16) and the decoder's concatenated input:
+// one box per workgroup, chunk after chunk.  Each chunk's box is staged with all
+// of a thread's loads issued before its first LDS store; the weights come from
+// global memory (cache hits), loaded one K step ahead; outputs go through LDS.
+template <int COUT, bool UPCAT>
+__global__ __launch_bounds__(256) void k_conv3_boxm(const u16 *__restrict__ in, Geo5 g, int cin,
+                                                    const u16 *__restrict__ w,
+                                                    const float *__restrict__ bias, int relu,
+                                                    u16 *__restrict__ out,
+                                                    const u16 *__restrict__ skip, int ca) {
+    using B = BoxCfg<COUT, 16>;
+    constexpr int MT = 4, NT = B::NT, SI = B::SI;
+    __shared__ uint4 tile[B::NAO];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int r = lane & 15, kg = lane >> 4;
+    const int nbw = (g.w + kBoxW - 1) / kBoxW, nbh = (g.h + kBoxH - 1) / kBoxH;
+    const int nbd = (g.d + kBoxD - 1) / kBoxD;
+    int b = blockIdx.x;
+    const int bw = b % nbw;
+    b /= nbw;
+    const int bh = b % nbh;
+    b /= nbh;
+    const int bd = b % nbd;
+    const int n = b / nbd;
+    const int x0 = bw * kBoxW, y0 = bh * kBoxH, z0 = bd * kBoxD;
+    const int wz = wv >> 1, wy = wv & 1;   // this wave's row within the box
+    const int ch = g.h >> 1, cw = g.w >> 1, cs = cin - ca;
+    const int vb = n * g.d * g.h * g.w;       // voxel index of (n, 0, 0, 0)
+    const int vc = n * (g.d >> 1) * ch * cw;  // coarse voxel of (n, 0, 0, 0)
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int q = 0; q < NT; ++q) acc[m][q] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int cb = 0; cb < cin; cb += 16) {
+        // all of a thread's staging loads are issued before the first LDS store
+        uint4 sv[SI];
+        int sidx[SI];
+#pragma unroll
+        for (int k = 0; k < SI; ++k) {
+            const int i = tid + k * 256;
+            const int v = i >> 1, grp = i & 1;   // group fastest: coalesced
+            const int hx = v % kHaloW, t = v / kHaloW;
+            const int hy = t % kHaloH, hz = t / kHaloH;
+            const int x = x0 - 1 + hx, y = y0 - 1 + hy, z = z0 - 1 + hz;
+            sv[k] = make_uint4(0, 0, 0, 0);
+            sidx[k] = i < B::NA ? grp * kHaloVox + v : -1;
+            if (sidx[k] >= 0 && (unsigned)x < (unsigned)g.w && (unsigned)y < (unsigned)g.h &&
+                (unsigned)z < (unsigned)g.d) {
+                const int ci = cb + 8 * grp;
+                if constexpr (UPCAT) {
+                    if (ci < ca) {
+                        const int q = vc + ((z >> 1) * ch + (y >> 1)) * cw + (x >> 1);
+                        sv[k] = *(const uint4 *)(in + (size_t)q * ca + ci);
+                    } else {
+                        const int q = vb + (z * g.h + y) * g.w + x;
+                        sv[k] = *(const uint4 *)(skip + (size_t)q * cs + (ci - ca));
+                    }
+                } else {
+                    const int q = vb + (z * g.h + y) * g.w + x;
+                    sv[k] = *(const uint4 *)(in + (size_t)q * cin + ci);
+                }
+            }
+        }
+        // weights of K step ks (zero past the last tap), loaded one step ahead
+        auto load_b = [&](int ks, bf16x8 *bb) {
+            const int kk = ks * 32 + 8 * kg;
+            const int tap = kk >> 4, grp = (kk & 15) >> 3;
+#pragma unroll
+            for (int q = 0; q < NT; ++q) {
+                uint4 v = make_uint4(0, 0, 0, 0);
+                if (tap < 27)
+                    v = *(const uint4 *)(w + ((size_t)tap * COUT + q * 16 + r) * cin + cb + 8 * grp);
+                bb[q] = __builtin_bit_cast(bf16x8, v);
+            }
+        };
+        bf16x8 bcur[NT], bnext[NT];
+        load_b(0, bcur);
+        __syncthreads();   // the previous chunk's reads are done
+#pragma unroll
+        for (int k = 0; k < SI; ++k)
+            if (sidx[k] >= 0) tile[sidx[k]] = sv[k];
+        __syncthreads();
+#pragma unroll 1
+        for (int ks = 0; ks < B::KS; ++ks) {
+            const int kk = ks * 32 + 8 * kg;
+            const int tap = kk >> 4, grp = (kk & 15) >> 3;
+            const bool tv = tap < 27;
+            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+            const int base = grp * kHaloVox + ((wz + 1 + dz) * kHaloH + (wy + 1 + dy)) * kHaloW +
+                             1 + dx + r;
+            if (ks + 1 < B::KS) load_b(ks + 1, bnext);
+            bf16x8 a[MT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                uint4 v = make_uint4(0, 0, 0, 0);
+                if (tv) v = tile[base + 16 * m];
+                a[m] = __builtin_bit_cast(bf16x8, v);
+            }
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int q = 0; q < NT; ++q)
+                    acc[m][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], bcur[q], acc[m][q], 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < NT; ++q) bcur[q] = bnext[q];
+        }
+    }
+    const int z = z0 + wz, y = y0 + wy;
+    const bool rowok = z < g.d && y < g.h;
+    const int prow = vb + (z * g.h + y) * g.w;
+    // outputs: into LDS in the MFMA layout, then 16 contiguous bytes per lane
+    __syncthreads();   // every wave has read its last A operand
+    u16 *ot = (u16 *)tile + wv * 64 * COUT;
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int q = 0; q < NT; ++q) {
+                const int co = q * 16 + r;
+                float v = acc[m][q][i] + (bias ? bias[co] : 0.0f);
+                if (relu) v = v > 0.0f ? v : 0.0f;
+                ot[(m * 16 + 4 * kg + i) * COUT + co] = f2bf_hw(v);
+            }
+    __syncthreads();
+    constexpr int G = COUT / 8;   // 16-byte groups per voxel
+#pragma unroll
+    for (int e = lane; e < 64 * G; e += 64) {
+        const int x = x0 + e / G;
+        if (rowok && x < g.w)
+            *(uint4 *)(out + (size_t)(prow + x) * COUT + (e % G) * 8) = *(const uint4 *)(ot + e * 8);
+    }
 }
 
 // [27][cout][cin] -> [27][cin][cout] with the taps mirrored (k -> 26-k): the
@@ -542,6 +905,71 @@ constexpr int kB = 256;
 
 inline unsigned nblk(int64_t n) { return (unsigned)cdiv(n, kB); }
 
+// Persistent grid of the LDS-staged kernel: the workgroups resident at once on
+// the current device (occupancy x CUs, queried once per kernel and device), or
+// one per box when there are fewer boxes.
+template <int COUT, int CC>
+unsigned box_grid(const Geo5 &g) {
+    static std::mutex mu;
+    static std::map<int, int> resident;   // device -> workgroups resident at once
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int res;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = resident.find(dev);
+        if (it == resident.end()) {
+            int per_cu = 0, cus = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per_cu, k_conv3_box<COUT, CC, false>, 256, 0) != hipSuccess ||
+                per_cu < 1)
+                per_cu = 1;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                    hipSuccess ||
+                cus < 1)
+                cus = 1;
+            it = resident.emplace(dev, per_cu * cus).first;
+        }
+        res = it->second;
+    }
+    const int64_t nbox = (int64_t)g.n * cdiv(g.d, kBoxD) * cdiv(g.h, kBoxH) * cdiv(g.w, kBoxW);
+    return (unsigned)(nbox < res ? nbox : res);
+}
+
+// Which convolution kernel a layer runs: rows of at least one box width take the
+// LDS-staged kernels -- the persistent one for cin = 8 or 16 and 16 or 32 output
+// channels, the per-box one for cin a multiple of 16 up to 64 output channels
+// (for UPCAT with a coarse channel count that is a multiple of 16).  Other layers
+// (the coarse levels of small inputs among them) keep the gathering kernel.
+template <int COUT, bool UPCAT>
+void launch_conv(const u16 *x, const Geo5 &g, int cin, const u16 *wb, const float *bias,
+                 int relu, u16 *y, const u16 *skip, int ca, hipStream_t s) {
+    if constexpr (COUT <= 64) {
+        if (g.w >= kBoxW) {
+            if constexpr (COUT <= 32 && !UPCAT) {
+                if (cin == 8) {
+                    k_conv3_box<COUT, 8, false><<<box_grid<COUT, 8>(g), 256, 0, s>>>(
+                        x, g, cin, wb, bias, relu, y, skip, ca);
+                    return;
+                }
+                if (cin == 16) {
+                    k_conv3_box<COUT, 16, false><<<box_grid<COUT, 16>(g), 256, 0, s>>>(
+                        x, g, cin, wb, bias, relu, y, skip, ca);
+                    return;
+                }
+            }
+            if (cin % 16 == 0 && (!UPCAT || ca % 16 == 0)) {
+                const unsigned nbox = (unsigned)((int64_t)g.n * cdiv(g.d, kBoxD) * cdiv(g.h, kBoxH) *
+                                                 cdiv(g.w, kBoxW));
+                k_conv3_boxm<COUT, UPCAT><<<nbox, 256, 0, s>>>(x, g, cin, wb, bias, relu, y, skip, ca);
+                return;
+            }
+        }
+    }
+    k_conv3_fwd<COUT, UPCAT><<<(unsigned)cdiv((int64_t)g.n * g.d * g.h * g.w, 256), 256, 0, s>>>(
+        x, g, cin, wb, bias, relu, y, skip, ca);
+}
+
 asets_status check_geo(int n, int d, int h, int w, int c) {
     if (n <= 0 || d <= 0 || h <= 0 || w <= 0 || c <= 0)
         return nn_fail(ASETS_EBADARG, "sizes must be positive");
@@ -563,17 +991,16 @@ asets_status asets_nn_conv3(const void *in, int n, int d, int h, int w, int cin,
     if (st) return st;
     if (!in || !wt || !out) return nn_fail(ASETS_EBADARG, "null pointer");
     const Geo5 g{n, d, h, w};
-    const unsigned grid = (unsigned)cdiv((int64_t)n * d * h * w, 256);
     hipStream_t s = (hipStream_t)stream;
     const u16 *x = (const u16 *)in, *wb = (const u16 *)wt;
     u16 *y = (u16 *)out;
     switch (cout) {
-    case 16: k_conv3_fwd<16><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y); break;
-    case 32: k_conv3_fwd<32><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y); break;
-    case 48: k_conv3_fwd<48><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y); break;
-    case 64: k_conv3_fwd<64><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y); break;
-    case 96: k_conv3_fwd<96><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y); break;
-    case 128: k_conv3_fwd<128><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y); break;
+    case 16: launch_conv<16, false>(x, g, cin, wb, bias, relu, y, nullptr, 0, s); break;
+    case 32: launch_conv<32, false>(x, g, cin, wb, bias, relu, y, nullptr, 0, s); break;
+    case 48: launch_conv<48, false>(x, g, cin, wb, bias, relu, y, nullptr, 0, s); break;
+    case 64: launch_conv<64, false>(x, g, cin, wb, bias, relu, y, nullptr, 0, s); break;
+    case 96: launch_conv<96, false>(x, g, cin, wb, bias, relu, y, nullptr, 0, s); break;
+    case 128: launch_conv<128, false>(x, g, cin, wb, bias, relu, y, nullptr, 0, s); break;
     default: return nn_fail(ASETS_EBADARG, "cout must be 16, 32, 48, 64, 96 or 128");
     }
     NNCHK();
@@ -591,14 +1018,13 @@ asets_status asets_nn_conv3_upcat(const void *coarse, int ca, const void *skip, 
     if (d % 2 || h % 2 || w % 2) return nn_fail(ASETS_EBADARG, "upsample needs even extents");
     if (!coarse || !skip || !wt || !out) return nn_fail(ASETS_EBADARG, "null pointer");
     const Geo5 g{n, d, h, w};
-    const unsigned grid = (unsigned)cdiv((int64_t)n * d * h * w, 256);
     hipStream_t s = (hipStream_t)stream;
     const u16 *x = (const u16 *)coarse, *k = (const u16 *)skip, *wb = (const u16 *)wt;
     u16 *y = (u16 *)out;
     switch (cout) {
-    case 16: k_conv3_fwd<16, true><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y, k, ca); break;
-    case 32: k_conv3_fwd<32, true><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y, k, ca); break;
-    case 64: k_conv3_fwd<64, true><<<grid, 256, 0, s>>>(x, g, cin, wb, bias, relu, y, k, ca); break;
+    case 16: launch_conv<16, true>(x, g, cin, wb, bias, relu, y, k, ca, s); break;
+    case 32: launch_conv<32, true>(x, g, cin, wb, bias, relu, y, k, ca, s); break;
+    case 64: launch_conv<64, true>(x, g, cin, wb, bias, relu, y, k, ca, s); break;
     default: return nn_fail(ASETS_EBADARG, "cout must be 16, 32 or 64");
     }
     NNCHK();

@@ -35,7 +35,8 @@ def _assert_bf16_close(got, ref, scale=None):
     assert not bad.any(), f"{bad.sum()} of {bad.size} out of tolerance; max err {err.max():.3e}"
 
 
-SHAPES = [(2, 3, 5, 7), (1, 8, 8, 8), (1, 4, 6, 40)]
+# rows narrower than 64 voxels run the gathering kernel, wider ones the LDS-staged one
+SHAPES = [(2, 3, 5, 7), (1, 8, 8, 8), (1, 4, 6, 40), (1, 3, 4, 70), (2, 2, 3, 64), (1, 4, 4, 128)]
 CHANNELS = [(8, 16), (16, 32), (48, 16), (32, 64), (96, 32), (128, 64), (16, 48), (32, 96),
             (64, 128)]
 
@@ -61,7 +62,7 @@ def test_conv3_forward(cuda, geo, cin, cout, relu):
     _assert_bf16_close(_f(out), NO.from_ncdhw(ref).numpy())
 
 
-@pytest.mark.parametrize("geo", SHAPES[:2] + [(2, 4, 6, 8)])
+@pytest.mark.parametrize("geo", SHAPES[:2] + [(2, 4, 6, 8), (1, 3, 2, 66), (2, 2, 2, 64)])
 @pytest.mark.parametrize("cin,cout", [(8, 16), (16, 16), (48, 16), (96, 32), (128, 64),
                                       (64, 64), (32, 64)])
 def test_conv3_backward(cuda, geo, cin, cout):
@@ -170,7 +171,9 @@ def test_pool_and_upsample(cuda, geo, c):
 
 
 @pytest.mark.parametrize("geo,ca,cs,cout", [((2, 4, 6, 8), 64, 64, 64), ((1, 6, 4, 10), 64, 32, 32),
-                                           ((2, 2, 8, 6), 32, 16, 16), ((1, 8, 8, 8), 16, 8, 16)])
+                                           ((2, 2, 8, 6), 32, 16, 16), ((1, 8, 8, 8), 16, 8, 16),
+                                           ((1, 4, 2, 128), 32, 16, 16), ((2, 2, 4, 64), 64, 32, 32),
+                                           ((1, 2, 2, 66), 64, 64, 64)])
 def test_conv3_upcat_equals_materialised(cuda, geo, ca, cs, cout):
     """The decoder convolution gathering [up2(coarse), skip] in place gives the same bits
     as upsample + channel copy + convolution."""
@@ -194,10 +197,11 @@ def test_conv3_upcat_equals_materialised(cuda, geo, ca, cs, cout):
     assert torch.equal(out.view(torch.int16), ref.view(torch.int16))
 
 
-def test_inference_forward_equals_training_forward(cuda):
+@pytest.mark.parametrize("shape,n", [((16, 24, 8), 2), ((128, 16, 8), 1)])
+def test_inference_forward_equals_training_forward(cuda, shape, n):
     """The inference forward (decoder inputs gathered in the convolution) and the training
-    forward (decoder inputs materialised) predict the same bits."""
-    shape, n = (16, 24, 8), 2
+    forward (decoder inputs materialised) predict the same bits, with rows narrower and
+    wider than the LDS-staged kernel's 64 voxels."""
     Cs, Ct, A, _, _ = _net_inputs(shape, n, 5)
     net = W.WarmStartNet(cuda, seed=4)
     d = [torch.from_numpy(x).to(cuda) for x in (Cs, Ct, A)]
@@ -309,12 +313,13 @@ def _net_inputs(shape, n, seed):
     return [np.concatenate(x) for x in (Cs, Ct, A, TU, TPT)]
 
 
-def test_network_forward_and_gradients(cuda):
+@pytest.mark.parametrize("shape,n", [((16, 24, 8), 2), ((64, 8, 8), 2)])
+def test_network_forward_and_gradients(cuda, shape, n):
     """Whole U-Net: loss and every parameter gradient of the project's kernels against
     float64 autograd through the oracle (bf16 rounding of activations mirrored in the
     forward pass; the activation gradients are bf16 only on the GPU, hence the 3 %
-    relative-L2 bar per parameter tensor)."""
-    shape, n = (16, 24, 8), 2
+    relative-L2 bar per parameter tensor).  The second shape's 64-voxel rows run the
+    LDS-staged convolution at full resolution."""
     Cs, Ct, A, TU, TPT = _net_inputs(shape, n, 77)
     net = W.WarmStartNet(cuda, seed=3)
     # non-zero biases so their gradients and the bias path are exercised
