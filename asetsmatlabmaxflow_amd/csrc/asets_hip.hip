@@ -2286,8 +2286,8 @@ asets_status asets_hbm_stream_dev(int mode, void *buf, int64_t bytes, int32_t fl
                                   int32_t grid, int32_t iters, double *ms, int64_t *traffic,
                                   const asets_opts *o) {
     if (!buf || !ms || !traffic) return fail(ASETS_EBADARG, "null argument");
-    if (mode < kStreamRead || mode > kStreamMixState)
-        return fail(ASETS_EBADARG, "mode 0 to 4");
+    if (mode < kStreamRead || mode > kStreamMixBlk)
+        return fail(ASETS_EBADARG, "mode 0 to 5");
     if (((uintptr_t)buf & 255) != 0) return fail(ASETS_EBADARG, "buf must be 256-byte aligned");
     if (iters < 1 || grid < 0) return fail(ASETS_EBADARG, "iters >= 1 and grid >= 0");
     const bool nt = (flags & 1) != 0, unroll1 = (flags & 2) != 0;
@@ -2335,6 +2335,18 @@ asets_status asets_hbm_stream_dev(int mode, void *buf, int64_t bytes, int32_t fl
         launch = [=]() {
             if (nt) k_stream_mix_state<true><<<g, 256, 0, s>>>(cs, ct, al, sin, sout, n16);
             else k_stream_mix_state<false><<<g, 256, 0, s>>>(cs, ct, al, sin, sout, n16);
+        };
+    } else if (mode == kStreamMixBlk) {
+        const int64_t N = bytes / 60 / 1024 * 1024;
+        if (N < 1024) return fail(ASETS_EBADARG, "buffer too small");
+        *traffic = 60 * N;
+        const float *cs = (const float *)b, *ct = (const float *)(b + 4 * N);
+        const float *al = (const float *)(b + 8 * N);
+        const char *sin = b + 12 * N;
+        char *sout = b + 36 * N;
+        launch = [=]() {
+            if (nt) k_stream_mix_blk<true><<<g, 256, 0, s>>>(cs, ct, al, sin, sout, N);
+            else k_stream_mix_blk<false><<<g, 256, 0, s>>>(cs, ct, al, sin, sout, N);
         };
     } else if (mode == kStreamMixPacked) {
         const int64_t N = bytes / 60 / 1024 * 1024;

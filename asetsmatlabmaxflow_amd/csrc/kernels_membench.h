@@ -21,6 +21,11 @@
 //   kStreamMixState  the same bytes with the inputs in their own three arrays (Cs,
 //                Ct, alpha as the caller holds them) and only the state block-
 //                interleaved (state read, state written): five arrays.
+//   kStreamMixBlk  the same five arrays with the fused binary kernel's own layout and
+//                access widths: one voxel per lane, Cs, Ct, alpha as 4-byte loads, the
+//                state in 1536-byte blocks of 64 voxels (u at 4 B, the (ps, pt) and
+//                (bx, by) pairs at 8 B, bz at 4 B per lane), read from one buffer and
+//                written to another.
 #pragma once
 #include "common.h"
 
@@ -31,7 +36,8 @@ enum StreamMode {
     kStreamCopy = 1,
     kStreamMix = 2,
     kStreamMixPacked = 3,
-    kStreamMixState = 4
+    kStreamMixState = 4,
+    kStreamMixBlk = 5
 };
 
 // 16-byte element (a clang vector: the non-temporal builtins take those)
@@ -160,6 +166,41 @@ __global__ __launch_bounds__(256) void k_stream_mix_state(const f32x4 *cs, const
         const f32x4 s = (a0 + a1) + a2;
 #pragma unroll
         for (int k = 0; k < 6; ++k) st16<NT>(sout + (blk * 6 + k) * 64 + lane, q[k] + s);
+    }
+}
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+template <bool NT, typename V>
+__device__ __forceinline__ V ldn(const V *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, typename V>
+__device__ __forceinline__ void stn(V *p, V v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+// kStreamMixBlk: N voxels (a multiple of 64), one per lane per step.
+template <bool NT>
+__global__ __launch_bounds__(256) void k_stream_mix_blk(const float *cs, const float *ct,
+                                                        const float *al, const char *sin,
+                                                        char *sout, int64_t N) {
+    const int64_t st = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += st) {
+        const int64_t base = (i >> 6) * 1536;
+        const int ln = (int)(i & 63);
+        const float a = ldn<NT>(cs + i), b = ldn<NT>(ct + i), c = ldn<NT>(al + i);
+        const float u = ldn<NT>((const float *)(sin + base) + ln);
+        const f32x2 p1 = ldn<NT>((const f32x2 *)(sin + base + 256) + ln);
+        const f32x2 p2 = ldn<NT>((const f32x2 *)(sin + base + 768) + ln);
+        const float bz = ldn<NT>((const float *)(sin + base + 1280) + ln);
+        const float q = (a + b) + c;
+        stn<NT>((float *)(sout + base) + ln, u + q);
+        stn<NT>((f32x2 *)(sout + base + 256) + ln, p1 + q);
+        stn<NT>((f32x2 *)(sout + base + 768) + ln, p2 + q);
+        stn<NT>((float *)(sout + base + 1280) + ln, bz + q);
     }
 }
 

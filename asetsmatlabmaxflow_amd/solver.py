@@ -291,8 +291,10 @@ def hbm_stream(mode: str, buf, *, nontemporal: bool = True, one_in_flight: bool 
     """Time a plain stream over the device tensor ``buf`` (asets_hbm_stream_dev):
     mode "read", "copy", "mix" (the binary solver's 36 B read + 24 B written per voxel over
     its eleven arrays, without the stencil), "mix_packed" (the same bytes through three
-    interleaved arrays) or "mix_state" (inputs in three arrays, the state interleaved).  Returns (ms per launch, bytes read plus written per launch)."""
-    m = {"read": 0, "copy": 1, "mix": 2, "mix_packed": 3, "mix_state": 4}[mode]
+    interleaved arrays), "mix_state" (inputs in three arrays, the state interleaved) or
+    "mix_blk" (those five arrays with the fused binary kernel's layout and 4- and 8-byte
+    lane accesses).  Returns (ms per launch, bytes read plus written per launch)."""
+    m = {"read": 0, "copy": 1, "mix": 2, "mix_packed": 3, "mix_state": 4, "mix_blk": 5}[mode]
     if not (buf.is_cuda and buf.is_contiguous()):
         raise ValueError("expected a contiguous CUDA tensor")
     ms, traffic = C.c_double(), C.c_int64()

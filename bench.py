@@ -623,7 +623,7 @@ def bench_hbm(args):
     buf = torch.ones(60 * N // 4, dtype=torch.float32, device=dev)   # 1006.6 MB
     torch.cuda.synchronize()
     rows = []
-    for mode in ("read", "copy", "mix", "mix_packed", "mix_state"):
+    for mode in ("read", "copy", "mix", "mix_packed", "mix_state", "mix_blk"):
         for nt in (True, False):
             for one in ((False, True) if mode in ("read", "copy") else (False,)):
                 for per_cu in (2, 4, 8, 16):
@@ -635,7 +635,7 @@ def bench_hbm(args):
                                  "workgroups_per_cu": per_cu, "ms": ms, "bytes": traffic,
                                  "GBps": traffic / (ms * 1e-3) / 1e9})
     best = {m: max((r for r in rows if r["mode"] == m), key=lambda r: r["GBps"])
-            for m in ("read", "copy", "mix", "mix_packed", "mix_state")}
+            for m in ("read", "copy", "mix", "mix_packed", "mix_state", "mix_blk")}
     mix = best["mix"]
     out = _base(args, "HBM GB/s ceiling: plain streams with 16-byte accesses (read, copy, "
                       "binary-solver traffic mix)", mix["GBps"], "GB/s", mix["ms"], "f32",

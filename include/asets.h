@@ -364,7 +364,10 @@ asets_status asets_selftest_division(int mode, float d, uint64_t *mismatches);
  * 36 B read and 24 B written per voxel, without the stencil; mode 3 moves the
  * same bytes through three arrays (inputs interleaved at 12 B per voxel, state
  * read at 24 B, state written at 24 B); mode 4 keeps Cs, Ct, alpha in three
- * arrays and interleaves only the state (five arrays).  flags bit 0:
+ * arrays and interleaves only the state (five arrays); mode 5 moves the same five
+ * arrays with the fused binary kernel's layout and access widths (one voxel per
+ * lane: 4-byte inputs, 1536-byte state blocks of 64 voxels read as 4- and 8-byte
+ * fields).  flags bit 0:
  * non-temporal cache policy; bit 1: one element per lane in flight instead of four
  * (modes 0, 1).  grid: workgroups of 256 threads, 0 = 8 per CU.  One untimed launch,
  * then `iters` launches timed with HIP events on the stream of o: *ms = mean time
