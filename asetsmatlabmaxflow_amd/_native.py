@@ -23,6 +23,7 @@ EXPORTED = (
     "asets_binary_dev_noalloc",
     "asets_potts_dev",
     "asets_binary3d_batched_dev", "asets_binary_energy_dev", "asets_potts_energy_dev",
+    "asets_binary_bounds_dev",
     "asets_cost_loglik_dev", "asets_cost_neglog_dev", "asets_alpha_contrast_dev",
     "asets_labels_dev", "asets_cost_l1_dev", "asets_cost_smooth_labels_dev", "asets_hmf_dev",
     "asets_selftest_division", "asets_hbm_stream_dev", "asets_last_error",
@@ -100,6 +101,8 @@ def lib() -> C.CDLL:
         L.asets_binary3d_batched_dev.argtypes = [C.c_int32, C.POINTER(Volume), PP, PD, PO]
         L.asets_binary_energy_dev.argtypes = [C.c_int, P, P, P, I64, I64, I64, P, PD, PO]
         L.asets_potts_energy_dev.argtypes = [C.c_int, P, P, I64, I64, I64, I64, P, PD, PO]
+        L.asets_binary_bounds_dev.argtypes = [C.c_int, P, P, P, P, P, P, P, I64, I64, I64,
+                                              C.c_int32, PD, PO]
         L.asets_cost_loglik_dev.argtypes = [P, P, I64, C.c_int32, C.c_double, P, PO]
         L.asets_cost_neglog_dev.argtypes = [P, I64, C.c_double, P, PO]
         L.asets_alpha_contrast_dev.argtypes = [C.c_int, P, I64, I64, I64, C.c_double, C.c_double,

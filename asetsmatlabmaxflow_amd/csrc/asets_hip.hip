@@ -26,6 +26,7 @@
 #include "asets.h"
 #include "common.h"
 #include "kernels_batched.h"
+#include "kernels_bounds.h"
 #include "kernels_energy.h"
 #include "kernels_prepost.h"
 #include "kernels_fused.h"
@@ -1873,6 +1874,55 @@ asets_status energy_dev(int ndim, const float *Cs, const float *Ct, const float 
     return ASETS_OK;
 }
 
+// FP64 primal-dual bounds of a binary state (kernels_bounds.h): out = {primal
+// P_lambda(u) after `sweeps` red-black sweeps, the voxel-centred primal bound, dual
+// D(p~)}.  Scratch: one double per voxel (the feasibility factors, then lambda).
+asets_status bounds_dev(int ndim, const float *Cs, const float *Ct, const float *alpha,
+                        const float *u, const float *bx, const float *by, const float *bz,
+                        int64_t ny, int64_t nx, int64_t nz, int32_t sweeps, double *out,
+                        const asets_opts *o) {
+    if (ndim != 2 && ndim != 3) return fail(ASETS_EBADARG, "ndim must be 2 or 3");
+    if (ndim == 2 && nz != 1) return fail(ASETS_EBADARG, "2D bounds need nz == 1");
+    asets_status st = check_dims(ny, nx, nz, 1);
+    if (st) return st;
+    if (!Cs || !Ct || !alpha || !u || !bx || !by || (ndim == 3 && !bz) || !out)
+        return fail(ASETS_EBADARG, "null array argument");
+    if (sweeps < 0 || sweeps > 100000) return fail(ASETS_EBADARG, "sweeps must be in [0, 1e5]");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(ASETS_EDEVICE, "device id %d", dev);
+    DeviceCache &dc = g_cache[dev];
+    std::lock_guard<std::mutex> lock(dc.mu);
+    hipStream_t s = o ? (hipStream_t)o->stream : nullptr;
+    if (!dc.host_flag) HIPCHK(hipHostMalloc((void **)&dc.host_flag, 64));
+    const BGeo G{ny, nx, nz, ny, ny * nx, ny * nx * nz};
+    const int nb = kBoundsBlocks;
+    const size_t need = Carver::sz(G.n, 8) + Carver::sz(4 * nb, 8) + Carver::sz(4, 8);
+    st = ensure(&dc.dev, &dc.dev_bytes, need);
+    if (st) return st;
+    Carver cv(dc.dev);
+    double *sl = cv.take<double>(G.n);   // feasibility factors, then lambda
+    double *part = cv.take<double>(4 * nb);
+    double *res = cv.take<double>(4);
+    const float *fz = ndim == 3 ? bz : nullptr;
+    const unsigned gv = (unsigned)std::min<int64_t>(cdiv(G.n, 256), 65536);
+    k_bnd_scale<<<gv, 256, 0, s>>>(G, alpha, bx, by, fz, sl);
+    k_bnd_dual<<<nb, kBoundsThreads, 0, s>>>(G, Cs, Ct, bx, by, fz, sl, part);
+    k_bnd_init<<<nb, kBoundsThreads, 0, s>>>(G, Cs, Ct, alpha, u, sl, part + nb);
+    k_bnd_F<<<nb, kBoundsThreads, 0, s>>>(G, alpha, u, sl, part + 2 * nb);
+    for (int32_t i = 0; i < sweeps; ++i) k_bnd_sweep<<<gv, 256, 0, s>>>(G, alpha, u, sl, i & 1);
+    k_bnd_F<<<nb, kBoundsThreads, 0, s>>>(G, alpha, u, sl, part + 3 * nb);
+    k_bnd_sum<<<1, 256, 0, s>>>(part, nb, 4, res);
+    HIPCHK(hipGetLastError());
+    double *h = (double *)dc.host_flag + 4;   // 4 doubles of the pinned slot
+    HIPCHK(hipMemcpyAsync(h, res, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    out[0] = h[1] + h[3];   // data + F(lambda after the sweeps)
+    out[1] = h[1] + h[2];   // data + F(voxel-centred lambda)
+    out[2] = h[0];          // dual
+    return ASETS_OK;
+}
+
 // ---------------------------------------------------------------------------
 // costs, weights and labels on device memory
 // ---------------------------------------------------------------------------
@@ -2223,6 +2273,14 @@ asets_status asets_potts_energy_dev(int ndim, const float *Ct, const float *alph
                                     int64_t nx, int64_t nz, int64_t nlab, const float *u,
                                     double *energy, const asets_opts *o) {
     return energy_dev(ndim, nullptr, Ct, alpha, ny, nx, nz, nlab, u, energy, o);
+}
+
+asets_status asets_binary_bounds_dev(int ndim, const float *Cs, const float *Ct,
+                                     const float *alpha, const float *u, const float *bx,
+                                     const float *by, const float *bz, int64_t ny, int64_t nx,
+                                     int64_t nz, int32_t sweeps, double *bounds,
+                                     const asets_opts *o) {
+    return bounds_dev(ndim, Cs, Ct, alpha, u, bx, by, bz, ny, nx, nz, sweeps, bounds, o);
 }
 
 asets_status asets_cost_loglik_dev(const float *img, const int32_t *scribbles, int64_t n,

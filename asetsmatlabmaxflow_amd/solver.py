@@ -275,6 +275,22 @@ def energy_binary(Cs, Ct, alpha, u, shape, *, stream=None) -> np.ndarray:
     return out
 
 
+def binary_bounds(Cs, Ct, alpha, u, bx, by, bz, shape, *, sweeps: int = 30,
+                  stream=None) -> dict:
+    """FP64 bracket of the binary model's optimum for the state (u, bx, by, bz) on the
+    device (asets_binary_bounds_dev): {"primal": P(u) with the regulariser bounded after
+    `sweeps` red-black lambda sweeps, "primal_tv": the voxel-centred bound, "dual":
+    D(p~), "width": (primal - dual) / |dual|}.  bz may be None in 2D."""
+    ndim, ny, nx, nz = _ndim_dims(shape)
+    out = np.zeros(3)
+    opts = nat.make_opts(_stream_handle(stream), nat.FLOW_F32, nat.KERNEL_AUTO, 0, 0, 0, None)
+    nat.check(nat.lib().asets_binary_bounds_dev(
+        ndim, _ptr(Cs), _ptr(Ct), _ptr(alpha), _ptr(u), _ptr(bx), _ptr(by), _ptr(bz), ny, nx,
+        nz, sweeps, out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(opts)))
+    return {"primal": float(out[0]), "primal_tv": float(out[1]), "dual": float(out[2]),
+            "width": float((out[0] - out[2]) / abs(out[2])) if out[2] else float("inf")}
+
+
 def energy_potts(Ct, alpha, u, shape, nlab: int, *, stream=None) -> np.ndarray:
     """FP64 energy {E, data, regularisation} of the Potts label fields u (device tensors)."""
     ndim, ny, nx, nz = _ndim_dims(shape)

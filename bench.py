@@ -255,6 +255,24 @@ def bench_binary3d(args):
             "iterations_at_that_err_bound": first["iterations"] if first else None,
             "sweep": sweep}
         _progress("label agreement sweep done")
+        # certified bracket D(p~) <= min <= P(u) (FP64, asets_binary_bounds_dev) of the
+        # errBound-1e-4 state and of the converged one; it levels off instead of closing
+        # (the reference's rescale is not a projection), so it is reported, not a stop rule
+        bnd = {}
+        for label, its, eb in (("err_bound_1e-4", 20000, 1e-4), ("fixed_2000", REF_ITERS, 0.0)):
+            rs = solve(its, eb, want_state=True)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            b = solver.binary_bounds(dCs, dCt, dA, rs.u, rs.state["bx"], rs.state["by"],
+                                     rs.state["bz"], shape, sweeps=30)
+            torch.cuda.synchronize()
+            bnd[label] = {**b, "iterations": rs.iterations, "seconds": time.perf_counter() - t1}
+            del rs
+        bnd["note"] = ("primal: P(u) with the regulariser bounded after 30 red-black lambda "
+                       "sweeps; primal_tv: the voxel-centred bound; width = (primal - dual) / "
+                       "|dual|")
+        out["time_to_tol"]["bounds"] = bnd
+        _progress("primal-dual bounds done")
 
     # learned warm start (network inference + solve from (u0, pt0)) vs the cold start,
     # both to errBound 1e-4; energies against a 2000-iteration reference solve

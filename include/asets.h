@@ -254,6 +254,30 @@ asets_status asets_potts_energy_dev(int ndim, const float *Ct, const float *alph
                                     int64_t nx, int64_t nz, int64_t nlab, const float *u,
                                     double *energy, const asets_opts *o);
 
+/* ---- primal-dual bounds of a binary state (FP64) ------------------------------
+ * No reference counterpart (the reference stops on the mean |du| trace,
+ * asetsBinaryMF3D_mex.c:183-188).  For the model max sum ps, ps <= Cs, pt <= Ct,
+ * div p - ps + pt = 0, per-voxel capacity 0.5 * (sum of the six faces^2) <= alpha^2
+ * (projStep, asetsBinaryMF3D_mex.c:321-331), and a state u, bx, by, bz (device
+ * arrays, column-major, own-face convention as asets_binary_dev_state returns them;
+ * bz unused and may be NULL in 2D):
+ *   bounds[2] = D(p~) = sum min(Cs, Ct + div p~), the flows scaled face by face to
+ *               the smaller feasibility factor of their two voxels (a lower bound of
+ *               the optimum);
+ *   bounds[0] = P(u) with the regulariser bounded by F(lambda) after `sweeps`
+ *               red-black Newton sweeps over lambda (an upper bound);
+ *   bounds[1] = the same with the voxel-centred lambda (the looser TV bound).
+ * u is clipped to [0, 1]; alpha must be > 0.  On the reference iteration the bracket
+ * does not close: its rescale (asetsBinaryMF3D_mex.c:337-397) averages the two
+ * voxels' factors, so its fixed point is not this problem's saddle point, and the
+ * relative width levels off (about 0.7 % on the 256^3 workload).  It certifies how
+ * far a state can be from the optimum; it is not a stop rule. */
+asets_status asets_binary_bounds_dev(int ndim, const float *Cs, const float *Ct,
+                                     const float *alpha, const float *u, const float *bx,
+                                     const float *by, const float *bz, int64_t ny, int64_t nx,
+                                     int64_t nz, int32_t sweeps, double *bounds,
+                                     const asets_opts *o);
+
 /* ---- before and after the solve: data costs, weights, labels (device arrays) --
  * The steps the reference's applications and tests run around the solvers, on the
  * GPU.  No reference C counterpart (they are MATLAB code); semantics as cited. */

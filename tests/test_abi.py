@@ -96,6 +96,17 @@ def test_bad_arguments_rejected_before_device_work():
     # oversized plane
     assert L.asets_binary3d(ptr, ptr, ptr, 1 << 16, 1 << 15, 1, C.byref(p), ptr, ptr,
                             C.byref(itn), None, None) == nat.ASETS_ETOOBIG
+    # bounds: bad ndim, 2D with nz > 1, missing bz in 3D, negative sweeps
+    out = np.zeros(3)
+    o = out.ctypes.data_as(C.POINTER(C.c_double))
+    assert L.asets_binary_bounds_dev(4, ptr, ptr, ptr, ptr, ptr, ptr, ptr, 4, 4, 4, 3, o,
+                                     None) == nat.ASETS_EBADARG
+    assert L.asets_binary_bounds_dev(2, ptr, ptr, ptr, ptr, ptr, ptr, None, 4, 4, 2, 3, o,
+                                     None) == nat.ASETS_EBADARG
+    assert L.asets_binary_bounds_dev(3, ptr, ptr, ptr, ptr, ptr, ptr, None, 4, 4, 2, 3, o,
+                                     None) == nat.ASETS_EBADARG
+    assert L.asets_binary_bounds_dev(3, ptr, ptr, ptr, ptr, ptr, ptr, ptr, 4, 4, 2, -1, o,
+                                     None) == nat.ASETS_EBADARG
     # batched: 1 to 65,535 volumes
     vols = (nat.Volume * 1)()
     assert L.asets_binary3d_batched_dev(0, vols, C.byref(p), None, None) == nat.ASETS_EBADARG

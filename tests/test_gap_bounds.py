@@ -20,117 +20,37 @@ stalls.  These tests pin the bounds (weak duality, tightness of F) and that find
 """
 import numpy as np
 
+import bounds_oracle as BO
 import oracle as O
 from asetsmatlabmaxflow_amd import workloads
 
 SHAPE = (24, 20, 16)   # (Ny, Nx, Nz)
+AX = BO.AX
 
 
 def _grid(a):
-    ny, nx, nz = SHAPE
-    a = np.asarray(a, np.float64)
-    return (a.ravel(order="F") if a.ndim > 1 else a).reshape(nz, nx, ny)
+    return BO.grid(a, SHAPE)
 
 
-def _hi(f, ax):   # own face of the +ax neighbour (0 past the volume); axes (z, x, y)
-    g = np.zeros_like(f)
-    sl = [slice(None)] * 3
-    src = list(sl)
-    sl[ax] = slice(None, -1)
-    src[ax] = slice(1, None)
-    g[tuple(sl)] = f[tuple(src)]
-    return g
-
-
-def _lo(f, ax):   # value at the -ax neighbour (0 outside)
-    g = np.zeros_like(f)
-    sl = [slice(None)] * 3
-    src = list(sl)
-    sl[ax] = slice(1, None)
-    src[ax] = slice(None, -1)
-    g[tuple(sl)] = f[tuple(src)]
-    return g
-
-
-AX = (1, 2, 0)   # bx along x (axis 1), by along y (axis 2), bz along z (axis 0)
-
-
-def _norm(f):
-    return np.sqrt(0.5 * sum(f[k] ** 2 + _hi(f[k], AX[k]) ** 2 for k in range(3)))
-
-
-def _div(f):
-    return sum(_hi(f[k], AX[k]) - f[k] for k in range(3))
+_hi, _lo, _norm, wfaces, sweep, primal_bound = BO.hi, BO.lo, BO.norm, BO.wfaces, BO.sweep, \
+    BO.primal_bound
+_wfaces = wfaces
 
 
 def dual(Cs, Ct, al, faces):
-    """D(p~): the faces scaled to the smaller feasibility factor of their two voxels."""
+    """D(p~), after checking that the scaled flows are feasible."""
     n = _norm(faces)
     s = np.where(n > al, al / np.where(n > 0, n, 1.0), 1.0)
     sc = [faces[k] * np.minimum(s, _lo(s, AX[k])) for k in range(3)]
     assert (_norm(sc) <= al * (1 + 1e-12)).all()
-    return np.minimum(Cs, Ct + _div(sc)).sum()
-
-
-def _wfaces(u):
-    w = [u - _lo(u, AX[k]) for k in range(3)]
-    for k in range(3):
-        idx = [slice(None)] * 3
-        idx[AX[k]] = 0
-        w[k][tuple(idx)] = 0.0
-    return w
+    return BO.dual(Cs, Ct, al, faces)
 
 
 def F(lam, al, w):
-    tot = (al ** 2 * lam).sum()
     for k in range(3):
         c = 0.5 * w[k] ** 2
-        L = lam + _lo(lam, AX[k])
-        tot += np.where(c > 0, c / np.where(L > 0, L, 1.0), 0.0).sum()
-        assert not ((c > 0) & (L <= 0)).any()
-    return tot
-
-
-def sweep(lam, al, w, color, newton=40):
-    """Minimise F over the voxels of one checkerboard colour, neighbours fixed."""
-    cs, ms = [], []
-    for k in range(3):
-        c = 0.5 * w[k] ** 2
-        cs += [c, _hi(c, AX[k])]
-        ms += [_lo(lam, AX[k]), _hi(lam, AX[k])]
-    Cm, Mm = np.stack(cs), np.stack(ms)
-    a2 = al ** 2
-    csum = Cm.sum(0)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        d0 = a2 - np.where(Cm > 0, Cm / np.where(Mm > 0, Mm, 0.0) ** 2, 0.0).sum(0)
-    d0 = np.where(np.isnan(d0), -np.inf, d0)
-    active = (csum > 0) & (d0 < 0)
-    mmin = np.where(Cm > 0, Mm, np.inf).min(0)
-    x = np.where(active, np.sqrt(csum) / al - np.where(np.isfinite(mmin), mmin, 0.0), 1.0)
-    x = np.where(x > 0, x, 1e-3)
-    for _ in range(newton):
-        den = x + Mm
-        den = np.where(den > 0, den, 1.0)
-        t = np.where(Cm > 0, Cm / den ** 2, 0.0)
-        g = a2 - t.sum(0)
-        h = 2.0 * np.where(Cm > 0, t / den, 0.0).sum(0)
-        xn = x - g / np.where(h > 0, h, 1.0)
-        x = np.where(xn > 0, xn, x / 4)
-    x = np.where(active, x, 0.0)
-    zz, xx, yy = np.indices(lam.shape)
-    return np.where(((zz + xx + yy) % 2) == color, x, lam)
-
-
-def primal_bound(Cs, Ct, al, u, sweeps=30):
-    ub = np.clip(u, 0.0, 1.0)
-    w = _wfaces(ub)
-    nrm = np.sqrt(sum(w[k] ** 2 + _hi(w[k], AX[k]) ** 2 for k in range(3)))
-    lam = nrm / (2.0 * np.sqrt(2.0) * al)   # equal split of every face: the voxel-centred TV
-    data = ((1 - ub) * Cs + ub * Ct).sum()
-    vox = data + F(lam, al, w)
-    for i in range(sweeps):
-        lam = sweep(lam, al, w, i % 2)
-    return data + F(lam, al, w), vox, lam, w
+        assert not ((c > 0) & ((lam + _lo(lam, AX[k])) <= 0)).any()
+    return BO.F(lam, al, w)
 
 
 def _problem():
