@@ -556,157 +556,264 @@ __global__ void k_relu_mask(u16 *g, const u16 *y, int n) {
 // ---------------------------------------------------------------------------
 // weight gradient on the matrix cores: for each tap k,
 //   dw[k, co, ci] = sum_p g[p, co] * in[p + o(k), ci],
-// a GEMM with M = cout, N = cin and K = voxels.  Workgroup = (NQ taps, chunk of
-// wg_chunk(P) voxels), 256 threads: NQ = 3 (the taps dx = -1, 0, 1 of one (dz, dy))
-// when a tap has at most 8 output tiles of 16 x 16, else NQ = 1.  For each slab of
-// 32 voxels, G (32 x cout) is staged once and the tap-shifted input (32 x cin; zero
-// where p + o(k) leaves the volume) once per tap, in LDS and transposed, voxels
-// contiguous, so the 8 consecutive K values of an MFMA lane are one 16-byte LDS
-// read.  The NQ x (cout/16) x (cin/16) (tap, output tile) units (cin 8 padded to
-// 16 with zeros) are dealt round-robin to the 4 waves, at most UPW per wave.  Each
-// chunk writes its own partial; k_sum_chunks adds them in order (deterministic).
+// a GEMM with M = cout, N = cin and K = voxels.  A workgroup (256 threads) owns
+// one 16-channel chunk of cin (8 for cin = 8) and walks boxes of 64 (w) x 2 (h) x
+// 2 (d) output voxels (blockIdx.x, blockIdx.x + gridDim.x, ...), accumulating all
+// 27 taps of its chunk in registers; each workgroup writes its own partial and
+// k_sum_chunks adds them in order (deterministic).
+//
+// Per box, two LDS tiles, voxels contiguous so the 8 consecutive K values of an
+// MFMA lane are one 16-byte read:
+//   Gt[co][4 box rows][64]        the output gradient of the box;
+//   Xt[ci][16 rows][80]           the input over the 4 x 4 (z, y) rows of the box and
+//                                 its halo, w from x0 - 8 (16-byte aligned windows);
+// both filled by 8 x 8 (voxel x channel) blocks transposed in registers and stored
+// as 16-byte rows, zero outside the volume.  The waves split the nine (dz, dy) rows
+// of taps (wave w: w, w + 4, w + 8).  For each of a box row's two 32-voxel K steps
+// and each of its (dz, dy) rows, a lane reads its aligned 8-voxel input window and
+// the 4 bytes either side; the dx = -1 and +1 windows are the same bytes shifted by
+// one voxel (v_alignbyte), so the three taps of the row share one LDS read.  Tile
+// rows are padded by 16 bytes so the 16 channels (co) of a fragment fall in
+// different banks.
 // ---------------------------------------------------------------------------
 constexpr int kWgCoMax = 64, kWgCiMax = 128;
-constexpr int kWgLd = 40;   // LDS row stride in bf16 (80 bytes: 16-byte aligned, spread banks)
+constexpr int kWbLen = 80;                          // input row: w0 - 8 .. w0 + 71
+constexpr int kWbXs = 16 * kWbLen + 8;              // Xt elements per channel
+constexpr int kWbGs = 4 * 64 + 8;                   // Gt elements per channel
 
-// voxels per weight-gradient chunk: about 128 chunks, 1024 to 16384 voxels each,
-// a multiple of the 32-voxel slab
-__host__ __device__ inline int wg_chunk(int64_t P) {
-    int64_t c = (P + 127) / 128;
-    c = c < 1024 ? 1024 : (c > 16384 ? 16384 : c);
-    return (int)((c + 31) / 32 * 32);
+// 8 x 8 transpose of 16-bit values: v[j] holds channels 0..7 of voxel j (dwords of
+// channel pairs); o[c] receives voxels 0..7 of channel c
+__device__ __forceinline__ void tr8x8(const uint4 *v, uint4 *o) {
+    uint32_t d[8][4];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        d[j][0] = v[j].x;
+        d[j][1] = v[j].y;
+        d[j][2] = v[j].z;
+        d[j][3] = v[j].w;
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        uint32_t r[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const uint32_t a = d[2 * m][c >> 1], b = d[2 * m + 1][c >> 1];
+            r[m] = (c & 1) ? ((a >> 16) | (b & 0xffff0000u)) : ((a & 0xffffu) | (b << 16));
+        }
+        o[c] = make_uint4(r[0], r[1], r[2], r[3]);
+    }
 }
 
-// NQ = 3: ntile <= 8, UPW = 6; NQ = 1: ntile <= 32, UPW = 8
-template <int NQ, int UPW>
-__global__ __launch_bounds__(256) void k_conv3_wgrad(const u16 *__restrict__ in, Geo5 g, int cin,
-                                                     const u16 *__restrict__ go, int cout,
-                                                     int chunk, float *__restrict__ part) {
-    __shared__ __attribute__((aligned(16))) u16 Gt[kWgCoMax][kWgLd];
-    __shared__ __attribute__((aligned(16))) u16 Xt[NQ][kWgCiMax][kWgLd];
-    // taps tap0 .. tap0 + NQ - 1: (dz, dy) fixed, dx = dx0 .. dx0 + NQ - 1
-    const int tap0 = NQ * (int)blockIdx.x;
-    const int dz = tap0 / 9 - 1, dy = (tap0 / 3) % 3 - 1, dx0 = tap0 % 3 - 1;
-    const int P = g.n * g.d * g.h * g.w;
-    const int pb = blockIdx.y * chunk;
-    const int pe = min(P, pb + chunk);
-    const int cinp = cin < 16 ? 16 : cin;   // padded N
-    const int nct = cout / 16, nit = cinp / 16, ntile = nct * nit, nunit = NQ * ntile;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+template <int NCT>
+__global__ __launch_bounds__(256) void k_conv3_wgrad_box(const u16 *__restrict__ in, Geo5 g,
+                                                         int cin, const u16 *__restrict__ go,
+                                                         float *__restrict__ part,
+                                                         float *__restrict__ bpart) {
+    constexpr int COUT = NCT * 16;
+    __shared__ __attribute__((aligned(16))) u16 xt[16 * kWbXs];
+    __shared__ __attribute__((aligned(16))) u16 gt[COUT * kWbGs];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int r = lane & 15, kg = lane >> 4;
-    f32x4 acc[UPW];
+    const int cb = blockIdx.y * 16;                      // first input channel of the chunk
+    const int ng = (cin - cb) >= 16 ? 2 : 1;             // 8-channel groups in the chunk
+    const int nbw = (g.w + 63) / 64, nbh = (g.h + 1) / 2, nbd = (g.d + 1) / 2;
+    const int nbox = g.n * nbd * nbh * nbw;
+    f32x4 acc[3][3][NCT];
 #pragma unroll
-    for (int t = 0; t < UPW; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // padding rows of an 8-channel input stay zero
-    if (cin < 16)
-        for (int e = threadIdx.x; e < NQ * (16 - cin) * kWgLd; e += 256) {
-            const int q = e / ((16 - cin) * kWgLd), rem = e % ((16 - cin) * kWgLd);
-            Xt[q][cin + rem / kWgLd][rem % kWgLd] = 0;
-        }
-    const int zyshift = (dz * g.h + dy) * g.w;
-    for (int s0 = pb; s0 < pe; s0 += 32) {
-        __syncthreads();   // the previous slab's MFMA reads are done
-        // G slab: 32 voxels x cout, 8 channels (16 bytes) per load
-        const int gper = cout / 8;
-        for (int e = threadIdx.x; e < 32 * gper; e += 256) {
-            const int v = e / gper, c8 = (e % gper) * 8;
-            const int p = s0 + v;
-            uint4 val = make_uint4(0, 0, 0, 0);
-            if (p < pe) val = *(const uint4 *)(go + (size_t)p * cout + c8);
-            const u16 *h = (const u16 *)&val;
+    for (int rr = 0; rr < 3; ++rr)
 #pragma unroll
-            for (int j = 0; j < 8; ++j) Gt[c8 + j][v] = h[j];
-        }
-        // shifted input slabs of the NQ taps: NQ x 32 voxels x cin
-        const int xper = cin / 8;
-        for (int e = threadIdx.x; e < NQ * 32 * xper; e += 256) {
-            const int q = e / (32 * xper), rem = e % (32 * xper);
-            const int v = rem / xper, c8 = (rem % xper) * 8;
-            const int dx = dx0 + q;
-            const int p = s0 + v;
-            uint4 val = make_uint4(0, 0, 0, 0);
-            if (p < pe) {
-                const int x = p % g.w;
-                int t = p / g.w;
-                const int y = t % g.h;
-                t /= g.h;
-                const int z = t % g.d;
-                if ((unsigned)(z + dz) < (unsigned)g.d && (unsigned)(y + dy) < (unsigned)g.h &&
-                    (unsigned)(x + dx) < (unsigned)g.w)
-                    val = *(const uint4 *)(in + (size_t)(p + zyshift + dx) * cin + c8);
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct) acc[rr][q][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // bias gradient (first channel chunk only): thread t < 64*NCT stages one block of
+    // 8 voxels x 8 output channels per box and sums it, voxels in order
+    const bool bias = bpart && blockIdx.y == 0;
+    float bsum[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) bsum[c] = 0.0f;
+    for (int b = blockIdx.x; b < nbox; b += gridDim.x) {
+        int t = b;
+        const int x0 = (t % nbw) * 64;
+        t /= nbw;
+        const int y0 = (t % nbh) * 2;
+        t /= nbh;
+        const int z0 = (t % nbd) * 2;
+        const int n = t / nbd;
+        __syncthreads();   // the previous box's reads are done
+        // input rows: 16 rows x 10 blocks of 8 voxels x 2 channel halves
+        for (int blk = tid; blk < 16 * 10 * 2; blk += 256) {
+            const int half = blk & 1, tt = blk >> 1;
+            const int eb = tt % 10, row = tt / 10;
+            const int zz = z0 - 1 + (row >> 2), yy = y0 - 1 + (row & 3);
+            const bool rok = half < ng && (unsigned)zz < (unsigned)g.d &&
+                             (unsigned)yy < (unsigned)g.h;
+            const size_t rb = (((size_t)n * g.d + zz) * g.h + yy) * g.w;
+            uint4 v[8], o[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int x = x0 - 8 + eb * 8 + j;
+                v[j] = make_uint4(0, 0, 0, 0);
+                if (rok && (unsigned)x < (unsigned)g.w)
+                    v[j] = *(const uint4 *)(in + (rb + x) * cin + cb + 8 * half);
             }
-            const u16 *h = (const u16 *)&val;
+            tr8x8(v, o);
 #pragma unroll
-            for (int j = 0; j < 8; ++j) Xt[q][c8 + j][v] = h[j];
+            for (int c = 0; c < 8; ++c)
+                *(uint4 *)&xt[(half * 8 + c) * kWbXs + row * kWbLen + eb * 8] = o[c];
+        }
+        // output gradient: 4 box rows x 8 blocks of 8 voxels x NCT*2 channel groups
+        for (int blk = tid; blk < 4 * 8 * 2 * NCT; blk += 256) {
+            const int cg = blk % (2 * NCT), tt = blk / (2 * NCT);
+            const int vb = tt & 7, br = tt >> 3;
+            const int zz = z0 + (br >> 1), yy = y0 + (br & 1);
+            const bool rok = zz < g.d && yy < g.h;
+            const size_t rb = (((size_t)n * g.d + zz) * g.h + yy) * g.w;
+            uint4 v[8], o[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int x = x0 + vb * 8 + j;
+                v[j] = make_uint4(0, 0, 0, 0);
+                if (rok && x < g.w) v[j] = *(const uint4 *)(go + (rb + x) * COUT + cg * 8);
+            }
+            tr8x8(v, o);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) *(uint4 *)&gt[(cg * 8 + c) * kWbGs + br * 64 + vb * 8] = o[c];
+            if (bias) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const uint32_t q[4] = {o[c].x, o[c].y, o[c].z, o[c].w};
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        bsum[c] += bf2f((u16)(q[m] & 0xffffu));
+                        bsum[c] += bf2f((u16)(q[m] >> 16));
+                    }
+                }
+            }
         }
         __syncthreads();
 #pragma unroll
-        for (int t = 0; t < UPW; ++t) {
-            const int u = wv + 4 * t;
-            if (u < nunit) {
-                const int q = u / ntile, tile = u % ntile;
-                const int ct = tile / nit, it = tile % nit;
-                const bf16x8 a = *(const bf16x8 *)&Gt[ct * 16 + r][8 * kg];
-                const bf16x8 b = *(const bf16x8 *)&Xt[q][it * 16 + r][8 * kg];
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+        for (int ks = 0; ks < 8; ++ks) {
+            const int br = ks >> 1, hh = ks & 1;
+            const int bz = br >> 1, by = br & 1;
+            bf16x8 a[NCT];
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct)
+                a[ct] = *(const bf16x8 *)&gt[(ct * 16 + r) * kWbGs + br * 64 + hh * 32 + kg * 8];
+            const int e0 = 8 + hh * 32 + kg * 8;   // element of voxel x0 + hh*32 + kg*8
+#pragma unroll
+            for (int rr = 0; rr < 3; ++rr) {
+                const int row9 = wv + 4 * rr;   // (dz + 1) * 3 + (dy + 1)
+                if (row9 < 9) {
+                    const int dz = row9 / 3 - 1, dy = row9 % 3 - 1;
+                    const u16 *rp = &xt[r * kWbXs + ((bz + 1 + dz) * 4 + (by + 1 + dy)) * kWbLen];
+                    const uint4 cur = *(const uint4 *)(rp + e0);
+                    const uint32_t pv = *(const uint32_t *)(rp + e0 - 2);
+                    const uint32_t nx = *(const uint32_t *)(rp + e0 + 8);
+                    const uint4 wm = make_uint4(__builtin_amdgcn_alignbyte(cur.x, pv, 2),
+                                                __builtin_amdgcn_alignbyte(cur.y, cur.x, 2),
+                                                __builtin_amdgcn_alignbyte(cur.z, cur.y, 2),
+                                                __builtin_amdgcn_alignbyte(cur.w, cur.z, 2));
+                    const uint4 wp = make_uint4(__builtin_amdgcn_alignbyte(cur.y, cur.x, 2),
+                                                __builtin_amdgcn_alignbyte(cur.z, cur.y, 2),
+                                                __builtin_amdgcn_alignbyte(cur.w, cur.z, 2),
+                                                __builtin_amdgcn_alignbyte(nx, cur.w, 2));
+                    const bf16x8 bm = __builtin_bit_cast(bf16x8, wm);
+                    const bf16x8 b0 = __builtin_bit_cast(bf16x8, cur);
+                    const bf16x8 bp = __builtin_bit_cast(bf16x8, wp);
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ++ct) {
+                        acc[rr][0][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ct], bm, acc[rr][0][ct], 0, 0, 0);
+                        acc[rr][1][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ct], b0, acc[rr][1][ct], 0, 0, 0);
+                        acc[rr][2][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ct], bp, acc[rr][2][ct], 0, 0, 0);
+                    }
+                }
             }
         }
     }
-    // lane holds rows (co) 4*kg + i of column (ci) r of each of its units
+    if (bias) {   // the workgroup's bias partial: the 32 threads of each channel group in order
+        __syncthreads();   // every wave is done with xt
+        float *bred = (float *)xt;
+        if (tid < 64 * NCT)
 #pragma unroll
-    for (int t = 0; t < UPW; ++t) {
-        const int u = wv + 4 * t;
-        if (u >= nunit) continue;
-        const int q = u / ntile, tile = u % ntile;
-        const int tap = tap0 + q;
-        const int ct = tile / nit, it = tile % nit;
-        const int ci = it * 16 + r;
-        if (ci >= cin) continue;
+            for (int c = 0; c < 8; ++c) bred[tid * 8 + c] = bsum[c];
+        __syncthreads();
+        if (tid < COUT) {
+            const int cg = tid >> 3, c = tid & 7;
+            float t = 0.0f;
+            for (int q = cg; q < 64 * NCT; q += 2 * NCT) t += bred[q * 8 + c];
+            bpart[(size_t)blockIdx.x * COUT + tid] = t;
+        }
+    }
+    // lane holds rows (co) 4*kg + i of column (ci) r of each (tap, co tile)
+    const int ci = cb + r;
+    if (r >= 8 * ng) return;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int co = ct * 16 + 4 * kg + i;
-            part[(((size_t)blockIdx.y * 27 + tap) * cout + co) * cin + ci] = acc[t][i];
+    for (int rr = 0; rr < 3; ++rr) {
+        const int row9 = wv + 4 * rr;
+        if (row9 >= 9) continue;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int tap = row9 * 3 + q;
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int co = ct * 16 + 4 * kg + i;
+                    part[(((size_t)blockIdx.x * 27 + tap) * COUT + co) * cin + ci] = acc[rr][q][ct][i];
+                }
         }
     }
 }
 
-// out[i] (+)= sum_c part[c*n + i], chunks added in order; the loads are issued eight
-// at a time ahead of their in-order adds
-__global__ void k_sum_chunks(const float *part, int nchunk, int n, float *out, int accumulate) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float s = 0.0f;
-    int c = 0;
-    for (; c + 8 <= nchunk; c += 8) {
-        float t[8];
-#pragma unroll
-        for (int b = 0; b < 8; ++b) t[b] = part[(size_t)(c + b) * n + i];
-#pragma unroll
-        for (int b = 0; b < 8; ++b) s += t[b];
-    }
-    for (; c < nchunk; ++c) s += part[(size_t)c * n + i];
-    out[i] = accumulate ? out[i] + s : s;
+// workgroups per input chunk of the weight gradient: about 512 in all, at most one
+// per box
+inline int wg_groups(int64_t nbox, int cin) {
+    const int64_t chunks = cin < 16 ? 1 : cin / 16;
+    int64_t gx = (512 + chunks - 1) / chunks;
+    if (gx > nbox) gx = nbox;
+    return (int)(gx < 1 ? 1 : gx);
 }
 
-// per-chunk partial sums of g over the voxels: the bias gradient.  Thread (j, co)
-// of 256 sums the chunk's voxels p = pb + j (mod 256 / cout) in order; the partial
-// sums are then added in j order (deterministic).
-__global__ __launch_bounds__(256) void k_bias_part(const u16 *go, int P, int cout, int chunk,
-                                                   float *part) {
+// out[i] (+)= sum over the chunks c of part[c*n + i].  A block of 256 threads owns
+// EPB elements; each of its 256/EPB groups adds a contiguous run of chunks in order
+// (loads issued eight ahead of their adds), and the group sums are added in group
+// order: a fixed order for every element (deterministic).
+template <int EPB>
+__global__ __launch_bounds__(256) void k_sum_chunks(const float *part, int nchunk, int n,
+                                                    float *out, int accumulate) {
+    constexpr int NG = 256 / EPB;
     __shared__ float red[256];
-    const int nj = 256 / cout;
-    const int co = threadIdx.x % cout, j = threadIdx.x / cout;
-    const int pb = blockIdx.x * chunk, pe = min(P, pb + chunk);
+    const int e = threadIdx.x % EPB, gi = threadIdx.x / EPB;
+    const int i = blockIdx.x * EPB + e;
+    const int per = (nchunk + NG - 1) / NG;
+    const int c0 = gi * per, c1 = min(nchunk, c0 + per);
     float s = 0.0f;
-    if (j < nj)
-        for (int p = pb + j; p < pe; p += nj) s += bf2f(go[(size_t)p * cout + co]);
+    if (i < n) {
+        int c = c0;
+        for (; c + 8 <= c1; c += 8) {
+            float t[8];
+#pragma unroll
+            for (int b = 0; b < 8; ++b) t[b] = part[(size_t)(c + b) * n + i];
+#pragma unroll
+            for (int b = 0; b < 8; ++b) s += t[b];
+        }
+        for (; c < c1; ++c) s += part[(size_t)c * n + i];
+    }
     red[threadIdx.x] = s;
     __syncthreads();
-    if (threadIdx.x < cout) {
+    if (gi == 0 && i < n) {
         float t = 0.0f;
-        for (int q = 0; q < nj; ++q) t += red[q * cout + threadIdx.x];
-        part[(size_t)blockIdx.x * cout + threadIdx.x] = t;
+        for (int q = 0; q < NG; ++q) t += red[q * EPB + e];
+        out[i] = accumulate ? out[i] + t : t;
     }
+}
+
+void sum_chunks(const float *part, int nchunk, int n, float *out, int accumulate,
+                hipStream_t s) {
+    if (n >= 2048)
+        k_sum_chunks<32><<<(unsigned)cdiv(n, 32), 256, 0, s>>>(part, nchunk, n, out, accumulate);
+    else
+        k_sum_chunks<8><<<(unsigned)cdiv(n, 8), 256, 0, s>>>(part, nchunk, n, out, accumulate);
 }
 
 // ---------------------------------------------------------------------------
@@ -1049,9 +1156,9 @@ asets_status asets_nn_relu_mask(void *g, const void *y, int64_t n, void *stream)
 }
 
 size_t asets_nn_wgrad_scratch_bytes(int n, int d, int h, int w, int cin, int cout) {
-    const int64_t P = (int64_t)n * d * h * w;
-    const int64_t nch = cdiv(P, wg_chunk(P));
-    return (size_t)(nch * 27LL * cout * cin + nch * cout) * sizeof(float);
+    const int64_t nbox = (int64_t)n * cdiv(d, 2) * cdiv(h, 2) * cdiv(w, 64);
+    const int64_t gx = wg_groups(nbox, cin);
+    return (size_t)(gx * 27LL * cout * cin + gx * cout) * sizeof(float);
 }
 
 asets_status asets_nn_conv3_wgrad(const void *in, int n, int d, int h, int w, int cin,
@@ -1066,24 +1173,22 @@ asets_status asets_nn_conv3_wgrad(const void *in, int n, int d, int h, int w, in
     if (!scratch || scratch_bytes < asets_nn_wgrad_scratch_bytes(n, d, h, w, cin, cout))
         return nn_fail(ASETS_EBADARG, "scratch too small");
     const Geo5 g{n, d, h, w};
-    const int P = n * d * h * w;
-    const int chunk = wg_chunk(P);
-    const int nch = (int)cdiv(P, chunk);
+    const int64_t nbox = (int64_t)n * cdiv(d, 2) * cdiv(h, 2) * cdiv(w, 64);
+    const int gx = wg_groups(nbox, cin);
+    const dim3 grid((unsigned)gx, (unsigned)(cin < 16 ? 1 : cin / 16));
     hipStream_t s = (hipStream_t)stream;
-    float *part = (float *)scratch;
-    if ((cout / 16) * ((cin < 16 ? 16 : cin) / 16) <= 8)
-        k_conv3_wgrad<3, 6><<<dim3(9, nch), 256, 0, s>>>((const u16 *)in, g, cin,
-                                                         (const u16 *)gout, cout, chunk, part);
-    else
-        k_conv3_wgrad<1, 8><<<dim3(27, nch), 256, 0, s>>>((const u16 *)in, g, cin,
-                                                          (const u16 *)gout, cout, chunk, part);
     const int nw = 27 * cout * cin;
-    k_sum_chunks<<<(unsigned)cdiv(nw, 64), 64, 0, s>>>(part, nch, nw, dw, accumulate);
-    if (dbias) {
-        float *bp = part + (size_t)nch * nw;
-        k_bias_part<<<nch, 256, 0, s>>>((const u16 *)gout, P, cout, chunk, bp);
-        k_sum_chunks<<<1, 128, 0, s>>>(bp, nch, cout, dbias, accumulate);
+    float *part = (float *)scratch;
+    float *bp = dbias ? part + (size_t)gx * nw : nullptr;
+    const u16 *x = (const u16 *)in, *gg = (const u16 *)gout;
+    switch (cout) {
+    case 16: k_conv3_wgrad_box<1><<<grid, 256, 0, s>>>(x, g, cin, gg, part, bp); break;
+    case 32: k_conv3_wgrad_box<2><<<grid, 256, 0, s>>>(x, g, cin, gg, part, bp); break;
+    case 48: k_conv3_wgrad_box<3><<<grid, 256, 0, s>>>(x, g, cin, gg, part, bp); break;
+    default: k_conv3_wgrad_box<4><<<grid, 256, 0, s>>>(x, g, cin, gg, part, bp); break;
     }
+    sum_chunks(part, gx, nw, dw, accumulate, s);
+    if (dbias) sum_chunks(bp, gx, cout, dbias, accumulate, s);
     NNCHK();
     return ASETS_OK;
 }
@@ -1176,7 +1281,7 @@ asets_status asets_nn_head_bwd(const void *x, int64_t P, const float *w, const f
     const int nb = (int)cdiv(P, kB);
     float *part = (float *)scratch;
     k_head_bwd<<<nb, kB, 0, s>>>((const u16 *)x, (int)P, w, u0, pt0, tu, tpt, (u16 *)dx, part);
-    k_sum_chunks<<<1, 64, 0, s>>>(part, nb, 35, grads, 0);
+    sum_chunks(part, nb, 35, grads, 0, s);
     NNCHK();
     return ASETS_OK;
 }

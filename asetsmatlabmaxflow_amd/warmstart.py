@@ -236,15 +236,9 @@ class WarmStartNet:
                       "g2_128": E(2, 128), "g2_32": E(2, 32), "g3_64a": E(3, 64),
                       "g3_64b": E(3, 64), "g3_64c": E(3, 64)})
             L = nnlib()
-            need = max(L.asets_nn_wgrad_scratch_bytes(*G[0], 48, 16),
-                       L.asets_nn_wgrad_scratch_bytes(*G[0], 16, 16),
-                       L.asets_nn_wgrad_scratch_bytes(*G[0], 8, 16),
-                       L.asets_nn_wgrad_scratch_bytes(*G[1], 96, 32),
-                       L.asets_nn_wgrad_scratch_bytes(*G[1], 32, 32),
-                       L.asets_nn_wgrad_scratch_bytes(*G[2], 128, 64),
-                       L.asets_nn_wgrad_scratch_bytes(*G[2], 64, 64),
-                       L.asets_nn_wgrad_scratch_bytes(*G[3], 64, 64),
-                       L.asets_nn_head_scratch_bytes(vox[0]))
+            # every convolution's weight gradient (level = the digit in its name) and the head
+            need = max([L.asets_nn_wgrad_scratch_bytes(*G[int(nm[3])], ci, co)
+                        for nm, ci, co in CONVS] + [L.asets_nn_head_scratch_bytes(vox[0])])
             b["scratch"] = torch.empty(need, dtype=torch.uint8, device=self.device)
             b["hgrad"] = torch.empty(64, dtype=torch.float32, device=self.device)
         self._bufs[key] = (G, vox, b)

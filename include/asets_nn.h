@@ -61,7 +61,9 @@ asets_status asets_nn_relu_mask(void *g, const void *y, int64_t n, void *stream)
  * dbias [cout] fp32 (may be NULL); accumulate != 0 adds to them.  cout a
  * multiple of 16 up to 64; cin 8 or a multiple of 16 up to 128.  scratch: device memory of
  * asets_nn_wgrad_scratch_bytes().  Computed on the matrix cores (bf16 products, fp32
- * sums) per tap and voxel chunk; the chunks are added in a fixed order (deterministic). */
+ * sums): each workgroup accumulates all 27 taps of one 16-channel input chunk over its
+ * share of 64x2x2-voxel boxes, and the workgroups' partials are added in a fixed order
+ * (deterministic). */
 size_t asets_nn_wgrad_scratch_bytes(int n, int d, int h, int w, int cin, int cout);
 asets_status asets_nn_conv3_wgrad(const void *in, int n, int d, int h, int w, int cin,
                                   const void *gout, int cout, float *dw, float *dbias,

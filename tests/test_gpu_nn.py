@@ -356,6 +356,24 @@ def test_network_forward_and_gradients(cuda, shape, n):
     print(f"network gradient check: worst relative L2 error {worst:.3e}")
 
 
+def test_training_step_many_boxes(cuda):
+    """A training step where every level has more weight-gradient boxes than workgroups
+    (scratch sized from every layer): finite loss and gradients, and the same bits when
+    repeated (deterministic reductions)."""
+    shape, n = (128, 64, 64), 2
+    Cs, Ct, A, TU, TPT = _net_inputs(shape, n, 21)
+    d = [torch.from_numpy(x).to(cuda) for x in (Cs, Ct, A, TU, TPT)]
+    net = W.WarmStartNet(cuda, seed=2)
+    grads, losses = [], []
+    for _ in range(2):
+        u0, pt0 = net.forward(d[0], d[1], d[2], shape, n, train=True)
+        losses.append(float(net.backward(d[0], d[1], u0, pt0, d[3], d[4], shape, n)))
+        grads.append(net.grad.clone())
+    torch.cuda.synchronize()
+    assert np.isfinite(losses).all() and torch.isfinite(grads[0]).all()
+    assert losses[0] == losses[1] and torch.equal(grads[0], grads[1])
+
+
 def test_training_reduces_loss(cuda):
     shape, n = (24, 24, 16), 2
     Cs, Ct, A, TU, TPT = _net_inputs(shape, n, 99)
