@@ -84,17 +84,17 @@ struct Geo5 {
 // multiple of 8, so a lane's 8 channels come from one source, and the values are
 // those asets_nn_up2 + asets_nn_chan_copy would have stored: same bits.
 // ---------------------------------------------------------------------------
-template <int COUT, bool UPCAT = false>
+template <int COUT, bool UPCAT = false, int MT = 4>
 __global__ __launch_bounds__(256) void k_conv3_fwd(const u16 *__restrict__ in, Geo5 g, int cin,
                                                    const u16 *__restrict__ w,
                                                    const float *__restrict__ bias, int relu,
                                                    u16 *__restrict__ out,
                                                    const u16 *__restrict__ skip = nullptr,
                                                    int ca = 0) {
-    constexpr int MT = 4, NT = COUT / 16;
+    constexpr int NT = COUT / 16;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int P = g.n * g.d * g.h * g.w;
-    const int p0 = ((int)blockIdx.x * 4 + wv) * 64;
+    const int p0 = ((int)blockIdx.x * 4 + wv) * (16 * MT);
     const int r = lane & 15, kg = lane >> 4;
     const int ch = g.h >> 1, cw = g.w >> 1, cs = cin - ca;   // coarse extents (UPCAT)
     int pz[MT], py[MT], px[MT], pb[MT], pc[MT];
@@ -987,6 +987,223 @@ __global__ __launch_bounds__(256) void k_head_bwd(const u16 *x, int P, const flo
     }
 }
 
+// ---------------------------------------------------------------------------
+// Space-to-depth variant: the network runs at half resolution on 2x2x2 blocks of
+// voxels and predicts the full solver state of every voxel of the block.
+// Coarse voxel (v, z2, x2, y2) of geometry gc (the fine extents halved) holds the
+// sub-voxels s = (dz*2 + dx)*2 + dy at fine (2*z2+dz, 2*x2+dx, 2*y2+dy).
+// ---------------------------------------------------------------------------
+// fine voxel index of sub-voxel s of coarse voxel (v, z2, x2, y2)
+__device__ __forceinline__ int s2d_fine(const Geo5 &gc, int v, int z2, int x2, int y2, int s) {
+    const int dz = s >> 2, dx = (s >> 1) & 1, dy = s & 1;
+    const int H = 2 * gc.h, W = 2 * gc.w;
+    return ((v * 2 * gc.d + 2 * z2 + dz) * H + 2 * x2 + dx) * W + 2 * y2 + dy;
+}
+
+// input features [P2, 32]: channel s*4 + f of sub-voxel s, f = Cs/s, Ct/s, alpha,
+// (Ct-Cs)/s with s = kCostScale
+__global__ void k_features_s2d(const float *Cs, const float *Ct, const float *alpha, Geo5 gc,
+                               u16 *out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int P2 = gc.n * gc.d * gc.h * gc.w;
+    if (p >= P2) return;
+    const int y2 = p % gc.w, x2 = (p / gc.w) % gc.h, z2 = (p / (gc.w * gc.h)) % gc.d;
+    const int v = p / (gc.w * gc.h * gc.d);
+    const float k = 1.0f / kCostScale;
+    uint32_t wd[16];
+#pragma unroll
+    for (int zx = 0; zx < 4; ++zx) {              // the dy = 0, 1 pair is one 8-byte load
+        const int g = s2d_fine(gc, v, z2, x2, y2, 2 * zx);
+        const float2 cs2 = *(const float2 *)(Cs + g), ct2 = *(const float2 *)(Ct + g);
+        const float2 a2 = *(const float2 *)(alpha + g);
+        const float cs[2] = {cs2.x, cs2.y}, ct[2] = {ct2.x, ct2.y}, a[2] = {a2.x, a2.y};
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy) {
+            const int s = 2 * zx + dy;
+            wd[2 * s] = (uint32_t)f2bf(cs[dy] * k) | ((uint32_t)f2bf(ct[dy] * k) << 16);
+            wd[2 * s + 1] = (uint32_t)f2bf(a[dy]) | ((uint32_t)f2bf((ct[dy] - cs[dy]) * k) << 16);
+        }
+    }
+    uint4 *o = (uint4 *)(out + (size_t)p * 32);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) o[q] = make_uint4(wd[4 * q], wd[4 * q + 1], wd[4 * q + 2], wd[4 * q + 3]);
+}
+
+// head: 1x1x1 convolution of x [P2, 16] to 32 outputs o = s*4 + q, written to
+// sub-voxel s of the fine field q (u0, bx0, by0, bz0).  w [32][16], bias [32], staged in
+// LDS once per workgroup (every lane then reads the same 16 bytes: a broadcast).  The
+// two sub-voxels dy = 0, 1 of a (dz, dx) pair are adjacent in a fine field: one 8-byte
+// store, contiguous across the lanes of a row.
+__global__ __launch_bounds__(256) void k_head_s2d_fwd(const u16 *x, Geo5 gc, const float *w,
+                                                      const float *bias, float *u0, float *bx0,
+                                                      float *by0, float *bz0) {
+    __shared__ float4 Wl[32 * 4];
+    __shared__ float Bl[32];
+    for (int i = threadIdx.x; i < 32 * 4; i += blockDim.x) Wl[i] = ((const float4 *)w)[i];
+    if (threadIdx.x < 32) Bl[threadIdx.x] = bias[threadIdx.x];
+    __syncthreads();
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int P2 = gc.n * gc.d * gc.h * gc.w;
+    if (p >= P2) return;
+    const int y2 = p % gc.w, x2 = (p / gc.w) % gc.h, z2 = (p / (gc.w * gc.h)) % gc.d;
+    const int v = p / (gc.w * gc.h * gc.d);
+    float xv[16];
+    const uint4 *xp = (const uint4 *)(x + (size_t)p * 16);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const uint4 t = xp[q];
+        const uint32_t wd[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            xv[8 * q + 2 * j] = bf2f((u16)(wd[j] & 0xffffu));
+            xv[8 * q + 2 * j + 1] = bf2f((u16)(wd[j] >> 16));
+        }
+    }
+    float *dst[4] = {u0, bx0, by0, bz0};
+#pragma unroll
+    for (int zx = 0; zx < 4; ++zx) {              // (dz, dx) = (zx >> 1, zx & 1)
+        const int g = s2d_fine(gc, v, z2, x2, y2, 2 * zx);   // dy = 0; dy = 1 is g + 1
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float a2[2];
+#pragma unroll
+            for (int dy = 0; dy < 2; ++dy) {
+                const int o = (2 * zx + dy) * 4 + q;
+                float a = Bl[o];
+#pragma unroll
+                for (int c4 = 0; c4 < 4; ++c4) {
+                    const float4 wv = Wl[o * 4 + c4];
+                    a += wv.x * xv[4 * c4];
+                    a += wv.y * xv[4 * c4 + 1];
+                    a += wv.z * xv[4 * c4 + 2];
+                    a += wv.w * xv[4 * c4 + 3];
+                }
+                a2[dy] = a;
+            }
+            *(float2 *)(dst[q] + g) = make_float2(a2[0], a2[1]);
+        }
+    }
+}
+
+// loss = (1/P) sum over the P fine voxels of (u0 - tu)^2 + lam * sum_q (b_q - tb_q)^2
+// (q over bx, by, bz).  Per coarse voxel: the 32 output gradients into gbuf [P2][32],
+// the input gradient dx [P2, 16] (bf16) and the voxel's loss sum into lossv [P2].
+__global__ void k_head_s2d_bwd(const u16 *x, Geo5 gc, const float *w, const float *u0,
+                               const float *bx0, const float *by0, const float *bz0,
+                               const float *tu, const float *tbx, const float *tby,
+                               const float *tbz, float lam, u16 *dx, float *gbuf,
+                               float *lossv) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int P2 = gc.n * gc.d * gc.h * gc.w;
+    if (p >= P2) return;
+    const int y2 = p % gc.w, x2 = (p / gc.w) % gc.h, z2 = (p / (gc.w * gc.h)) % gc.d;
+    const int v = p / (gc.w * gc.h * gc.d);
+    const float inv = 2.0f / (8.0f * (float)P2);
+    const float *out[4] = {u0, bx0, by0, bz0};
+    const float *tgt[4] = {tu, tbx, tby, tbz};
+    float gx[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) gx[c] = 0.0f;
+    float l = 0.0f;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int g = s2d_fine(gc, v, z2, x2, y2, s);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int o = s * 4 + q;
+            const float wq = q == 0 ? 1.0f : lam;
+            const float e = out[q][g] - tgt[q][g];
+            l += wq * e * e;
+            const float go = wq * e * inv;
+            gbuf[(size_t)p * 32 + o] = go;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) gx[c] += go * w[o * 16 + c];
+        }
+    }
+    uint32_t wd[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        wd[j] = (uint32_t)f2bf(gx[2 * j]) | ((uint32_t)f2bf(gx[2 * j + 1]) << 16);
+    uint4 *d = (uint4 *)(dx + (size_t)p * 16);
+    d[0] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+    d[1] = make_uint4(wd[4], wd[5], wd[6], wd[7]);
+    lossv[p] = l;
+}
+
+// Per chunk of kHeadChunk coarse voxels: partial sums part[chunk][545] of
+// dL/dw[o][c] = sum_p g[p][o] x[p][c] (512), dL/db[o] = sum_p g[p][o] (32) and the
+// loss sum (1), each thread one value, voxels in order (k_sum_chunks adds the chunks
+// in order: deterministic).
+constexpr int kHeadChunk = 128;
+constexpr int kHeadGrads = 545;
+__global__ __launch_bounds__(576) void k_head_s2d_wgrad(const u16 *x, const float *gbuf,
+                                                        const float *lossv, int P2,
+                                                        float *part) {
+    __shared__ float G[kHeadChunk][33];
+    __shared__ float X[kHeadChunk][17];
+    __shared__ float Lv[kHeadChunk];
+    const int p0 = blockIdx.x * kHeadChunk;
+    const int np = min(kHeadChunk, P2 - p0);
+    for (int i = threadIdx.x; i < kHeadChunk * 32; i += blockDim.x) {
+        const int pp = i / 32, o = i % 32;
+        G[pp][o] = pp < np ? gbuf[(size_t)(p0 + pp) * 32 + o] : 0.0f;
+    }
+    for (int i = threadIdx.x; i < kHeadChunk * 16; i += blockDim.x) {
+        const int pp = i / 16, c = i % 16;
+        X[pp][c] = pp < np ? bf2f(x[(size_t)(p0 + pp) * 16 + c]) : 0.0f;
+    }
+    for (int i = threadIdx.x; i < kHeadChunk; i += blockDim.x) Lv[i] = i < np ? lossv[p0 + i] : 0.0f;
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t >= kHeadGrads) return;
+    float s = 0.0f;
+    if (t < 512) {
+        const int o = t / 16, c = t % 16;
+        for (int pp = 0; pp < kHeadChunk; ++pp) s += G[pp][o] * X[pp][c];
+    } else if (t < 544) {
+        const int o = t - 512;
+        for (int pp = 0; pp < kHeadChunk; ++pp) s += G[pp][o];
+    } else {
+        for (int pp = 0; pp < kHeadChunk; ++pp) s += Lv[pp];
+    }
+    part[(size_t)blockIdx.x * kHeadGrads + t] = s;
+}
+
+// Solver state from a predicted (u0, bx0, by0, bz0), per voxel of n volumes of
+// (ny, nx, nz) (column-major, back to back), in place: u0 clamped to [0, 1], the
+// zero-flux faces (axis index 0) set to 0; ps, pt the source and sink flows that
+// the iteration's fixed point pairs with them (asetsBinaryMF3D_mex.c:411-429 at
+// du = 0): dv = div b; u >= 1/2: pt = Ct (the sink saturated), else
+// pt = min(Cs - dv, Ct); ps = pt + dv.
+__global__ void k_warm_state(float *u0, float *bx0, float *by0, float *bz0, const float *Cs,
+                             const float *Ct, int ny, int nx, int nz, int64_t total, float *ps,
+                             float *pt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int64_t plane = (int64_t)nx * ny, N = plane * nz;
+    const int64_t gl = i % N;
+    const int y = (int)(gl % ny), x = (int)((gl / ny) % nx), z = (int)(gl / plane);
+    const float byl = y > 0 ? by0[i] : 0.0f;
+    const float bxl = x > 0 ? bx0[i] : 0.0f;
+    const float bzl = z > 0 ? bz0[i] : 0.0f;
+    const float byh = y + 1 < ny ? by0[i + 1] : 0.0f;
+    const float bxh = x + 1 < nx ? bx0[i + ny] : 0.0f;
+    const float bzh = z + 1 < nz ? bz0[i + plane] : 0.0f;
+    if (y == 0) by0[i] = 0.0f;
+    if (x == 0) bx0[i] = 0.0f;
+    if (z == 0) bz0[i] = 0.0f;
+    const float dv = byh - byl + bxh - bxl + bzh - bzl;
+    float u = u0[i];
+    u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
+    u0[i] = u;
+    const float cs = Cs[i], ct = Ct[i];
+    float t = cs - dv;
+    t = t < ct ? t : ct;
+    const float p_t = u >= 0.5f ? ct : t;
+    pt[i] = p_t;
+    ps[i] = p_t + dv;
+}
+
 // AdamW on fp32 master weights; refreshes the bf16 copy when wb != null
 __global__ void k_adamw(float *w, u16 *wb, const float *g, float *m, float *v, int n, float lr,
                         float b1, float b2, float eps, float wd, float bc1, float bc2) {
@@ -1048,6 +1265,9 @@ unsigned box_grid(const Geo5 &g) {
 // channels, the per-box one for cin a multiple of 16 up to 64 output channels
 // (for UPCAT with a coarse channel count that is a multiple of 16).  Other layers
 // (the coarse levels of small inputs among them) keep the gathering kernel.
+// voxel count below which the gathering convolution runs 16 voxels per wave
+constexpr int64_t kSmallConv = (int64_t)1 << 18;
+
 template <int COUT, bool UPCAT>
 void launch_conv(const u16 *x, const Geo5 &g, int cin, const u16 *wb, const float *bias,
                  int relu, u16 *y, const u16 *skip, int ca, hipStream_t s) {
@@ -1073,7 +1293,16 @@ void launch_conv(const u16 *x, const Geo5 &g, int cin, const u16 *wb, const floa
             }
         }
     }
-    k_conv3_fwd<COUT, UPCAT><<<(unsigned)cdiv((int64_t)g.n * g.d * g.h * g.w, 256), 256, 0, s>>>(
+    // small tensors (the coarse levels): 16 voxels per wave instead of 64, so the grid
+    // fills the chip; each output keeps its K order (same bits).  (The gathered decoder
+    // input measured slower that way: 112 -> 124 us at 32^3, 128 -> 64 channels.)
+    const int64_t P = (int64_t)g.n * g.d * g.h * g.w;
+    if (!UPCAT && P < kSmallConv) {
+        k_conv3_fwd<COUT, UPCAT, 1><<<(unsigned)cdiv(P, 64), 256, 0, s>>>(
+            x, g, cin, wb, bias, relu, y, skip, ca);
+        return;
+    }
+    k_conv3_fwd<COUT, UPCAT><<<(unsigned)cdiv(P, 256), 256, 0, s>>>(
         x, g, cin, wb, bias, relu, y, skip, ca);
 }
 
@@ -1282,6 +1511,95 @@ asets_status asets_nn_head_bwd(const void *x, int64_t P, const float *w, const f
     float *part = (float *)scratch;
     k_head_bwd<<<nb, kB, 0, s>>>((const u16 *)x, (int)P, w, u0, pt0, tu, tpt, (u16 *)dx, part);
     sum_chunks(part, nb, 35, grads, 0, s);
+    NNCHK();
+    return ASETS_OK;
+}
+
+static asets_status s2d_geo(int n, int d, int h, int w, Geo5 *gc) {
+    if (n <= 0 || d <= 0 || h <= 0 || w <= 0 || (d | h | w) & 1)
+        return nn_fail(ASETS_EBADARG, "extents must be positive and even");
+    if ((int64_t)n * d * h * w * 4 >= kMaxElems)
+        return nn_fail(ASETS_ETOOBIG, "tensor of more than 2^31 elements");
+    *gc = Geo5{n, d / 2, h / 2, w / 2};
+    return ASETS_OK;
+}
+
+asets_status asets_nn_features_s2d(const float *Cs, const float *Ct, const float *alpha, int n,
+                                   int d, int h, int w, void *out, void *stream) {
+    Geo5 gc;
+    asets_status st = s2d_geo(n, d, h, w, &gc);
+    if (st) return st;
+    if (!Cs || !Ct || !alpha || !out) return nn_fail(ASETS_EBADARG, "null pointer");
+    if (((uintptr_t)Cs | (uintptr_t)Ct | (uintptr_t)alpha) & 7)
+        return nn_fail(ASETS_EBADARG, "Cs, Ct and alpha must be 8-byte aligned");
+    const int64_t P2 = (int64_t)gc.n * gc.d * gc.h * gc.w;
+    k_features_s2d<<<nblk(P2), kB, 0, (hipStream_t)stream>>>(Cs, Ct, alpha, gc, (u16 *)out);
+    NNCHK();
+    return ASETS_OK;
+}
+
+asets_status asets_nn_head_s2d(const void *x, int n, int d, int h, int w, const float *wt,
+                               const float *b, float *u0, float *bx0, float *by0, float *bz0,
+                               void *stream) {
+    Geo5 gc;
+    asets_status st = s2d_geo(n, d, h, w, &gc);
+    if (st) return st;
+    if (!x || !wt || !b || !u0 || !bx0 || !by0 || !bz0)
+        return nn_fail(ASETS_EBADARG, "null pointer");
+    if (((uintptr_t)u0 | (uintptr_t)bx0 | (uintptr_t)by0 | (uintptr_t)bz0) & 7)
+        return nn_fail(ASETS_EBADARG, "output fields must be 8-byte aligned");
+    if ((uintptr_t)wt & 15) return nn_fail(ASETS_EBADARG, "head weights must be 16-byte aligned");
+    const int64_t P2 = (int64_t)gc.n * gc.d * gc.h * gc.w;
+    k_head_s2d_fwd<<<nblk(P2), kB, 0, (hipStream_t)stream>>>((const u16 *)x, gc, wt, b, u0, bx0,
+                                                             by0, bz0);
+    NNCHK();
+    return ASETS_OK;
+}
+
+size_t asets_nn_head_s2d_scratch_bytes(int n, int d, int h, int w) {
+    const int64_t P2 = (int64_t)n * (d / 2) * (h / 2) * (w / 2);
+    if (P2 <= 0) return 0;
+    return (size_t)(P2 * 33 + cdiv(P2, kHeadChunk) * kHeadGrads) * sizeof(float);
+}
+
+asets_status asets_nn_head_s2d_bwd(const void *x, int n, int d, int h, int w, const float *wt,
+                                   const float *u0, const float *bx0, const float *by0,
+                                   const float *bz0, const float *tu, const float *tbx,
+                                   const float *tby, const float *tbz, float lam, void *dx,
+                                   float *grads, void *scratch, size_t scratch_bytes,
+                                   void *stream) {
+    Geo5 gc;
+    asets_status st = s2d_geo(n, d, h, w, &gc);
+    if (st) return st;
+    if (!x || !wt || !u0 || !bx0 || !by0 || !bz0 || !tu || !tbx || !tby || !tbz || !dx ||
+        !grads)
+        return nn_fail(ASETS_EBADARG, "null pointer");
+    if (!scratch || scratch_bytes < asets_nn_head_s2d_scratch_bytes(n, d, h, w))
+        return nn_fail(ASETS_EBADARG, "scratch too small");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t P2 = (int64_t)gc.n * gc.d * gc.h * gc.w;
+    float *gbuf = (float *)scratch;
+    float *lossv = gbuf + P2 * 32;
+    float *part = lossv + P2;
+    k_head_s2d_bwd<<<nblk(P2), kB, 0, s>>>((const u16 *)x, gc, wt, u0, bx0, by0, bz0, tu, tbx,
+                                           tby, tbz, lam, (u16 *)dx, gbuf, lossv);
+    const int nch = (int)cdiv(P2, kHeadChunk);
+    k_head_s2d_wgrad<<<nch, 576, 0, s>>>((const u16 *)x, gbuf, lossv, (int)P2, part);
+    sum_chunks(part, nch, kHeadGrads, grads, 0, s);
+    NNCHK();
+    return ASETS_OK;
+}
+
+asets_status asets_nn_warm_state(float *u0, float *bx0, float *by0, float *bz0, const float *Cs,
+                                 const float *Ct, int64_t ny, int64_t nx, int64_t nz, int n,
+                                 float *ps, float *pt, void *stream) {
+    if (ny <= 0 || nx <= 0 || nz <= 0 || n <= 0 || ny * nx >= kMaxElems)
+        return nn_fail(ASETS_EBADARG, "bad extents");
+    if (!u0 || !bx0 || !by0 || !bz0 || !Cs || !Ct || !ps || !pt)
+        return nn_fail(ASETS_EBADARG, "null pointer");
+    const int64_t total = ny * nx * nz * n;
+    k_warm_state<<<(unsigned)cdiv(total, kB), kB, 0, (hipStream_t)stream>>>(
+        u0, bx0, by0, bz0, Cs, Ct, (int)ny, (int)nx, (int)nz, total, ps, pt);
     NNCHK();
     return ASETS_OK;
 }

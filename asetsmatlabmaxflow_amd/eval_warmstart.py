@@ -6,9 +6,11 @@ For each volume (workloads.binary3d_ellipsoids, the given seeds), with inputs re
 the GPU:
   reference  plain solver, --ref-iters fixed iterations: E* and the reference labels
   cold       plain solver from the initial cut to errBound (the reference's rule)
-  warm       network inference + plain solver from (u0, pt0) to errBound
+  warm       network inference + plain solver from (u0, pt0) to errBound; with
+             --variant s2d, inference of the full state + asets_binary_dev_resume
   ideal      plain solver from the reference's own (u, pt) to errBound: the bound on what
              any (u0, pt0) predictor can gain (the flows restart from 0 either way)
+  ideal_full_state  (--variant s2d) resume from the reference's whole state
 Reported per run: wall seconds (warm includes inference), iterations, relative energy gap
 (E - E*) / |E*| and label agreement with the reference; for the network output alone its
 energy gap and label agreement.  Also the iterations each start needs to reach a relative
@@ -31,7 +33,7 @@ def _sync():
 
 
 def evaluate_volume(net, seed: int, size: int, err_bound: float, ref_iters: int,
-                    gap_target: float = 1e-5):
+                    gap_target: float = 1e-5, gap: bool = True):
     import torch
 
     from asetsmatlabmaxflow_amd import solver, workloads
@@ -66,13 +68,28 @@ def evaluate_volume(net, seed: int, size: int, err_bound: float, ref_iters: int,
     u_id, pt_id = ref.u.clone(), ref.state["pt"].clone()
     r, s = timed(lambda: solver.binary(*d, shape, 20000, err_bound, cc, st, u0=u_id, pt0=pt_id))
     out["ideal_warm"] = {"seconds": s, "iterations": r.iterations, **quality(r.u)}
+    full = hasattr(net, "predict_state")
+    if full:
+        # the exact full state of the reference solve: the bound on what a full-state
+        # predictor can gain
+        ref_st = {"u": ref.u, **ref.state}
+        r, s = timed(lambda: solver.binary_resume(*d, shape, ref_st, 20000, err_bound, cc, st))
+        out["ideal_full_state"] = {"seconds": s, "iterations": r.iterations, **quality(r.u)}
     if net is not None:
-        (u0, pt0), s_inf = timed(lambda: net.forward(d[0], d[1], d[2], shape, 1))
-        out["network_output"] = {"inference_seconds": s_inf, **quality(u0)}
+        if full:
+            pst, s_inf = timed(lambda: net.predict_state(d[0], d[1], d[2], shape, 1))
+            out["network_output"] = {"inference_seconds": s_inf, **quality(pst["u"])}
 
-        def warm():
-            w0, w1 = net.forward(d[0], d[1], d[2], shape, 1)
-            return solver.binary(*d, shape, 20000, err_bound, cc, st, u0=w0, pt0=w1)
+            def warm():
+                w = net.predict_state(d[0], d[1], d[2], shape, 1)
+                return solver.binary_resume(*d, shape, w, 20000, err_bound, cc, st)
+        else:
+            (u0, pt0), s_inf = timed(lambda: net.forward(d[0], d[1], d[2], shape, 1))
+            out["network_output"] = {"inference_seconds": s_inf, **quality(u0)}
+
+            def warm():
+                w0, w1 = net.forward(d[0], d[1], d[2], shape, 1)
+                return solver.binary(*d, shape, 20000, err_bound, cc, st, u0=w0, pt0=w1)
 
         r, s = timed(warm)
         out["warm"] = {"seconds": s, "iterations": r.iterations, **quality(r.u)}
@@ -87,9 +104,11 @@ def evaluate_volume(net, seed: int, size: int, err_bound: float, ref_iters: int,
             k *= 2
         return None
 
+    if not gap:
+        return out
     out[f"iters_to_gap_{gap_target:g}"] = {"cold": iters_to_gap(),
                                            "ideal_warm": iters_to_gap(u0=u_id, pt0=pt_id)}
-    if net is not None:
+    if net is not None and not full:
         u0, pt0 = net.forward(d[0], d[1], d[2], shape, 1)
         out[f"iters_to_gap_{gap_target:g}"]["warm"] = iters_to_gap(u0=u0.clone(),
                                                                    pt0=pt0.clone())
@@ -103,22 +122,28 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--err-bound", type=float, default=1e-4)
     ap.add_argument("--ref-iters", type=int, default=2000)
+    ap.add_argument("--variant", default="full", choices=["full", "s2d"])
+    ap.add_argument("--no-gap", action="store_true",
+                    help="skip the iterations-to-energy-gap search")
     args = ap.parse_args()
 
     import torch
 
-    from asetsmatlabmaxflow_amd.warmstart import WarmStartNet
+    from asetsmatlabmaxflow_amd.warmstart import WarmStartNet, WarmStartNetS2D
 
     net = None
     if args.weights:
-        net = WarmStartNet(torch.device("cuda:0"))
+        net = (WarmStartNetS2D if args.variant == "s2d" else WarmStartNet)(
+            torch.device("cuda:0"))
         net.load(args.weights)
     rows = []
     for seed in args.seeds:
-        row = evaluate_volume(net, seed, args.size, args.err_bound, args.ref_iters)
+        row = evaluate_volume(net, seed, args.size, args.err_bound, args.ref_iters,
+                              gap=not args.no_gap)
         rows.append(row)
         print(json.dumps(row), flush=True)
-    keys = ["cold", "ideal_warm"] + (["warm"] if net is not None else [])
+    keys = ["cold", "ideal_warm"] + (["ideal_full_state"] if "ideal_full_state" in rows[0]
+                                     else []) + (["warm"] if net is not None else [])
     summary = {k: {"mean_seconds": float(np.mean([r[k]["seconds"] for r in rows])),
                    "mean_iterations": float(np.mean([r[k]["iterations"] for r in rows])),
                    "mean_gap": float(np.mean([r[k]["gap"] for r in rows])),

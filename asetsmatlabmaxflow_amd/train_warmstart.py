@@ -7,8 +7,10 @@ A long run can be split: --checkpoint saves weights, AdamW moments and the step 
 --resume continues from them, and --total-steps keeps one cosine schedule across parts.
 
 Data: volumes of the 256^3 ellipsoid generator (workloads.binary3d_ellipsoids) at
---size^3, seeds 2000, 2001, ...; targets (u, pt) are the plain solver's state after
---target-iters fixed iterations (errBound 0).  Loss: mean (u0-u)^2 + mean ((pt0-pt)/10)^2.
+--size^3, seeds 2000, 2001, ...; targets are the plain solver's state after
+--target-iters fixed iterations (errBound 0): (u, pt) for --variant full, with loss
+mean (u0-u)^2 + mean ((pt0-pt)/10)^2; (u, bx, by, bz) for --variant s2d, with loss
+mean (u0-u)^2 + 2 * mean of the squared face-flow errors.
 Optimiser: AdamW (lr --lr, betas 0.9/0.999, weight decay 0.01), cosine decay to 10 %.
 Every forward/backward/optimiser operation is a kernel of libasets_nn.so.
 Prints one progress line every --log-every steps and a JSON summary at the end.
@@ -35,17 +37,18 @@ def _gen(args):
 
 
 def make_dataset(nvol: int, size: int, seed0: int, target_iters: int, device, workers: int,
-                 log=print):
+                 log=print, flows: bool = False):
     """Generate nvol volumes on the CPU (process pool), solve each on the GPU, and return
-    device tensors Cs, Ct, alpha, tu, tpt of shape [nvol * size^3]."""
+    device tensors Cs, Ct, alpha, tu, tpt (and with flows the face flows tbx, tby, tbz)
+    of shape [nvol * size^3]."""
     import torch
 
     from asetsmatlabmaxflow_amd import solver, workloads
 
     N = size ** 3
     shape = (size, size, size)
-    bufs = {k: torch.empty(nvol * N, dtype=torch.float32, device=device)
-            for k in ("Cs", "Ct", "alpha", "tu", "tpt")}
+    keys = ("Cs", "Ct", "alpha", "tu") + (("tbx", "tby", "tbz") if flows else ("tpt",))
+    bufs = {k: torch.empty(nvol * N, dtype=torch.float32, device=device) for k in keys}
     cc, st = workloads.BINARY3D_CC, workloads.BINARY3D_STEPS
     t0 = time.time()
     # spawned workers (numpy only) never inherit this process's GPU context
@@ -56,7 +59,12 @@ def make_dataset(nvol: int, size: int, seed0: int, target_iters: int, device, wo
                 bufs[k][sl].copy_(torch.from_numpy(arr))
             r = solver.binary(bufs["Cs"][sl], bufs["Ct"][sl], bufs["alpha"][sl], shape,
                               target_iters, 0.0, cc, st, want_state=True, out=bufs["tu"][sl])
-            bufs["tpt"][sl].copy_(r.state["pt"])
+            if flows:
+                for k in ("bx", "by", "bz"):
+                    bufs["t" + k][sl].copy_(r.state[k])
+            else:
+                bufs["tpt"][sl].copy_(r.state["pt"])
+            del r
             if (v + 1) % max(1, nvol // 8) == 0:
                 log(f"dataset {v + 1}/{nvol} volumes, {time.time() - t0:.1f} s")
     torch.cuda.synchronize()
@@ -83,8 +91,14 @@ def train(net, data, size: int, nvol: int, steps: int, batch: int, lr: float, se
         g = int(rng.integers(ngroups))
         sl = slice(g * batch * N, (g + 1) * batch * N)
         Cs, Ct, a = data["Cs"][sl], data["Ct"][sl], data["alpha"][sl]
-        u0, pt0 = net.forward(Cs, Ct, a, shape, batch, train=True)
-        loss = net.backward(Cs, Ct, u0, pt0, data["tu"][sl], data["tpt"][sl], shape, batch)
+        if "tbx" in data:   # space-to-depth network, full state
+            pred = net.forward(Cs, Ct, a, shape, batch, train=True)
+            loss = net.backward_state(pred, {k: data["t" + k][sl] for k in
+                                             ("u", "bx", "by", "bz")}, shape, batch)
+        else:
+            u0, pt0 = net.forward(Cs, Ct, a, shape, batch, train=True)
+            loss = net.backward(Cs, Ct, u0, pt0, data["tu"][sl], data["tpt"][sl], shape,
+                                batch)
         lr_s = lr * (0.1 + 0.9 * 0.5 * (1 + math.cos(math.pi * s / max(1, total))))
         net.adamw_step(lr=lr_s)
         if (i + 1) % log_every == 0 or i == 0:
@@ -114,19 +128,23 @@ def main():
     ap.add_argument("--total-steps", type=int, default=0,
                     help="length of the cosine schedule (default: this run's last step)")
     ap.add_argument("--checkpoint", default=None, help="write a checkpoint here at the end")
+    ap.add_argument("--variant", default="full", choices=["full", "s2d"],
+                    help="full: WarmStartNet predicting (u0, pt0); s2d: WarmStartNetS2D "
+                         "predicting (u0, bx0, by0, bz0)")
     ap.add_argument("--out", required=True)
     args = ap.parse_args()
 
     import torch
 
-    from asetsmatlabmaxflow_amd.warmstart import WarmStartNet
+    from asetsmatlabmaxflow_amd.warmstart import WarmStartNet, WarmStartNetS2D
 
     dev = torch.device("cuda:0")
     t0 = time.time()
     data = make_dataset(args.volumes, args.size, args.seed0, args.target_iters, dev,
-                        args.workers, log=lambda m: print(m, flush=True))
+                        args.workers, log=lambda m: print(m, flush=True),
+                        flows=args.variant == "s2d")
     data_s = time.time() - t0
-    net = WarmStartNet(dev, seed=args.seed)
+    net = (WarmStartNetS2D if args.variant == "s2d" else WarmStartNet)(dev, seed=args.seed)
     if args.init:
         net.load(args.init)
     if args.resume:
@@ -137,7 +155,8 @@ def main():
     net.save(args.out)
     if args.checkpoint:
         net.save_checkpoint(args.checkpoint)
-    print(json.dumps({"volumes": args.volumes, "size": args.size, "steps": args.steps,
+    print(json.dumps({"variant": args.variant, "volumes": args.volumes, "size": args.size,
+                      "steps": args.steps,
                       "batch": args.batch, "lr": args.lr, "target_iters": args.target_iters,
                       "dataset_s": data_s, "train_s": train_s,
                       "ms_per_step": train_s / max(1, args.steps) * 1e3,

@@ -51,7 +51,9 @@ NN_EXPORTED = (
     "asets_nn_wgrad_scratch_bytes", "asets_nn_conv3_wgrad", "asets_nn_pool2",
     "asets_nn_pool2_bwd", "asets_nn_up2", "asets_nn_up2_bwd", "asets_nn_chan_copy",
     "asets_nn_features", "asets_nn_head", "asets_nn_head_scratch_bytes", "asets_nn_head_bwd",
-    "asets_nn_adamw", "asets_nn_to_bf16", "asets_nn_last_error",
+    "asets_nn_adamw", "asets_nn_to_bf16", "asets_nn_last_error", "asets_nn_features_s2d",
+    "asets_nn_head_s2d", "asets_nn_head_s2d_scratch_bytes", "asets_nn_head_s2d_bwd",
+    "asets_nn_warm_state",
 )
 
 _nn = None
@@ -89,10 +91,17 @@ def nnlib() -> C.CDLL:
         L.asets_nn_adamw.argtypes = [P, P, P, P, P, I64, F, F, F, F, F, I, P]
         L.asets_nn_to_bf16.argtypes = [P, P, I64, P]
         L.asets_nn_last_error.restype = C.c_char_p
+        L.asets_nn_features_s2d.argtypes = [P, P, P, I, I, I, I, P, P]
+        L.asets_nn_head_s2d.argtypes = [P, I, I, I, I, P, P, P, P, P, P, P]
+        L.asets_nn_head_s2d_scratch_bytes.argtypes = [I, I, I, I]
+        L.asets_nn_head_s2d_scratch_bytes.restype = C.c_size_t
+        L.asets_nn_head_s2d_bwd.argtypes = [P, I, I, I, I, P, P, P, P, P, P, P, P, P, F, P, P, P,
+                                            C.c_size_t, P]
+        L.asets_nn_warm_state.argtypes = [P, P, P, P, P, P, I64, I64, I64, I, P, P, P]
         for name in NN_EXPORTED:
             f = getattr(L, name)
             if name not in ("asets_nn_wgrad_scratch_bytes", "asets_nn_head_scratch_bytes",
-                            "asets_nn_last_error"):
+                            "asets_nn_head_s2d_scratch_bytes", "asets_nn_last_error"):
                 f.restype = C.c_int
         _nn = L
     return _nn
@@ -117,20 +126,26 @@ class WarmStartNet:
     All parameters live in one flat fp32 buffer (conv weights [27][cout][cin], then biases,
     then the head), mirrored in bf16 for the kernels; AdamW updates both in one launch."""
 
+    CONVS = CONVS
+    HEAD_W, HEAD_B = HEAD_W, HEAD_B
+    NFEAT = 8           # input feature channels at level 0
+    FAN_IN0 = 5         # of which non-zero
+    SHIFT = 0           # level l runs at the input extents >> (l + SHIFT)
+
     def __init__(self, device="cuda", seed: int = 0):
         self.device = torch.device(device)
         self.layout = {}
         off = 0
-        for name, cin, cout in CONVS:
+        for name, cin, cout in self.CONVS:
             self.layout[name + ".w"] = (off, 27 * cout * cin)
             off = _align(off + 27 * cout * cin)
-        for name, cin, cout in CONVS:
+        for name, cin, cout in self.CONVS:
             self.layout[name + ".b"] = (off, cout)
             off = _align(off + cout)
-        self.layout["head.w"] = (off, HEAD_W)
-        off = _align(off + HEAD_W)
-        self.layout["head.b"] = (off, HEAD_B)
-        off = _align(off + HEAD_B)
+        self.layout["head.w"] = (off, self.HEAD_W)
+        off = _align(off + self.HEAD_W)
+        self.layout["head.b"] = (off, self.HEAD_B)
+        off = _align(off + self.HEAD_B)
         self.nparam = off
         self.w32 = torch.zeros(off, dtype=torch.float32, device=self.device)
         self.grad = torch.zeros_like(self.w32)
@@ -138,7 +153,7 @@ class WarmStartNet:
         self.v = torch.zeros_like(self.w32)
         self.w16 = torch.zeros(off, dtype=torch.bfloat16, device=self.device)
         self.wflip = {name: torch.zeros(27 * cin * cout, dtype=torch.bfloat16, device=self.device)
-                      for name, cin, cout in CONVS}
+                      for name, cin, cout in self.CONVS}
         self.step_count = 0
         self._bufs = {}
         self._scratch = None
@@ -153,14 +168,17 @@ class WarmStartNet:
         """He-normal convolution weights, zero biases; head starts at u0 = 0.5, pt0 = min."""
         rng = np.random.default_rng(seed)
         host = np.zeros(self.nparam, dtype=np.float32)
-        for name, cin, cout in CONVS:
+        for name, cin, cout in self.CONVS:
             off, n = self.layout[name + ".w"]
-            fan_in = 27 * (5 if name == "enc0a" else cin)   # 5 non-zero input features
+            fan_in = 27 * (self.FAN_IN0 if name == "enc0a" else cin)
             host[off:off + n] = rng.normal(0.0, np.sqrt(2.0 / fan_in), n)
         off, n = self.layout["head.w"]
         host[off:off + n] = rng.normal(0.0, 0.01, n)
-        host[self.layout["head.b"][0]] = 0.5
+        self._init_head_bias(host, self.layout["head.b"][0])
         self.load_flat(host)
+
+    def _init_head_bias(self, host, off) -> None:
+        host[off] = 0.5   # u0 starts at 0.5
 
     def load_flat(self, host: np.ndarray) -> None:
         self.w32.copy_(torch.from_numpy(np.ascontiguousarray(host, dtype=np.float32)))
@@ -209,10 +227,12 @@ class WarmStartNet:
 
     def _geom(self, n, shape):
         ny, nx, nz = shape
-        if ny % 8 or nx % 8 or nz % 8:
-            raise ValueError("every extent must be a multiple of 8")
+        m = 8 << self.SHIFT
+        if ny % m or nx % m or nz % m:
+            raise ValueError(f"every extent must be a multiple of {m}")
         # NDHWC geometry per level: (n, d, h, w)
-        return [(n, nz >> l, nx >> l, ny >> l) for l in range(4)]
+        return [(n, nz >> (l + self.SHIFT), nx >> (l + self.SHIFT), ny >> (l + self.SHIFT))
+                for l in range(4)]
 
     def _buffers(self, n, shape, train: bool):
         key = (n, tuple(shape), train)
@@ -223,7 +243,7 @@ class WarmStartNet:
         vox = [g[0] * g[1] * g[2] * g[3] for g in G]
         bf = torch.bfloat16
         E = lambda l, c: torch.empty(vox[l] * c, dtype=bf, device=self.device)
-        b = {"x0": E(0, 8), "e0a": E(0, 16), "e0": E(0, 16), "p1": E(1, 16), "e1a": E(1, 32),
+        b = {"x0": E(0, self.NFEAT), "e0a": E(0, 16), "e0": E(0, 16), "p1": E(1, 16), "e1a": E(1, 32),
              "e1": E(1, 32), "p2": E(2, 32), "e2a": E(2, 64), "e2": E(2, 64), "p3": E(3, 64),
              "e3a": E(3, 64), "e3": E(3, 64), "d2": E(2, 64), "d1": E(1, 32), "d0": E(0, 16)}
         if train:
@@ -238,15 +258,19 @@ class WarmStartNet:
             L = nnlib()
             # every convolution's weight gradient (level = the digit in its name) and the head
             need = max([L.asets_nn_wgrad_scratch_bytes(*G[int(nm[3])], ci, co)
-                        for nm, ci, co in CONVS] + [L.asets_nn_head_scratch_bytes(vox[0])])
+                        for nm, ci, co in self.CONVS] + [self._head_scratch(n, shape, vox)])
             b["scratch"] = torch.empty(need, dtype=torch.uint8, device=self.device)
-            b["hgrad"] = torch.empty(64, dtype=torch.float32, device=self.device)
+            b["hgrad"] = torch.empty(_align(self.HEAD_W + self.HEAD_B + 1), dtype=torch.float32,
+                                     device=self.device)
         self._bufs[key] = (G, vox, b)
         return self._bufs[key]
 
+    def _head_scratch(self, n, shape, vox):
+        return nnlib().asets_nn_head_scratch_bytes(vox[0])
+
     # -- forward ---------------------------------------------------------------------
     def _conv(self, name, src, g, dst, relu=True):
-        cin, cout = next((ci, co) for nm, ci, co in CONVS if nm == name)
+        cin, cout = next((ci, co) for nm, ci, co in self.CONVS if nm == name)
         _ck(nnlib().asets_nn_conv3(_p(src), *g, cin, _p(self.view(name + ".w", "w16")),
                                    _p(self.view(name + ".b")), cout, int(relu), _p(dst),
                                    self._stream()))
@@ -254,7 +278,7 @@ class WarmStartNet:
     def _dec(self, name, coarse, ca, skip, cat, g, dst, train):
         """Decoder convolution over [up2(coarse), skip]: materialised into `cat` when
         training (the backward pass reads it), gathered in the convolution otherwise."""
-        cin, cout = next((ci, co) for nm, ci, co in CONVS if nm == name)
+        cin, cout = next((ci, co) for nm, ci, co in self.CONVS if nm == name)
         L, s = nnlib(), self._stream()
         if train:
             _ck(L.asets_nn_up2(_p(coarse), *g, ca, _p(cat), cin, 0, s))
@@ -266,16 +290,9 @@ class WarmStartNet:
                                        _p(self.view(name + ".w", "w16")),
                                        _p(self.view(name + ".b")), cout, 1, _p(dst), s))
 
-    def forward(self, Cs, Ct, alpha, shape, n=1, *, u0=None, pt0=None, train=False):
-        """Predict (u0, pt0) for n volumes of `shape` stored back to back in the flat
-        float32 device tensors Cs, Ct, alpha (column-major each).  Returns (u0, pt0)."""
-        G, vox, b = self._buffers(n, shape, train)
-        P = vox[0]
-        for t in (Cs, Ct, alpha):
-            if t.numel() != P or t.dtype != torch.float32 or not t.is_cuda:
-                raise ValueError(f"expected {P} float32 device values")
+    def _trunk(self, b, G, train) -> None:
+        """Encoder and decoder: b["x0"] -> b["d0"] (16 channels at level 0)."""
         L, s = nnlib(), self._stream()
-        _ck(L.asets_nn_features(_p(Cs), _p(Ct), _p(alpha), P, _p(b["x0"]), s))
         self._conv("enc0a", b["x0"], G[0], b["e0a"])
         self._conv("enc0b", b["e0a"], G[0], b["e0"])
         _ck(L.asets_nn_pool2(_p(b["e0"]), *G[0], 16, _p(b["p1"]), s))
@@ -290,6 +307,21 @@ class WarmStartNet:
         self._dec("dec2", b["e3"], 64, b["e2"], b.get("c2"), G[2], b["d2"], train)
         self._dec("dec1", b["d2"], 64, b["e1"], b.get("c1"), G[1], b["d1"], train)
         self._dec("dec0", b["d1"], 32, b["e0"], b.get("c0"), G[0], b["d0"], train)
+
+    def _check_inputs(self, vox_fine, *ts):
+        for t in ts:
+            if t.numel() != vox_fine or t.dtype != torch.float32 or not t.is_cuda:
+                raise ValueError(f"expected {vox_fine} float32 device values")
+
+    def forward(self, Cs, Ct, alpha, shape, n=1, *, u0=None, pt0=None, train=False):
+        """Predict (u0, pt0) for n volumes of `shape` stored back to back in the flat
+        float32 device tensors Cs, Ct, alpha (column-major each).  Returns (u0, pt0)."""
+        G, vox, b = self._buffers(n, shape, train)
+        P = vox[0]
+        self._check_inputs(P, Cs, Ct, alpha)
+        L, s = nnlib(), self._stream()
+        _ck(L.asets_nn_features(_p(Cs), _p(Ct), _p(alpha), P, _p(b["x0"]), s))
+        self._trunk(b, G, train)
         if u0 is None:
             u0 = torch.empty(P, dtype=torch.float32, device=self.device)
         if pt0 is None:
@@ -302,7 +334,7 @@ class WarmStartNet:
     def _conv_bwd(self, name, src, g, gout, gin, first=False):
         """Weight/bias gradients of conv `name` (input src) from gout; input gradient into
         gin unless first."""
-        cin, cout = next((ci, co) for nm, ci, co in CONVS if nm == name)
+        cin, cout = next((ci, co) for nm, ci, co in self.CONVS if nm == name)
         L, s = nnlib(), self._stream()
         scratch = self._cur["scratch"]
         _ck(L.asets_nn_conv3_wgrad(_p(src), *g, cin, _p(gout), cout,
@@ -322,13 +354,20 @@ class WarmStartNet:
         self._cur = b
         L, s = nnlib(), self._stream()
         P = vox[0]
-        R = lambda g, y, nel: _ck(L.asets_nn_relu_mask(_p(g), _p(y), nel, s))
         hg = b["hgrad"]
         _ck(L.asets_nn_head_bwd(_p(b["d0"]), P, _p(self.view("head.w")), _p(u0), _p(pt0),
                                 _p(tu), _p(tpt), _p(b["g0_16a"]), _p(hg), _p(b["scratch"]),
                                 b["scratch"].numel(), s))
         self.view("head.w", "grad").copy_(hg[:32])
         self.view("head.b", "grad").copy_(hg[32:34])
+        self._trunk_bwd(b, G, vox)
+        return hg[34] / P
+
+    def _trunk_bwd(self, b, G, vox) -> None:
+        """Backward through decoder and encoder from d loss / d d0 in b["g0_16a"]."""
+        L, s = nnlib(), self._stream()
+        P = vox[0]
+        R = lambda g, y, nel: _ck(L.asets_nn_relu_mask(_p(g), _p(y), nel, s))
         # decoder level 0: d0 = relu(conv(c0))
         R(b["g0_16a"], b["d0"], P * 16)
         self._conv_bwd("dec0", b["c0"], G[0], b["g0_16a"], b["g0_48"])
@@ -369,7 +408,6 @@ class WarmStartNet:
         self._conv_bwd("enc0b", b["e0a"], G[0], b["g0_16b"], b["g0_16a"])
         R(b["g0_16a"], b["e0a"], P * 16)
         self._conv_bwd("enc0a", b["x0"], G[0], b["g0_16a"], None, first=True)
-        return hg[34] / P
 
     def adamw_step(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, wd=0.01) -> None:
         self.step_count += 1
@@ -386,3 +424,98 @@ def warm_solve(net: WarmStartNet, Cs, Ct, alpha, shape, max_iter, err_bound, cc,
     u0, pt0 = net.forward(Cs, Ct, alpha, shape, 1)
     return solver.binary(Cs, Ct, alpha, shape, max_iter, err_bound, cc, steps, u0=u0, pt0=pt0,
                          **kw)
+
+
+# The space-to-depth network: the same trunk one level coarser (levels at 1/2 ... 1/16 of
+# the input extents), fed the 2x2x2 blocks' costs as 32 channels and predicting the
+# solver's whole state per voxel.
+CONVS_S2D = (("enc0a", 32, 16),) + CONVS[1:]
+FLOW_LOSS_WEIGHT = 2.0
+
+
+class WarmStartNetS2D(WarmStartNet):
+    """4-level U-Net (16/32/64/64 channels) on 2x2x2 voxel blocks: input features
+    (Cs/10, Ct/10, alpha, (Ct-Cs)/10) of the block's 8 voxels (asets_nn_features_s2d),
+    output (u0, bx0, by0, bz0) of the 8 voxels from a 1x1x1 head (asets_nn_head_s2d).
+    With ps and pt paired to them (asets_nn_warm_state) that is a full solver state,
+    resumed by asets_binary_dev_resume.  Every level runs at half the resolution of
+    WarmStartNet's, so inference moves about an eighth of the activation bytes.
+    Extents must be multiples of 16.
+
+    A start from (u0, pt0) alone restarts the spatial flows at 0, and the flows are what
+    the cold start spends its iterations building: from the exact converged (u, pt)
+    with flows 0 the reference stop rule takes as many iterations as from the initial
+    cut, while the exact full state stops after one."""
+
+    CONVS = CONVS_S2D
+    HEAD_W, HEAD_B = 32 * 16, 32
+    NFEAT = 32
+    FAN_IN0 = 32
+    SHIFT = 1
+
+    def _init_head_bias(self, host, off) -> None:
+        host[off:off + 32:4] = 0.5   # u0 of the 8 sub-voxels starts at 0.5, flows at 0
+
+    def _head_scratch(self, n, shape, vox):
+        ny, nx, nz = shape
+        return nnlib().asets_nn_head_s2d_scratch_bytes(n, nz, nx, ny)
+
+    def forward(self, Cs, Ct, alpha, shape, n=1, *, out=None, train=False):
+        """Predict (u0, bx0, by0, bz0) for n volumes of `shape` stored back to back in the
+        flat float32 device tensors Cs, Ct, alpha.  Returns a dict of fine fields."""
+        G, vox, b = self._buffers(n, shape, train)
+        ny, nx, nz = shape
+        P = n * ny * nx * nz
+        self._check_inputs(P, Cs, Ct, alpha)
+        L, s = nnlib(), self._stream()
+        _ck(L.asets_nn_features_s2d(_p(Cs), _p(Ct), _p(alpha), n, nz, nx, ny, _p(b["x0"]), s))
+        self._trunk(b, G, train)
+        out = out or {}
+        for k in ("u", "bx", "by", "bz"):
+            if k not in out:
+                out[k] = torch.empty(P, dtype=torch.float32, device=self.device)
+        _ck(L.asets_nn_head_s2d(_p(b["d0"]), n, nz, nx, ny, _p(self.view("head.w")),
+                                _p(self.view("head.b")), _p(out["u"]), _p(out["bx"]),
+                                _p(out["by"]), _p(out["bz"]), s))
+        return out
+
+    def backward_state(self, pred: dict, target: dict, shape, n=1,
+                       lam: float = FLOW_LOSS_WEIGHT) -> "torch.Tensor":
+        """Gradients of (1/P) sum (u0-tu)^2 + lam*((bx0-tbx)^2 + (by0-tby)^2 + (bz0-tbz)^2)
+        into self.grad after a forward(train=True) on the same inputs.  Returns the loss."""
+        G, vox, b = self._buffers(n, shape, True)
+        self._cur = b
+        ny, nx, nz = shape
+        L, s = nnlib(), self._stream()
+        hg = b["hgrad"]
+        _ck(L.asets_nn_head_s2d_bwd(
+            _p(b["d0"]), n, nz, nx, ny, _p(self.view("head.w")), _p(pred["u"]), _p(pred["bx"]),
+            _p(pred["by"]), _p(pred["bz"]), _p(target["u"]), _p(target["bx"]),
+            _p(target["by"]), _p(target["bz"]), lam, _p(b["g0_16a"]), _p(hg),
+            _p(b["scratch"]), b["scratch"].numel(), s))
+        self.view("head.w", "grad").copy_(hg[:512])
+        self.view("head.b", "grad").copy_(hg[512:544])
+        self._trunk_bwd(b, G, vox)
+        return hg[544] / (n * ny * nx * nz)
+
+    def predict_state(self, Cs, Ct, alpha, shape, n=1) -> dict:
+        """A full solver state {u, ps, pt, bx, by, bz} for asets_binary_dev_resume."""
+        st = self.forward(Cs, Ct, alpha, shape, n)
+        ny, nx, nz = shape
+        P = n * ny * nx * nz
+        st["ps"] = torch.empty(P, dtype=torch.float32, device=self.device)
+        st["pt"] = torch.empty(P, dtype=torch.float32, device=self.device)
+        _ck(nnlib().asets_nn_warm_state(_p(st["u"]), _p(st["bx"]), _p(st["by"]), _p(st["bz"]),
+                                        _p(Cs), _p(Ct), ny, nx, nz, n, _p(st["ps"]),
+                                        _p(st["pt"]), self._stream()))
+        return st
+
+
+def warm_solve_state(net: WarmStartNetS2D, Cs, Ct, alpha, shape, max_iter, err_bound, cc,
+                     steps, **kw):
+    """Predict a full state with the space-to-depth network, then resume the plain
+    solver from it (asets_binary_dev_resume)."""
+    from . import solver
+
+    st = net.predict_state(Cs, Ct, alpha, shape, 1)
+    return solver.binary_resume(Cs, Ct, alpha, shape, st, max_iter, err_bound, cc, steps, **kw)

@@ -274,45 +274,92 @@ def bench_binary3d(args):
         out["time_to_tol"]["bounds"] = bnd
         _progress("primal-dual bounds done")
 
-    # learned warm start (network inference + solve from (u0, pt0)) vs the cold start,
-    # both to errBound 1e-4; energies against a 2000-iteration reference solve
-    wpath = ROOT / "asetsmatlabmaxflow_amd" / "weights" / "warmstart_unet.npz"
-    if not args.no_tol and not args.no_warm and wpath.exists() and n % 8 == 0:
-        from asetsmatlabmaxflow_amd.warmstart import WarmStartNet
+    # learned warm starts vs the cold start, both to errBound 1e-4, energies against a
+    # 2000-iteration reference solve: "warm_start" = the space-to-depth network's full state
+    # (u, faces; ps, pt paired to them) resumed by asets_binary_dev_resume;
+    # "warm_start_u_pt" = the full-resolution network's (u0, pt0) with the flows at 0
+    wdir = ROOT / "asetsmatlabmaxflow_amd" / "weights"
+    variants = (("warm_start", "warmstart_s2d.npz", 16), ("warm_start_u_pt",
+                                                          "warmstart_unet.npz", 8))
+    if not args.no_tol and not args.no_warm and any((wdir / w).exists() for _, w, _ in variants):
+        from asetsmatlabmaxflow_amd.warmstart import WarmStartNet, WarmStartNetS2D
 
-        net = WarmStartNet(dev)
-        net.load(wpath)
         with torch.cuda.stream(stream):
             ref = solver.binary(dCs, dCt, dA, shape, REF_ITERS, 0.0, cc, steps, stream=stream)
             e_star = solver.energy_binary(dCs, dCt, dA, ref.u, shape, stream=stream)[0]
-            net.forward(dCs, dCt, dA, shape, 1)   # warm-up
-            torch.cuda.synchronize()
-            # median of three (inference + solve), as the cold median above
-            t_infs, t_warms = [], []
-            for _ in range(3):
-                t3 = time.perf_counter()
-                u0, pt0 = net.forward(dCs, dCt, dA, shape, 1)
-                torch.cuda.synchronize()
-                t_infs.append(time.perf_counter() - t3)
-                rw = solver.binary(dCs, dCt, dA, shape, 20000, 1e-4, cc, steps, u0=u0,
-                                   pt0=pt0, stream=stream)
-                torch.cuda.synchronize()
-                t_warms.append(time.perf_counter() - t3)
-            t_inf, t_warm = sorted(t_infs)[1], sorted(t_warms)[1]
-            e_warm = solver.energy_binary(dCs, dCt, dA, rw.u, shape, stream=stream)[0]
-            e_net = solver.energy_binary(dCs, dCt, dA, u0, shape, stream=stream)[0]
             lab_ref = ref.u > 0.5
-        out["warm_start"] = {
-            "err_bound": 1e-4, "inference_seconds": t_inf, "seconds": t_warm,
-            "iterations": rw.iterations, "cold_seconds": out["time_to_tol"]["seconds"],
-            "cold_iterations": out["time_to_tol"]["iterations"],
-            "speedup_vs_cold": out["time_to_tol"]["seconds"] / t_warm,
-            "energy_gap_warm": (e_warm - e_star) / abs(e_star),
-            "energy_gap_cold": (out["time_to_tol"]["energy"] - e_star) / abs(e_star),
-            "energy_gap_network_output": (e_net - e_star) / abs(e_star),
-            "label_agreement_warm": float(((rw.u > 0.5) == lab_ref).float().mean().item()),
-            "label_agreement_cold": float(((rt.u > 0.5) == lab_ref).float().mean().item()),
-            "weights": "asetsmatlabmaxflow_amd/weights/warmstart_unet.npz"}
+            e_cold = out["time_to_tol"]["energy"]
+            lab_cold = float(((rt.u > 0.5) == lab_ref).float().mean().item())
+            for key, wname, mult in variants:
+                if not (wdir / wname).exists() or n % mult:
+                    continue
+                full = key == "warm_start"
+                net = (WarmStartNetS2D if full else WarmStartNet)(dev)
+                net.load(wdir / wname)
+
+                def predict():
+                    if full:
+                        return net.predict_state(dCs, dCt, dA, shape, 1)
+                    u0, pt0 = net.forward(dCs, dCt, dA, shape, 1)
+                    return {"u": u0, "pt": pt0}
+
+                def warm_solve(st):
+                    if full:
+                        return solver.binary_resume(dCs, dCt, dA, shape, st, 20000, 1e-4, cc,
+                                                    steps, stream=stream)
+                    return solver.binary(dCs, dCt, dA, shape, 20000, 1e-4, cc, steps,
+                                         u0=st["u"], pt0=st["pt"], stream=stream)
+
+                warm_solve(predict())   # warm-up
+                torch.cuda.synchronize()
+                # median of three (inference + solve), as the cold median above
+                t_infs, t_warms = [], []
+                for _ in range(3):
+                    t3 = time.perf_counter()
+                    st = predict()
+                    torch.cuda.synchronize()
+                    t_infs.append(time.perf_counter() - t3)
+                    rw = warm_solve(st)
+                    torch.cuda.synchronize()
+                    t_warms.append(time.perf_counter() - t3)
+                t_inf, t_warm = sorted(t_infs)[1], sorted(t_warms)[1]
+                e_warm = solver.energy_binary(dCs, dCt, dA, rw.u, shape, stream=stream)[0]
+                st = predict()
+                e_net = solver.energy_binary(dCs, dCt, dA, st["u"], shape, stream=stream)[0]
+                out[key] = {
+                    "start": ("network full state (u, faces; ps, pt paired), resumed"
+                              if full else "network (u0, pt0), flows 0"),
+                    "err_bound": 1e-4, "inference_seconds": t_inf, "seconds": t_warm,
+                    "iterations": rw.iterations, "cold_seconds": out["time_to_tol"]["seconds"],
+                    "cold_iterations": out["time_to_tol"]["iterations"],
+                    "speedup_vs_cold": out["time_to_tol"]["seconds"] / t_warm,
+                    "energy_gap_warm": (e_warm - e_star) / abs(e_star),
+                    "energy_gap_cold": (e_cold - e_star) / abs(e_star),
+                    "energy_gap_network_output": (e_net - e_star) / abs(e_star),
+                    "label_agreement_warm": float(((rw.u > 0.5) == lab_ref).float().mean()
+                                                  .item()),
+                    "label_agreement_cold": lab_cold,
+                    "label_agreement_network_output": float(((st["u"] > 0.5) == lab_ref)
+                                                            .float().mean().item()),
+                    "weights": f"asetsmatlabmaxflow_amd/weights/{wname}"}
+                del net, st, rw
+            # the bound on any full-state predictor: resume from the reference's own state
+            rs = solver.binary(dCs, dCt, dA, shape, REF_ITERS, 0.0, cc, steps, stream=stream,
+                               want_state=True)
+            ideal = {"u": rs.u, **rs.state}
+            solver.binary_resume(dCs, dCt, dA, shape, ideal, 20000, 1e-4, cc, steps,
+                                 stream=stream)
+            torch.cuda.synchronize()
+            t4 = time.perf_counter()
+            ri = solver.binary_resume(dCs, dCt, dA, shape, ideal, 20000, 1e-4, cc, steps,
+                                      stream=stream)
+            torch.cuda.synchronize()
+            if "warm_start" in out:
+                out["warm_start"]["ideal_full_state"] = {
+                    "seconds": time.perf_counter() - t4, "iterations": ri.iterations,
+                    "note": "resume from the 2000-iteration solve's own state: what an exact "
+                            "full-state predictor would reach, inference excluded"}
+            del rs, ideal, ri
 
     _progress("GPU legs done")
     if not args.no_cpu:

@@ -3,7 +3,8 @@
  * for the binary max-flow solver: a small 3D U-Net (3x3x3 convolutions on the
  * gfx950 matrix cores, bf16 activations and weights, fp32 accumulation) that
  * maps (Cs, Ct, alpha) to a starting point (u0, pt0) for asets_binary_warm /
- * asets_binary_dev.
+ * asets_binary_dev, or, in its space-to-depth form, to a full solver state
+ * (u, ps, pt and the face flows) for asets_binary_dev_resume.
  *
  * None of these entry points has a counterpart in the reference: its solvers
  * always start from the initial cut of asetsBinaryMF3D_mex.c:205-224
@@ -110,6 +111,48 @@ asets_status asets_nn_head_bwd(const void *x, int64_t P, const float *w, const f
                                const float *pt0, const float *tu, const float *tpt, void *dx,
                                float *grads, void *scratch, size_t scratch_bytes,
                                void *stream);
+
+/* ---- space-to-depth network: half resolution in, the full solver state out ----
+ * Coarse voxel (v, z2, x2, y2) of the half-resolution tensors [n, d/2, h/2, w/2, c]
+ * stands for the 2x2x2 fine voxels s = (dz*2 + dx)*2 + dy at (2*z2+dz, 2*x2+dx,
+ * 2*y2+dy) (dz along d = Nz, dx along h = Nx, dy along w = Ny).  (d, h, w) are the
+ * fine extents, all even; the fine float fields are 8-byte aligned (a dy pair is one
+ * 8-byte access). */
+
+/* Input features [n, d/2, h/2, w/2, 32]: channel s*4 + f of sub-voxel s holds
+ * f = Cs/10, Ct/10, alpha, (Ct-Cs)/10. */
+asets_status asets_nn_features_s2d(const float *Cs, const float *Ct, const float *alpha, int n,
+                                   int d, int h, int w, void *out, void *stream);
+
+/* Head: 1x1x1 convolution of x [n, d/2, h/2, w/2, 16] with w [32][16] and b [32];
+ * output o = s*4 + q goes to sub-voxel s of field q: u0, then the face flows bx0,
+ * by0, bz0 in the solver's own-face convention (bx0[g] is the face between x-1 and
+ * x).  Fine fields of n*d*h*w floats; wt 16-byte aligned. */
+asets_status asets_nn_head_s2d(const void *x, int n, int d, int h, int w, const float *wt,
+                               const float *b, float *u0, float *bx0, float *by0, float *bz0,
+                               void *stream);
+
+/* Loss (1/P) sum over the P = n*d*h*w fine voxels of (u0-tu)^2 + lam *
+ * ((bx0-tbx)^2 + (by0-tby)^2 + (bz0-tbz)^2) and its gradients: dx [n,d/2,h/2,w/2,16]
+ * bf16; grads (fp32, 545 values) = d/dw (512, [32][16]), d/db (32), then the loss * P.
+ * Deterministic (fixed-order sums).  scratch: asets_nn_head_s2d_scratch_bytes(). */
+size_t asets_nn_head_s2d_scratch_bytes(int n, int d, int h, int w);
+asets_status asets_nn_head_s2d_bwd(const void *x, int n, int d, int h, int w, const float *wt,
+                                   const float *u0, const float *bx0, const float *by0,
+                                   const float *bz0, const float *tu, const float *tbx,
+                                   const float *tby, const float *tbz, float lam, void *dx,
+                                   float *grads, void *scratch, size_t scratch_bytes,
+                                   void *stream);
+
+/* Solver state for asets_binary_dev_resume from a predicted (u0, bx0, by0, bz0) of n
+ * column-major volumes (ny, nx, nz) stored back to back, in place: u0 clamped to
+ * [0, 1] and the zero-flux faces (axis index 0) set to 0; ps and pt written as the
+ * fixed point of the reference's source/sink update (asetsBinaryMF3D_mex.c:411-429)
+ * pairs them with u and the faces: dv = div b, pt = Ct where u >= 1/2 and
+ * min(Cs - dv, Ct) elsewhere, ps = pt + dv. */
+asets_status asets_nn_warm_state(float *u0, float *bx0, float *by0, float *bz0, const float *Cs,
+                                 const float *Ct, int64_t ny, int64_t nx, int64_t nz, int n,
+                                 float *ps, float *pt, void *stream);
 
 /* AdamW step `step` (1-based) on fp32 weights w with moments m, v; refreshes the
  * bf16 copy wb when it is not NULL. */

@@ -119,3 +119,78 @@ def network(params: dict, Cs, Ct, alpha, n, shape, tu=None, tpt=None):
         tpt = torch.as_tensor(np.asarray(tpt, np.float64))
         loss = ((u0 - tu) ** 2).mean() + (((pt0 - tpt) / COST_SCALE) ** 2).mean()
     return u0, pt0, loss
+
+
+# ---- space-to-depth network (WarmStartNetS2D) ------------------------------------
+# Coarse voxel (v, z2, x2, y2) holds the sub-voxels s = (dz*2 + dx)*2 + dy at fine
+# (2*z2+dz, 2*x2+dx, 2*y2+dy); fine fields are column-major (y fastest), i.e. numpy
+# arrays of shape (n, nz, nx, ny) in C order.
+
+def features_s2d(Cs, Ct, alpha, n, shape):
+    """[P] float arrays -> [P/8, 32] features of asets_nn_features_s2d (bf16 values)."""
+    ny, nx, nz = shape
+    Cs, Ct, alpha = (np.asarray(a, np.float32).reshape(n, nz, nx, ny) for a in (Cs, Ct, alpha))
+    k = np.float32(1.0 / COST_SCALE)
+    f = np.stack([Cs * k, Ct * k, alpha, (Ct - Cs) * k], axis=-1)   # (n, nz, nx, ny, 4)
+    f = f.reshape(n, nz // 2, 2, nx // 2, 2, ny // 2, 2, 4).transpose(0, 1, 3, 5, 2, 4, 6, 7)
+    return bf16_round(f.reshape(-1, 32))
+
+
+def head_s2d(xf, hw, hb, n, shape):
+    """xf [P/8, 16] (torch float64), hw [32*16], hb [32] -> (u0, bx0, by0, bz0) fine
+    [P] float64 tensors."""
+    ny, nx, nz = shape
+    o = xf @ hw.reshape(32, 16).T + hb                                 # [P2, 32], o = s*4 + q
+    o = o.reshape(n, nz // 2, nx // 2, ny // 2, 2, 2, 2, 4).permute(7, 0, 1, 4, 2, 5, 3, 6)
+    o = o.reshape(4, -1)
+    return o[0], o[1], o[2], o[3]
+
+
+CONVS_S2D = (("enc0a", 32, 16),) + CONVS[1:]
+
+
+def network_s2d(params: dict, Cs, Ct, alpha, n, shape, targets=None, lam=2.0):
+    """Forward pass of the space-to-depth U-Net (and its loss if targets (tu, tbx, tby,
+    tbz) are given).  params as for network(), 'head.w' [512], 'head.b' [32].
+    Returns ((u0, bx0, by0, bz0), loss)."""
+    ny, nx, nz = shape
+    x = to_ncdhw(features_s2d(Cs, Ct, alpha, n, shape), n, nz // 2, nx // 2, ny // 2, 32)
+
+    def C(name, inp, relu=True):
+        cin, cout = next((ci, co) for nm, ci, co in CONVS_S2D if nm == name)
+        return rb(conv3(inp, weight_to_torch(params[name + ".w"], cin, cout),
+                        params[name + ".b"], relu))
+
+    e0 = C("enc0b", C("enc0a", x))
+    e1 = C("enc1b", C("enc1a", rb(pool2(e0))))
+    e2 = C("enc2b", C("enc2a", rb(pool2(e1))))
+    e3 = C("enc3b", C("enc3a", rb(pool2(e2))))
+    d2 = C("dec2", torch.cat([up2(e3), e2], 1))
+    d1 = C("dec1", torch.cat([up2(d2), e1], 1))
+    d0 = C("dec0", torch.cat([up2(d1), e0], 1))
+    xf = from_ncdhw(d0).reshape(-1, 16)
+    out = head_s2d(xf, params["head.w"], params["head.b"], n, shape)
+    loss = None
+    if targets is not None:
+        t = [torch.as_tensor(np.asarray(a, np.float64)).reshape(-1) for a in targets]
+        loss = ((out[0] - t[0]) ** 2).mean() + lam * sum(((out[q] - t[q]) ** 2).mean()
+                                                         for q in (1, 2, 3))
+    return out, loss
+
+
+def warm_state(u0, bx0, by0, bz0, Cs, Ct, n, shape):
+    """asets_nn_warm_state in float32 numpy: (u, ps, pt, bx, by, bz) fine arrays."""
+    ny, nx, nz = shape
+    r = lambda a: np.array(a, np.float32).reshape(n, nz, nx, ny)
+    u, bx, by, bz, Cs, Ct = (r(a) for a in (u0, bx0, by0, bz0, Cs, Ct))
+    bx[:, :, 0, :] = 0
+    by[:, :, :, 0] = 0
+    bz[:, 0, :, :] = 0
+    hi = lambda b, ax: np.concatenate([np.take(b, range(1, b.shape[ax]), axis=ax),
+                                       np.zeros_like(np.take(b, [0], axis=ax))], axis=ax)
+    dv = ((((hi(by, 3) - by) + hi(bx, 2)) - bx) + hi(bz, 1)) - bz
+    u = np.clip(u, 0, 1).astype(np.float32)
+    pt = np.where(u >= 0.5, Ct, np.minimum(Cs - dv, Ct)).astype(np.float32)
+    ps = (pt + dv).astype(np.float32)
+    f = lambda a: a.reshape(-1)
+    return f(u), f(ps), f(pt), f(bx), f(by), f(bz)

@@ -388,3 +388,176 @@ def test_training_reduces_loss(cuda):
         net.adamw_step(lr=3e-3)
     print(f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
     assert losses[-1] < 0.5 * losses[0]
+
+
+# ---- space-to-depth network (WarmStartNetS2D) ------------------------------------
+def test_features_s2d(cuda):
+    rng = np.random.default_rng(31)
+    n, shape = 2, (6, 4, 8)          # (ny, nx, nz)
+    P = n * int(np.prod(shape))
+    Cs, Ct = rng.uniform(0, 30, P).astype(np.float32), rng.uniform(0, 30, P).astype(np.float32)
+    a = rng.uniform(0.2, 1.0, P).astype(np.float32)
+    feat = torch.empty(P * 4, dtype=torch.bfloat16, device=cuda)
+    dev = [torch.from_numpy(v).to(cuda) for v in (Cs, Ct, a)]
+    ny, nx, nz = shape
+    W._ck(W.nnlib().asets_nn_features_s2d(*(_p(t) for t in dev), n, nz, nx, ny, _p(feat), None))
+    torch.cuda.synchronize()
+    want = NO.features_s2d(Cs, Ct, a, n, shape)
+    assert np.array_equal(_f(feat).reshape(-1, 32), want)
+    # sub-voxel s = (dz*2 + dx)*2 + dy of coarse voxel (0, 1, 1, 2) is fine (2+dz, 2+dx, 4+dy)
+    c = ((0 * (nz // 2) + 1) * (nx // 2) + 1) * (ny // 2) + 2
+    for s in range(8):
+        dz, dx, dy = s >> 2, (s >> 1) & 1, s & 1
+        g = ((2 + dz) * nx + 2 + dx) * ny + 4 + dy
+        assert want[c, s * 4 + 1] == NO.bf16_round(np.float32(Ct[g] * np.float32(0.1)))
+
+
+def test_head_s2d_forward_backward(cuda):
+    rng = np.random.default_rng(32)
+    L = W.nnlib()
+    n, shape = 2, (10, 6, 4)
+    ny, nx, nz = shape
+    P = n * ny * nx * nz
+    P2 = P // 8
+    x = NO.bf16_round(np.maximum(rng.normal(0, 1, (P2, 16)), 0))
+    w = rng.normal(0, 0.3, 512).astype(np.float32)
+    b = rng.normal(0, 0.2, 32).astype(np.float32)
+    tg = [rng.uniform(0, 1, P).astype(np.float32)] + \
+        [rng.normal(0, 0.5, P).astype(np.float32) for _ in range(3)]
+    lam = 2.0
+    xd = _bf(x, cuda)
+    wd, bd = torch.from_numpy(w).to(cuda), torch.from_numpy(b).to(cuda)
+    tgd = [torch.from_numpy(t).to(cuda) for t in tg]
+    out = [torch.empty(P, device=cuda) for _ in range(4)]
+    W._ck(L.asets_nn_head_s2d(_p(xd), n, nz, nx, ny, _p(wd), _p(bd), *(_p(t) for t in out),
+                              None))
+    dx = torch.empty(P2 * 16, dtype=torch.bfloat16, device=cuda)
+    grads = torch.empty(545, device=cuda)
+    nb = L.asets_nn_head_s2d_scratch_bytes(n, nz, nx, ny)
+    scratch = torch.empty(nb, dtype=torch.uint8, device=cuda)
+    W._ck(L.asets_nn_head_s2d_bwd(_p(xd), n, nz, nx, ny, _p(wd), *(_p(t) for t in out),
+                                  *(_p(t) for t in tgd), lam, _p(dx), _p(grads), _p(scratch),
+                                  nb, None))
+    torch.cuda.synchronize()
+    X = torch.as_tensor(x.astype(np.float64)).requires_grad_(True)
+    Wt = torch.as_tensor(w.astype(np.float64)).requires_grad_(True)
+    Bt = torch.as_tensor(b.astype(np.float64)).requires_grad_(True)
+    ro = NO.head_s2d(X, Wt, Bt, n, shape)
+    t64 = [torch.as_tensor(t.astype(np.float64)) for t in tg]
+    loss = ((ro[0] - t64[0]) ** 2).mean() + lam * sum(((ro[q] - t64[q]) ** 2).mean()
+                                                      for q in (1, 2, 3))
+    loss.backward()
+    for q in range(4):
+        np.testing.assert_allclose(out[q].cpu().numpy(), ro[q].detach().numpy(), rtol=1e-5,
+                                   atol=1e-5)
+    g = grads.cpu().numpy().astype(np.float64)
+    np.testing.assert_allclose(g[:512], Wt.grad.numpy(), rtol=1e-4, atol=1e-6)
+    np.testing.assert_allclose(g[512:544], Bt.grad.numpy(), rtol=1e-4, atol=1e-6)
+    np.testing.assert_allclose(g[544] / P, loss.item(), rtol=1e-5)
+    _assert_bf16_close(_f(dx), X.grad.numpy().ravel())
+
+
+@pytest.mark.parametrize("n,shape", [(1, (5, 4, 3)), (2, (7, 3, 4)), (1, (9, 1, 1))])
+def test_warm_state(cuda, n, shape):
+    """ps and pt paired to a predicted (u, faces): bit for bit against the float32
+    restatement, u clamped to [0, 1], zero-flux faces zeroed; the sink is saturated where
+    u >= 1/2 and the source stays within Cs elsewhere."""
+    rng = np.random.default_rng(33)
+    P = n * int(np.prod(shape))
+    u0 = rng.uniform(-0.2, 1.2, P).astype(np.float32)
+    fl = [rng.normal(0, 0.4, P).astype(np.float32) for _ in range(3)]
+    Cs, Ct = rng.uniform(0, 3, P).astype(np.float32), rng.uniform(0, 3, P).astype(np.float32)
+    d = [torch.from_numpy(a.copy()).to(cuda) for a in [u0] + fl + [Cs, Ct]]
+    ps, pt = torch.empty(P, device=cuda), torch.empty(P, device=cuda)
+    ny, nx, nz = shape
+    W._ck(W.nnlib().asets_nn_warm_state(*(_p(t) for t in d), ny, nx, nz, n, _p(ps), _p(pt),
+                                        None))
+    torch.cuda.synchronize()
+    want = NO.warm_state(u0, *fl, Cs, Ct, n, shape)
+    got = [d[0], ps, pt, d[1], d[2], d[3]]
+    for g_, w_ in zip(got, want):
+        assert np.array_equal(g_.cpu().numpy(), w_)
+    u, ps_, pt_, bx, by, bz = want
+    assert u.min() >= 0 and u.max() <= 1
+    lo = u < 0.5
+    assert np.all(ps_[lo] <= Cs[lo] + 1e-6)
+    assert np.all(pt_[~lo] == Ct[~lo])
+    f = lambda a: a.reshape(n, shape[2], shape[1], shape[0])
+    assert not f(bx)[:, :, 0, :].any() and not f(by)[:, :, :, 0].any()
+    assert not f(bz)[:, 0, :, :].any()
+
+
+def _net_state_inputs(shape, n, seed):
+    Cs, Ct, A, TU, _ = _net_inputs(shape, n, seed)
+    rng = np.random.default_rng(seed + 1)
+    P = Cs.size
+    TB = [rng.normal(0, 0.3, P).astype(np.float32) for _ in range(3)]
+    return Cs, Ct, A, TU, TB
+
+
+@pytest.mark.parametrize("shape,n", [((32, 16, 16), 2), ((128, 16, 16), 1)])
+def test_network_s2d_forward_and_gradients(cuda, shape, n):
+    """Space-to-depth U-Net: predicted state, loss and every parameter gradient against
+    float64 autograd through the oracle (3 % relative-L2 bar per parameter tensor, as for
+    the full-resolution network).  The second shape's level-0 rows are 64 voxels wide
+    (LDS-staged convolutions)."""
+    Cs, Ct, A, TU, TB = _net_state_inputs(shape, n, 78)
+    net = W.WarmStartNetS2D(cuda, seed=3)
+    flat = net.w32.cpu().numpy()
+    rng = np.random.default_rng(2)
+    for name, _, cout in W.CONVS_S2D:
+        o, k = net.layout[name + ".b"]
+        flat[o:o + k] = rng.normal(0, 0.05, k)
+    o, k = net.layout["head.w"]
+    flat[o:o + k] = rng.normal(0, 0.2, k)
+    net.load_flat(flat)
+    d = [torch.from_numpy(x).to(cuda) for x in (Cs, Ct, A, TU, *TB)]
+    pred = net.forward(d[0], d[1], d[2], shape, n, train=True)
+    tgt = dict(zip(("u", "bx", "by", "bz"), d[3:]))
+    loss = net.backward_state(pred, tgt, shape, n)
+    torch.cuda.synchronize()
+    w32 = net.w32.cpu().numpy()
+    w16 = net.w16.float().cpu().numpy()
+    params = {}
+    for key, (o, k) in net.layout.items():
+        src = w16 if (key.endswith(".w") and not key.startswith("head")) else w32
+        params[key] = torch.as_tensor(src[o:o + k].astype(np.float64)).requires_grad_(True)
+    rout, rloss = NO.network_s2d(params, Cs, Ct, A, n, shape, [TU, *TB], W.FLOW_LOSS_WEIGHT)
+    rloss.backward()
+    for q, k in enumerate(("u", "bx", "by", "bz")):
+        np.testing.assert_allclose(pred[k].cpu().numpy(), rout[q].detach().numpy(), rtol=0,
+                                   atol=2e-2)
+    np.testing.assert_allclose(loss.item(), rloss.item(), rtol=1e-2)
+    grad = net.grad.cpu().numpy().astype(np.float64)
+    for key, (o, k) in net.layout.items():
+        ref = params[key].grad.numpy()
+        rel = np.linalg.norm(grad[o:o + k] - ref) / max(np.linalg.norm(ref), 1e-30)
+        assert rel < 3e-2, f"{key}: relative L2 error {rel:.3e}"
+
+
+def test_network_s2d_inference_equals_training_forward(cuda):
+    shape, n = (64, 32, 16), 1
+    Cs, Ct, A, _, _ = _net_state_inputs(shape, n, 6)
+    net = W.WarmStartNetS2D(cuda, seed=5)
+    d = [torch.from_numpy(x).to(cuda) for x in (Cs, Ct, A)]
+    p1 = {k: v.clone() for k, v in net.forward(d[0], d[1], d[2], shape, n, train=True).items()}
+    p2 = net.forward(d[0], d[1], d[2], shape, n, train=False)
+    torch.cuda.synchronize()
+    assert all(torch.equal(p1[k], p2[k]) for k in p1)
+
+
+def test_network_s2d_training_reduces_loss(cuda):
+    shape, n = (32, 32, 16), 2
+    Cs, Ct, A, _, _ = _net_state_inputs(shape, n, 98)
+    TU = (Cs > Ct).astype(np.float32)    # learnable targets: the initial cut, flows 0
+    d = [torch.from_numpy(x).to(cuda) for x in (Cs, Ct, A, TU)]
+    z = torch.zeros_like(d[3])
+    net = W.WarmStartNetS2D(cuda, seed=0)
+    losses = []
+    for _ in range(60):
+        pred = net.forward(d[0], d[1], d[2], shape, n, train=True)
+        losses.append(float(net.backward_state(pred, {"u": d[3], "bx": z, "by": z, "bz": z},
+                                               shape, n)))
+        net.adamw_step(lr=3e-3)
+    print(f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    assert losses[-1] < 0.5 * losses[0]
