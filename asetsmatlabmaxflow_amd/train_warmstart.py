@@ -72,7 +72,8 @@ def make_dataset(nvol: int, size: int, seed0: int, target_iters: int, device, wo
 
 
 def train(net, data, size: int, nvol: int, steps: int, batch: int, lr: float, seed: int,
-          log_every: int = 50, log=print, total_steps: int | None = None):
+          log_every: int = 50, log=print, total_steps: int | None = None,
+          flow_weight: float | None = None):
     """Run `steps` AdamW steps from the network's own step count (0, or where a
     checkpoint left it) on a cosine schedule over `total_steps` (default: steps).  Each
     step takes `batch` volumes drawn at random (batch groups of consecutive volumes)."""
@@ -93,8 +94,9 @@ def train(net, data, size: int, nvol: int, steps: int, batch: int, lr: float, se
         Cs, Ct, a = data["Cs"][sl], data["Ct"][sl], data["alpha"][sl]
         if "tbx" in data:   # space-to-depth network, full state
             pred = net.forward(Cs, Ct, a, shape, batch, train=True)
+            kw = {} if flow_weight is None else {"lam": flow_weight}
             loss = net.backward_state(pred, {k: data["t" + k][sl] for k in
-                                             ("u", "bx", "by", "bz")}, shape, batch)
+                                             ("u", "bx", "by", "bz")}, shape, batch, **kw)
         else:
             u0, pt0 = net.forward(Cs, Ct, a, shape, batch, train=True)
             loss = net.backward(Cs, Ct, u0, pt0, data["tu"][sl], data["tpt"][sl], shape,
@@ -131,6 +133,8 @@ def main():
     ap.add_argument("--variant", default="full", choices=["full", "s2d"],
                     help="full: WarmStartNet predicting (u0, pt0); s2d: WarmStartNetS2D "
                          "predicting (u0, bx0, by0, bz0)")
+    ap.add_argument("--flow-weight", type=float, default=None,
+                    help="s2d: weight of the face-flow errors in the loss (default 2)")
     ap.add_argument("--out", required=True)
     args = ap.parse_args()
 
@@ -151,7 +155,7 @@ def main():
         net.load_checkpoint(args.resume)
     hist, train_s = train(net, data, args.size, args.volumes, args.steps, args.batch, args.lr,
                           args.seed, args.log_every, log=lambda m: print(m, flush=True),
-                          total_steps=args.total_steps or None)
+                          total_steps=args.total_steps or None, flow_weight=args.flow_weight)
     net.save(args.out)
     if args.checkpoint:
         net.save_checkpoint(args.checkpoint)
