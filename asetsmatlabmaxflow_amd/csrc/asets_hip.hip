@@ -416,51 +416,62 @@ __global__ void k_to_f32(float *dst, const T *src, int64_t n, int stride) {
     if (i < n) dst[i] = Io<T>::get(src + stride * i);
 }
 
-// Initial block-interleaved binary state (BlkLayout), one thread per lane of a
-// block.  Cold start (u0 null): u = (Cs - Ct >= 0), ps = pt = min(Cs, Ct), flows 0
-// (asetsBinaryMF3D_mex.c:215-219).  Warm start or resume: u0, ps0 (null: pt0), pt0
-// and the flows b0 (null: 0), whose faces with axis index 0 -- the zero-flux
-// boundary, never updated -- are forced to 0.  Padding lanes y >= ny: all 0.  Flows
-// are rounded as the storage type stores them.
+// Initial block-interleaved binary state (BlkLayout).  Cold start (u0 null): u =
+// (Cs - Ct >= 0), ps = pt = min(Cs, Ct), flows 0 (asetsBinaryMF3D_mex.c:215-219).
+// Warm start or resume: u0, ps0 (null: pt0), pt0 and the flows b0 (null: 0), whose faces
+// with axis index 0 -- the zero-flux boundary, never updated -- are forced to 0.  Padding
+// lanes y >= ny: all 0.  Flows are rounded as the storage type stores them.  One
+// workgroup per column (x, z) of the volume (blk_col_grid), its threads over the lanes
+// of the column's nyb blocks: the column's position is divided out once per workgroup,
+// not per lane.
 template <typename T>
 __global__ void k_blk_init(const float *Cs, const float *Ct, const float *u0, const float *ps0,
                            const float *pt0, const float *bx0, const float *by0,
                            const float *bz0, char *st, int ny, int nx, int nz, int nyb) {
     using BL = BlkLayout<T>;
     // 32-bit indices: the layout's buffers stay below 2^31 bytes (fused_fits_blk)
-    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (t >= nz * nx * nyb * 64) return;
-    const int b = t >> 6;
-    const int ln = t & 63;
-    const int yb = b % nyb;
-    const int xz = b / nyb;
-    const int x = xz % nx, z = xz / nx;
-    const int y = yb * 64 + ln;
-    float u = 0.0f, ps = 0.0f, pt = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
-    if (y < ny) {
-        const int64_t g = ((int64_t)z * nx + x) * ny + y;
-        if (u0) {
-            u = u0[g];
-            pt = pt0[g];
-            ps = ps0 ? ps0[g] : pt;
-        } else {
-            const float cs = Cs[g], ct = Ct[g];
-            u = (cs - ct >= 0.0f) ? 1.0f : 0.0f;
-            ps = pt = fmin_ref(cs, ct);
+    const int c = (int)blockIdx.x;            // column z * nx + x
+    const int x = c % nx, z = c / nx;
+    const int64_t g0 = (int64_t)c * ny;       // voxel (x, y = 0, z)
+    for (int i = threadIdx.x; i < nyb * 64; i += blockDim.x) {
+        const int yb = i >> 6, ln = i & 63;
+        const int y = i;
+        float u = 0.0f, ps = 0.0f, pt = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
+        if (y < ny) {
+            const int64_t g = g0 + y;
+            if (u0) {
+                u = u0[g];
+                pt = pt0[g];
+                ps = ps0 ? ps0[g] : pt;
+            } else {
+                const float cs = Cs[g], ct = Ct[g];
+                u = (cs - ct >= 0.0f) ? 1.0f : 0.0f;
+                ps = pt = fmin_ref(cs, ct);
+            }
+            if (bx0 && x > 0) bx = bx0[g];
+            if (by0 && y > 0) by = by0[g];
+            if (bz0 && z > 0) bz = bz0[g];
         }
-        if (bx0 && x > 0) bx = bx0[g];
-        if (by0 && y > 0) by = by0[g];
-        if (bz0 && z > 0) bz = bz0[g];
+        char *blk = st + ((size_t)c * nyb + yb) * BL::kBytes;
+        ((float *)(blk + BL::kU))[ln] = u;
+        T *pp = (T *)(blk + BL::kPS) + 2 * ln;
+        T *bb = (T *)(blk + BL::kBB) + 2 * ln;
+        Io<T>::put(pp, ps);
+        Io<T>::put(pp + 1, pt);
+        Io<T>::put(bb, bx);
+        Io<T>::put(bb + 1, by);
+        Io<T>::put((T *)(blk + BL::kBZ) + ln, bz);
     }
-    char *blk = st + (size_t)b * BL::kBytes;
-    ((float *)(blk + BL::kU))[ln] = u;
-    T *pp = (T *)(blk + BL::kPS) + 2 * ln;
-    T *bb = (T *)(blk + BL::kBB) + 2 * ln;
-    Io<T>::put(pp, ps);
-    Io<T>::put(pp + 1, pt);
-    Io<T>::put(bb, bx);
-    Io<T>::put(bb + 1, by);
-    Io<T>::put((T *)(blk + BL::kBZ) + ln, bz);
+}
+
+// Launch shape of the per-column block-interleaved kernels: one workgroup per column,
+// up to 256 threads over its 64 * nyb lanes.
+struct BlkColGrid {
+    unsigned grid, threads;
+};
+inline BlkColGrid blk_col_grid(int64_t nx, int64_t nz, int nyb) {
+    const int lanes = 64 * nyb;
+    return BlkColGrid{(unsigned)(nx * nz), (unsigned)(lanes < 256 ? lanes : 256)};
 }
 
 // Initial block-interleaved Potts flows (PottsBlkLayout) of every label: pt = ps, the
@@ -484,31 +495,27 @@ __global__ void k_init_potts_blk(const float *ps, char *st, int ny, int nx, int 
 }
 
 // u and, where requested, the flows of a block-interleaved state to planar fp32
-// arrays (own-face convention), one thread per lane of a block.
+// arrays (own-face convention).  One workgroup per column, as k_blk_init.
 template <typename T>
 __global__ void k_blk_take(const char *st, float *u, float *ps, float *pt, float *bx, float *by,
                            float *bz, int ny, int nx, int nz, int nyb) {
     using BL = BlkLayout<T>;
     // 32-bit indices: the layout's buffers stay below 2^31 bytes (fused_fits_blk)
-    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (t >= nz * nx * nyb * 64) return;
-    const int b = t >> 6;
-    const int ln = t & 63;
-    const int yb = b % nyb;
-    const int xz = b / nyb;
-    const int x = xz % nx, z = xz / nx;
-    const int y = yb * 64 + ln;
-    if (y >= ny) return;
-    const int64_t g = ((int64_t)z * nx + x) * ny + y;
-    const char *blk = st + (size_t)b * BL::kBytes;
-    const T *pp = (const T *)(blk + BL::kPS) + 2 * ln;
-    const T *bb = (const T *)(blk + BL::kBB) + 2 * ln;
-    if (u) u[g] = ((const float *)(blk + BL::kU))[ln];
-    if (ps) ps[g] = Io<T>::get(pp);
-    if (pt) pt[g] = Io<T>::get(pp + 1);
-    if (bx) bx[g] = Io<T>::get(bb);
-    if (by) by[g] = Io<T>::get(bb + 1);
-    if (bz) bz[g] = Io<T>::get((const T *)(blk + BL::kBZ) + ln);
+    const int c = (int)blockIdx.x;            // column z * nx + x
+    const int64_t g0 = (int64_t)c * ny;
+    for (int y = threadIdx.x; y < ny; y += blockDim.x) {
+        const int yb = y >> 6, ln = y & 63;
+        const int64_t g = g0 + y;
+        const char *blk = st + ((size_t)c * nyb + yb) * BL::kBytes;
+        const T *pp = (const T *)(blk + BL::kPS) + 2 * ln;
+        const T *bb = (const T *)(blk + BL::kBB) + 2 * ln;
+        if (u) u[g] = ((const float *)(blk + BL::kU))[ln];
+        if (ps) ps[g] = Io<T>::get(pp);
+        if (pt) pt[g] = Io<T>::get(pp + 1);
+        if (bx) bx[g] = Io<T>::get(bb);
+        if (by) by[g] = Io<T>::get(bb + 1);
+        if (bz) bz[g] = Io<T>::get((const T *)(blk + BL::kBZ) + ln);
+    }
 }
 
 // Everything the fused solve loop needs (one volume).
@@ -566,12 +573,14 @@ asets_status run_fused(const FusedRun &r, Carver &cv, int32_t *hflag) {
     // cold start: iteration 0 forms the initial state itself (and checks alpha);
     // with no iteration buf[0] must hold it for the outputs
     const bool first = !r.u0;
-    const unsigned nbb = (unsigned)cdiv(r.nz * r.nx * (int64_t)r.nyb * 64, kBlock);
     if (r.blk) {
-        if (!first || max_iter == 0)
-            k_blk_init<T><<<nbb, kBlock, 0, s>>>(r.Cs, r.Ct, r.u0, r.ps0, r.pt0, r.b0[0],
-                                                 r.b0[1], r.b0[2], (char *)buf[0].u, (int)r.ny,
-                                                 (int)r.nx, (int)r.nz, r.nyb);
+        if (!first || max_iter == 0) {
+            const BlkColGrid cg = blk_col_grid(r.nx, r.nz, r.nyb);
+            k_blk_init<T><<<cg.grid, cg.threads, 0, s>>>(r.Cs, r.Ct, r.u0, r.ps0, r.pt0,
+                                                         r.b0[0], r.b0[1], r.b0[2],
+                                                         (char *)buf[0].u, (int)r.ny,
+                                                         (int)r.nx, (int)r.nz, r.nyb);
+        }
     } else if (first && max_iter == 0) {
         k_init_binary_cold_t<T><<<nb, kBlock, 0, s>>>(r.Cs, r.Ct, buf[0].u, buf[0].ps,
                                                      buf[0].bx, buf[0].bz, N);
@@ -638,9 +647,10 @@ asets_status run_fused(const FusedRun &r, Carver &cv, int32_t *hflag) {
     const int32_t n_exec = (itn < max_iter) ? itn + 1 : max_iter;
     const FusedState<T> &fin = buf[n_exec & 1];
     if (r.blk) {
-        k_blk_take<T><<<nbb, kBlock, 0, s>>>((const char *)fin.u, r.u, r.o_ps, r.o_pt, r.o_bx,
-                                             r.o_by, r.o_bz, (int)r.ny, (int)r.nx, (int)r.nz,
-                                             r.nyb);
+        const BlkColGrid cg = blk_col_grid(r.nx, r.nz, r.nyb);
+        k_blk_take<T><<<cg.grid, cg.threads, 0, s>>>((const char *)fin.u, r.u, r.o_ps, r.o_pt,
+                                                     r.o_bx, r.o_by, r.o_bz, (int)r.ny,
+                                                     (int)r.nx, (int)r.nz, r.nyb);
         HIPCHK(hipGetLastError());
         return ASETS_OK;
     }
@@ -1398,11 +1408,13 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
             const int64_t N = (int64_t)b.ny * b.nx * b.nz;
             const unsigned nb = (unsigned)cdiv(N, kBlock);
             // buffer 1 needs none: a volume's first iteration writes every voxel of it
-            if (blk)
-                k_blk_init<T><<<(unsigned)cdiv((int64_t)b.nz * b.nx * b.nyb * 64, kBlock), kBlock,
-                                0, s>>>(b.Cs, b.Ct, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                        nullptr, (char *)b.u[0], b.ny, b.nx, b.nz, b.nyb);
-            else
+            if (blk) {
+                const BlkColGrid cg = blk_col_grid(b.nx, b.nz, b.nyb);
+                k_blk_init<T><<<cg.grid, cg.threads, 0, s>>>(b.Cs, b.Ct, nullptr, nullptr,
+                                                             nullptr, nullptr, nullptr, nullptr,
+                                                             (char *)b.u[0], b.ny, b.nx, b.nz,
+                                                             b.nyb);
+            } else
                 k_init_binary_cold_t<T><<<nb, kBlock, 0, s>>>(b.Cs, b.Ct, b.u[0], b.ps[0],
                                                              b.bx[0], b.bz[0], N);
         }
