@@ -174,8 +174,12 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
     // slot k holds a position (only the last halo slot can run past the halo)
     auto in2 = [&](int k) { return k < KO || (NH % NT) == 0 || tid + (k - KO) * NT < NH; };
     auto in1 = [&](int k) { return k < KO || (NH1 % NT) == 0 || tid + (k - KO) * NT < NH1; };
-    const int slot = lb % a.nflow;
-    const int rest = lb / a.nflow;
+    // node slowest: a node's neighbouring tiles, which re-read each other's halo, run
+    // together (measured 2.4 % faster at 192^3 than node fastest, where siblings
+    // shared their parent's sink-flow halo instead)
+    const int per_node = a.nblocks / a.nflow;
+    const int slot = lb / per_node;
+    const int rest = lb % per_node;
     const int n = a.flow[slot];
     const HmfNode nd = a.nodes[n];
     const int64_t on = (int64_t)n * a.N, op = (int64_t)nd.parent * a.N;
