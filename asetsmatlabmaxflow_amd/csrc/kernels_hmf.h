@@ -338,8 +338,8 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
         }
     };
 
-    // S2a: stepped in-plane faces of plane q
-    auto s2a = [&](int q, const Raw &r) {
+    // S2a: stepped in-plane faces of plane q from its stored faces fxl, fyl
+    auto s2a = [&](int q, const float (&fxl)[K2], const float (&fyl)[K2]) {
         const float *gq = GK[q & 1];
         const int sl = (q + 3) % 3;
         const bool qin = q >= 0 && q < nz;
@@ -354,11 +354,11 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
                 const float cy = gq[(f & 8) ? idx - 1 : idx];
                 float bx1, by1;
                 if (IS3D) {
-                    bx1 = (r.bxl[k] + c) - cx;
-                    by1 = (r.byl[k] + c) - cy;
+                    bx1 = (fxl[k] + c) - cx;
+                    by1 = (fyl[k] + c) - cy;
                 } else {
-                    bx1 = r.bxl[k] + (c - cx);
-                    by1 = r.byl[k] + (c - cy);
+                    bx1 = fxl[k] + (c - cx);
+                    by1 = fyl[k] + (c - cy);
                 }
                 B1X[sl][idx] = (live && (f & 2)) ? bx1 : 0.0f;
                 B1Y[sl][idx] = (live && (f & 4)) ? by1 : 0.0f;
@@ -366,8 +366,13 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
         }
     };
 
-    // S2b: capacity factor of plane q over the radius-1 region
-    auto s2b = [&](int q, const Raw &r) {
+    // S2b: capacity factor of plane q over the radius-1 region, with alpha of plane q
+    // in al (a map's values; a scalar alpha otherwise); the own positions' factors
+    // also go to g_hi, where the output stage of plane q-1 reads them (same thread)
+    float g_lo[KO], g_hi[KO];   // g at the own positions: planes q-1 and q
+#pragma unroll
+    for (int k = 0; k < KO; ++k) g_lo[k] = g_hi[k] = 1.0f;
+    auto s2b = [&](int q, const float (&al_map)[K1]) {
         const int sl = (q + 3) % 3, sn = (q + 4) % 3;
         float *go = G[q & 1];
         const bool qin = q >= 0 && q < nz;
@@ -383,10 +388,11 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
                 const float zl = B1Z[sl][l], zr = B1Z[sn][l];
                 t = (t + zr * zr) + zl * zl;
             }
-            const float al = amap ? r.al[k] : as;
+            const float al = amap ? al_map[k] : as;
             const bool live = ((fl1 >> k) & 1) && qin;
             const float v = live ? hmf_cap(t, al) : 1.0f;
             if (in1(k)) go[idx] = v;
+            if (k < KO) g_hi[k] = v;
         }
     };
 
@@ -398,22 +404,19 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
         if (!(IS3D && s + 1 < nz)) return 0.0f;
         return (0.5f * B1Z[sn][l2]) * (gup + gc);
     };
+    // g of planes s and s+1 at the own positions in g_lo and g_hi
     auto s3_carry = [&](int s) {
-        const float *g0 = G[s & 1];
-        const float *g1 = G[(s + 1) & 1];
 #pragma unroll
         for (int k = 0; k < KO; ++k) {
             const int i = tid % TY, j = tid / TY + k * (NT / TY);
             const int l2 = (j + 2) * W2 + (i + 2);
-            const int l1 = (j + 1) * W1 + (i + 1);
-            carry[k] = z_face_up(s, g0[l1], g1[l1], l2);
+            carry[k] = z_face_up(s, g_lo[k], g_hi[k], l2);
         }
     };
     // S3: rescaled faces and divergence of plane s
     auto s3 = [&](int s) {
         const int sl = (s + 3) % 3;
         const float *g0 = G[s & 1];
-        const float *g1 = G[(s + 1) & 1];
         const PRef wd = pref(dvo, s, nz, 4, plane);
         const PRef wx = pref(ox, s, nz, 4, plane);
         const PRef wy = pref(oy, s, nz, 4, plane);
@@ -424,8 +427,8 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
             const int y = y0 + i, x = x0 + j;
             const int l2 = (j + 2) * W2 + (i + 2);
             const int l1 = (j + 1) * W1 + (i + 1);
-            const float gc = g0[l1];
-            const float bzr = z_face_up(s, gc, g1[l1], l2);
+            const float gc = g_lo[k];
+            const float bzr = z_face_up(s, gc, g_hi[k], l2);
             float bxl = (0.5f * B1X[sl][l2]) * (gc + g0[l1 - W1]);
             float bxr = (0.5f * B1X[sl][l2 + W2]) * (g0[l1 + W1] + gc);
             float byl = (0.5f * B1Y[sl][l2]) * (gc + g0[l1 - 1]);
@@ -451,14 +454,36 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
         }
     };
 
-    auto step = [&](int s, Raw &r) {
-        if (s + 2 <= z1 + 1) load_raw(s + 2, r);
-        if (s - 1 >= z0) s3(s - 1);
-        else if (s - 1 == z0 - 1) s3_carry(s - 1);
-        if (s + 2 <= z1 + 1) s1(s + 2, r);
-        __syncthreads();
-        if (s + 2 <= z1 + 1) s2a(s + 2, r);
-        if (s + 1 >= z0 - 1 && s + 1 <= z1) s2b(s + 1, r);
+    // One barrier per plane step.  Step t: the loads of raw plane t+2, S2b(t), S3(t-1),
+    // S2a(t+1), S1(t+2); barrier.  Each stage reads in LDS only what an earlier step
+    // wrote, and each ring slot it writes was last read in an earlier step.  In
+    // registers instead: g of plane t at S3(t-1)'s own positions (computed by the same
+    // thread in S2b(t)), alpha of plane t for S2b and the stored bx, by of plane t+1 for
+    // S2a, taken from the raw plane S1 read one step earlier (alc, fxl, fyl).
+    float alc[K1], fxl[K2], fyl[K2];
+    auto keep_raw = [&](const Raw &rr) {
+#pragma unroll
+        for (int k = 0; k < K1; ++k) alc[k] = rr.al[k];
+#pragma unroll
+        for (int k = 0; k < K2; ++k) {
+            fxl[k] = rr.bxl[k];
+            fyl[k] = rr.byl[k];
+        }
+    };
+    auto step = [&](int t, Raw &r) {
+        if (t + 2 <= z1 + 1) load_raw(t + 2, r);
+        if (t >= z0 - 1) {
+#pragma unroll
+            for (int k = 0; k < KO; ++k) g_lo[k] = g_hi[k];
+            s2b(t, alc);
+        }
+        if (t - 1 >= z0) s3(t - 1);
+        else if (t - 1 == z0 - 1) s3_carry(t - 1);
+        if (t >= z0 - 2 && t + 1 <= z1) s2a(t + 1, fxl, fyl);
+        if (t + 2 <= z1 + 1) {
+            s1(t + 2, r);
+            keep_raw(r);
+        }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < K2; ++k) bz_cur[k] = r.bzn[k];
@@ -467,14 +492,14 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
 #pragma unroll
     for (int k = 0; k < K2; ++k) r.bzn[k] = 0.0f;
 #pragma unroll
-    for (int k = 0; k < K1; ++k) r.al[k] = as;
+    for (int k = 0; k < K1; ++k) r.al[k] = alc[k] = as;
     if (IS3D) {
         const PRef rz = pref(fz, z0 - 2, nz, 4, plane);
 #pragma unroll
         for (int k = 0; k < K2; ++k) bz_cur[k] = ldf(rz, off2[k]);
     }
     if constexpr (IS3D) {
-        for (int s = z0 - 4; s <= z1; ++s) step(s, r);
+        for (int t = z0 - 4; t <= z1; ++t) step(t, r);
     } else {
         // one plane: its stages in dependency order, no z pipeline
         load_raw(0, r);
@@ -485,9 +510,11 @@ __device__ __forceinline__ void hmf_tile(const HmfArgs &a, int lb) {
         }
         s1(0, r);
         __syncthreads();
-        s2a(0, r);
+        s2a(0, r.bxl, r.byl);
         __syncthreads();
-        s2b(0, r);
+        s2b(0, r.al);
+#pragma unroll
+        for (int k = 0; k < KO; ++k) g_lo[k] = g_hi[k];   // S3(0) reads g of plane 0
         __syncthreads();
         s3(0);
     }
