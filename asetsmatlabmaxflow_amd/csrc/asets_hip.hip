@@ -59,6 +59,9 @@ asets_status fail(asets_status s, const char *fmt, ...) {
 
 constexpr int kBlock = 256;
 constexpr int kMaxDevices = 64;
+// Most partial sums for which every workgroup of a fused binary launch runs the
+// previous iteration's stop test itself (each reads all of them: 16 KiB at most).
+constexpr int64_t kStopAllParts = 2048;
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
@@ -182,10 +185,19 @@ inline void fill(float *p, float v, int64_t n, hipStream_t s) {
     if (n > 0) k_fill_f32<<<(unsigned)cdiv(n, kBlock), kBlock, 0, s>>>(p, v, n);
 }
 
+// A solve's flags cleared, *it_num = max_iter and cvg[0, max_iter) = 0, in one
+// launch (before any alpha check writes into the flags).
+void solve_setup(Flags *f, int32_t *it_num, int32_t max_iter, float *cvg, hipStream_t s) {
+    constexpr int kWords = (int)(sizeof(Flags) / sizeof(int32_t));
+    const int64_t n = std::max<int64_t>(std::max<int64_t>(max_iter, kWords), 1);
+    k_solve_setup<<<(unsigned)cdiv(n, kBlock), kBlock, 0, s>>>((int32_t *)f, kWords, it_num,
+                                                              max_iter, cvg, max_iter);
+}
+
 // Enqueue iterations [0, max_iter) via enqueue_iter(it), polling the stop flag
 // on the host every poll iterations.  Returns the number of iterations whose
 // kernels were enqueued.
-// hflag: host memory for the three flag words read at each poll.
+// hflag: host memory for the four flag words read at each poll.
 template <typename F>
 asets_status run_loop(int32_t max_iter, float err_bound, int32_t poll_opt, hipStream_t s,
                       int32_t *hflag, Flags *flags, int32_t *itn, F &&enqueue_iter) {
@@ -200,12 +212,12 @@ asets_status run_loop(int32_t max_iter, float err_bound, int32_t poll_opt, hipSt
         }
         it += k;
         HIPCHK(hipGetLastError());
-        // done, bad_alpha and it_num in one read
-        HIPCHK(hipMemcpyAsync(hflag, &flags->done, 3 * sizeof(int32_t), hipMemcpyDeviceToHost,
+        // done, bad_alpha, it_num and stop_at in one read
+        HIPCHK(hipMemcpyAsync(hflag, &flags->done, 4 * sizeof(int32_t), hipMemcpyDeviceToHost,
                               s));
         HIPCHK(hipStreamSynchronize(s));
         if (hflag[1]) return alpha_error();
-        if (hflag[0]) {
+        if (hflag[0] || hflag[3]) {
             *itn = hflag[2];
             break;
         }
@@ -621,7 +633,17 @@ asets_status run_fused(const FusedRun &r, Carver &cv, int32_t *hflag) {
     // on partials double-buffered by iteration parity; k_finalize only after the
     // last.  A stop found one launch late is still exact: that launch wrote the
     // other state buffer, and every later launch returns at once.
-    const bool deferred = (r.p->err_bound == 0.0f);
+    // err_bound > 0: every workgroup of the next launch runs the stop test at its
+    // start (stop_all) and returns when it fires, which costs each workgroup one
+    // read of the partials instead of a k_finalize launch per iteration; the launch
+    // after the last iteration then costs no more than an empty one.  Only where
+    // the partials are few (each workgroup reads all of them).
+    const bool stop_all = (r.p->err_bound != 0.0f) && r.nparts <= kStopAllParts;
+    const bool deferred = (r.p->err_bound == 0.0f) || stop_all;
+    // and when only u is asked for, the launch that finds the stop also writes it
+    // (k_blk_take's work, without its launch after the host's poll)
+    const bool take_u = stop_all && r.blk && !r.o_ps && !r.o_pt && !r.o_bx && !r.o_by &&
+                        !r.o_bz && r.ny % 4 == 0 && ((uintptr_t)r.u & 15) == 0;
     double *const P[2] = {r.partials, r.partials + r.nparts};
     if (r.ev0) HIPCHK(hipEventRecord(r.ev0, s));
     auto iter = [&](int32_t it) -> asets_status {
@@ -629,6 +651,8 @@ asets_status run_fused(const FusedRun &r, Carver &cv, int32_t *hflag) {
         fa.out = buf[(it + 1) & 1];
         fa.partials = deferred ? P[it & 1] : r.partials;
         fa.stop = DeferredStop{};
+        fa.stop_all = stop_all ? 1 : 0;
+        fa.take_u = take_u ? r.u : nullptr;
         if (deferred && it > 0)
             fa.stop = DeferredStop{P[(it - 1) & 1], r.nparts, (double)N, it - 1,
                                    r.p->err_bound, r.cvg, r.it_num, r.flags};
@@ -647,6 +671,7 @@ asets_status run_fused(const FusedRun &r, Carver &cv, int32_t *hflag) {
     const int32_t n_exec = (itn < max_iter) ? itn + 1 : max_iter;
     const FusedState<T> &fin = buf[n_exec & 1];
     if (r.blk) {
+        if (take_u && hflag[3] != 0) return ASETS_OK;   // the stop's launch wrote u
         const BlkColGrid cg = blk_col_grid(r.nx, r.nz, r.nyb);
         k_blk_take<T><<<cg.grid, cg.threads, 0, s>>>((const char *)fin.u, r.u, r.o_ps, r.o_pt,
                                                      r.o_bx, r.o_by, r.o_bz, (int)r.ny,
@@ -839,11 +864,9 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
     Flags *flags = cv.take<Flags>(1);
     double *partials = cv.take<double>(2 * nparts);
 
-    k_fill_i32<<<1, 64, 0, s>>>((int32_t *)flags, 0, sizeof(Flags) / 4);
+    solve_setup(flags, it_num, max_iter, cvg, s);
     // a fused cold start checks alpha in its first iteration
     if (max_iter > 0 && !(fused && !u0)) alpha_check_async(alpha, N, flags, s);
-    k_fill_i32<<<1, 1, 0, s>>>(it_num, max_iter, 1);
-    fill(cvg, 0.0f, max_iter, s);
     HIPCHK(hipGetLastError());
     int32_t itn = max_iter;
 
@@ -974,10 +997,8 @@ asets_status potts_dev(int ndim, const float *Ct, const float *alpha, int64_t ny
     double *partials = cv.take<double>(nparts);
     float *ps = cv.take<float>(N);
 
-    k_fill_i32<<<1, 64, 0, s>>>((int32_t *)flags, 0, sizeof(Flags) / 4);
+    solve_setup(flags, it_num, max_iter, cvg, s);
     if (max_iter > 0) alpha_check_async(alpha, NL, flags, s);
-    k_fill_i32<<<1, 1, 0, s>>>(it_num, max_iter, 1);
-    fill(cvg, 0.0f, max_iter, s);
 
     EventPair evp;
     hipEvent_t &ev0 = evp.a, &ev1 = evp.b;
@@ -1760,11 +1781,9 @@ asets_status ishikawa_dev(int ndim, const float *Ct, const float *alpha, int64_t
     for (int b = 0; b < 2; ++b)
         for (int k = 0; k < 3; ++k) bb[b][k] = cv.take<float>(NM);
 
-    k_fill_i32<<<1, 64, 0, s>>>((int32_t *)flags, 0, sizeof(Flags) / 4);
+    solve_setup(flags, it_num, max_iter, cvg, s);
     // only the first L-1 alpha fields are used (asetsIshikawa3D_mex.c:377-379)
     if (max_iter > 0) alpha_check_async(alpha, NM, flags, s);
-    k_fill_i32<<<1, 1, 0, s>>>(it_num, max_iter, 1);
-    fill(cvg, 0.0f, max_iter, s);
     for (int b = 0; b < 2; ++b)
         for (int k = 0; k < 3; ++k) fill(bb[b][k], 0.0f, NM, s);
     k_init_ishikawa<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Ct, u, pt, N, (int)L);

@@ -18,8 +18,10 @@ constexpr int kWave = 64;
 struct Flags {
     int32_t done;       // stop: converged, or alpha invalid (every later iteration returns)
     int32_t bad_alpha;  // alpha <= 0 somewhere (set before the first iteration)
-    int32_t it_num;     // iteration at which the stop test fired (valid when done)
-    int32_t pad[61];
+    int32_t it_num;     // iteration at which the stop test fired (valid when done or stop_at)
+    int32_t stop_at;    // fused binary, stop test in every workgroup: that iteration + 1,
+                        // written once by the launch that finds the stop (0: none)
+    int32_t pad[60];
 };
 
 __device__ __forceinline__ float fmin_ref(float a, float b) {

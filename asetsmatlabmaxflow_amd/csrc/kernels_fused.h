@@ -86,6 +86,8 @@ struct FusedArgs {
     // set, when not null, so that the following iterations return at once)
     int32_t *alpha_bad, *alpha_stop;
     DeferredStop stop;      // the previous iteration's stop test (kf_binary)
+    int stop_all;           // 1: every workgroup runs it at its start (else workgroup 0, last)
+    float *take_u;          // stop_all: planar u that the launch finding the stop writes
     // block-interleaved binary state (BlkLayout; in.u / out.u are the buffers' bases)
     int nyb;                // 64-row blocks per column: ceil(ny / 64)
     int pb;                 // bytes per plane of the state: nx * nyb * block bytes
@@ -1202,8 +1204,12 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
 // by a workgroup of blockDim.x threads (a divisor of 1024): each thread plays
 // 1024 / blockDim.x of k_finalize's threads -- the same strided sums, the same
 // wave sums of the same 64 values, the 16 wave sums added in the same order --
-// so cvg has the same bits.  scratch: >= 16 doubles of LDS, free on entry.
-__device__ __forceinline__ void deferred_stop(const DeferredStop &d, double *scratch) {
+// so cvg has the same bits.  Every thread forms the final sum, so every thread
+// returns the decision; thread 0 writes cvg and the stop outputs when `write`:
+// flags->done, or with `at` flags->stop_at (the launch that every workgroup of
+// tests it).  scratch: >= 16 doubles of LDS, free on entry and on return.
+__device__ __forceinline__ bool stop_test(const DeferredStop &d, double *scratch, bool write,
+                                          bool at = false) {
     constexpr int kFinalizeThreads = 1024;
     for (int vt = threadIdx.x; vt < kFinalizeThreads; vt += blockDim.x) {
         double v = strided_sum(d.partials, d.nparts, vt, kFinalizeThreads);
@@ -1211,16 +1217,42 @@ __device__ __forceinline__ void deferred_stop(const DeferredStop &d, double *scr
         if ((vt & (kWave - 1)) == 0) scratch[vt / kWave] = v;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = 0.0;
-        for (int w = 0; w < kFinalizeThreads / kWave; ++w) r += scratch[w];
-        const float c = (float)(r / d.denom);
+    double r = 0.0;
+    for (int w = 0; w < kFinalizeThreads / kWave; ++w) r += scratch[w];
+    __syncthreads();
+    const float c = (float)(r / d.denom);
+    const bool stop = c <= d.err_bound;   // asetsBinaryMF3D_mex.c:183-188
+    if (write && threadIdx.x == 0) {
         d.cvg[d.it] = c;
-        if (c <= d.err_bound) {   // asetsBinaryMF3D_mex.c:183-188
-            d.flags->done = 1;
+        if (stop) {
             d.flags->it_num = d.it;
+            if (at) d.flags->stop_at = d.it + 1;
+            else d.flags->done = 1;
             *d.it_num = d.it;
         }
+    }
+    return stop;
+}
+
+__device__ __forceinline__ void deferred_stop(const DeferredStop &d, double *scratch) {
+    (void)stop_test(d, scratch, true);
+}
+
+// This workgroup's share of u out of the block-interleaved input state a.in into
+// the planar a.take_u, four voxels per 16-byte access (ny a multiple of 4, take_u
+// 16-byte aligned; the state's blocks are).
+template <typename T>
+__device__ void take_u_share(const FusedArgs<T> &a) {
+    using BL = BlkLayout<T>;
+    const char *st = (const char *)a.in.u;
+    const int ny = a.ny;
+    const int nq = a.plane / 4 * a.nz;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+        const int g = 4 * q;
+        const int c = g / ny, y = g - c * ny;
+        const float4 v = *(const float4 *)(st + ((size_t)c * a.nyb + (y >> 6)) * BL::kBytes +
+                                           BL::kU + 4 * (y & 63));
+        *(float4 *)(a.take_u + g) = v;
     }
 }
 
@@ -1238,12 +1270,32 @@ template <bool IS3D, int TX, int NT, int WAVES, typename T, bool FIRST = false, 
 __global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
     __shared__ double red[16];
     if (a.flags->done) return;
+    // stop_all: every workgroup runs the previous iteration's stop test first (workgroup
+    // 0 writes its outputs).  When it fires, the workgroup writes its share of u
+    // (take_u) from this launch's input, the final state, and returns, so this launch
+    // costs no more than that copy.  stop_at (that iteration + 1) is written once;
+    // a workgroup that reads it skips the test: it returns at once in a later launch
+    // and takes its share in this one.  A stale 0 only makes it run the test.  One
+    // thread reads stop_at for the whole workgroup (its waves must agree).
+    if (a.stop_all && a.stop.partials) {
+        __shared__ int32_t at_s;
+        if (threadIdx.x == 0) at_s = a.flags->stop_at;
+        __syncthreads();
+        const int32_t at = at_s;
+        if (at != 0 && at - 1 != a.stop.it) return;
+        if (at != 0 || stop_test(a.stop, red, blockIdx.x == 0, true)) {
+            if constexpr (BLK) {
+                if (a.take_u) take_u_share<T>(a);
+            }
+            return;
+        }
+    }
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const double e =
         (lb < a.nblocks) ? fused_tile<IS3D, TX, NT, T, 0, FIRST, IS3D, BLK>(a, lb) : 0.0;
     const double s = block_sum(e, red);
     if (threadIdx.x == 0) a.partials[blockIdx.x] = s;
-    if (blockIdx.x == 0 && a.stop.partials) {
+    if (blockIdx.x == 0 && a.stop.partials && !a.stop_all) {
         __syncthreads();   // thread 0 has read red
         deferred_stop(a.stop, red);
     }

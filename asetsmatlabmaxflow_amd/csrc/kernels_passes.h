@@ -46,9 +46,14 @@ __global__ void k_fill_f32(float *p, float v, int64_t n) {
     if (i < n) p[i] = v;
 }
 
-__global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
-    int64_t i = gid();
-    if (i < n) p[i] = v;
+// Start of a solve in one launch: nflag flag words to 0, *it_num = max_iter and
+// cvg[0, n) = 0.
+__global__ void k_solve_setup(int32_t *flags, int nflag, int32_t *it_num, int32_t max_iter,
+                              float *cvg, int64_t n) {
+    const int64_t i = gid();
+    if (i < nflag) flags[i] = 0;
+    if (i == 0) *it_num = max_iter;
+    if (i < n) cvg[i] = 0.0f;
 }
 
 // asetsBinaryMF3D_mex.c:205-224; warm start (u0, pt0 and optionally ps0) has no
@@ -251,7 +256,7 @@ template <bool STRICT = false>
 __global__ void k_finalize(const double *partials, int64_t nparts, double denom, int32_t it,
                            float err_bound, float *cvg, int32_t *it_num, Flags *f) {
     __shared__ double red[16];
-    if (f->done) return;
+    if (f->done || f->stop_at) return;
     const double v = strided_sum(partials, nparts, threadIdx.x, blockDim.x);
     double s = block_sum(v, red);
     if (threadIdx.x == 0) {

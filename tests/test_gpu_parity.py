@@ -112,6 +112,58 @@ def test_stop_rule_matches(cuda, kernel):
     assert _bits_equal(r.u.cpu().numpy(), ref.u)
 
 
+def _first_new_minimum(cvg64, lo):
+    """An iteration k >= lo whose cvg is below every earlier one, and an errBound halfway
+    between, so that the first iteration with cvg <= errBound is k by a wide margin."""
+    for k in range(lo, len(cvg64)):
+        prev = cvg64[:k].min()
+        if cvg64[k] < prev and (prev - cvg64[k]) > 1e-4 * prev:
+            return k, float(np.float32(0.5 * (cvg64[k] + prev)))
+    raise AssertionError("no new minimum of cvg")
+
+
+@pytest.mark.parametrize("flow", ["f32", "f16"])
+@pytest.mark.parametrize("shape", [(64, 29, 17), (36, 20, 9), (33, 29, 17), (128, 40)])
+def test_stop_found_by_next_launch(cuda, flow, shape):
+    """errBound > 0 on the fused kernel: every workgroup of the launch after iteration k
+    runs k's stop test, and when it fires that launch writes u out of the final state
+    (ny a multiple of 4, u asked for alone; else the copy after the loop does).  The stop
+    falls inside a poll batch, on a batch's last launch, on the solve's last iteration
+    (the test after the loop finds it) and one before (the last launch finds it); with
+    and without the state outputs.  fp32 against the oracle; fp16 against its own run
+    of the same iteration count with errBound 0."""
+    cc, steps = 0.25, 0.11
+    Cs, Ct, a = workloads.random_smooth_binary(shape, seed=21 + sum(shape), alpha=0.2)
+    d = [solver.to_device(x) for x in (Cs, Ct, a)]
+    if flow == "f32":
+        trace = orc.binary(Cs, Ct, a, shape, 120, 0.0, cc, steps, stop_rule=1).cvg64
+    else:
+        trace = solver.binary(*d, shape, 120, 0.0, cc, steps, kernel="fused",
+                              flow=flow).cvg.cpu().numpy().astype(np.float64)
+    k, eb = _first_new_minimum(trace, 20)
+    cases = [(120, 7, False), (120, k + 1, False), (k + 1, 0, False), (k + 2, 0, False),
+             (120, 7, True), (k + 2, 0, True)]
+    for maxit, poll, state in cases:
+        print(f"stop case shape={shape} flow={flow} k={k} maxit={maxit} poll={poll} "
+              f"state={state}", flush=True)
+        r = solver.binary(*d, shape, maxit, eb, cc, steps, kernel="fused", flow=flow,
+                          poll_every=poll, want_state=state)
+        assert r.it_num == k, (r.it_num, k)
+        cvg = r.cvg.cpu().numpy()
+        assert np.all(cvg[k + 1:] == 0)
+        u = r.u.cpu().numpy()
+        if flow == "f32":
+            ref = orc.binary(Cs, Ct, a, shape, maxit, eb, cc, steps, stop_rule=1)
+            assert ref.it_num == k
+            assert _bits_equal(u, ref.u), _report(u, ref.u)
+            np.testing.assert_allclose(cvg[:k + 1], ref.cvg64[:k + 1].astype(np.float32),
+                                       rtol=1e-6, atol=0)
+        else:
+            fixed = solver.binary(*d, shape, k + 1, 0.0, cc, steps, kernel="fused", flow=flow)
+            assert _bits_equal(u, fixed.u.cpu().numpy())
+            assert np.array_equal(cvg[:k + 1], fixed.cvg.cpu().numpy())
+
+
 @pytest.mark.parametrize("kernel", ["fused", "passes"])
 def test_warm_start(cuda, kernel):
     shape = (40, 21, 13)
