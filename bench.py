@@ -43,6 +43,22 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 REF_ITERS = 2000        # fixed iterations of the converged reference solve
 LABEL_TARGET = 0.9999   # label agreement the errBound 1e-4 solve is held to
+TIMED_CALLS = 9         # whole solve calls timed per figure (after an untimed first call)
+
+
+def _median_call(fn, n=TIMED_CALLS):
+    """Median wall time of n calls of fn() (each bracketed by device synchronisations)
+    and the last call's result."""
+    import torch
+
+    ts, res = [], None
+    for _ in range(n):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        res = fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+    return sorted(ts)[n // 2], res, ts
 
 
 def _oracle():
@@ -206,15 +222,10 @@ def bench_binary3d(args):
 
     if not args.no_tol:
         # whole solve calls (setup, iterations, stop polls, output copy) from the host:
-        # the first call of this configuration, then the median of the next five
-        runs = []
-        for _ in range(6):
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            rt = solve(20000, 1e-4)
-            torch.cuda.synchronize()
-            runs.append(time.perf_counter() - t1)
-        ttt = sorted(runs[1:])[2]
+        # the first call of this configuration, then the median of the next TIMED_CALLS
+        first, _, _ = _median_call(lambda: solve(20000, 1e-4), 1)
+        ttt, rt, runs = _median_call(lambda: solve(20000, 1e-4))
+        runs = [first] + runs
         _progress(f"time to tolerance: {ttt * 1e3:.2f} ms")
         lab = (rt.u > 0.5)
         e = solver.energy_binary(dCs, dCt, dA, rt.u, shape)
@@ -234,14 +245,8 @@ def bench_binary3d(args):
 
         sweep = []
         for eb in (1e-4, 5e-5, 2e-5, 1e-5, 5e-6, 2e-6, 1e-6, 5e-7):
-            ts = []
-            for _ in range(3):
-                torch.cuda.synchronize()
-                t1 = time.perf_counter()
-                r_eb = solve(20000, eb)
-                torch.cuda.synchronize()
-                ts.append(time.perf_counter() - t1)
-            sweep.append({"err_bound": eb, "seconds": sorted(ts)[1],
+            t_eb, r_eb, _ = _median_call(lambda: solve(20000, eb))
+            sweep.append({"err_bound": eb, "seconds": t_eb,
                           "iterations": r_eb.iterations, "label_agreement": agree(r_eb.u)})
             if sweep[-1]["label_agreement"] >= LABEL_TARGET:
                 break
@@ -311,18 +316,10 @@ def bench_binary3d(args):
                                          u0=st["u"], pt0=st["pt"], stream=stream)
 
                 warm_solve(predict())   # warm-up
-                torch.cuda.synchronize()
-                # median of three (inference + solve), as the cold median above
-                t_infs, t_warms = [], []
-                for _ in range(3):
-                    t3 = time.perf_counter()
-                    st = predict()
-                    torch.cuda.synchronize()
-                    t_infs.append(time.perf_counter() - t3)
-                    rw = warm_solve(st)
-                    torch.cuda.synchronize()
-                    t_warms.append(time.perf_counter() - t3)
-                t_inf, t_warm = sorted(t_infs)[1], sorted(t_warms)[1]
+                # medians of TIMED_CALLS calls, as the cold figure: inference alone, and
+                # inference + solve
+                t_inf, _, _ = _median_call(predict)
+                t_warm, rw, _ = _median_call(lambda: warm_solve(predict()))
                 e_warm = solver.energy_binary(dCs, dCt, dA, rw.u, shape, stream=stream)[0]
                 st = predict()
                 e_net = solver.energy_binary(dCs, dCt, dA, st["u"], shape, stream=stream)[0]
@@ -346,23 +343,18 @@ def bench_binary3d(args):
                 # which the warm start's energy gap is no worse than the cold 1e-4 stop's
                 gap_cold = (e_cold - e_star) / abs(e_star)
                 for eb in (1e-4, 5e-5, 3e-5, 2e-5, 1.5e-5, 1e-5, 5e-6):
-                    ts = []
-                    for _ in range(3):
-                        torch.cuda.synchronize()
-                        t5 = time.perf_counter()
+                    def warm_eb():
                         st = predict()
                         if full:
-                            rq = solver.binary_resume(dCs, dCt, dA, shape, st, 20000, eb, cc,
-                                                      steps, stream=stream)
-                        else:
-                            rq = solver.binary(dCs, dCt, dA, shape, 20000, eb, cc, steps,
-                                               u0=st["u"], pt0=st["pt"], stream=stream)
-                        torch.cuda.synchronize()
-                        ts.append(time.perf_counter() - t5)
+                            return solver.binary_resume(dCs, dCt, dA, shape, st, 20000, eb,
+                                                        cc, steps, stream=stream)
+                        return solver.binary(dCs, dCt, dA, shape, 20000, eb, cc, steps,
+                                             u0=st["u"], pt0=st["pt"], stream=stream)
+
+                    tq, rq, _ = _median_call(warm_eb)
                     eq = solver.energy_binary(dCs, dCt, dA, rq.u, shape, stream=stream)[0]
                     gq = (eq - e_star) / abs(e_star)
                     if gq <= gap_cold:
-                        tq = sorted(ts)[1]
                         out[key]["same_energy"] = {
                             "err_bound": eb, "iterations": rq.iterations, "seconds": tq,
                             "energy_gap": gq,
@@ -373,16 +365,15 @@ def bench_binary3d(args):
             rs = solver.binary(dCs, dCt, dA, shape, REF_ITERS, 0.0, cc, steps, stream=stream,
                                want_state=True)
             ideal = {"u": rs.u, **rs.state}
-            solver.binary_resume(dCs, dCt, dA, shape, ideal, 20000, 1e-4, cc, steps,
-                                 stream=stream)
-            torch.cuda.synchronize()
-            t4 = time.perf_counter()
-            ri = solver.binary_resume(dCs, dCt, dA, shape, ideal, 20000, 1e-4, cc, steps,
-                                      stream=stream)
-            torch.cuda.synchronize()
+            def resume_ideal():
+                return solver.binary_resume(dCs, dCt, dA, shape, ideal, 20000, 1e-4, cc,
+                                            steps, stream=stream)
+
+            resume_ideal()
+            t_ideal, ri, _ = _median_call(resume_ideal)
             if "warm_start" in out:
                 out["warm_start"]["ideal_full_state"] = {
-                    "seconds": time.perf_counter() - t4, "iterations": ri.iterations,
+                    "seconds": t_ideal, "iterations": ri.iterations,
                     "note": "resume from the 2000-iteration solve's own state: what an exact "
                             "full-state predictor would reach, inference excluded"}
             del rs, ideal, ri

@@ -492,17 +492,21 @@ def test_batched_zero_and_one_iteration(cuda):
 
 
 @pytest.mark.parametrize("flow", ["f32", "f16"])
-@pytest.mark.parametrize("shape,tile_x", [((70, 45, 33), 8), ((70, 45, 33), 16),
-                                          ((70, 45, 33), 32), ((130, 77), 16)])
-def test_deferred_stop_test_same_bits(cuda, flow, shape, tile_x):
-    """errBound 0 folds each iteration's stop test into the next iteration's kernel
-    (workgroups of 256, 512 or 1024 threads playing the finalize kernel's 1024); a
-    negative errBound (never met) keeps the separate finalize kernel.  The cvg traces
-    and u are the same bits."""
+@pytest.mark.parametrize("shape,tile_x,zchunk", [((70, 45, 33), 8, 0), ((70, 45, 33), 16, 0),
+                                                 ((70, 45, 33), 32, 0), ((130, 77), 16, 0),
+                                                 ((200, 96, 96), 8, 1)])
+def test_deferred_stop_test_same_bits(cuda, flow, shape, tile_x, zchunk):
+    """errBound 0 folds each iteration's stop test into the end of the next iteration's
+    first workgroup (workgroups of 256, 512 or 1024 threads playing the finalize kernel's
+    1024).  A negative errBound (never met) runs it at the start of every workgroup of
+    the next iteration's kernel, or, past 2048 workgroups (4608 here with 64 x 8 tiles
+    of one plane), in the separate finalize kernel.  The cvg traces and u are the same
+    bits."""
     Cs, Ct, a = workloads.random_smooth_binary(shape, seed=23, alpha=0.15)
     d = [solver.to_device(x) for x in (Cs, Ct, a)]
-    r0 = solver.binary(*d, shape, 25, 0.0, 0.3, 0.14, kernel="fused", flow=flow, tile_x=tile_x)
-    rn = solver.binary(*d, shape, 25, -1.0, 0.3, 0.14, kernel="fused", flow=flow, tile_x=tile_x)
+    kw = dict(kernel="fused", flow=flow, tile_x=tile_x, zchunk=zchunk)
+    r0 = solver.binary(*d, shape, 25, 0.0, 0.3, 0.14, **kw)
+    rn = solver.binary(*d, shape, 25, -1.0, 0.3, 0.14, **kw)
     assert r0.it_num == rn.it_num == 25
     assert torch.equal(r0.cvg.view(torch.int32), rn.cvg.view(torch.int32))
     assert torch.equal(r0.u.view(torch.int32), rn.u.view(torch.int32))
