@@ -1593,7 +1593,7 @@ asets_status asets_nn_head_s2d_bwd(const void *x, int n, int d, int h, int w, co
 asets_status asets_nn_warm_state(float *u0, float *bx0, float *by0, float *bz0, const float *Cs,
                                  const float *Ct, int64_t ny, int64_t nx, int64_t nz, int n,
                                  float *ps, float *pt, void *stream) {
-    if (ny <= 0 || nx <= 0 || nz <= 0 || n <= 0 || ny * nx >= kMaxElems ||
+    if (ny <= 0 || nx <= 0 || nz <= 0 || n <= 0 || ny * nx >= kMaxElems || nz >= kMaxElems ||
         ny * nx * nz >= ((int64_t)1 << 40) / n)
         return nn_fail(ASETS_EBADARG, "bad extents");
     if (!u0 || !bx0 || !by0 || !bz0 || !Cs || !Ct || !ps || !pt)

@@ -53,6 +53,18 @@ def test_nn_bad_arguments_rejected_before_device_work():
     assert b"scratch" in L.asets_nn_last_error()
     assert L.asets_nn_conv3(1, 1 << 12, 1 << 12, 1 << 12, 8, 16, 1, None, 16, 1, 1,
                             None) == nat.ASETS_ETOOBIG
+    # space-to-depth: odd extents, misaligned fields, too many elements
+    assert L.asets_nn_features_s2d(8, 8, 8, 1, 4, 4, 5, 8, None) == nat.ASETS_EBADARG
+    assert L.asets_nn_features_s2d(8, 12, 8, 1, 4, 4, 4, 8, None) == nat.ASETS_EBADARG
+    assert L.asets_nn_head_s2d(8, 1, 4, 4, 4, 8, 8, 8, 8, 8, 8, None) == nat.ASETS_EBADARG
+    assert L.asets_nn_features_s2d(8, 8, 8, 1, 1 << 10, 1 << 10, 1 << 10, 8,
+                                   None) == nat.ASETS_ETOOBIG
+    # warm state: every extent must fit the kernel's 32-bit coordinates
+    one = 8
+    assert L.asets_nn_warm_state(*([one] * 6), 1, 1, 1 << 31, 1, one, one,
+                                 None) == nat.ASETS_EBADARG
+    assert L.asets_nn_warm_state(*([one] * 6), 4, 4, 4, 0, one, one, None) == nat.ASETS_EBADARG
+    assert L.asets_nn_warm_state(*([one] * 6), 4, 4, 4, 1, None, one, None) == nat.ASETS_EBADARG
 
 
 def test_mex_modules_export_mexFunction():
