@@ -235,12 +235,14 @@ struct TileCfg {
 // chunk layer (all labels or flow fields of a launch).  A workgroup streams its
 // chunk plus the 4-plane prologue; the 64 x 16 tile's LDS allows two resident
 // workgroups per CU, and two give about 1.25x the throughput of one.  A CU given
-// k = ceil(W / CUs) of the W workgroups then takes about (k == 1 ? 1 : k / 1.25)
-// workgroup-times, and the chunk (at least 8 planes) with the least modelled
-// time wins.  A grid just past one resident round is what it avoids: Ishikawa at
-// 192^3 with 5 flow fields (180 tiles) took 0.731 ms per iteration with 64-plane
-// chunks (540 workgroups) and 0.623 ms with the 48 planes the model picks; for
-// the binary kernel it picks what a resident-round rule picked (256^3: 32).
+// k = ceil(W / CUs) of the W workgroups runs them in pairs, the odd one alone:
+// about (k / 2) * 2 / 1.25 + (k % 2) workgroup-times, and the chunk (at least 8
+// planes) with the least modelled time wins.  A grid just past one resident round
+// is what it avoids: Ishikawa at 192^3 with 5 flow fields (180 tiles) took 0.731
+// ms per iteration with 64-plane chunks (540 workgroups); a sweep of 20 to 64
+// planes ranked the chunks as this model does (28 best, 0.591 ms; 48, 0.600 ms;
+// 32, 0.626 ms), and so for Potts (28, 0.630 ms; 48, 0.661 ms).  For the binary
+// kernel it picks what a resident-round rule picked (256^3: 32).
 int pick_zchunk(int64_t tiles, int64_t nz) {
     if (nz <= 8) return (int)std::max<int64_t>(nz, 1);
     int cus = 0, dev = 0;
@@ -255,7 +257,8 @@ int pick_zchunk(int64_t tiles, int64_t nz) {
     for (int64_t nzc = 1; nzc <= nz / 8; ++nzc) {
         const int64_t zc = cdiv(nz, nzc);
         const int64_t k = cdiv(tiles * cdiv(nz, zc), cus);
-        const double cost = (k == 1 ? 1.0 : (double)k / 1.25) * (double)(zc + 4);
+        const double cost = ((double)(k / 2) * (2.0 / 1.25) + (double)(k % 2)) *
+                            (double)(zc + 4);
         if (nzc == 1 || cost < best) {
             best = cost;
             best_zc = zc;

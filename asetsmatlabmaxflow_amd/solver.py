@@ -143,8 +143,8 @@ def binary_resume(Cs, Ct, alpha, shape, state: dict, max_iter: int, err_bound: f
 
 
 def potts(Ct, alpha, shape, nlab: int, max_iter: int, err_bound: float, cc: float,
-          steps: float, *, kernel: str = "auto", poll_every: int = 0, stream=None,
-          time_kernel: bool = False) -> Result:
+          steps: float, *, kernel: str = "auto", tile_x: int = 0, zchunk: int = 0,
+          poll_every: int = 0, stream=None, time_kernel: bool = False) -> Result:
     """Potts continuous max-flow (asetsPotts2D/3D_mex semantics) on device tensors."""
     if len(shape) == 2:
         ndim, (ny, nx), nz = 2, shape, 1
@@ -163,7 +163,7 @@ def potts(Ct, alpha, shape, nlab: int, max_iter: int, err_bound: float, cc: floa
     secs = C.c_double(0.0)
     kms = C.c_double(0.0) if time_kernel else None
     opts = nat.make_opts(_stream_handle(stream), nat.FLOW_F32, _KERNELS[kernel], poll_every,
-                         0, 0, kms)
+                         tile_x, zchunk, kms)
     p = nat.make_params(max_iter, err_bound, cc, steps)
     rc = nat.lib().asets_potts_dev(ndim, _ptr(Ct), _ptr(alpha), ny, nx, nz, nlab, C.byref(p),
                                    _ptr(u), _ptr(cvg), itn.data_ptr(), C.byref(secs),

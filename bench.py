@@ -450,11 +450,13 @@ def bench_potts3d(args):
     cc, st = workloads.POTTS3D_CC, workloads.POTTS3D_STEPS
     kern = "passes" if args.kernel == "passes" else "fused"
     for _ in range(args.warmup):
-        solver.potts(*d, shape, L, 5, 0.0, cc, st, kernel=kern)
+        solver.potts(*d, shape, L, 5, 0.0, cc, st, kernel=kern, tile_x=args.tile_x,
+                     zchunk=args.zchunk)
     torch.cuda.synchronize()
     t = time.perf_counter()
     for _ in range(args.steps):
-        r = solver.potts(*d, shape, L, iters, 0.0, cc, st, kernel=kern, time_kernel=True)
+        r = solver.potts(*d, shape, L, iters, 0.0, cc, st, kernel=kern, tile_x=args.tile_x,
+                         zchunk=args.zchunk, time_kernel=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t
     out = _base(args, "voxel-iters/s, 192^3 Potts L=6 (300 fixed iterations)",
@@ -490,11 +492,12 @@ def bench_ishikawa3d(args):
     d = [solver.to_device(x) for x in (Ct, a)]
     cc, st = workloads.ISHIKAWA3D_CC, workloads.ISHIKAWA3D_STEPS
     for _ in range(args.warmup):
-        solver.ishikawa(*d, shape, L, 5, 0.0, cc, st)
+        solver.ishikawa(*d, shape, L, 5, 0.0, cc, st, tile_x=args.tile_x, zchunk=args.zchunk)
     torch.cuda.synchronize()
     t = time.perf_counter()
     for _ in range(args.steps):
-        r = solver.ishikawa(*d, shape, L, iters, 0.0, cc, st, time_kernel=True)
+        r = solver.ishikawa(*d, shape, L, iters, 0.0, cc, st, tile_x=args.tile_x,
+                            zchunk=args.zchunk, time_kernel=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t
     out = _base(args, "voxel-iters/s, 192^3 Ishikawa L=6 (300 fixed iterations)",
