@@ -11,7 +11,8 @@ the GPU:
   ideal      plain solver from the reference's own (u, pt) to errBound: the bound on what
              any (u0, pt0) predictor can gain (the flows restart from 0 either way)
   ideal_full_state  (--variant s2d) resume from the reference's whole state
-Reported per run: wall seconds (warm includes inference), iterations, relative energy gap
+Reported per run: wall seconds (the median of 7 calls after an untimed one; warm includes
+inference), iterations, relative energy gap
 (E - E*) / |E*| and label agreement with the reference; for the network output alone its
 energy gap and label agreement.  Also the iterations each start needs to reach a relative
 energy gap of 1e-5 (searched over a doubling schedule of fixed iteration counts).
@@ -51,12 +52,18 @@ def evaluate_volume(net, seed: int, size: int, err_bound: float, ref_iters: int,
         return {"gap": (e - Estar) / abs(Estar),
                 "label_agreement": float(((u > 0.5) == lab_ref).float().mean())}
 
-    def timed(fn):
-        _sync()
-        t = time.perf_counter()
-        r = fn()
-        _sync()
-        return r, time.perf_counter() - t
+    def timed(fn, n=7):
+        # one untimed call (the GPU idled while the volume was generated on the CPU),
+        # then the median of n
+        fn()
+        ts = []
+        for _ in range(n):
+            _sync()
+            t = time.perf_counter()
+            r = fn()
+            _sync()
+            ts.append(time.perf_counter() - t)
+        return r, sorted(ts)[n // 2]
 
     out = {"seed": seed, "shape": list(shape), "E_star": Estar, "ref_iters": ref_iters}
     # warm-up of both paths (kernel loading, workspace allocation)
