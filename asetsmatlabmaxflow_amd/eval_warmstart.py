@@ -11,6 +11,9 @@ the GPU:
   ideal      plain solver from the reference's own (u, pt) to errBound: the bound on what
              any (u0, pt0) predictor can gain (the flows restart from 0 either way)
   ideal_full_state  (--variant s2d) resume from the reference's whole state
+With --sensitivity it instead resumes from the reference's converged state with its face
+flows perturbed (noise, scaling, truncation away from the label boundary, blurring,
+coarsening) and reports the iterations each start needs.
 Reported per run: wall seconds (the median of 7 calls after an untimed one; warm includes
 inference), iterations, relative energy gap
 (E - E*) / |E*| and label agreement with the reference; for the network output alone its
@@ -122,6 +125,89 @@ def evaluate_volume(net, seed: int, size: int, err_bound: float, ref_iters: int,
     return out
 
 
+def sensitivity(seed: int, size: int, err_bound: float, ref_iters: int, calls: int = 3):
+    """How accurate a full-state start must be: resume the plain solver from the
+    reference's converged state with its face flows perturbed in stated ways (ps and pt
+    re-paired to them each time, as for the network's prediction) and report the
+    iterations to err_bound and the wall time (median of `calls` calls)."""
+    import torch
+    import torch.nn.functional as F
+
+    from asetsmatlabmaxflow_amd import solver, workloads
+    from asetsmatlabmaxflow_amd.warmstart import pair_state
+
+    shape = (size, size, size)
+    Cs, Ct, a, _ = workloads.binary3d_ellipsoids(shape, seed=seed)
+    d = [solver.to_device(x) for x in (Cs, Ct, a)]
+    cc, st = workloads.BINARY3D_CC, workloads.BINARY3D_STEPS
+    ref = solver.binary(*d, shape, ref_iters, 0.0, cc, st, want_state=True)
+    cold = solver.binary(*d, shape, 20000, err_bound, cc, st)
+    gen = torch.Generator(device=d[0].device).manual_seed(seed)
+    vol = (size, size, size)   # (nz, nx, ny) view of a column-major field
+    faces = {k: ref.state[k] for k in ("bx", "by", "bz")}
+    alpha = d[2]
+
+    def noise():
+        return torch.randn(ref.u.numel(), generator=gen, device=ref.u.device)
+
+    def blur(f, sigma):
+        r = int(3 * sigma + 0.5)
+        x = torch.arange(-r, r + 1, device=f.device, dtype=torch.float32)
+        k = torch.exp(-0.5 * (x / sigma) ** 2)
+        k = k / k.sum()
+        t = f.view(1, 1, *vol)
+        for dim in range(3):
+            shp = [1, 1, 1, 1, 1]
+            shp[2 + dim] = 2 * r + 1
+            pad = [0, 0, 0, 0, 0, 0]
+            pad[2 * (2 - dim)] = pad[2 * (2 - dim) + 1] = r
+            t = F.conv3d(F.pad(t, pad, mode="replicate"), k.view(shp))
+        return t.reshape(-1)
+
+    def coarse(f):
+        t = F.avg_pool3d(f.view(1, 1, *vol), 2)
+        return F.interpolate(t, scale_factor=2, mode="nearest").reshape(-1)
+
+    def near_boundary(u, r):
+        lab = (u > 0.5).float().view(1, 1, *vol)
+        edge = (F.max_pool3d(lab, 3, 1, 1) != -F.max_pool3d(-lab, 3, 1, 1)).float()
+        return F.max_pool3d(edge, 2 * r + 1, 1, r).reshape(-1) > 0
+
+    def case(name, fx, u=None):
+        state = {"u": (ref.u if u is None else u).clone(),
+                 **{k: fx(k, v).contiguous().clone() for k, v in faces.items()}}
+        pair_state(state, d[0], d[1], shape)
+        r = None
+        ts = []
+        for _ in range(calls):
+            _sync()
+            t = time.perf_counter()
+            r = solver.binary_resume(*d, shape, state, 20000, err_bound, cc, st)
+            _sync()
+            ts.append(time.perf_counter() - t)
+        return {"start": name, "iterations": r.iterations, "seconds": sorted(ts)[calls // 2]}
+
+    rows = [{"start": "cold (reference initial cut)", "iterations": cold.iterations}]
+    rows.append(case("exact", lambda k, v: v))
+    for eps in (0.05, 0.1, 0.2, 0.3):
+        rows.append(case(f"x (1 + {eps:g} N(0,1))", lambda k, v, e=eps: v * (1 + e * noise())))
+    for dl in (0.01, 0.03, 0.1):
+        rows.append(case(f"+ {dl:g} alpha N(0,1)", lambda k, v, e=dl: v + e * alpha * noise()))
+    for sc in (0.9, 1.1):
+        rows.append(case(f"x {sc:g}", lambda k, v, c=sc: v * c))
+    for r in (8, 4, 2):
+        m = near_boundary(ref.u, r)
+        rows.append(case(f"exact within {r} voxels of the label boundary, 0 beyond",
+                         lambda k, v, mm=m: torch.where(mm, v, torch.zeros_like(v))))
+    for sg in (0.7, 1.5):
+        rows.append(case(f"Gaussian blur sigma {sg:g}", lambda k, v, g=sg: blur(v, g)))
+    rows.append(case("2x coarser field, upsampled", lambda k, v: coarse(v)))
+    rows.append(case("exact faces, u binarised", lambda k, v: v, u=(ref.u > 0.5).float()))
+    rows.append(case("exact faces, u blurred sigma 0.7", lambda k, v: v, u=blur(ref.u, 0.7)))
+    return {"seed": seed, "shape": list(shape), "err_bound": err_bound, "ref_iters": ref_iters,
+            "sensitivity": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--weights", default=None)
@@ -132,7 +218,14 @@ def main():
     ap.add_argument("--variant", default="full", choices=["full", "s2d"])
     ap.add_argument("--no-gap", action="store_true",
                     help="skip the iterations-to-energy-gap search")
+    ap.add_argument("--sensitivity", action="store_true",
+                    help="instead: resume from the reference's state with perturbed flows")
     args = ap.parse_args()
+    if args.sensitivity:
+        for seed in args.seeds:
+            print(json.dumps(sensitivity(seed, args.size, args.err_bound, args.ref_iters)),
+                  flush=True)
+        return
 
     import torch
 

@@ -501,14 +501,23 @@ class WarmStartNetS2D(WarmStartNet):
     def predict_state(self, Cs, Ct, alpha, shape, n=1) -> dict:
         """A full solver state {u, ps, pt, bx, by, bz} for asets_binary_dev_resume."""
         st = self.forward(Cs, Ct, alpha, shape, n)
-        ny, nx, nz = shape
-        P = n * ny * nx * nz
-        st["ps"] = torch.empty(P, dtype=torch.float32, device=self.device)
-        st["pt"] = torch.empty(P, dtype=torch.float32, device=self.device)
-        _ck(nnlib().asets_nn_warm_state(_p(st["u"]), _p(st["bx"]), _p(st["by"]), _p(st["bz"]),
-                                        _p(Cs), _p(Ct), ny, nx, nz, n, _p(st["ps"]),
-                                        _p(st["pt"]), self._stream()))
-        return st
+        return pair_state(st, Cs, Ct, shape, n, stream=self._stream())
+
+
+def pair_state(st: dict, Cs, Ct, shape, n=1, stream=None) -> dict:
+    """Complete {u, bx, by, bz} (device tensors of n volumes, modified in place: u clamped
+    to [0, 1], zero-flux faces set to 0) with the ps and pt that the reference's
+    source/sink update pairs them with (asets_nn_warm_state): a full solver state."""
+    ny, nx, nz = shape
+    P = n * ny * nx * nz
+    dev = st["u"].device
+    st["ps"] = torch.empty(P, dtype=torch.float32, device=dev)
+    st["pt"] = torch.empty(P, dtype=torch.float32, device=dev)
+    _ck(nnlib().asets_nn_warm_state(_p(st["u"]), _p(st["bx"]), _p(st["by"]), _p(st["bz"]),
+                                    _p(Cs), _p(Ct), ny, nx, nz, n, _p(st["ps"]),
+                                    _p(st["pt"]),
+                                    stream if stream is not None else WarmStartNet._stream()))
+    return st
 
 
 def warm_solve_state(net: WarmStartNetS2D, Cs, Ct, alpha, shape, max_iter, err_bound, cc,
