@@ -6,6 +6,22 @@
 // directions normalised), numIts iterations of k_hmf_sink + the spatial step
 // (kf_hmf_flow, or the kh_* passes), then CalculateCostsFullFlow.
 
+// One launch for a solve's output copies: segment blockIdx.y copies n floats from src
+// to dst (src null: fills 0).
+struct CopySeg {
+    float *dst;
+    const float *src;
+    int64_t n;
+};
+constexpr int kMaxCopySegsPerLaunch = 65535;   // grid height
+
+__global__ void k_multi_copy(const CopySeg *segs) {
+    const CopySeg sg = segs[blockIdx.y];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sg.n; i += stride)
+        sg.dst[i] = sg.src ? sg.src[i] : 0.0f;
+}
+
 struct HmfPlan {
     std::vector<HmfNode> nodes;
     std::vector<int32_t> flow;      // non-root nodes in index order
@@ -227,7 +243,8 @@ asets_status hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz, asets_hmf_nod
     size_t need = Carver::sz(M, sizeof(HmfNode)) + Carver::sz(nflow, 4) +
                   Carver::sz(nev, sizeof(HmfEvent)) + Carver::sz(std::max(nshape, 1), sizeof(HmfShapeSrc)) +
                   Carver::sz((int64_t)M * kHmfCostBlocks * 2, 8) + Carver::sz(2 * M, 8) +
-                  (size_t)(nbuf * (1 + nd) + 2) * Carver::sz(MN, 4);
+                  (size_t)(nbuf * (1 + nd) + 2) * Carver::sz(MN, 4) +
+                  Carver::sz((int64_t)M * 9, sizeof(CopySeg));
     if (!fused) need += Carver::sz((int64_t)nflow * N, 4);
     if (nshape) need += Carver::sz((int64_t)nshape * N, 4) + 2 * Carver::sz((int64_t)nshape * 3 * N, 4);
     st = ensure(&dc.dev, &dc.dev_bytes, need);
@@ -239,6 +256,7 @@ asets_status hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz, asets_hmf_nod
     HmfShapeSrc *d_shape = cv.take<HmfShapeSrc>(std::max(nshape, 1));
     double *d_cpart = cv.take<double>((int64_t)M * kHmfCostBlocks * 2);
     double *d_cost = cv.take<double>(2 * M);
+    CopySeg *d_segs = cv.take<CopySeg>((int64_t)M * 9);   // u, pt, div, 3 faces, 3 shape dirs
     float *ub[2] = {nullptr, nullptr}, *fb[2][3] = {};
     for (int b = 0; b < nbuf; ++b) {
         ub[b] = cv.take<float>(MN);
@@ -370,28 +388,29 @@ asets_status hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz, asets_hmf_nod
     HIPCHK(hipGetLastError());
     std::vector<double> cost(2 * M);
     HIPCHK(hipMemcpyAsync(cost.data(), d_cost, 2 * M * sizeof(double), hipMemcpyDeviceToHost, s));
-    // outputs
+    // outputs, every copy and fill in one launch (a 40-leaf tree has about 200)
+    std::vector<CopySeg> segs;
+    segs.reserve((size_t)M * 9);
     for (int32_t n = 0; n < M; ++n) {
         const int64_t on = (int64_t)n * N;
-        if (in[n].u) HIPCHK(hipMemcpyAsync(in[n].u, ub[fin] + on, N * 4, hipMemcpyDeviceToDevice, s));
-        if (in[n].pt) HIPCHK(hipMemcpyAsync(in[n].pt, pt + on, N * 4, hipMemcpyDeviceToDevice, s));
-        if (in[n].div) HIPCHK(hipMemcpyAsync(in[n].div, dv + on, N * 4, hipMemcpyDeviceToDevice, s));
-        for (int k = 0; k < 3; ++k) {
-            if (!in[n].faces[k]) continue;
-            if (k < nd)
-                HIPCHK(hipMemcpyAsync(in[n].faces[k], fb[fin][k] + on, N * 4,
-                                      hipMemcpyDeviceToDevice, s));
-            else
-                fill(in[n].faces[k], 0.0f, N, s);
-        }
+        if (in[n].u) segs.push_back({in[n].u, ub[fin] + on, N});
+        if (in[n].pt) segs.push_back({in[n].pt, pt + on, N});
+        if (in[n].div) segs.push_back({in[n].div, dv + on, N});
+        for (int k = 0; k < 3; ++k)
+            if (in[n].faces[k]) segs.push_back({in[n].faces[k], k < nd ? fb[fin][k] + on : nullptr, N});
         const int sh = pl.nodes[n].shape;
-        for (int k = 0; k < 3; ++k) {
-            if (!in[n].l_out[k]) continue;
-            if (sh >= 0)
-                HIPCHK(hipMemcpyAsync(in[n].l_out[k], Lbuf + ((int64_t)sh * 3 + k) * N, N * 4,
-                                      hipMemcpyDeviceToDevice, s));
-            else
-                fill(in[n].l_out[k], 0.0f, N, s);
+        for (int k = 0; k < 3; ++k)
+            if (in[n].l_out[k])
+                segs.push_back({in[n].l_out[k],
+                                sh >= 0 ? Lbuf + ((int64_t)sh * 3 + k) * N : nullptr, N});
+    }
+    if (!segs.empty()) {
+        HIPCHK(hipMemcpyAsync(d_segs, segs.data(), segs.size() * sizeof(CopySeg),
+                              hipMemcpyHostToDevice, s));
+        const unsigned gx = (unsigned)std::min<int64_t>(cdiv(N, kBlock), 1024);
+        for (size_t i0 = 0; i0 < segs.size(); i0 += kMaxCopySegsPerLaunch) {
+            const unsigned gy = (unsigned)std::min<size_t>(segs.size() - i0, kMaxCopySegsPerLaunch);
+            k_multi_copy<<<dim3(gx, gy), kBlock, 0, s>>>(d_segs + i0);
         }
     }
     HIPCHK(hipGetLastError());
