@@ -22,6 +22,22 @@ __global__ void k_multi_copy(const CopySeg *segs) {
         sg.dst[i] = sg.src ? sg.src[i] : 0.0f;
 }
 
+// Enqueue the segments through the device table d_segs (room for segs.size()).
+inline hipError_t multi_copy(const std::vector<CopySeg> &segs, CopySeg *d_segs, hipStream_t s) {
+    if (segs.empty()) return hipSuccess;
+    const hipError_t e = hipMemcpyAsync(d_segs, segs.data(), segs.size() * sizeof(CopySeg),
+                                        hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    int64_t nmax = 0;
+    for (const CopySeg &c : segs) nmax = std::max(nmax, c.n);
+    const unsigned gx = (unsigned)std::max<int64_t>(std::min<int64_t>(cdiv(nmax, kBlock), 1024), 1);
+    for (size_t i0 = 0; i0 < segs.size(); i0 += kMaxCopySegsPerLaunch) {
+        const unsigned gy = (unsigned)std::min<size_t>(segs.size() - i0, kMaxCopySegsPerLaunch);
+        k_multi_copy<<<dim3(gx, gy), kBlock, 0, s>>>(d_segs + i0);
+    }
+    return hipGetLastError();
+}
+
 struct HmfPlan {
     std::vector<HmfNode> nodes;
     std::vector<int32_t> flow;      // non-root nodes in index order
@@ -244,7 +260,7 @@ asets_status hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz, asets_hmf_nod
                   Carver::sz(nev, sizeof(HmfEvent)) + Carver::sz(std::max(nshape, 1), sizeof(HmfShapeSrc)) +
                   Carver::sz((int64_t)M * kHmfCostBlocks * 2, 8) + Carver::sz(2 * M, 8) +
                   (size_t)(nbuf * (1 + nd) + 2) * Carver::sz(MN, 4) +
-                  Carver::sz((int64_t)M * 9, sizeof(CopySeg));
+                  Carver::sz(std::max<int64_t>((int64_t)M * 9, 16), sizeof(CopySeg));
     if (!fused) need += Carver::sz((int64_t)nflow * N, 4);
     if (nshape) need += Carver::sz((int64_t)nshape * N, 4) + 2 * Carver::sz((int64_t)nshape * 3 * N, 4);
     st = ensure(&dc.dev, &dc.dev_bytes, need);
@@ -256,7 +272,9 @@ asets_status hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz, asets_hmf_nod
     HmfShapeSrc *d_shape = cv.take<HmfShapeSrc>(std::max(nshape, 1));
     double *d_cpart = cv.take<double>((int64_t)M * kHmfCostBlocks * 2);
     double *d_cost = cv.take<double>(2 * M);
-    CopySeg *d_segs = cv.take<CopySeg>((int64_t)M * 9);   // u, pt, div, 3 faces, 3 shape dirs
+    // segment tables of the zero fills at the start (up to 11) and of the output copies
+    // (u, pt, div, 3 faces, 3 shape directions per node)
+    CopySeg *d_segs = cv.take<CopySeg>(std::max<int64_t>((int64_t)M * 9, 16));
     float *ub[2] = {nullptr, nullptr}, *fb[2][3] = {};
     for (int b = 0; b < nbuf; ++b) {
         ub[b] = cv.take<float>(MN);
@@ -278,15 +296,20 @@ asets_status hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz, asets_hmf_nod
     if (nshape)
         HIPCHK(hipMemcpyAsync(d_shape, pl.shape.data(), nshape * sizeof(HmfShapeSrc),
                               hipMemcpyHostToDevice, s));
-    // InitializeFullFlow: zero state (asetsHMF3D.m:109-115), normalised directions
-    for (int b = 0; b < nbuf; ++b) {
-        fill(ub[b], 0.0f, MN, s);
-        for (int k = 0; k < nd; ++k) fill(fb[b][k], 0.0f, MN, s);
+    // InitializeFullFlow: zero state (asetsHMF3D.m:109-115) in one launch, normalised
+    // directions
+    {
+        std::vector<CopySeg> zero;
+        for (int b = 0; b < nbuf; ++b) {
+            zero.push_back({ub[b], nullptr, MN});
+            for (int k = 0; k < nd; ++k) zero.push_back({fb[b][k], nullptr, MN});
+        }
+        zero.push_back({pt, nullptr, MN});
+        zero.push_back({dv, nullptr, MN});
+        if (nshape) zero.push_back({exbuf, nullptr, (int64_t)nshape * 3 * N});
+        HIPCHK(multi_copy(zero, d_segs, s));
     }
-    fill(pt, 0.0f, MN, s);
-    fill(dv, 0.0f, MN, s);
     if (nshape) {
-        fill(exbuf, 0.0f, (int64_t)nshape * 3 * N, s);
         const dim3 grid((unsigned)cdiv(N, kBlock), (unsigned)nshape);
         if (is3d) k_hmf_lnorm<true><<<grid, kBlock, 0, s>>>(d_shape, Lbuf, N);
         else k_hmf_lnorm<false><<<grid, kBlock, 0, s>>>(d_shape, Lbuf, N);
@@ -404,16 +427,7 @@ asets_status hmf_dev(int ndim, int64_t ny, int64_t nx, int64_t nz, asets_hmf_nod
                 segs.push_back({in[n].l_out[k],
                                 sh >= 0 ? Lbuf + ((int64_t)sh * 3 + k) * N : nullptr, N});
     }
-    if (!segs.empty()) {
-        HIPCHK(hipMemcpyAsync(d_segs, segs.data(), segs.size() * sizeof(CopySeg),
-                              hipMemcpyHostToDevice, s));
-        const unsigned gx = (unsigned)std::min<int64_t>(cdiv(N, kBlock), 1024);
-        for (size_t i0 = 0; i0 < segs.size(); i0 += kMaxCopySegsPerLaunch) {
-            const unsigned gy = (unsigned)std::min<size_t>(segs.size() - i0, kMaxCopySegsPerLaunch);
-            k_multi_copy<<<dim3(gx, gy), kBlock, 0, s>>>(d_segs + i0);
-        }
-    }
-    HIPCHK(hipGetLastError());
+    HIPCHK(multi_copy(segs, d_segs, s));
     HIPCHK(hipStreamSynchronize(s));
     for (int32_t n = 0; n < M; ++n) {
         in[n].data_cost = cost[2 * n];
