@@ -369,6 +369,53 @@ __global__ void k_init_binary_t(const float *Cs, const float *Ct, const float *u
     Io<T>::put(pt + kPair * g, mt);
 }
 
+// A resumed state's source and sink flows when the caller gives only u and the faces:
+// the fixed point of the source/sink update (asetsBinaryMF3D_mex.c:411-429 at du = 0)
+// pairs them as dv = div b (the faces with axis index 0 taken as 0), pt = Ct where
+// u >= 1/2 and min(Cs - dv, Ct) elsewhere, ps = pt + dv, with u clamped to [0, 1].
+// The same operations, in the same order, as asets_nn_warm_state (2D: no z terms).
+__device__ __forceinline__ void pair_voxel(const float *u0, const float *bx0, const float *by0,
+                                           const float *bz0, const float *Cs, const float *Ct,
+                                           int64_t g, int x, int y, int z, int ny, int nx,
+                                           int nz, int64_t plane, float &u, float &ps,
+                                           float &pt) {
+    const float byl = y > 0 ? by0[g] : 0.0f;
+    const float bxl = x > 0 ? bx0[g] : 0.0f;
+    const float byh = y + 1 < ny ? by0[g + 1] : 0.0f;
+    const float bxh = x + 1 < nx ? bx0[g + ny] : 0.0f;
+    float dv = byh - byl + bxh - bxl;
+    if (bz0) {
+        const float bzl = z > 0 ? bz0[g] : 0.0f;
+        const float bzh = z + 1 < nz ? bz0[g + plane] : 0.0f;
+        dv = dv + bzh - bzl;
+    }
+    float uu = u0[g];
+    uu = uu < 0.0f ? 0.0f : (uu > 1.0f ? 1.0f : uu);
+    const float cs = Cs[g], ct = Ct[g];
+    float t = cs - dv;
+    t = t < ct ? t : ct;
+    u = uu;
+    pt = uu >= 0.5f ? ct : t;
+    ps = pt + dv;
+}
+
+// u, ps and pt of a resumed solve paired as pair_voxel, into planar state buffers
+// (ps and pt every `stride` elements: 1, or kPair for the fused kernels' pairs).
+template <typename T>
+__global__ void k_pair_init_t(const float *Cs, const float *Ct, const float *u0,
+                              const float *bx0, const float *by0, const float *bz0, int ny,
+                              int nx, int nz, float *u, T *ps, T *pt, int stride) {
+    const int64_t plane = (int64_t)ny * nx, N = plane * nz;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    const int y = (int)(g % ny), x = (int)((g / ny) % nx), z = (int)(g / plane);
+    float uu, ms, mt;
+    pair_voxel(u0, bx0, by0, bz0, Cs, Ct, g, x, y, z, ny, nx, nz, plane, uu, ms, mt);
+    u[g] = uu;
+    Io<T>::put(ps + (int64_t)stride * g, ms);
+    Io<T>::put(pt + (int64_t)stride * g, mt);
+}
+
 // The whole initial state of a cold start in one pass: u and the (ps, pt) pair
 // as k_init_binary_t, the (bx, by) pair and bz zero.
 template <typename T>
@@ -442,7 +489,8 @@ __global__ void k_to_f32(float *dst, const T *src, int64_t n, int stride) {
 template <typename T>
 __global__ void k_blk_init(const float *Cs, const float *Ct, const float *u0, const float *ps0,
                            const float *pt0, const float *bx0, const float *by0,
-                           const float *bz0, char *st, int ny, int nx, int nz, int nyb) {
+                           const float *bz0, char *st, int ny, int nx, int nz, int nyb,
+                           int pair) {
     using BL = BlkLayout<T>;
     // 32-bit indices: the layout's buffers stay below 2^31 bytes (fused_fits_blk)
     const int c = (int)blockIdx.x;            // column z * nx + x
@@ -454,7 +502,10 @@ __global__ void k_blk_init(const float *Cs, const float *Ct, const float *u0, co
         float u = 0.0f, ps = 0.0f, pt = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
         if (y < ny) {
             const int64_t g = g0 + y;
-            if (u0) {
+            if (pair) {   // ps, pt paired to u and the faces (pair_voxel)
+                pair_voxel(u0, bx0, by0, bz0, Cs, Ct, g, x, y, z, ny, nx, nz,
+                           (int64_t)ny * nx, u, ps, pt);
+            } else if (u0) {
                 u = u0[g];
                 pt = pt0[g];
                 ps = ps0 ? ps0[g] : pt;
@@ -556,6 +607,7 @@ struct FusedRun {
     bool blk;       // block-interleaved state (BlkLayout) instead of planar fields
     int nyb;        // blocks per column (blk)
     int pb;         // state bytes per plane (blk)
+    bool pair;      // resume: ps and pt paired to u0 and the faces (pair_voxel)
 };
 
 // Workspace bytes of a fused solve with flow element size es.
@@ -594,14 +646,20 @@ asets_status run_fused(const FusedRun &r, Carver &cv, int32_t *hflag) {
             k_blk_init<T><<<cg.grid, cg.threads, 0, s>>>(r.Cs, r.Ct, r.u0, r.ps0, r.pt0,
                                                          r.b0[0], r.b0[1], r.b0[2],
                                                          (char *)buf[0].u, (int)r.ny,
-                                                         (int)r.nx, (int)r.nz, r.nyb);
+                                                         (int)r.nx, (int)r.nz, r.nyb,
+                                                         r.pair ? 1 : 0);
         }
     } else if (first && max_iter == 0) {
         k_init_binary_cold_t<T><<<nb, kBlock, 0, s>>>(r.Cs, r.Ct, buf[0].u, buf[0].ps,
                                                      buf[0].bx, buf[0].bz, N);
     } else if (!first) {
-        k_init_binary_t<T><<<nb, kBlock, 0, s>>>(r.Cs, r.Ct, r.u0, r.ps0, r.pt0, buf[0].u,
-                                                buf[0].ps, buf[0].pt, N);
+        if (r.pair)
+            k_pair_init_t<T><<<nb, kBlock, 0, s>>>(r.Cs, r.Ct, r.u0, r.b0[0], r.b0[1], r.b0[2],
+                                                  (int)r.ny, (int)r.nx, (int)r.nz, buf[0].u,
+                                                  buf[0].ps, buf[0].pt, kPair);
+        else
+            k_init_binary_t<T><<<nb, kBlock, 0, s>>>(r.Cs, r.Ct, r.u0, r.ps0, r.pt0, buf[0].u,
+                                                    buf[0].ps, buf[0].pt, N);
         T *const f0[3] = {buf[0].bx, buf[0].by, buf[0].bz};
         init_faces<T>(f0, r.b0, r.ny, r.nx, r.nz, kPair, s);
     }
@@ -771,6 +829,7 @@ void launch_ishikawa_couple(int L, const float *dv, float *pt, float *u, const f
 struct WarmIn {
     const float *u0 = nullptr, *ps0 = nullptr, *pt0 = nullptr;
     const float *b0[3] = {nullptr, nullptr, nullptr};
+    bool pair = false;   // ps0, pt0 NULL: paired to u0 and the faces (pair_voxel)
 };
 
 // A workspace the caller fixes instead of the library's cached one (the
@@ -800,7 +859,7 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
     if (!Cs || !Ct || !alpha || !u || !it_num || (p->max_iter > 0 && !cvg))
         return fail(ASETS_EBADARG, "null array argument");
     const float *u0 = w.u0, *pt0 = w.pt0;
-    if ((u0 == nullptr) != (pt0 == nullptr))
+    if (w.pair ? !u0 : (u0 == nullptr) != (pt0 == nullptr))
         return fail(ASETS_EBADARG, "warm start needs both u0 and pt0");
     if (!u0 && (w.ps0 || w.b0[0] || w.b0[1] || w.b0[2]))
         return fail(ASETS_EBADARG, "a resumed state needs u0 and pt0");
@@ -887,8 +946,12 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
         float *const f0[3] = {bx, by, bz};
         init_faces<float>(f0, w.b0, ny, nx, nz, 1, s);
         fill(dv, 0.0f, N, s);
-        k_init_binary<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Cs, Ct, u0, w.ps0, pt0, u, ps,
-                                                                   pt, N);
+        if (w.pair)
+            k_pair_init_t<float><<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(
+                Cs, Ct, u0, w.b0[0], w.b0[1], w.b0[2], (int)ny, (int)nx, (int)nz, u, ps, pt, 1);
+        else
+            k_init_binary<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(Cs, Ct, u0, w.ps0, pt0, u,
+                                                                       ps, pt, N);
         if (w.b0[0] || w.b0[1] || w.b0[2]) {
             // dv of the resumed flows: the divergence the previous iteration left
             kp_div<<<(unsigned)cdiv(N, kBlock), kBlock, 0, s>>>(bx, by, bz, dv, G, is3d);
@@ -928,7 +991,7 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
                     {w.b0[0], w.b0[1], w.b0[2]}, u, cvg, it_num,
                     o_ps, o_pt, o_bx, o_by, o_bz, tc, ntile_y, ntile_x, nzc, nblocks,
                     grid_fused, nparts, partials, flags, s, ev0, ev1,
-                    o ? o->poll_every : 0, &itn, blk, (int)cdiv(ny, 64), (int)pb};
+                    o ? o->poll_every : 0, &itn, blk, (int)cdiv(ny, 64), (int)pb, w.pair};
         st = (flow16 == ASETS_FLOW_F16) ? run_fused<__half>(fr, cv, hflag)
                                           : run_fused<float>(fr, cv, hflag);
         if (st) return st;
@@ -1437,7 +1500,7 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
                 k_blk_init<T><<<cg.grid, cg.threads, 0, s>>>(b.Cs, b.Ct, nullptr, nullptr,
                                                              nullptr, nullptr, nullptr, nullptr,
                                                              (char *)b.u[0], b.ny, b.nx, b.nz,
-                                                             b.nyb);
+                                                             b.nyb, 0);
             } else
                 k_init_binary_cold_t<T><<<nb, kBlock, 0, s>>>(b.Cs, b.Ct, b.u[0], b.ps[0],
                                                              b.bx[0], b.bz[0], N);
@@ -2267,12 +2330,15 @@ asets_status asets_binary_dev_resume(int ndim, const float *Cs, const float *Ct,
                                      const float *bz0, float *u, float *cvg, int32_t *it_num,
                                      double *seconds, float *ps, float *pt, float *bx, float *by,
                                      float *bz, const asets_opts *o) {
-    if (!u0 || !ps0 || !pt0 || !bx0 || !by0 || (ndim == 3 && !bz0))
-        return fail(ASETS_EBADARG, "resume needs u0, ps0, pt0 and the flow faces");
+    if (!u0 || !bx0 || !by0 || (ndim == 3 && !bz0))
+        return fail(ASETS_EBADARG, "resume needs u0 and the flow faces");
+    if (!ps0 != !pt0)
+        return fail(ASETS_EBADARG, "resume takes ps0 and pt0 both, or neither (paired)");
     WarmIn w;
     w.u0 = u0;
     w.ps0 = ps0;
     w.pt0 = pt0;
+    w.pair = !ps0;
     w.b0[0] = bx0;
     w.b0[1] = by0;
     w.b0[2] = ndim == 3 ? bz0 : nullptr;

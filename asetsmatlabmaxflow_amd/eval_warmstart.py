@@ -87,11 +87,14 @@ def evaluate_volume(net, seed: int, size: int, err_bound: float, ref_iters: int,
         out["ideal_full_state"] = {"seconds": s, "iterations": r.iterations, **quality(r.u)}
     if net is not None:
         if full:
-            pst, s_inf = timed(lambda: net.predict_state(d[0], d[1], d[2], shape, 1))
+            # inference: the network alone (the resume pairs ps and pt); quality of its
+            # output with u clamped to [0, 1], as the resume takes it
+            _, s_inf = timed(lambda: net.forward(d[0], d[1], d[2], shape, 1))
+            pst = net.predict_state(d[0], d[1], d[2], shape, 1)
             out["network_output"] = {"inference_seconds": s_inf, **quality(pst["u"])}
 
-            def warm():
-                w = net.predict_state(d[0], d[1], d[2], shape, 1)
+            def warm():   # u and the faces; the resume pairs ps and pt to them
+                w = net.forward(d[0], d[1], d[2], shape, 1)
                 return solver.binary_resume(*d, shape, w, 20000, err_bound, cc, st)
         else:
             (u0, pt0), s_inf = timed(lambda: net.forward(d[0], d[1], d[2], shape, 1))

@@ -114,7 +114,9 @@ def binary_resume(Cs, Ct, alpha, shape, state: dict, max_iter: int, err_bound: f
                   tile_x: int = 0, zchunk: int = 0, poll_every: int = 0, stream=None,
                   want_state: bool = False, time_kernel: bool = False, out=None) -> Result:
     """Continue a binary solve from a full state {u, ps, pt, bx, by, bz} (device tensors,
-    the convention of Result.state); k iterations then a resume of m equal k+m iterations."""
+    the convention of Result.state); k iterations then a resume of m equal k+m iterations.
+    Without "ps" and "pt" the library pairs them to u and the faces (the fixed point of
+    the source/sink update, as asets_nn_warm_state)."""
     ndim, ny, nx, nz = _ndim_dims(shape)
     N = ny * nx * nz
     dev = Cs.device
@@ -130,7 +132,7 @@ def binary_resume(Cs, Ct, alpha, shape, state: dict, max_iter: int, err_bound: f
           for k in ("ps", "pt", "bx", "by", "bz")} if want_state else {}
     rc = nat.lib().asets_binary_dev_resume(
         ndim, _ptr(Cs), _ptr(Ct), _ptr(alpha), ny, nx, nz, C.byref(p), _ptr(state["u"]),
-        _ptr(state["ps"]), _ptr(state["pt"]), _ptr(state["bx"]), _ptr(state["by"]),
+        _ptr(state.get("ps")), _ptr(state.get("pt")), _ptr(state["bx"]), _ptr(state["by"]),
         _ptr(state.get("bz") if ndim == 3 else None), _ptr(u), _ptr(cvg), itn.data_ptr(),
         C.byref(secs), *(_ptr(st.get(k)) for k in ("ps", "pt", "bx", "by", "bz")),
         C.byref(opts))

@@ -522,9 +522,10 @@ def pair_state(st: dict, Cs, Ct, shape, n=1, stream=None) -> dict:
 
 def warm_solve_state(net: WarmStartNetS2D, Cs, Ct, alpha, shape, max_iter, err_bound, cc,
                      steps, **kw):
-    """Predict a full state with the space-to-depth network, then resume the plain
-    solver from it (asets_binary_dev_resume)."""
+    """Predict u and the face flows with the space-to-depth network, then resume the
+    plain solver from them: asets_binary_dev_resume pairs ps and pt to them (as
+    predict_state does) while it lays the state out."""
     from . import solver
 
-    st = net.predict_state(Cs, Ct, alpha, shape, 1)
+    st = net.forward(Cs, Ct, alpha, shape, 1)
     return solver.binary_resume(Cs, Ct, alpha, shape, st, max_iter, err_bound, cc, steps, **kw)

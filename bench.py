@@ -306,8 +306,9 @@ def bench_binary3d(args):
                 net.load(wdir / wname)
 
                 def predict():
+                    # full state: u and the faces; the resume pairs ps and pt to them
                     if full:
-                        return net.predict_state(dCs, dCt, dA, shape, 1)
+                        return net.forward(dCs, dCt, dA, shape, 1)
                     u0, pt0 = net.forward(dCs, dCt, dA, shape, 1)
                     return {"u": u0, "pt": pt0}
 
@@ -324,7 +325,8 @@ def bench_binary3d(args):
                 t_inf, _, _ = _median_call(predict)
                 t_warm, rw, _ = _median_call(lambda: warm_solve(predict()))
                 e_warm = solver.energy_binary(dCs, dCt, dA, rw.u, shape, stream=stream)[0]
-                st = predict()
+                # the network's own output (u clamped to [0, 1] as the resume takes it)
+                st = net.predict_state(dCs, dCt, dA, shape, 1) if full else predict()
                 e_net = solver.energy_binary(dCs, dCt, dA, st["u"], shape, stream=stream)[0]
                 out[key] = {
                     "start": ("network full state (u, faces; ps, pt paired), resumed"

@@ -199,7 +199,12 @@ asets_status asets_binary_dev_state(int ndim, const float *Cs, const float *Ct,
  * the faces with axis index 0 are the zero-flux boundary and are taken as 0 whatever
  * the input holds.  A solve of k iterations followed by a resume of m iterations from
  * its state gives the same u (bit for bit) as one solve of k+m iterations.  bz0 is
- * ignored (may be NULL) when ndim == 2; ps..bz out may be NULL. */
+ * ignored (may be NULL) when ndim == 2; ps..bz out may be NULL.
+ * ps0 and pt0 may both be NULL (a predicted state of u and the faces): they are then
+ * paired to them as the fixed point of the source/sink update
+ * (asetsBinaryMF3D_mex.c:411-429): dv = div b, pt = Ct where u >= 1/2 and
+ * min(Cs - dv, Ct) elsewhere, ps = pt + dv, with u clamped to [0, 1] -- the pairing of
+ * asets_nn_warm_state, formed while the state is laid out, without its passes. */
 asets_status asets_binary_dev_resume(int ndim, const float *Cs, const float *Ct,
                                      const float *alpha, int64_t ny, int64_t nx, int64_t nz,
                                      const asets_params *p, const float *u0, const float *ps0,
