@@ -1206,8 +1206,8 @@ __device__ __forceinline__ double fused_tile(const FusedArgs<T> a, int lb) {
 // wave sums of the same 64 values, the 16 wave sums added in the same order --
 // so cvg has the same bits.  Every thread forms the final sum, so every thread
 // returns the decision; thread 0 writes cvg and the stop outputs when `write`:
-// flags->done, or with `at` flags->stop_at (the launch that every workgroup of
-// tests it).  scratch: >= 16 doubles of LDS, free on entry and on return.
+// flags->done, or, with `at` (every workgroup of the launch runs the test),
+// flags->stop_at.  scratch: >= 16 doubles of LDS, free on entry and on return.
 __device__ __forceinline__ bool stop_test(const DeferredStop &d, double *scratch, bool write,
                                           bool at = false) {
     constexpr int kFinalizeThreads = 1024;
@@ -1226,8 +1226,11 @@ __device__ __forceinline__ bool stop_test(const DeferredStop &d, double *scratch
         d.cvg[d.it] = c;
         if (stop) {
             d.flags->it_num = d.it;
-            if (at) d.flags->stop_at = d.it + 1;
-            else d.flags->done = 1;
+            if (at)   // read by other workgroups of this launch
+                __hip_atomic_store(&d.flags->stop_at, d.it + 1, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            else
+                d.flags->done = 1;
             *d.it_num = d.it;
         }
     }
@@ -1279,7 +1282,9 @@ __global__ __launch_bounds__(NT, WAVES) void kf_binary(FusedArgs<T> a) {
     // thread reads stop_at for the whole workgroup (its waves must agree).
     if (a.stop_all && a.stop.partials) {
         __shared__ int32_t at_s;
-        if (threadIdx.x == 0) at_s = a.flags->stop_at;
+        if (threadIdx.x == 0)
+            at_s = __hip_atomic_load(&a.flags->stop_at, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         const int32_t at = at_s;
         if (at != 0 && at - 1 != a.stop.it) return;
