@@ -487,21 +487,25 @@ __global__ void k_to_f32(float *dst, const T *src, int64_t n, int stride) {
 // of the column's nyb blocks: the column's position is divided out once per workgroup,
 // not per lane.
 template <typename T>
+// alpha (may be NULL): checked on the way (alpha <= 0 somewhere sets *bad and *done, as
+// k_alpha_check does), for a resumed solve whose first iteration does not check it.
 __global__ void k_blk_init(const float *Cs, const float *Ct, const float *u0, const float *ps0,
                            const float *pt0, const float *bx0, const float *by0,
                            const float *bz0, char *st, int ny, int nx, int nz, int nyb,
-                           int pair) {
+                           int pair, const float *alpha, int32_t *bad, int32_t *done) {
     using BL = BlkLayout<T>;
     // 32-bit indices: the layout's buffers stay below 2^31 bytes (fused_fits_blk)
     const int c = (int)blockIdx.x;            // column z * nx + x
     const int x = c % nx, z = c / nx;
     const int64_t g0 = (int64_t)c * ny;       // voxel (x, y = 0, z)
+    int alpha_bad = 0;
     for (int i = threadIdx.x; i < nyb * 64; i += blockDim.x) {
         const int yb = i >> 6, ln = i & 63;
         const int y = i;
         float u = 0.0f, ps = 0.0f, pt = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
         if (y < ny) {
             const int64_t g = g0 + y;
+            if (alpha && alpha[g] <= 0.0f) alpha_bad = 1;
             if (pair) {   // ps, pt paired to u and the faces (pair_voxel)
                 pair_voxel(u0, bx0, by0, bz0, Cs, Ct, g, x, y, z, ny, nx, nz,
                            (int64_t)ny * nx, u, ps, pt);
@@ -527,6 +531,10 @@ __global__ void k_blk_init(const float *Cs, const float *Ct, const float *u0, co
         Io<T>::put(bb, bx);
         Io<T>::put(bb + 1, by);
         Io<T>::put((T *)(blk + BL::kBZ) + ln, bz);
+    }
+    if (alpha_bad) {
+        atomicOr(bad, 1);
+        atomicOr(done, 1);
     }
 }
 
@@ -647,7 +655,9 @@ asets_status run_fused(const FusedRun &r, Carver &cv, int32_t *hflag) {
                                                          r.b0[0], r.b0[1], r.b0[2],
                                                          (char *)buf[0].u, (int)r.ny,
                                                          (int)r.nx, (int)r.nz, r.nyb,
-                                                         r.pair ? 1 : 0);
+                                                         r.pair ? 1 : 0,
+                                                         max_iter > 0 ? r.alpha : nullptr,
+                                                         &r.flags->bad_alpha, &r.flags->done);
         }
     } else if (first && max_iter == 0) {
         k_init_binary_cold_t<T><<<nb, kBlock, 0, s>>>(r.Cs, r.Ct, buf[0].u, buf[0].ps,
@@ -927,8 +937,9 @@ asets_status binary_dev(int ndim, const float *Cs, const float *Ct, const float 
     double *partials = cv.take<double>(2 * nparts);
 
     solve_setup(flags, it_num, max_iter, cvg, s);
-    // a fused cold start checks alpha in its first iteration
-    if (max_iter > 0 && !(fused && !u0)) alpha_check_async(alpha, N, flags, s);
+    // a fused cold start checks alpha in its first iteration, a fused block-interleaved
+    // resume while it lays the state out (k_blk_init)
+    if (max_iter > 0 && !(fused && (!u0 || blk))) alpha_check_async(alpha, N, flags, s);
     HIPCHK(hipGetLastError());
     int32_t itn = max_iter;
 
@@ -1500,7 +1511,8 @@ asets_status batched_dev(int32_t nvol, const asets_volume *vols, const asets_par
                 k_blk_init<T><<<cg.grid, cg.threads, 0, s>>>(b.Cs, b.Ct, nullptr, nullptr,
                                                              nullptr, nullptr, nullptr, nullptr,
                                                              (char *)b.u[0], b.ny, b.nx, b.nz,
-                                                             b.nyb, 0);
+                                                             b.nyb, 0, nullptr, nullptr,
+                                                             nullptr);
             } else
                 k_init_binary_cold_t<T><<<nb, kBlock, 0, s>>>(b.Cs, b.Ct, b.u[0], b.ps[0],
                                                              b.bx[0], b.bz[0], N);

@@ -531,7 +531,8 @@ def test_potts_deferred_stop_test_same_bits(cuda, shape, nlab):
 
 def test_alpha_error_every_solve(cuda):
     """alpha <= 0 anywhere (asetsBinaryMF3D_mex.c:317-319): the binary solves (fused fp32
-    and fp16, per-pass), Potts and the batched solve report it, the batched one naming
+    and fp16, per-pass; cold, warm and resumed), Potts and the batched solve report it,
+    the batched one naming
     the volume.  The check runs on the device before the first iteration and is read
     with the first stop-flag poll; an error leaves the library usable, and a following
     solve equals the oracle."""
@@ -547,6 +548,21 @@ def test_alpha_error_every_solve(cuda):
                              ("passes", "f32", 0.0)):
         with pytest.raises(nat.AsetsError, match="alpha"):
             solver.binary(*b, shape, 40, eb, 0.25, 0.11, kernel=kernel, flow=flow)
+    # warm starts and resumes (checked while the state is laid out, or by the per-pass
+    # solver's own check)
+    u0 = torch.full((int(np.prod(shape)),), 0.5, dtype=torch.float32, device=cuda)
+    pt0 = torch.minimum(b[0], b[1])
+    part = solver.binary(*g, shape, 3, 0.0, 0.25, 0.11, want_state=True)
+    full_state = dict(part.state, u=part.u)
+    paired = {k: full_state[k] for k in ("u", "bx", "by", "bz")}
+    for kernel, flow in (("fused", "f32"), ("fused", "f16"), ("passes", "f32")):
+        with pytest.raises(nat.AsetsError, match="alpha"):
+            solver.binary(*b, shape, 40, 0.0, 0.25, 0.11, u0=u0, pt0=pt0, kernel=kernel,
+                          flow=flow)
+        for st in (full_state, paired):
+            with pytest.raises(nat.AsetsError, match="alpha"):
+                solver.binary_resume(*b, shape, st, 10, 1e-4, 0.25, 0.11, kernel=kernel,
+                                     flow=flow)
     with pytest.raises(nat.AsetsError, match="volume 1: alpha"):
         solver.binary_batched([tuple(g) + (shape,), tuple(b) + (shape,)], 10, 0.0, 0.25, 0.11)
     Ctp, ap = workloads.random_smooth_potts((8, 6, 4), 3, seed=2)
