@@ -105,6 +105,15 @@ def test_bad_arguments_rejected_before_device_work():
                                C.byref(itn), None, None) == nat.ASETS_EBADARG
     assert L.asets_binary_warm(3, ptr, ptr, ptr, 4, 4, 2, C.byref(p), ptr, None, ptr, ptr,
                                C.byref(itn), None, None) == nat.ASETS_EBADARG
+    # resume: ps0 without pt0 (both or neither: neither pairs them), or no faces
+    cvg = ptr
+    assert L.asets_binary_dev_resume(3, ptr, ptr, ptr, 4, 4, 2, C.byref(p), ptr, ptr, None,
+                                     ptr, ptr, ptr, ptr, cvg, ptr, None, None, None, None,
+                                     None, None, None) == nat.ASETS_EBADARG
+    assert b"both, or neither" in L.asets_last_error()
+    assert L.asets_binary_dev_resume(3, ptr, ptr, ptr, 4, 4, 2, C.byref(p), ptr, None, None,
+                                     ptr, None, ptr, ptr, cvg, ptr, None, None, None, None,
+                                     None, None, None) == nat.ASETS_EBADARG
     # oversized plane
     assert L.asets_binary3d(ptr, ptr, ptr, 1 << 16, 1 << 15, 1, C.byref(p), ptr, ptr,
                             C.byref(itn), None, None) == nat.ASETS_ETOOBIG
